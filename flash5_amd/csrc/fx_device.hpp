@@ -1,0 +1,424 @@
+// fx_device.hpp -- device-side math of the unsplit hydro block update.
+//
+// Reference semantics (paths relative to
+// source/physics/Hydro/HydroMain/unsplit):
+//   primitive eigen system      hy_eigenParameters.F90:42-89,
+//                               hy_eigenValue.F90:33-50,
+//                               OnlyHydro/hy_eigenVector.F90:46-151
+//   slope limiters              hy_slopeLimiters.F90:38-60
+//   TVD slopes                  hy_TVDslope.F90:54-265
+//   transverse flux sigma       hy_upwindTransverseFlux.F90:43-211
+//   PPM + characteristic trace  hy_DataReconstructNormalDir_PPM.F90:73-958
+//   HLLC                        hy_HLLC.F90:53-324, hy_prim2con.F90,
+//                               hy_prim2flx.F90
+//
+// The eigenvectors are sparse (each left/right vector has at most three
+// non-zero entries); the products below keep only those entries, in the
+// reference's summation order, so that every non-zero result rounds the
+// same way as the dense 5x5 form.
+//
+// Primitive vectors are double[8]: dens, velx, vely, velz, pres, gamc, game,
+// eint.  Waves: 0 fast-left, 1 slow-left, 2 entropy, 3 slow-right,
+// 4 fast-right.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fx_hydro.h"
+
+namespace fx {
+
+enum { P_DENS = 0, P_VELX = 1, P_VELY = 2, P_VELZ = 3, P_PRES = 4, P_GAMC = 5,
+       P_GAME = 6, P_EINT = 7 };
+
+struct DevParams {
+  int ndim, nxb, nyb, nzb, ng, nunk;
+  int nx, ny, nz;            // guard-inclusive extents
+  long ncell;                // cells per tile
+  long voff[FX_NVAR_USED];   // var slot * ncell
+  int lo[3], hi[3];          // interior limits, 0-based inclusive
+  double gamma;
+  int limiter;
+  double beta;
+  int charLim, flat, steep, ctu3d, aux, riemann;
+  double fP, smlrho, smallE, eintSwitch, tiny;
+  double gammam1, gam1inv, ggprodinv, abar;
+  int exactMaxIter;
+  double exactTol;
+  int bext[3][3];            // [dir][axis] extent of the dir-band of state cells
+  long bcells[3];            // cells per dir-band
+  long boff[3];              // offset of each band in a tile's state scratch
+  long scellsPerTile;        // sum of band cells
+  int fext[3][3];            // [dir][axis] flux-array extents
+  long nface[3];
+  long icells;               // interior cells per tile
+};
+
+struct Eig {
+  double un, cf, ahinv, lfp, a2inv, rp, rho;
+};
+
+__device__ __forceinline__ bool eig_make(const double* V, int dir, Eig& e) {
+  e.rho = V[P_DENS];
+  e.un = V[P_VELX + dir];
+  const double a2 = V[P_GAMC] * V[P_PRES] / V[P_DENS];
+  if (!(a2 > 0.0)) return false;
+  e.cf = sqrt(a2);
+  const double dinv = 1.0 / V[P_DENS];
+  const double a2e = e.cf * e.cf;
+  e.a2inv = 1.0 / a2e;
+  e.ahinv = 0.5 / e.cf;
+  const double Na = 0.5 * e.a2inv;
+  e.lfp = dinv * Na;
+  e.rp = V[P_DENS] * a2e;
+  return true;
+}
+
+__device__ __forceinline__ void eig_lambda(const Eig& e, double lam[5]) {
+  lam[0] = e.un - e.cf;
+  lam[1] = e.un;
+  lam[2] = e.un;
+  lam[3] = e.un;
+  lam[4] = e.un + e.cf;
+}
+
+// L_n . d over the five primitive slots
+__device__ __forceinline__ double eig_left1(const Eig& e, int dir, int n, const double* d) {
+  const int v1 = 1 + dir, v2 = 1 + (dir + 1) % 3, v3 = 1 + (dir + 2) % 3;
+  switch (n) {
+    case 0: return (-e.ahinv) * d[v1] + e.lfp * d[P_PRES];
+    case 1: return d[v2];
+    case 2: return d[P_DENS] + (-e.a2inv) * d[P_PRES];
+    case 3: return d[v3];
+    default: return e.ahinv * d[v1] + e.lfp * d[P_PRES];
+  }
+}
+
+// out[0..4] = sum_n R_n * w[n]
+__device__ __forceinline__ void eig_right(const Eig& e, int dir, const double w[5], double* out) {
+  const int v1 = 1 + dir, v2 = 1 + (dir + 1) % 3, v3 = 1 + (dir + 2) % 3;
+  out[P_DENS] = e.rho * w[0] + w[2] + e.rho * w[4];
+  out[v1] = (-e.cf) * w[0] + e.cf * w[4];
+  out[v2] = w[1];
+  out[v3] = w[3];
+  out[P_PRES] = e.rp * w[0] + e.rp * w[4];
+}
+
+__device__ __forceinline__ double sgn1(double a) { return copysign(1.0, a); }
+
+__device__ __forceinline__ double limit(int kind, double a, double b, double beta) {
+  switch (kind) {
+    case FX_MINMOD: return .5 * (sgn1(a) + sgn1(b)) * fmin(fabs(a), fabs(b));
+    case FX_VANLEER: return (a * b <= 0.) ? 0. : 2. * a * b / (a + b);
+    case FX_LIMITED:
+      if (b > 0.) return fmax(fmax(0., fmin(beta * a, b)), fmin(a, beta * b));
+      return fmin(fmin(0., fmax(beta * a, b)), fmax(a, beta * b));
+    default:  // FX_MC
+      return (sgn1(a) + sgn1(b)) * fmin(fmin(fabs(a), .25 * fabs(a + b)), fabs(b));
+  }
+}
+
+// Limited slope of the cell whose eigen system is e (hy_TVDslope), projected
+// back to primitive slopes when characteristic limiting is on
+// (PPM.F90:523-570).  out[0..7].
+__device__ __forceinline__ void tvd_slope(const DevParams& P, int dir, const double* VL,
+                                          const double* V0, const double* VR, const Eig& e,
+                                          double out[8]) {
+  double dL[8], dR[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    dL[k] = V0[k] - VL[k];
+    dR[k] = VR[k] - V0[k];
+  }
+  if (P.limiter == FX_HYBRID) {
+    if (P.charLim) {
+      double w[5];
+#pragma unroll
+      for (int n = 0; n < 5; ++n) {
+        const double a = eig_left1(e, dir, n, dL), b = eig_left1(e, dir, n, dR);
+        w[n] = (n == 2) ? limit(FX_MC, a, b, P.beta) : limit(FX_MINMOD, a, b, P.beta);
+      }
+      eig_right(e, dir, w, out);
+      for (int k = P_GAMC; k <= P_EINT; ++k) out[k] = limit(FX_MC, dL[k], dR[k], P.beta);
+    } else {
+      out[P_DENS] = limit(FX_MC, dL[P_DENS], dR[P_DENS], P.beta);
+      for (int k = P_VELX; k <= P_PRES; ++k) out[k] = limit(FX_MINMOD, dL[k], dR[k], P.beta);
+      out[P_GAMC] = limit(FX_MC, dL[P_GAMC], dR[P_GAMC], P.beta);
+      out[P_GAME] = limit(FX_MC, dL[P_GAME], dR[P_GAME], P.beta);
+      out[P_EINT] = 0.;
+    }
+    return;
+  }
+  if (P.charLim) {
+    double w[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n)
+      w[n] = limit(P.limiter, eig_left1(e, dir, n, dL), eig_left1(e, dir, n, dR), P.beta);
+    eig_right(e, dir, w, out);
+    for (int k = P_GAMC; k <= P_EINT; ++k) out[k] = limit(P.limiter, dL[k], dR[k], P.beta);
+  } else {
+    for (int k = 0; k < 8; ++k) out[k] = limit(P.limiter, dL[k], dR[k], P.beta);
+  }
+}
+
+// Transverse flux sigma of one cell (hy_upwindTransverseFlux, order 1).
+__device__ __forceinline__ void trans_sigma(const DevParams& P, int dir, const double* vm1,
+                                            const double* vc0, const double* vp1, const Eig& e,
+                                            double sig[8]) {
+  double lam[5];
+  eig_lambda(e, lam);
+  const int v1 = 1 + dir, v2 = 1 + (dir + 1) % 3, v3 = 1 + (dir + 2) % 3;
+#pragma unroll
+  for (int v = 0; v < 8; ++v) sig[v] = 0.;
+  double tf[5];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    if (lam[n] > 0.) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) tf[v] = vc0[v] - vm1[v];
+    } else {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) tf[v] = vp1[v] - vc0[v];
+    }
+    const double db = eig_left1(e, dir, n, tf);
+    switch (n) {
+      case 0:
+        sig[P_DENS] = sig[P_DENS] + lam[0] * e.rho * db;
+        sig[v1] = sig[v1] + lam[0] * (-e.cf) * db;
+        sig[P_PRES] = sig[P_PRES] + lam[0] * e.rp * db;
+        break;
+      case 1: sig[v2] = sig[v2] + lam[1] * db; break;
+      case 2: sig[P_DENS] = sig[P_DENS] + lam[2] * db; break;
+      case 3: sig[v3] = sig[v3] + lam[3] * db; break;
+      default:
+        sig[P_DENS] = sig[P_DENS] + lam[4] * e.rho * db;
+        sig[v1] = sig[v1] + lam[4] * e.cf * db;
+        sig[P_PRES] = sig[P_PRES] + lam[4] * e.rp * db;
+        break;
+    }
+  }
+  if (P.aux) {
+    const double d = (lam[2] > 0.) ? vc0[P_EINT] - vm1[P_EINT] : vp1[P_EINT] - vc0[P_EINT];
+    sig[P_EINT] = lam[2] * d;
+  }
+}
+
+// Full PPM reconstruction of one cell along dir (PPM.F90:378-842).
+// Returns false on an imaginary sound speed in a neighbour's eigen system.
+__device__ __forceinline__ bool ppm_full(const DevParams& P, int dir, double dtn, double delta,
+                                         const double* Vmm, const double* Vm, const double* Vc,
+                                         const double* Vp, const double* Vpp, double flatC,
+                                         const Eig& e0, double Wp[8], double Wm[8]) {
+  Eig eP, eN;
+  if (!eig_make(Vp, dir, eP)) return false;
+  if (!eig_make(Vm, dir, eN)) return false;
+  double dbN[8], db0[8], dbP[8];
+  tvd_slope(P, dir, Vmm, Vm, Vc, eN, dbN);
+  tvd_slope(P, dir, Vm, Vc, Vp, e0, db0);
+  tvd_slope(P, dir, Vc, Vp, Vpp, eP, dbP);
+  const int endVar = P.aux ? P_EINT : P_PRES;
+  double vL[8], vR[8];
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    vL[v] = 0.;
+    vR[v] = 0.;
+  }
+  for (int v = 0; v <= endVar; ++v) {
+    vL[v] = 0.5 * (Vc[v] + Vm[v]) - (db0[v] - dbN[v]) / 6.;
+    vR[v] = 0.5 * (Vc[v] + Vp[v]) - (dbP[v] - db0[v]) / 6.;
+  }
+  if (P.steep) {
+    const double temp1 = Vp[P_DENS] - Vm[P_DENS];
+    if (fabs(temp1) > P.tiny) {
+      const double d2R = (Vpp[P_DENS] - 2. * Vp[P_DENS] + Vc[P_DENS]) / (6. * delta * delta);
+      const double d2L = (Vc[P_DENS] - 2. * Vm[P_DENS] + Vmm[P_DENS]) / (6. * delta * delta);
+      double eta = (d2L - d2R) * (delta * delta) / temp1;
+      if (d2R * d2L >= 0.) eta = 0.;
+      if (0.01 * fmin(Vp[P_DENS], Vm[P_DENS]) - fabs(Vp[P_DENS] - Vm[P_DENS]) >= 0.) eta = 0.;
+      eta = fmax(0., fmin(1., 20.0 * (eta - 0.05)));
+      const double t2 = fabs(Vp[P_PRES] - Vm[P_PRES]) / fmin(Vp[P_PRES], Vm[P_PRES]);
+      const double t3 = fabs(Vp[P_DENS] - Vm[P_DENS]) / fmin(Vp[P_DENS], Vm[P_DENS]);
+      if (Vc[P_GAME] * 0.1 * t3 - t2 < 0.0) eta = 0.;
+      vL[P_DENS] = vL[P_DENS] * (1. - eta) + (Vm[P_DENS] + 0.5 * dbN[P_DENS]) * eta;
+      vR[P_DENS] = vR[P_DENS] * (1. - eta) + (Vp[P_DENS] - 0.5 * dbP[P_DENS]) * eta;
+    }
+  }
+  if (P.flat) {
+    for (int v = 0; v <= endVar; ++v) {
+      vL[v] = flatC * Vc[v] + (1.0 - flatC) * vL[v];
+      vR[v] = flatC * Vc[v] + (1.0 - flatC) * vR[v];
+    }
+  }
+  double dW[8], W6[8];
+  for (int n = 0; n <= endVar; ++n) {
+    if ((vR[n] - Vc[n]) * (Vc[n] - vL[n]) <= 0.) {
+      vL[n] = Vc[n];
+      vR[n] = Vc[n];
+    }
+    double dd = vR[n] - vL[n];
+    if (6. * (vR[n] - vL[n]) * (Vc[n] - 0.5 * (vL[n] + vR[n])) > dd * dd) vL[n] = 3. * Vc[n] - 2. * vR[n];
+    dd = vR[n] - vL[n];
+    if (6. * (vR[n] - vL[n]) * (Vc[n] - 0.5 * (vL[n] + vR[n])) < -(dd * dd)) vR[n] = 3. * Vc[n] - 2. * vL[n];
+    dW[n] = vR[n] - vL[n];
+    W6[n] = 6. * (Vc[n] - 0.5 * (vR[n] + vL[n]));
+  }
+  double lam[5];
+  eig_lambda(e0, lam);
+  const double hdtn = 0.5 * dtn;
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    Wp[v] = 0.;
+    Wm[v] = 0.;
+  }
+  {
+    double lmax = fmax(lam[4], 0.);
+    double lmin = -fmin(lam[0], 0.);
+    for (int v = 0; v <= P_PRES; ++v) {
+      Wp[v] = vR[v] - lmax * hdtn * (dW[v] - (1.0 - lmax * hdtn * 4. / 3.) * W6[v]);
+      Wm[v] = vL[v] + lmin * hdtn * (dW[v] + (1.0 - lmin * hdtn * 4. / 3.) * W6[v]);
+    }
+    lmax = fmax(lam[2], 0.);
+    lmin = -fmin(lam[2], 0.);
+    for (int v = P_PRES + 1; v <= endVar; ++v) {
+      Wp[v] = vR[v] - lmax * hdtn * (dW[v] - (1.0 - lmax * hdtn * 4. / 3.) * W6[v]);
+      Wm[v] = vL[v] + lmin * hdtn * (dW[v] + (1.0 - lmin * hdtn * 4. / 3.) * W6[v]);
+    }
+  }
+  Wp[P_GAMC] = Vc[P_GAMC];
+  Wp[P_GAME] = Vc[P_GAME];
+  Wm[P_GAMC] = Vc[P_GAMC];
+  Wm[P_GAME] = Vc[P_GAME];
+  // characteristic tracing (PPM.F90:751-800), HLL-type branch for all
+  // solvers but Roe
+  double sL[5] = {0., 0., 0., 0., 0.}, sR[5] = {0., 0., 0., 0., 0.};
+  double mW6[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) mW6[v] = -W6[v];
+  const int v1 = 1 + dir, v2 = 1 + (dir + 1) % 3, v3 = 1 + (dir + 2) % 3;
+  const bool roe = (P.riemann == FX_ROE);
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    const double cA = eig_left1(e0, dir, n, dW);
+    const double cB = eig_left1(e0, dir, n, mW6);
+    const double x = dtn * lam[n];
+    const double aL = .5 * (-1. - x), bL = .25 * (1. + 2. * dtn * lam[n] + 4. / 3. * (x * x));
+    const double aR = .5 * (1. - x), bR = .25 * (1. - 2. * dtn * lam[n] + 4. / 3. * (x * x));
+    const bool doL = !roe || lam[n] < 0.;
+    const bool doR = !roe || lam[n] > 0.;
+    // non-zero right-eigenvector entries of wave n
+    int rv[3];
+    double rc[3];
+    int nr;
+    switch (n) {
+      case 0: rv[0] = P_DENS; rc[0] = e0.rho; rv[1] = v1; rc[1] = -e0.cf; rv[2] = P_PRES; rc[2] = e0.rp; nr = 3; break;
+      case 1: rv[0] = v2; rc[0] = 1.; nr = 1; break;
+      case 2: rv[0] = P_DENS; rc[0] = 1.; nr = 1; break;
+      case 3: rv[0] = v3; rc[0] = 1.; nr = 1; break;
+      default: rv[0] = P_DENS; rc[0] = e0.rho; rv[1] = v1; rc[1] = e0.cf; rv[2] = P_PRES; rc[2] = e0.rp; nr = 3; break;
+    }
+    for (int q = 0; q < nr; ++q) {
+      const int v = rv[q];
+      const double R = rc[q];
+      if (doL) sL[v] = sL[v] + (aL * R * cA + bL * R * cB);
+      if (doR) sR[v] = sR[v] + (aR * R * cA + bR * R * cB);
+    }
+  }
+#pragma unroll
+  for (int v = 0; v <= P_PRES; ++v) {
+    Wm[v] = Vc[v] + W6[v] / 12. + sL[v];
+    Wp[v] = Vc[v] + W6[v] / 12. + sR[v];
+  }
+  return true;
+}
+
+// ---- HLLC (hy_HLLC.F90:53-324) ----
+__device__ __forceinline__ void prim2flx(int dir, const double* V, double fP, double F[6]) {
+  const double u2 = V[P_VELX] * V[P_VELX] + V[P_VELY] * V[P_VELY] + V[P_VELZ] * V[P_VELZ];
+  const double E = 0.5 * V[P_DENS] * u2 + V[P_PRES] / (V[P_GAME] - 1.);
+  const double un = V[P_VELX + dir];
+  F[5] = V[P_PRES];
+  F[0] = V[P_DENS] * un;
+  F[1] = F[0] * V[P_VELX];
+  F[2] = F[0] * V[P_VELY];
+  F[3] = F[0] * V[P_VELZ];
+  F[1 + dir] = F[1 + dir] + fP * V[P_PRES];
+  F[4] = (E + V[P_PRES]) * un;
+}
+
+// returns 0 ok, 1 imaginary sound speed
+__device__ __forceinline__ int hllc(const DevParams& P, int dir, const double* Vm,
+                                    const double* Vp, double F[6]) {
+  const double velNL = Vm[P_VELX + dir], velNR = Vp[P_VELX + dir];
+  const double aL2 = Vm[P_GAMC] * Vm[P_PRES] / Vm[P_DENS];
+  const double aR2 = Vp[P_GAMC] * Vp[P_PRES] / Vp[P_DENS];
+  if ((Vm[P_DENS] < P.tiny && Vm[P_DENS] > 0.) || (Vp[P_DENS] < P.tiny && Vp[P_DENS] > 0.) ||
+      (Vm[P_PRES] < P.tiny && Vm[P_PRES] > 0.) || (Vp[P_PRES] < P.tiny && Vp[P_PRES] > 0.)) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) F[f] = 0.;
+    return 0;
+  }
+  if (aL2 < 0. || aR2 < 0.) return 1;
+  const double cfL = sqrt(aL2), cfR = sqrt(aR2);
+  const double SL = fmin(velNL - cfL, velNR - cfR);
+  const double SR = fmax(velNL + cfL, velNR + cfR);
+  const double tpL = Vm[P_PRES], tpR = Vp[P_PRES];
+  double qStar = (Vp[P_DENS] * velNR * (SR - velNR) - Vm[P_DENS] * velNL * (SL - velNL) + tpL - tpR -
+                  0. * 0. + 0. * 0.);
+  qStar = qStar / (Vp[P_DENS] * (SR - velNR) - Vm[P_DENS] * (SL - velNL));
+  if (SL >= 0.) {
+    prim2flx(dir, Vm, P.fP, F);
+    return 0;
+  }
+  if (!(qStar < 0. && SR >= 0.) && !(SL < 0. && qStar >= 0.)) {
+    prim2flx(dir, Vp, P.fP, F);
+    return 0;
+  }
+  const bool left = (SL < 0. && qStar >= 0.);
+  const double* V = left ? Vm : Vp;
+  const double S = left ? SL : SR;
+  const double velN = left ? velNL : velNR;
+  const double tp = left ? tpL : tpR;
+  const double numer = S - velN, denom = S - qStar;
+  const double pStar = Vm[P_DENS] * (SL - velNL) * (qStar - velNL) + tpL;
+  double Fs[6];
+  prim2flx(dir, V, P.fP, Fs);
+  // conserved state of the chosen side (hy_prim2con.F90)
+  const double u2 = V[P_VELX] * V[P_VELX] + V[P_VELY] * V[P_VELY] + V[P_VELZ] * V[P_VELZ];
+  double Uc[5];
+  Uc[0] = V[P_DENS];
+  Uc[1] = V[P_DENS] * V[P_VELX];
+  Uc[2] = V[P_DENS] * V[P_VELY];
+  Uc[3] = V[P_DENS] * V[P_VELZ];
+  Uc[4] = 0.5 * V[P_DENS] * u2 + V[P_PRES] / (V[P_GAME] - 1.) + 0.5 * 0.;
+  double Us[5];
+  const double dStar = Uc[0] * numer / denom;
+  Us[0] = dStar;
+  Us[4] = Uc[4] * numer / denom + ((pStar * qStar - tp * velN)) / denom;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) Us[1 + m] = (m == dir) ? dStar * qStar : Uc[1 + m] * numer / denom;
+#pragma unroll
+  for (int f = 0; f < 5; ++f) F[f] = Fs[f] + S * (Us[f] - Uc[f]);
+  F[5] = Fs[5];
+  return 0;
+}
+
+// ---- gamma-law EOS, MODE_DENS_EI on one cell (Eos_getData.F90:189-215,
+// eos_idealGamma.F90:323-333, Eos_putData.F90:197-210) ----
+struct EosOut {
+  double pres, temp, eint, ener, game;
+};
+__device__ __forceinline__ EosOut eos_dens_ei(const DevParams& P, double dens, double vx, double vy,
+                                              double vz, double eintIn, double enerIn) {
+  const double ekin = 0.5 * (vx * vx + vy * vy + vz * vz);
+  double ie = eintIn;
+  if (enerIn - ekin > fmax(P.smallE, P.eintSwitch * ekin)) ie = enerIn - ekin;
+  ie = fmax(ie, P.smallE);
+  EosOut o;
+  o.eint = ie;
+  o.pres = dens * ie * P.gam1inv;
+  o.temp = ie * P.ggprodinv * P.abar;
+  o.ener = ie + ekin;
+  o.game = o.pres / (ie * dens) + 1;
+  return o;
+}
+
+}  // namespace fx

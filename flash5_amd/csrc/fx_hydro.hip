@@ -1,0 +1,852 @@
+// fx_hydro.hip -- MI355X (gfx950) kernels and C ABI of the unsplit hydro
+// block update on the ideal gamma-law EOS.
+//
+// Pipeline per call (one launch each, all tiles of the call batched):
+//   k_states   one thread per (tile, direction, state cell of that
+//              direction's band): flattening coefficient, PPM with
+//              characteristic tracing, 2-D CTU transverse corrections,
+//              3-D full-CTU corner terms, positivity fallback
+//              (hy_getRiemannState.F90:356-1327, hy_dataReconstOneStep.F90)
+//   k_fluxes   one thread per (tile, direction, face): Riemann solver and
+//              aux internal-energy fluxes (hy_getFaceFlux.F90:289-500)
+//   k_update   one thread per interior cell: conservative update, aux
+//              internal-energy update, energy fix, EOS
+//              (hy_unsplitUpdate.F90:440-925, hy_energyFix.F90:188-251,
+//              Eos_wrapped MODE_DENS_EI)
+// State cells of direction d are the cells whose d-coordinate lies in
+// [lo-1, hi+1] and whose other coordinates are interior: exactly the cells
+// the reference stores face states for (hy_getRiemannState.F90:553-591).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "fx_device.hpp"
+#include "fx_hydro.h"
+
+using namespace fx;
+
+namespace {
+
+constexpr int kStateComps = 12;  // Wp and Wn of dens, vx, vy, vz, pres, eint
+constexpr int kThreads = 256;
+
+struct DevStatus {
+  int code;
+  int tile, i, j, k, dir;
+  unsigned long long fallbacks;
+  unsigned long long newtonIters;
+  unsigned long long newtonFallbacks;
+};
+
+struct Launch {
+  const fx_tile* tiles;  // device array
+  int nTiles;
+  double* states;        // scratch: [tile][band][comp][cell]
+  DevStatus* status;
+  double dt;
+};
+
+__device__ void report_error(DevStatus* s, int code, int tile, int i, int j, int k, int dir) {
+  if (atomicCAS(&s->code, 0, code) == 0) {
+    s->tile = tile;
+    s->i = i;
+    s->j = j;
+    s->k = k;
+    s->dir = dir;
+  }
+}
+
+__device__ __forceinline__ long cidx(const DevParams& P, int i, int j, int k) {
+  return ((long)k * P.ny + j) * P.nx + i;
+}
+
+__device__ __forceinline__ void load_prim(const DevParams& P, const double* __restrict__ U, long c,
+                                          double V[8]) {
+  V[P_DENS] = U[P.voff[FX_DENS] + c];
+  V[P_VELX] = U[P.voff[FX_VELX] + c];
+  V[P_VELY] = U[P.voff[FX_VELY] + c];
+  V[P_VELZ] = U[P.voff[FX_VELZ] + c];
+  V[P_PRES] = U[P.voff[FX_PRES] + c];
+  V[P_GAMC] = U[P.voff[FX_GAMC] + c];
+  V[P_GAME] = U[P.voff[FX_GAME] + c];
+  V[P_EINT] = U[P.voff[FX_EINT] + c];
+}
+
+// FlatTilde of one cell along dir (hy_getRiemannState.F90:365-404)
+__device__ __forceinline__ double flat_tilde(const DevParams& P, const double* __restrict__ U,
+                                             long c, long s, int dir) {
+  const double* pr = U + P.voff[FX_PRES];
+  const double* vn = U + P.voff[FX_VELX + dir];
+  const double dp1 = (pr[c + s] - pr[c - s]);
+  const double dp2 = (pr[c + 2 * s] - pr[c - 2 * s]);
+  const double dv1 = vn[c + s] - vn[c - s];
+  const double presL = pr[c - s], presR = pr[c + s];
+  const double Sp = (fabs(dp2) > 1.e-15) ? dp1 / dp2 - 0.75 : 0.;
+  double ft = fmax(0.0, fmin(1.0, 10.0 * Sp));
+  if ((fabs(dp1) / fmin(presL, presR) < 1. / 3.) || dv1 > 0.) ft = 0.;
+  return ft;
+}
+
+// FlatCoeff (hy_getRiemannState.F90:408-453)
+__device__ __forceinline__ double flat_coeff(const DevParams& P, const double* __restrict__ U,
+                                             long c, long s, int dir) {
+  const double* pr = U + P.voff[FX_PRES];
+  const double dp1 = (pr[c + s] - pr[c - s]);
+  const double f0 = flat_tilde(P, U, c, s, dir);
+  if (dp1 < 0.0) return fmax(f0, flat_tilde(P, U, c + s, s, dir));
+  if (dp1 == 0.) return f0;
+  return fmax(f0, flat_tilde(P, U, c - s, s, dir));
+}
+
+__device__ __forceinline__ long axis_stride(const DevParams& P, int a) {
+  return a == 0 ? 1L : (a == 1 ? (long)P.nx : (long)P.nx * P.ny);
+}
+
+// sigma of cell c along direction sd (transverse-only PPM call)
+__device__ __forceinline__ bool sigma_at(const DevParams& P, const double* __restrict__ U, long c,
+                                         int sd, double sig[8], Eig* eout) {
+  const long s = axis_stride(P, sd);
+  double Vm[8], Vc[8], Vp[8];
+  load_prim(P, U, c - s, Vm);
+  load_prim(P, U, c, Vc);
+  load_prim(P, U, c + s, Vp);
+  Eig e;
+  if (!eig_make(Vc, sd, e)) return false;
+  trans_sigma(P, sd, Vm, Vc, Vp, e, sig);
+  if (eout) *eout = e;
+  return true;
+}
+
+__global__ void __launch_bounds__(kThreads) k_states(DevParams P, Launch L) {
+  const long perTile = P.scellsPerTile;
+  const long total = perTile * L.nTiles;
+  for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+       gid += (long)gridDim.x * blockDim.x) {
+    const int t = (int)(gid / perTile);
+    long r = gid - (long)t * perTile;
+    int d = 0;
+    while (d < P.ndim - 1 && r >= P.bcells[d]) {
+      r -= P.bcells[d];
+      ++d;
+    }
+    // band-local coordinates (x fastest)
+    const int x0 = P.bext[d][0], x1 = P.bext[d][1];
+    const int t0 = (int)(r % x0);
+    const int t1 = (int)((r / x0) % x1);
+    const int t2 = (int)(r / ((long)x0 * x1));
+    int ci[3] = {t0 + P.lo[0], t1 + P.lo[1], t2 + P.lo[2]};
+    ci[d] -= 1;
+    const fx_tile& T = L.tiles[t];
+    const double* __restrict__ U = T.U;
+    const long c = cidx(P, ci[0], ci[1], ci[2]);
+    const long sd = axis_stride(P, d);
+    const double dt = L.dt;
+
+    double Vc[8], Vm[8], Vp[8], Vmm[8], Vpp[8];
+    load_prim(P, U, c, Vc);
+    load_prim(P, U, c - sd, Vm);
+    load_prim(P, U, c + sd, Vp);
+    load_prim(P, U, c - 2 * sd, Vmm);
+    load_prim(P, U, c + 2 * sd, Vpp);
+    Eig e0;
+    double Wp[8], Wn[8];
+    const double flat = P.flat ? flat_coeff(P, U, c, sd, d) : 0.;
+    bool ok = eig_make(Vc, d, e0);
+    if (ok) ok = ppm_full(P, d, dt / T.del[d], T.del[d], Vmm, Vm, Vc, Vp, Vpp, flat, e0, Wp, Wn);
+    // transverse sigma at this cell and 2-D CTU corrections
+    // (hy_dataReconstOneStep.F90:906-951)
+    double sigo[3][8];
+    Eig eo[3];
+    for (int o = 0; o < P.ndim && ok; ++o) {
+      if (o == d) continue;
+      ok = sigma_at(P, U, c, o, sigo[o], &eo[o]);
+    }
+    if (!ok) {
+      report_error(L.status, 1, t, ci[0], ci[1], ci[2], d);
+      continue;
+    }
+    for (int o = 0; o < P.ndim; ++o) {
+      if (o == d) continue;
+      const double hdt = 0.5 * dt / T.del[o];
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        Wn[v] = Wn[v] - hdt * sigo[o][v];
+        Wp[v] = Wp[v] - hdt * sigo[o][v];
+      }
+    }
+    if (Wn[P_DENS] < 0. || Wp[P_DENS] < 0. || Wn[P_PRES] < 0. || Wp[P_PRES] < 0.) {
+      atomicAdd(&L.status->fallbacks, 1ULL);
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        Wn[v] = Vc[v];
+        Wp[v] = Vc[v];
+      }
+    }
+    // 3-D full CTU (hy_getRiemannState.F90:952-1327): two corner terms per
+    // state: sigma of direction sd at the +-1 neighbours along nbr, upwinded
+    // with the eigen system of direction nbr at this cell.
+    if (P.ndim == 3 && P.ctu3d) {
+      const int o1 = (d + 1) % 3, o2 = (d + 2) % 3;
+      // term order: X (ZY over j? no: YZ then ZY), Y (ZX then XZ), Z (XY then YX)
+      int sdA, nbA, sdB, nbB;
+      if (d == 0) { sdA = 2; nbA = 1; sdB = 1; nbB = 2; }
+      else if (d == 1) { sdA = 0; nbA = 2; sdB = 2; nbB = 0; }
+      else { sdA = 1; nbA = 0; sdB = 0; nbB = 1; }
+      (void)o1;
+      (void)o2;
+      double tf[2][8];
+      const int sds[2] = {sdA, sdB}, nbs[2] = {nbA, nbB};
+      for (int q = 0; q < 2 && ok; ++q) {
+        const long sn = axis_stride(P, nbs[q]);
+        double sm[8], sp[8];
+        ok = sigma_at(P, U, c - sn, sds[q], sm, nullptr) && sigma_at(P, U, c + sn, sds[q], sp, nullptr);
+        if (ok) trans_sigma(P, nbs[q], sm, sigo[sds[q]], sp, eo[nbs[q]], tf[q]);
+      }
+      if (!ok) {
+        report_error(L.status, 1, t, ci[0], ci[1], ci[2], d);
+        continue;
+      }
+      const int a = (d + 1) % 3, b = (d + 2) % 3;
+      const double fac = dt * dt / (6. * T.del[a] * T.del[b]);
+      // dt2dydz6 = dt*dt/(6*dy*dz), dt2dzdx6 = dt*dt/(6*dz*dx),
+      // dt2dxdy6 = dt*dt/(6*dx*dy): the (a, b) pairs (y,z), (z,x), (x,y)
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        Wp[v] = Wp[v] + (tf[0][v] + tf[1][v]) * fac;
+        Wn[v] = Wn[v] + (tf[0][v] + tf[1][v]) * fac;
+      }
+      if (Wp[P_DENS] <= 0. || Wn[P_DENS] <= 0. || Wp[P_PRES] <= 0. || Wn[P_PRES] <= 0.) {
+        atomicAdd(&L.status->fallbacks, 1ULL);
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+          Wn[v] = Vc[v];
+          Wp[v] = Vc[v];
+        }
+      }
+    }
+    // store Wp (comps 0..5) and Wn (6..11): dens, vx, vy, vz, pres, eint
+    double* S = L.states + (long)t * perTile * kStateComps + P.boff[d] * kStateComps;
+    const long bc = P.bcells[d];
+    const long ri = gid - (long)t * perTile - P.boff[d];
+    S[0 * bc + ri] = Wp[P_DENS];
+    S[1 * bc + ri] = Wp[P_VELX];
+    S[2 * bc + ri] = Wp[P_VELY];
+    S[3 * bc + ri] = Wp[P_VELZ];
+    S[4 * bc + ri] = Wp[P_PRES];
+    S[5 * bc + ri] = Wp[P_EINT];
+    S[6 * bc + ri] = Wn[P_DENS];
+    S[7 * bc + ri] = Wn[P_VELX];
+    S[8 * bc + ri] = Wn[P_VELY];
+    S[9 * bc + ri] = Wn[P_VELZ];
+    S[10 * bc + ri] = Wn[P_PRES];
+    S[11 * bc + ri] = Wn[P_EINT];
+  }
+}
+
+// band-local linear index of block cell (i,j,k) in direction d's band
+__device__ __forceinline__ long band_index(const DevParams& P, int d, int i, int j, int k) {
+  int ci[3] = {i - P.lo[0], j - P.lo[1], k - P.lo[2]};
+  ci[d] += 1;
+  return ((long)ci[2] * P.bext[d][1] + ci[1]) * P.bext[d][0] + ci[0];
+}
+
+__global__ void __launch_bounds__(kThreads) k_fluxes(DevParams P, Launch L) {
+  const long perTile = P.nface[0] + P.nface[1] + P.nface[2];
+  const long total = perTile * L.nTiles;
+  for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+       gid += (long)gridDim.x * blockDim.x) {
+    const int t = (int)(gid / perTile);
+    long r = gid - (long)t * perTile;
+    int d = 0;
+    while (d < P.ndim - 1 && r >= P.nface[d]) {
+      r -= P.nface[d];
+      ++d;
+    }
+    const int f0 = P.fext[d][0], f1 = P.fext[d][1];
+    const int a0 = (int)(r % f0);
+    const int a1 = (int)((r / f0) % f1);
+    const int a2 = (int)(r / ((long)f0 * f1));
+    const int i = a0 + P.lo[0], j = a1 + P.lo[1], k = a2 + P.lo[2];
+    int im[3] = {i, j, k};
+    im[d] -= 1;
+    const fx_tile& T = L.tiles[t];
+    const double* __restrict__ U = T.U;
+    const double* S = L.states + (long)t * P.scellsPerTile * kStateComps + P.boff[d] * kStateComps;
+    const long bc = P.bcells[d];
+    const long bl = band_index(P, d, im[0], im[1], im[2]);
+    const long br = band_index(P, d, i, j, k);
+    double VL[8], VR[8];
+    VL[P_DENS] = S[0 * bc + bl];
+    VL[P_VELX] = S[1 * bc + bl];
+    VL[P_VELY] = S[2 * bc + bl];
+    VL[P_VELZ] = S[3 * bc + bl];
+    VL[P_PRES] = S[4 * bc + bl];
+    VL[P_EINT] = S[5 * bc + bl];
+    VR[P_DENS] = S[6 * bc + br];
+    VR[P_VELX] = S[7 * bc + br];
+    VR[P_VELY] = S[8 * bc + br];
+    VR[P_VELZ] = S[9 * bc + br];
+    VR[P_PRES] = S[10 * bc + br];
+    VR[P_EINT] = S[11 * bc + br];
+    // face-state gammas equal the cell's (PPM.F90:733-736 under
+    // FLASH_EOS_GAMMA, and the first-order fallback)
+    const long cl = cidx(P, im[0], im[1], im[2]), cr = cidx(P, i, j, k);
+    VL[P_GAMC] = U[P.voff[FX_GAMC] + cl];
+    VL[P_GAME] = U[P.voff[FX_GAME] + cl];
+    VR[P_GAMC] = U[P.voff[FX_GAMC] + cr];
+    VR[P_GAME] = U[P.voff[FX_GAME] + cr];
+    double F[8];
+    int ierr = hllc(P, d, VL, VR, F);
+    if (ierr) {
+      report_error(L.status, 2, t, i, j, k, d);
+      continue;
+    }
+    F[FX_NFLUX - 2] = 0.;
+    F[FX_NFLUX - 1] = 0.;
+    if (P.aux) {
+      if (F[0] > 0.) {
+        F[7] = F[0] / VL[P_DENS];
+        F[6] = F[0] * VL[P_EINT];
+      } else {
+        F[7] = F[0] / VR[P_DENS];
+        F[6] = F[0] * VR[P_EINT];
+      }
+    }
+    double* Fo = T.flux[d];
+    const long nf = P.nface[d];
+#pragma unroll
+    for (int q = 0; q < FX_NFLUX; ++q) Fo[q * nf + r] = F[q];
+  }
+}
+
+__device__ __forceinline__ long face_index(const DevParams& P, int d, int i, int j, int k) {
+  return ((long)(k - P.lo[2]) * P.fext[d][1] + (j - P.lo[1])) * P.fext[d][0] + (i - P.lo[0]);
+}
+
+// FROM_STATES: face-averaged pressure from this call's states; otherwise
+// read from the tile's pavg array (hy_updateSolution path).
+template <bool FROM_STATES, bool WRITE_PAVG_ONLY>
+__global__ void __launch_bounds__(kThreads) k_update(DevParams P, Launch L,
+                                                     const unsigned char* __restrict__ inPlace) {
+  const long perTile = P.icells;
+  const long total = perTile * L.nTiles;
+  const int nxi = P.nxb, nyi = P.ndim > 1 ? P.nyb : 1;
+  for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+       gid += (long)gridDim.x * blockDim.x) {
+    const int t = (int)(gid / perTile);
+    const long r = gid - (long)t * perTile;
+    const fx_tile& T = L.tiles[t];
+    const bool doUpdate = WRITE_PAVG_ONLY ? false : (inPlace ? inPlace[t] != 0 : true);
+    const int i = (int)(r % nxi) + P.lo[0];
+    const int j = (int)((r / nxi) % nyi) + P.lo[1];
+    const int k = (int)(r / ((long)nxi * nyi)) + P.lo[2];
+    double pres = 0.;
+    if (P.aux) {
+      if (FROM_STATES) {
+        const double* Sb = L.states + (long)t * P.scellsPerTile * kStateComps;
+        for (int d = 0; d < P.ndim; ++d) {
+          const double* S = Sb + P.boff[d] * kStateComps;
+          const long bc = P.bcells[d];
+          const long bi = band_index(P, d, i, j, k);
+          const double pN = S[10 * bc + bi], pP = S[4 * bc + bi];
+          if (d == 0) pres = 0.5 * (pN + pP);
+          else if (d == 1) pres = pres * 0.5 + (pN + pP) * 0.25;
+          else pres = (pres * 4.0 + (pN + pP)) / 6.0;
+        }
+        if (!doUpdate && T.pavg) T.pavg[r] = pres;
+      } else {
+        pres = T.pavg[r];
+      }
+    }
+    if (!doUpdate) continue;
+    double* U = T.U;
+    const long c = cidx(P, i, j, k);
+    const double dx = T.del[0], dy = P.ndim > 1 ? T.del[1] : 1., dz = P.ndim > 2 ? T.del[2] : 1.;
+    const double dt = L.dt;
+    double FL[8], FR[8], GL[8], GR[8], HL[8], HR[8];
+    {
+      const long nf = P.nface[0];
+      const long f0 = face_index(P, 0, i, j, k);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        FL[q] = T.flux[0][q * nf + f0];
+        FR[q] = T.flux[0][q * nf + f0 + 1];
+        GL[q] = GR[q] = HL[q] = HR[q] = 0.0;
+      }
+    }
+    if (P.ndim >= 2) {
+      const long nf = P.nface[1];
+      const long g0 = face_index(P, 1, i, j, k);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        GL[q] = T.flux[1][q * nf + g0] * 1;
+        GR[q] = T.flux[1][q * nf + g0 + P.fext[1][0]] * 1;
+      }
+    }
+    if (P.ndim == 3) {
+      const long nf = P.nface[2];
+      const long h0 = face_index(P, 2, i, j, k);
+      const long hs = (long)P.fext[2][0] * P.fext[2][1];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        HL[q] = T.flux[2][q * nf + h0] * 1;
+        HR[q] = T.flux[2][q * nf + h0 + hs] * 1;
+      }
+    }
+    double Sphys[5] = {0., 0., 0., 0., 0.};
+    Sphys[1] = (FL[5] - FR[5]) * (1.0 - P.fP) / dx;
+    if (P.ndim >= 2) Sphys[2] = (GL[5] - GR[5]) * (1.0 - P.fP) / dy;
+    if (P.ndim == 3) Sphys[3] = (HL[5] - HR[5]) * (1.0 - P.fP) / dz;
+    const double dens0 = U[P.voff[FX_DENS] + c];
+    double IntEner = 0.;
+    if (P.aux) {
+      IntEner = dens0 * U[P.voff[FX_EINT] + c];
+      IntEner = IntEner + dt / dx * (FL[6] - FR[6] + pres * (FL[7] - FR[7]));
+      if (P.ndim > 1) {
+        IntEner = IntEner + dt / dy * (GL[6] - GR[6] + pres * (GL[7] - GR[7]));
+        if (P.ndim > 2) IntEner = IntEner + dt / dz * (HL[6] - HR[6] + pres * (HL[7] - HR[7]));
+      }
+      IntEner = IntEner + dt * 0.0;
+    }
+    double U0[5];
+    U0[0] = dens0;
+    U0[1] = U[P.voff[FX_VELX] + c] * dens0;
+    U0[2] = U[P.voff[FX_VELY] + c] * dens0;
+    U0[3] = U[P.voff[FX_VELZ] + c] * dens0;
+    U0[4] = dens0 * U[P.voff[FX_ENER] + c];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      U0[q] = U0[q] - dt / dx * (FR[q] - FL[q]);
+      if (P.ndim > 1) {
+        U0[q] = U0[q] - dt / dy * (GR[q] - GL[q]);
+        if (P.ndim > 2) U0[q] = U0[q] - dt / dz * (HR[q] - HL[q]);
+      }
+      U0[q] = U0[q] + dt * (0. + Sphys[q]);
+    }
+    const double densNew = fmax(U0[0], P.smlrho);
+    const double vx = U0[1] / densNew, vy = U0[2] / densNew, vz = U0[3] / densNew;
+    const double ener = U0[4] / densNew;
+    double eintAux = P.aux ? IntEner / densNew : U[P.voff[FX_EINT] + c];
+    // energy fix (hy_energyFix.F90:188-251)
+    const double ekin = 0.5 * (vx * vx + vy * vy + vz * vz);
+    const double eint = ener - ekin - 0.0;
+    double newEint;
+    if (!P.aux || eint > P.eintSwitch * (ekin + 0.0)) newEint = fmax(P.smallE, eint);
+    else newEint = fmax(P.smallE, eintAux);
+    const double enerFix = newEint + ekin;
+    const EosOut eo = eos_dens_ei(P, densNew, vx, vy, vz, newEint, enerFix);
+    U[P.voff[FX_DENS] + c] = densNew;
+    U[P.voff[FX_VELX] + c] = vx;
+    U[P.voff[FX_VELY] + c] = vy;
+    U[P.voff[FX_VELZ] + c] = vz;
+    U[P.voff[FX_ENER] + c] = eo.ener;
+    U[P.voff[FX_EINT] + c] = eo.eint;
+    U[P.voff[FX_PRES] + c] = eo.pres;
+    U[P.voff[FX_TEMP] + c] = eo.temp;
+    U[P.voff[FX_GAMC] + c] = P.gamma;
+    U[P.voff[FX_GAME] + c] = eo.game;
+  }
+}
+
+// per-workgroup max of (|v_d| + c)/del_d over interior cells
+__global__ void __launch_bounds__(kThreads) k_dt(DevParams P, const double* __restrict__ Ubase,
+                                                 long nBlocks, long stride, double idx, double idy,
+                                                 double idz, double* __restrict__ partial) {
+  __shared__ double red[kThreads];
+  const long perTile = P.icells;
+  const long total = perTile * nBlocks;
+  const int nxi = P.nxb, nyi = P.ndim > 1 ? P.nyb : 1;
+  double m = 0.;
+  for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+       gid += (long)gridDim.x * blockDim.x) {
+    const long b = gid / perTile;
+    const long r = gid - b * perTile;
+    const int i = (int)(r % nxi) + P.lo[0];
+    const int j = (int)((r / nxi) % nyi) + P.lo[1];
+    const int k = (int)(r / ((long)nxi * nyi)) + P.lo[2];
+    const double* U = Ubase + b * stride;
+    const long c = cidx(P, i, j, k);
+    const double snd2 = U[P.voff[FX_GAMC] + c] * U[P.voff[FX_PRES] + c] / U[P.voff[FX_DENS] + c];
+    const double cs = sqrt(snd2);
+    double dtl = (fabs(U[P.voff[FX_VELX] + c] - 0.) + cs) * idx;
+    if (P.ndim > 1) dtl = fmax(dtl, (fabs(U[P.voff[FX_VELY] + c] - 0.) + cs) * idy);
+    if (P.ndim > 2) dtl = fmax(dtl, (fabs(U[P.voff[FX_VELZ] + c] - 0.) + cs) * idz);
+    m = fmax(m, dtl);
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_reset_status(DevStatus* s) {
+  s->code = 0;
+  s->tile = s->i = s->j = s->k = s->dir = -1;
+  s->fallbacks = 0;
+  s->newtonIters = 0;
+  s->newtonFallbacks = 0;
+}
+
+// ---------------------------------------------------------------------
+// host state
+// ---------------------------------------------------------------------
+struct Lib {
+  bool init = false;
+  fx_params prm{};
+  DevParams dp{};
+  hipStream_t stream = nullptr;
+  int async = 0;
+  double* states = nullptr;
+  size_t statesCap = 0;   // doubles
+  double* flux = nullptr; // library-owned flux arrays for tiles without them
+  size_t fluxCap = 0;
+  fx_tile* dtiles = nullptr;
+  size_t dtilesCap = 0;
+  unsigned char* dinplace = nullptr;
+  size_t dinplaceCap = 0;
+  DevStatus* status = nullptr;
+  double* partial = nullptr;
+  int partialCap = 0;
+  std::vector<fx_tile> htiles;
+  int numCU = 256;
+  // batch tile-list cache (fx_hydro_advance_batch)
+  double* batchU = nullptr;
+  long batchN = -1, batchStride = -1;
+  double batchDel[3] = {0, 0, 0};
+  bool batchValid = false;
+  // per-stage timing (fx_hydro_set_timing)
+  int timing = 0;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  float stageMs[3] = {0.f, 0.f, 0.f};
+};
+Lib g;
+
+int hip_fail(hipError_t e, const char* what, fx_status* st) {
+  if (e == hipSuccess) return 0;
+  fprintf(stderr, "fx_hydro: %s failed: %s\n", what, hipGetErrorString(e));
+  if (st) st->code = 4;
+  return 4;
+}
+#define HIPCALL(x, st)                                    \
+  do {                                                    \
+    int rc__ = hip_fail((x), #x, (st));                   \
+    if (rc__) return rc__;                                \
+  } while (0)
+
+template <typename T>
+int grow(T** ptr, size_t* cap, size_t need, fx_status* st) {
+  if (*cap >= need) return 0;
+  if (*ptr) HIPCALL(hipFree(*ptr), st);
+  *ptr = nullptr;
+  *cap = 0;
+  HIPCALL(hipMalloc((void**)ptr, need * sizeof(T)), st);
+  *cap = need;
+  return 0;
+}
+
+DevParams make_dev_params(const fx_params& p) {
+  DevParams d{};
+  d.ndim = p.ndim;
+  d.nxb = p.nxb;
+  d.nyb = p.ndim > 1 ? p.nyb : 1;
+  d.nzb = p.ndim > 2 ? p.nzb : 1;
+  d.ng = p.ng;
+  d.nunk = p.nunk;
+  d.nx = p.nxb + 2 * p.ng;
+  d.ny = p.ndim > 1 ? p.nyb + 2 * p.ng : 1;
+  d.nz = p.ndim > 2 ? p.nzb + 2 * p.ng : 1;
+  d.ncell = (long)d.nx * d.ny * d.nz;
+  for (int v = 0; v < FX_NVAR_USED; ++v) d.voff[v] = (long)p.var[v] * d.ncell;
+  d.lo[0] = p.ng;
+  d.hi[0] = p.ng + p.nxb - 1;
+  d.lo[1] = p.ndim > 1 ? p.ng : 0;
+  d.hi[1] = p.ndim > 1 ? p.ng + p.nyb - 1 : 0;
+  d.lo[2] = p.ndim > 2 ? p.ng : 0;
+  d.hi[2] = p.ndim > 2 ? p.ng + p.nzb - 1 : 0;
+  d.gamma = p.gamma;
+  d.limiter = p.limiter;
+  d.beta = p.limitedSlopeBeta;
+  d.charLim = p.charLimiting;
+  d.flat = p.flattening;
+  d.steep = p.steepening;
+  d.ctu3d = p.use3dFullCTU;
+  d.aux = p.useAuxEint;
+  d.riemann = p.riemann;
+  d.fP = p.fP;
+  d.smlrho = p.smlrho;
+  d.smallE = p.smallE;
+  d.eintSwitch = p.eintSwitch;
+  d.tiny = p.tiny;
+  d.gammam1 = 1.0 / (p.gamma - 1.0);
+  const double ggprod = d.gammam1 * p.gasConstant;
+  d.ggprodinv = 1. / ggprod;
+  d.gam1inv = 1. / d.gammam1;
+  d.abar = p.singleSpeciesA;
+  d.exactMaxIter = p.exactMaxIter;
+  d.exactTol = p.exactTol;
+  const int n[3] = {d.nxb, d.nyb, d.nzb};
+  long off = 0;
+  for (int dir = 0; dir < 3; ++dir) {
+    for (int a = 0; a < 3; ++a) {
+      d.bext[dir][a] = n[a] + ((a == dir && dir < p.ndim) ? 2 : 0);
+      d.fext[dir][a] = n[a] + ((a == dir && dir < p.ndim) ? 1 : 0);
+    }
+    if (dir < p.ndim) {
+      d.bcells[dir] = (long)d.bext[dir][0] * d.bext[dir][1] * d.bext[dir][2];
+      d.nface[dir] = (long)d.fext[dir][0] * d.fext[dir][1] * d.fext[dir][2];
+    } else {
+      d.bcells[dir] = 0;
+      d.nface[dir] = 0;
+    }
+    d.boff[dir] = off;
+    off += d.bcells[dir];
+  }
+  d.scellsPerTile = off;
+  d.icells = (long)d.nxb * d.nyb * d.nzb;
+  return d;
+}
+
+int grid_for(long items) {
+  long blocks = (items + kThreads - 1) / kThreads;
+  const long cap = (long)g.numCU * 64;
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
+  return (int)blocks;
+}
+
+int validate(const fx_params* p) {
+  if (!p) return 3;
+  if (p->ndim < 1 || p->ndim > 3 || p->ng != 4 || p->nxb < 2) return 3;
+  if (p->ndim > 1 && p->nyb < 2) return 3;
+  if (p->ndim > 2 && p->nzb < 2) return 3;
+  for (int v = 0; v < FX_NVAR_USED; ++v)
+    if (p->var[v] < 0 || p->var[v] >= p->nunk) return 3;
+  if (!(p->gamma > 1.0)) return 3;
+  if (p->riemann != FX_HLLC) return 3;  // other solvers: not in this build yet
+  return 0;
+}
+
+int finish(fx_status* st) {
+  if (g.async) return 0;
+  HIPCALL(hipStreamSynchronize(g.stream), st);
+  DevStatus h{};
+  HIPCALL(hipMemcpy(&h, g.status, sizeof h, hipMemcpyDeviceToHost), st);
+  if (st) {
+    st->code = h.code;
+    st->first_tile = h.tile;
+    st->first_i = h.i;
+    st->first_j = h.j;
+    st->first_k = h.k;
+    st->first_dir = h.dir;
+    st->fallbacks = (long long)h.fallbacks;
+    st->newton_iters = (long long)h.newtonIters;
+    st->newton_fallbacks = (long long)h.newtonFallbacks;
+  }
+  return h.code;
+}
+
+// Uploads the tile list, substituting library-owned flux arrays where a
+// tile has none.
+int upload_tiles(const fx_tile* tiles, int nTiles, bool needFlux, fx_status* st) {
+  g.batchValid = false;
+  g.htiles.assign(tiles, tiles + nTiles);
+  const DevParams& P = g.dp;
+  const size_t perTileFlux = (size_t)(P.nface[0] + P.nface[1] + P.nface[2]) * FX_NFLUX;
+  size_t missing = 0;
+  for (int t = 0; t < nTiles; ++t)
+    for (int d = 0; d < P.ndim; ++d)
+      if (!g.htiles[t].flux[d]) ++missing;
+  if (needFlux && missing) {
+    if (grow(&g.flux, &g.fluxCap, perTileFlux * (size_t)nTiles, st)) return 4;
+    for (int t = 0; t < nTiles; ++t) {
+      double* base = g.flux + perTileFlux * t;
+      for (int d = 0; d < P.ndim; ++d) {
+        if (!g.htiles[t].flux[d]) g.htiles[t].flux[d] = base;
+        base += (size_t)P.nface[d] * FX_NFLUX;
+      }
+    }
+  }
+  if (grow(&g.dtiles, &g.dtilesCap, (size_t)nTiles, st)) return 4;
+  HIPCALL(hipMemcpyAsync(g.dtiles, g.htiles.data(), sizeof(fx_tile) * nTiles, hipMemcpyHostToDevice,
+                         g.stream),
+          st);
+  return 0;
+}
+
+int record(int n) {
+  if (!g.timing) return 0;
+  if (!g.ev[n]) {
+    if (hipEventCreate(&g.ev[n]) != hipSuccess) return 4;
+  }
+  return hipEventRecord(g.ev[n], g.stream) == hipSuccess ? 0 : 4;
+}
+
+int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char* inPlace,
+                fx_status* st, bool tilesUploaded = false) {
+  if (!g.init) {
+    if (st) st->code = 3;
+    return 3;
+  }
+  if (st) memset(st, 0, sizeof *st);
+  if (nTiles <= 0) return 0;
+  const DevParams& P = g.dp;
+  if (!tilesUploaded && upload_tiles(tiles, nTiles, true, st)) return 4;
+  const unsigned char* dIn = nullptr;
+  bool anyFluxOnly = false;
+  if (inPlace) {
+    for (int t = 0; t < nTiles; ++t)
+      if (!inPlace[t]) {
+        anyFluxOnly = true;
+        if (P.aux && !tiles[t].pavg) {
+          if (st) st->code = 3;
+          return 3;
+        }
+      }
+    if (grow(&g.dinplace, &g.dinplaceCap, (size_t)nTiles, st)) return 4;
+    HIPCALL(hipMemcpyAsync(g.dinplace, inPlace, nTiles, hipMemcpyHostToDevice, g.stream), st);
+    dIn = g.dinplace;
+  }
+  if (grow(&g.states, &g.statesCap, (size_t)P.scellsPerTile * kStateComps * nTiles, st)) return 4;
+  Launch L{g.dtiles, nTiles, g.states, g.status, dt};
+  hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
+  if (record(0)) return 4;
+  hipLaunchKernelGGL(k_states, dim3(grid_for(P.scellsPerTile * nTiles)), dim3(kThreads), 0, g.stream, P, L);
+  if (record(1)) return 4;
+  hipLaunchKernelGGL(k_fluxes, dim3(grid_for((P.nface[0] + P.nface[1] + P.nface[2]) * nTiles)),
+                     dim3(kThreads), 0, g.stream, P, L);
+  if (record(2)) return 4;
+  (void)anyFluxOnly;
+  hipLaunchKernelGGL((k_update<true, false>), dim3(grid_for(P.icells * nTiles)), dim3(kThreads), 0,
+                     g.stream, P, L, dIn);
+  if (record(3)) return 4;
+  HIPCALL(hipGetLastError(), st);
+  int rc = finish(st);
+  if (g.timing && !g.async) {
+    for (int k = 0; k < 3; ++k)
+      if (hipEventElapsedTime(&g.stageMs[k], g.ev[k], g.ev[k + 1]) != hipSuccess) g.stageMs[k] = -1.f;
+  }
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fx_hydro_init(const fx_params* p) {
+  int rc = validate(p);
+  if (rc) return rc;
+  if (!g.init || memcmp(&g.prm, p, sizeof *p) != 0) g.batchValid = false;
+  g.prm = *p;
+  g.dp = make_dev_params(*p);
+  if (!g.status) {
+    if (hipMalloc((void**)&g.status, sizeof(DevStatus)) != hipSuccess) return 4;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      g.numCU = prop.multiProcessorCount;
+  }
+  g.init = true;
+  return 0;
+}
+
+void fx_hydro_finalize(void) {
+  void* bufs[] = {g.states, g.flux, g.dtiles, g.dinplace, g.status, g.partial};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  for (hipEvent_t e : g.ev)
+    if (e) (void)hipEventDestroy(e);
+  g = Lib{};
+}
+
+int fx_hydro_compute_fluxes(const fx_tile* tiles, int nTiles, double dt, double dtOld,
+                            const unsigned char* updateInPlace, fx_status* st) {
+  (void)dtOld;
+  return run_compute(tiles, nTiles, dt, updateInPlace, st);
+}
+
+int fx_hydro_update_solution(const fx_tile* tiles, int nTiles, double dt, double dtOld,
+                             fx_status* st) {
+  (void)dtOld;
+  if (!g.init) return 3;
+  if (st) memset(st, 0, sizeof *st);
+  if (nTiles <= 0) return 0;
+  for (int t = 0; t < nTiles; ++t) {
+    for (int d = 0; d < g.dp.ndim; ++d)
+      if (!tiles[t].flux[d]) return 3;
+    if (g.dp.aux && !tiles[t].pavg) return 3;
+  }
+  const DevParams& P = g.dp;
+  if (upload_tiles(tiles, nTiles, false, st)) return 4;
+  Launch L{g.dtiles, nTiles, nullptr, g.status, dt};
+  hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
+  hipLaunchKernelGGL((k_update<false, false>), dim3(grid_for(P.icells * nTiles)), dim3(kThreads), 0,
+                     g.stream, P, L, (const unsigned char*)nullptr);
+  HIPCALL(hipGetLastError(), st);
+  return finish(st);
+}
+
+int fx_hydro_advance_batch(double* U, long nBlocks, long blockStride, const double del[3], double dt,
+                           fx_status* st) {
+  if (!g.init || !U || nBlocks < 0 || blockStride < g.dp.ncell * g.dp.nunk) return 3;
+  const bool same = g.batchValid && g.batchU == U && g.batchN == nBlocks &&
+                    g.batchStride == blockStride && g.batchDel[0] == del[0] &&
+                    g.batchDel[1] == del[1] && g.batchDel[2] == del[2];
+  if (!same) {
+    std::vector<fx_tile> tl((size_t)nBlocks);
+    for (long b = 0; b < nBlocks; ++b) {
+      fx_tile& t = tl[(size_t)b];
+      memset(&t, 0, sizeof t);
+      t.U = U + b * blockStride;
+      for (int d = 0; d < 3; ++d) t.del[d] = del[d];
+      t.level = 1;
+    }
+    if (upload_tiles(tl.data(), (int)nBlocks, true, st)) return 4;
+    g.batchU = U;
+    g.batchN = nBlocks;
+    g.batchStride = blockStride;
+    for (int d = 0; d < 3; ++d) g.batchDel[d] = del[d];
+    g.batchValid = true;
+  }
+  return run_compute(nullptr, (int)nBlocks, dt, nullptr, st, true);
+}
+
+int fx_hydro_compute_dt(const double* U, long nBlocks, long blockStride, const double del[3],
+                        double cfl, double* dtOut) {
+  if (!g.init || !U || nBlocks <= 0 || !dtOut) return 3;
+  const DevParams& P = g.dp;
+  const int grid = grid_for(P.icells * nBlocks);
+  if (g.partialCap < grid) {
+    if (g.partial) (void)hipFree(g.partial);
+    if (hipMalloc((void**)&g.partial, sizeof(double) * grid) != hipSuccess) return 4;
+    g.partialCap = grid;
+  }
+  hipLaunchKernelGGL(k_dt, dim3(grid), dim3(kThreads), 0, g.stream, P, U, nBlocks, blockStride,
+                     1. / del[0], 1. / del[1], 1. / del[2], g.partial);
+  std::vector<double> h((size_t)grid);
+  if (hipMemcpyAsync(h.data(), g.partial, sizeof(double) * grid, hipMemcpyDeviceToHost, g.stream) !=
+      hipSuccess)
+    return 4;
+  if (hipStreamSynchronize(g.stream) != hipSuccess) return 4;
+  double m = 0.;
+  for (double v : h) m = v > m ? v : m;
+  *dtOut = m != 0. ? cfl / m : 1.e300;
+  return 0;
+}
+
+void fx_hydro_set_stream(void* stream) { g.stream = (hipStream_t)stream; }
+void fx_hydro_set_timing(int on) { g.timing = on; }
+int fx_hydro_stage_times(float* ms, int n) {
+  for (int k = 0; k < n && k < 3; ++k) ms[k] = g.stageMs[k];
+  return 3;
+}
+void fx_hydro_set_async(int a) { g.async = a; }
+const char* fx_hydro_version(void) { return "flash5_amd fx_hydro 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,152 @@
+/*
+ * fx_hydro.h -- C ABI of the MI355X-native unsplit hydro block update.
+ *
+ * Drop-in boundary for FLASH5's unsplit Hydro on the ideal gamma-law EOS.
+ * The reference entry points this ABI replaces (paths relative to
+ * source/physics/Hydro):
+ *   Hydro(simTime, dt, dtOld)         Hydro_interface.F90:52-57,
+ *                                     HydroMain/unsplit/Hydro.F90:6-343
+ *   hy_computeFluxes(tile, Uin, Uout, del, timeEndAdv, dt, dtOld, sweep)
+ *                                     HydroMain/unsplit/hy_computeFluxes.F90:76
+ *   hy_updateSolution(tile, Uin, Uout, del, timeEndAdv, dt, dtOld, sweep)
+ *                                     HydroMain/unsplit/hy_updateSolution.F90:104
+ *   Hydro_init / Hydro_finalize       HydroMain/unsplit/Hydro_init.F90:24
+ *   Hydro_computeDt                   HydroMain/unsplit/Hydro_computeDt.F90:186
+ *   Grid_putFluxData / Grid_conserveFluxes (flux-register correction,
+ *   Cartesian, refinement ratio 2)    Grid/GridMain/AMR/Amrex/
+ *                                     Grid_addFineToFluxRegister.F90:144-171,
+ *                                     Grid_conserveFluxes.F90:108
+ *
+ * Layout (the index-reordered AMReX layout, Grid_tile.F90:285): a tile's
+ * cell data U(i,j,k,var) in Fortran order over the guard-inclusive box,
+ * i.e. U[var][k][j][i] with i fastest.  Face flux densities F(i,j,k,flux)
+ * per direction over the interior faces only (extent nxb+1 along the face
+ * normal), NFLUXES = 8 in the order dens, xmom, ymom, zmom, ener, pres,
+ * eint, volu (OnlyHydro/Config:6-13).  All pointers passed to compute
+ * entry points are device pointers (hipMalloc'd or managed memory); the
+ * library never frees caller memory.
+ *
+ * Errors: every entry returns 0 on success.  On failure the fx_status
+ * carries the first failing (tile, i, j, k, dir, code), where the Fortran
+ * host calls Driver_abortFlash as the reference would
+ * (hy_getFaceFlux.F90:1217, hy_eigenParameters.F90:84).
+ */
+#ifndef FX_HYDRO_H
+#define FX_HYDRO_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* variable slots inside fx_params.var (FLASH's *_VAR indices, 0-based) */
+enum { FX_DENS = 0, FX_VELX, FX_VELY, FX_VELZ, FX_PRES, FX_ENER, FX_GAMC,
+       FX_GAME, FX_TEMP, FX_EINT, FX_NVAR_USED };
+enum { FX_NFLUX = 8 };
+
+/* slope limiters (UHD.h:15-19) */
+enum { FX_MINMOD = 1, FX_MC = 2, FX_HYBRID = 3, FX_VANLEER = 4, FX_LIMITED = 5 };
+/* Riemann solvers: HLLC (UHD.h:369); FX_EXACT = exact iterative solver */
+enum { FX_ROE = 1, FX_HLL = 3, FX_HLLC = 4, FX_EXACT = 20 };
+/* precision modes */
+enum { FX_PREC_FP64 = 0,      /* plain FP64, IEEE ops without contraction */
+       FX_PREC_FP64_FAST = 1, /* FP64 with FMA contraction */
+       FX_PREC_MIXED = 2 };   /* FP32 reconstruction and fluxes, FP64 update */
+
+/* Runtime knobs read by Hydro_init (Hydro_init.F90:60-420) and the EOS
+ * (Eos_init.F90:66-73, eos_initGamma.F90:42). */
+typedef struct {
+  int ndim;                 /* NDIM: 1, 2, 3 */
+  int nxb, nyb, nzb;        /* NXB, NYB, NZB (1 for absent dims) */
+  int ng;                   /* NGUARD; must be 4 */
+  int nunk;                 /* NUNK_VARS: variables stored per cell */
+  int var[FX_NVAR_USED];    /* 0-based slot of DENS, VELX, ... in U's var axis */
+  double gamma;             /* gamma */
+  int limiter;              /* slopeLimiter */
+  double limitedSlopeBeta;  /* LimitedSlopeBeta */
+  int charLimiting;         /* charLimiting */
+  int flattening;           /* use_flattening */
+  int steepening;           /* use_steepening */
+  int use3dFullCTU;         /* use_3dFullCTU */
+  int useAuxEint;           /* use_auxEintEqn */
+  int riemann;              /* RiemannSolver: FX_HLLC or FX_EXACT */
+  double fP;                /* hy_fPresInMomFlux */
+  double smlrho, smallp, smallE, eintSwitch;
+  double tiny;              /* hy_tiny */
+  double gasConstant;       /* ideal gas constant */
+  double singleSpeciesA;    /* eos_singleSpeciesA */
+  int exactMaxIter;         /* Newton iteration cap of the exact solver */
+  double exactTol;          /* relative p* convergence tolerance */
+  int precision;            /* FX_PREC_* */
+  int learnedSeed;          /* 1: seed exact-solver Newton from the MLP */
+} fx_params;
+
+/* One tile as the reference's tile iterator yields it (Grid_tile.F90). */
+typedef struct {
+  double* U;                /* U[var][k][j][i], guard-inclusive */
+  double* flux[3];          /* flux densities per direction, or NULL */
+  double* pavg;             /* per-interior-cell face-averaged pressure for
+                               the aux internal-energy update (the
+                               reference's scrch_Ptr HY_VAR1), or NULL */
+  double del[3];            /* cell widths (tileDesc%deltas) */
+  int level;                /* refinement level, 1-based */
+  int pad;
+} fx_tile;
+
+typedef struct {
+  int code;                 /* 0 ok, 1 eigen (imaginary c), 2 Riemann error,
+                               3 bad argument, 4 HIP error */
+  int first_tile, first_i, first_j, first_k, first_dir;
+  long long fallbacks;      /* cells reset to first-order states */
+  long long newton_iters;   /* exact solver: total Newton iterations */
+  long long newton_fallbacks; /* exact solver: learned seeds that failed */
+  double min_cell_cfl;      /* reserved (hy_cfl side effect) */
+} fx_status;
+
+/* Hydro_init: set knobs, allocate nothing (scratch grows on first use). */
+int fx_hydro_init(const fx_params* p);
+/* Hydro_finalize: free library-owned scratch. */
+void fx_hydro_finalize(void);
+
+/* hy_computeFluxes over nTiles tiles of the configured block shape.
+ * updateInPlace[t] != 0 (or updateInPlace == NULL) updates U of tile t in
+ * place (the reference's Uout => Uin case, hy_computeFluxes.F90:334-378):
+ * conservative update, energy fix and EOS; otherwise only fluxes and pavg
+ * are produced, for hy_updateSolution after flux correction. */
+int fx_hydro_compute_fluxes(const fx_tile* tiles, int nTiles, double dt,
+                            double dtOld, const unsigned char* updateInPlace,
+                            fx_status* st);
+
+/* hy_updateSolution: update from each tile's (corrected) flux arrays and
+ * pavg, then energy fix and EOS. */
+int fx_hydro_update_solution(const fx_tile* tiles, int nTiles, double dt,
+                             double dtOld, fx_status* st);
+
+/* Strided batch of independent blocks: block b's U starts at
+ * U + b*blockStride (doubles).  Equivalent to fx_hydro_compute_fluxes with
+ * updateInPlace all 1 and no flux output; no host-side tile list. */
+int fx_hydro_advance_batch(double* U, long nBlocks, long blockStride,
+                           const double del[3], double dt, fx_status* st);
+
+/* Hydro_computeDt over a strided batch (device-side min reduction):
+ * dt = cfl / max over interior cells of max_d (|v_d| + c) / del_d. */
+int fx_hydro_compute_dt(const double* U, long nBlocks, long blockStride,
+                        const double del[3], double cfl, double* dtOut);
+
+/* Work on the given HIP stream (hipStream_t as void*); NULL = default. */
+void fx_hydro_set_stream(void* stream);
+/* 1: entry points return without synchronising (needed for graph capture). */
+void fx_hydro_set_async(int async_);
+/* Per-stage timing: when on, each compute call records HIP events on the
+ * library's stream around its kernels; fx_hydro_stage_times copies the last
+ * call's durations in ms (states, fluxes, update) and returns the count. */
+void fx_hydro_set_timing(int on);
+int fx_hydro_stage_times(float* ms, int n);
+/* Library version string. */
+const char* fx_hydro_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
