@@ -57,9 +57,10 @@ struct Eig {
   double un, cf, ahinv, lfp, a2inv, rp, rho;
 };
 
-__device__ __forceinline__ bool eig_make(const double* V, int dir, Eig& e) {
+template <int DIR>
+__device__ __forceinline__ bool eig_make(const double* V, Eig& e) {
   e.rho = V[P_DENS];
-  e.un = V[P_VELX + dir];
+  e.un = V[P_VELX + DIR];
   const double a2 = V[P_GAMC] * V[P_PRES] / V[P_DENS];
   if (!(a2 > 0.0)) return false;
   e.cf = sqrt(a2);
@@ -81,21 +82,35 @@ __device__ __forceinline__ void eig_lambda(const Eig& e, double lam[5]) {
   lam[4] = e.un + e.cf;
 }
 
+// velocity slots normal / first / second transverse to DIR
+template <int DIR> struct Vel {
+  static constexpr int v1 = 1 + DIR, v2 = 1 + (DIR + 1) % 3, v3 = 1 + (DIR + 2) % 3;
+};
+
 // L_n . d over the five primitive slots
-__device__ __forceinline__ double eig_left1(const Eig& e, int dir, int n, const double* d) {
-  const int v1 = 1 + dir, v2 = 1 + (dir + 1) % 3, v3 = 1 + (dir + 2) % 3;
-  switch (n) {
-    case 0: return (-e.ahinv) * d[v1] + e.lfp * d[P_PRES];
-    case 1: return d[v2];
-    case 2: return d[P_DENS] + (-e.a2inv) * d[P_PRES];
-    case 3: return d[v3];
-    default: return e.ahinv * d[v1] + e.lfp * d[P_PRES];
-  }
+template <int DIR, int N>
+__device__ __forceinline__ double eig_left1(const Eig& e, const double* d) {
+  constexpr int v1 = Vel<DIR>::v1, v2 = Vel<DIR>::v2, v3 = Vel<DIR>::v3;
+  if constexpr (N == 0) return (-e.ahinv) * d[v1] + e.lfp * d[P_PRES];
+  else if constexpr (N == 1) return d[v2];
+  else if constexpr (N == 2) return d[P_DENS] + (-e.a2inv) * d[P_PRES];
+  else if constexpr (N == 3) return d[v3];
+  else return e.ahinv * d[v1] + e.lfp * d[P_PRES];
+}
+
+template <int DIR>
+__device__ __forceinline__ void eig_left(const Eig& e, const double* d, double w[5]) {
+  w[0] = eig_left1<DIR, 0>(e, d);
+  w[1] = eig_left1<DIR, 1>(e, d);
+  w[2] = eig_left1<DIR, 2>(e, d);
+  w[3] = eig_left1<DIR, 3>(e, d);
+  w[4] = eig_left1<DIR, 4>(e, d);
 }
 
 // out[0..4] = sum_n R_n * w[n]
-__device__ __forceinline__ void eig_right(const Eig& e, int dir, const double w[5], double* out) {
-  const int v1 = 1 + dir, v2 = 1 + (dir + 1) % 3, v3 = 1 + (dir + 2) % 3;
+template <int DIR>
+__device__ __forceinline__ void eig_right(const Eig& e, const double w[5], double* out) {
+  constexpr int v1 = Vel<DIR>::v1, v2 = Vel<DIR>::v2, v3 = Vel<DIR>::v3;
   out[P_DENS] = e.rho * w[0] + w[2] + e.rho * w[4];
   out[v1] = (-e.cf) * w[0] + e.cf * w[4];
   out[v2] = w[1];
@@ -120,7 +135,8 @@ __device__ __forceinline__ double limit(int kind, double a, double b, double bet
 // Limited slope of the cell whose eigen system is e (hy_TVDslope), projected
 // back to primitive slopes when characteristic limiting is on
 // (PPM.F90:523-570).  out[0..7].
-__device__ __forceinline__ void tvd_slope(const DevParams& P, int dir, const double* VL,
+template <int DIR>
+__device__ __forceinline__ void tvd_slope(const DevParams& P, const double* VL,
                                           const double* V0, const double* VR, const Eig& e,
                                           double out[8]) {
   double dL[8], dR[8];
@@ -131,13 +147,13 @@ __device__ __forceinline__ void tvd_slope(const DevParams& P, int dir, const dou
   }
   if (P.limiter == FX_HYBRID) {
     if (P.charLim) {
-      double w[5];
+      double a[5], b[5], w[5];
+      eig_left<DIR>(e, dL, a);
+      eig_left<DIR>(e, dR, b);
 #pragma unroll
-      for (int n = 0; n < 5; ++n) {
-        const double a = eig_left1(e, dir, n, dL), b = eig_left1(e, dir, n, dR);
-        w[n] = (n == 2) ? limit(FX_MC, a, b, P.beta) : limit(FX_MINMOD, a, b, P.beta);
-      }
-      eig_right(e, dir, w, out);
+      for (int n = 0; n < 5; ++n)
+        w[n] = (n == 2) ? limit(FX_MC, a[n], b[n], P.beta) : limit(FX_MINMOD, a[n], b[n], P.beta);
+      eig_right<DIR>(e, w, out);
       for (int k = P_GAMC; k <= P_EINT; ++k) out[k] = limit(FX_MC, dL[k], dR[k], P.beta);
     } else {
       out[P_DENS] = limit(FX_MC, dL[P_DENS], dR[P_DENS], P.beta);
@@ -149,11 +165,12 @@ __device__ __forceinline__ void tvd_slope(const DevParams& P, int dir, const dou
     return;
   }
   if (P.charLim) {
-    double w[5];
+    double a[5], b[5], w[5];
+    eig_left<DIR>(e, dL, a);
+    eig_left<DIR>(e, dR, b);
 #pragma unroll
-    for (int n = 0; n < 5; ++n)
-      w[n] = limit(P.limiter, eig_left1(e, dir, n, dL), eig_left1(e, dir, n, dR), P.beta);
-    eig_right(e, dir, w, out);
+    for (int n = 0; n < 5; ++n) w[n] = limit(P.limiter, a[n], b[n], P.beta);
+    eig_right<DIR>(e, w, out);
     for (int k = P_GAMC; k <= P_EINT; ++k) out[k] = limit(P.limiter, dL[k], dR[k], P.beta);
   } else {
     for (int k = 0; k < 8; ++k) out[k] = limit(P.limiter, dL[k], dR[k], P.beta);
@@ -161,41 +178,39 @@ __device__ __forceinline__ void tvd_slope(const DevParams& P, int dir, const dou
 }
 
 // Transverse flux sigma of one cell (hy_upwindTransverseFlux, order 1).
-__device__ __forceinline__ void trans_sigma(const DevParams& P, int dir, const double* vm1,
+template <int DIR>
+__device__ __forceinline__ void trans_sigma(const DevParams& P, const double* vm1,
                                             const double* vc0, const double* vp1, const Eig& e,
                                             double sig[8]) {
   double lam[5];
   eig_lambda(e, lam);
-  const int v1 = 1 + dir, v2 = 1 + (dir + 1) % 3, v3 = 1 + (dir + 2) % 3;
+  constexpr int v1 = Vel<DIR>::v1, v2 = Vel<DIR>::v2, v3 = Vel<DIR>::v3;
 #pragma unroll
   for (int v = 0; v < 8; ++v) sig[v] = 0.;
-  double tf[5];
+  double dm[5], dp[5];
 #pragma unroll
-  for (int n = 0; n < 5; ++n) {
-    if (lam[n] > 0.) {
-#pragma unroll
-      for (int v = 0; v < 5; ++v) tf[v] = vc0[v] - vm1[v];
-    } else {
-#pragma unroll
-      for (int v = 0; v < 5; ++v) tf[v] = vp1[v] - vc0[v];
-    }
-    const double db = eig_left1(e, dir, n, tf);
-    switch (n) {
-      case 0:
-        sig[P_DENS] = sig[P_DENS] + lam[0] * e.rho * db;
-        sig[v1] = sig[v1] + lam[0] * (-e.cf) * db;
-        sig[P_PRES] = sig[P_PRES] + lam[0] * e.rp * db;
-        break;
-      case 1: sig[v2] = sig[v2] + lam[1] * db; break;
-      case 2: sig[P_DENS] = sig[P_DENS] + lam[2] * db; break;
-      case 3: sig[v3] = sig[v3] + lam[3] * db; break;
-      default:
-        sig[P_DENS] = sig[P_DENS] + lam[4] * e.rho * db;
-        sig[v1] = sig[v1] + lam[4] * e.cf * db;
-        sig[P_PRES] = sig[P_PRES] + lam[4] * e.rp * db;
-        break;
-    }
+  for (int v = 0; v < 5; ++v) {
+    dm[v] = vc0[v] - vm1[v];
+    dp[v] = vp1[v] - vc0[v];
   }
+  // upwinded characteristic amplitude of wave N
+#define FX_TS_DB(N) (lam[N] > 0. ? eig_left1<DIR, N>(e, dm) : eig_left1<DIR, N>(e, dp))
+  {
+    const double db = FX_TS_DB(0);
+    sig[P_DENS] = sig[P_DENS] + lam[0] * e.rho * db;
+    sig[v1] = sig[v1] + lam[0] * (-e.cf) * db;
+    sig[P_PRES] = sig[P_PRES] + lam[0] * e.rp * db;
+  }
+  { const double db = FX_TS_DB(1); sig[v2] = sig[v2] + lam[1] * db; }
+  { const double db = FX_TS_DB(2); sig[P_DENS] = sig[P_DENS] + lam[2] * db; }
+  { const double db = FX_TS_DB(3); sig[v3] = sig[v3] + lam[3] * db; }
+  {
+    const double db = FX_TS_DB(4);
+    sig[P_DENS] = sig[P_DENS] + lam[4] * e.rho * db;
+    sig[v1] = sig[v1] + lam[4] * e.cf * db;
+    sig[P_PRES] = sig[P_PRES] + lam[4] * e.rp * db;
+  }
+#undef FX_TS_DB
   if (P.aux) {
     const double d = (lam[2] > 0.) ? vc0[P_EINT] - vm1[P_EINT] : vp1[P_EINT] - vc0[P_EINT];
     sig[P_EINT] = lam[2] * d;
@@ -204,27 +219,28 @@ __device__ __forceinline__ void trans_sigma(const DevParams& P, int dir, const d
 
 // Full PPM reconstruction of one cell along dir (PPM.F90:378-842).
 // Returns false on an imaginary sound speed in a neighbour's eigen system.
-__device__ __forceinline__ bool ppm_full(const DevParams& P, int dir, double dtn, double delta,
+template <int DIR>
+__device__ __forceinline__ bool ppm_full(const DevParams& P, double dtn, double delta,
                                          const double* Vmm, const double* Vm, const double* Vc,
                                          const double* Vp, const double* Vpp, double flatC,
                                          const Eig& e0, double Wp[8], double Wm[8]) {
   Eig eP, eN;
-  if (!eig_make(Vp, dir, eP)) return false;
-  if (!eig_make(Vm, dir, eN)) return false;
+  if (!eig_make<DIR>(Vp, eP)) return false;
+  if (!eig_make<DIR>(Vm, eN)) return false;
   double dbN[8], db0[8], dbP[8];
-  tvd_slope(P, dir, Vmm, Vm, Vc, eN, dbN);
-  tvd_slope(P, dir, Vm, Vc, Vp, e0, db0);
-  tvd_slope(P, dir, Vc, Vp, Vpp, eP, dbP);
+  tvd_slope<DIR>(P, Vmm, Vm, Vc, eN, dbN);
+  tvd_slope<DIR>(P, Vm, Vc, Vp, e0, db0);
+  tvd_slope<DIR>(P, Vc, Vp, Vpp, eP, dbP);
   const int endVar = P.aux ? P_EINT : P_PRES;
   double vL[8], vR[8];
 #pragma unroll
   for (int v = 0; v < 8; ++v) {
     vL[v] = 0.;
     vR[v] = 0.;
-  }
-  for (int v = 0; v <= endVar; ++v) {
-    vL[v] = 0.5 * (Vc[v] + Vm[v]) - (db0[v] - dbN[v]) / 6.;
-    vR[v] = 0.5 * (Vc[v] + Vp[v]) - (dbP[v] - db0[v]) / 6.;
+    if (v <= endVar) {
+      vL[v] = 0.5 * (Vc[v] + Vm[v]) - (db0[v] - dbN[v]) / 6.;
+      vR[v] = 0.5 * (Vc[v] + Vp[v]) - (dbP[v] - db0[v]) / 6.;
+    }
   }
   if (P.steep) {
     const double temp1 = Vp[P_DENS] - Vm[P_DENS];
@@ -242,14 +258,16 @@ __device__ __forceinline__ bool ppm_full(const DevParams& P, int dir, double dtn
       vR[P_DENS] = vR[P_DENS] * (1. - eta) + (Vp[P_DENS] - 0.5 * dbP[P_DENS]) * eta;
     }
   }
-  if (P.flat) {
-    for (int v = 0; v <= endVar; ++v) {
-      vL[v] = flatC * Vc[v] + (1.0 - flatC) * vL[v];
-      vR[v] = flatC * Vc[v] + (1.0 - flatC) * vR[v];
-    }
-  }
   double dW[8], W6[8];
-  for (int n = 0; n <= endVar; ++n) {
+#pragma unroll
+  for (int n = 0; n < 8; ++n) {
+    dW[n] = 0.;
+    W6[n] = 0.;
+    if (n > endVar) continue;
+    if (P.flat) {
+      vL[n] = flatC * Vc[n] + (1.0 - flatC) * vL[n];
+      vR[n] = flatC * Vc[n] + (1.0 - flatC) * vR[n];
+    }
     if ((vR[n] - Vc[n]) * (Vc[n] - vL[n]) <= 0.) {
       vL[n] = Vc[n];
       vR[n] = Vc[n];
@@ -270,17 +288,15 @@ __device__ __forceinline__ bool ppm_full(const DevParams& P, int dir, double dtn
     Wm[v] = 0.;
   }
   {
-    double lmax = fmax(lam[4], 0.);
-    double lmin = -fmin(lam[0], 0.);
-    for (int v = 0; v <= P_PRES; ++v) {
-      Wp[v] = vR[v] - lmax * hdtn * (dW[v] - (1.0 - lmax * hdtn * 4. / 3.) * W6[v]);
-      Wm[v] = vL[v] + lmin * hdtn * (dW[v] + (1.0 - lmin * hdtn * 4. / 3.) * W6[v]);
-    }
-    lmax = fmax(lam[2], 0.);
-    lmin = -fmin(lam[2], 0.);
-    for (int v = P_PRES + 1; v <= endVar; ++v) {
-      Wp[v] = vR[v] - lmax * hdtn * (dW[v] - (1.0 - lmax * hdtn * 4. / 3.) * W6[v]);
-      Wm[v] = vL[v] + lmin * hdtn * (dW[v] + (1.0 - lmin * hdtn * 4. / 3.) * W6[v]);
+    const double lmax = fmax(lam[4], 0.), lmin = -fmin(lam[0], 0.);
+    const double emax = fmax(lam[2], 0.), emin = -fmin(lam[2], 0.);
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      if (v > endVar) continue;
+      const double a = v <= P_PRES ? lmax : emax;
+      const double b = v <= P_PRES ? lmin : emin;
+      Wp[v] = vR[v] - a * hdtn * (dW[v] - (1.0 - a * hdtn * 4. / 3.) * W6[v]);
+      Wm[v] = vL[v] + b * hdtn * (dW[v] + (1.0 - b * hdtn * 4. / 3.) * W6[v]);
     }
   }
   Wp[P_GAMC] = Vc[P_GAMC];
@@ -293,35 +309,32 @@ __device__ __forceinline__ bool ppm_full(const DevParams& P, int dir, double dtn
   double mW6[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) mW6[v] = -W6[v];
-  const int v1 = 1 + dir, v2 = 1 + (dir + 1) % 3, v3 = 1 + (dir + 2) % 3;
+  constexpr int v1 = Vel<DIR>::v1, v2 = Vel<DIR>::v2, v3 = Vel<DIR>::v3;
   const bool roe = (P.riemann == FX_ROE);
-#pragma unroll
-  for (int n = 0; n < 5; ++n) {
-    const double cA = eig_left1(e0, dir, n, dW);
-    const double cB = eig_left1(e0, dir, n, mW6);
-    const double x = dtn * lam[n];
-    const double aL = .5 * (-1. - x), bL = .25 * (1. + 2. * dtn * lam[n] + 4. / 3. * (x * x));
-    const double aR = .5 * (1. - x), bR = .25 * (1. - 2. * dtn * lam[n] + 4. / 3. * (x * x));
-    const bool doL = !roe || lam[n] < 0.;
-    const bool doR = !roe || lam[n] > 0.;
-    // non-zero right-eigenvector entries of wave n
-    int rv[3];
-    double rc[3];
-    int nr;
-    switch (n) {
-      case 0: rv[0] = P_DENS; rc[0] = e0.rho; rv[1] = v1; rc[1] = -e0.cf; rv[2] = P_PRES; rc[2] = e0.rp; nr = 3; break;
-      case 1: rv[0] = v2; rc[0] = 1.; nr = 1; break;
-      case 2: rv[0] = P_DENS; rc[0] = 1.; nr = 1; break;
-      case 3: rv[0] = v3; rc[0] = 1.; nr = 1; break;
-      default: rv[0] = P_DENS; rc[0] = e0.rho; rv[1] = v1; rc[1] = e0.cf; rv[2] = P_PRES; rc[2] = e0.rp; nr = 3; break;
-    }
-    for (int q = 0; q < nr; ++q) {
-      const int v = rv[q];
-      const double R = rc[q];
-      if (doL) sL[v] = sL[v] + (aL * R * cA + bL * R * cB);
-      if (doR) sR[v] = sR[v] + (aR * R * cA + bR * R * cB);
-    }
+  double cA[5], cB[5];
+  eig_left<DIR>(e0, dW, cA);
+  eig_left<DIR>(e0, mW6, cB);
+  // contribution of wave N through its right-eigenvector entry R at slot V
+#define FX_TRACE(N, V, R)                                                          \
+  {                                                                                \
+    const double x = dtn * lam[N];                                                 \
+    if (!roe || lam[N] < 0.)                                                       \
+      sL[V] = sL[V] + (.5 * (-1. - x) * (R) * cA[N] +                              \
+                       .25 * (1. + 2. * dtn * lam[N] + 4. / 3. * (x * x)) * (R) * cB[N]); \
+    if (!roe || lam[N] > 0.)                                                       \
+      sR[V] = sR[V] + (.5 * (1. - x) * (R) * cA[N] +                               \
+                       .25 * (1. - 2. * dtn * lam[N] + 4. / 3. * (x * x)) * (R) * cB[N]); \
   }
+  FX_TRACE(0, P_DENS, e0.rho)
+  FX_TRACE(0, v1, -e0.cf)
+  FX_TRACE(0, P_PRES, e0.rp)
+  FX_TRACE(1, v2, 1.)
+  FX_TRACE(2, P_DENS, 1.)
+  FX_TRACE(3, v3, 1.)
+  FX_TRACE(4, P_DENS, e0.rho)
+  FX_TRACE(4, v1, e0.cf)
+  FX_TRACE(4, P_PRES, e0.rp)
+#undef FX_TRACE
 #pragma unroll
   for (int v = 0; v <= P_PRES; ++v) {
     Wm[v] = Vc[v] + W6[v] / 12. + sL[v];
@@ -331,23 +344,25 @@ __device__ __forceinline__ bool ppm_full(const DevParams& P, int dir, double dtn
 }
 
 // ---- HLLC (hy_HLLC.F90:53-324) ----
-__device__ __forceinline__ void prim2flx(int dir, const double* V, double fP, double F[6]) {
+template <int DIR>
+__device__ __forceinline__ void prim2flx(const double* V, double fP, double F[6]) {
   const double u2 = V[P_VELX] * V[P_VELX] + V[P_VELY] * V[P_VELY] + V[P_VELZ] * V[P_VELZ];
   const double E = 0.5 * V[P_DENS] * u2 + V[P_PRES] / (V[P_GAME] - 1.);
-  const double un = V[P_VELX + dir];
+  const double un = V[P_VELX + DIR];
   F[5] = V[P_PRES];
   F[0] = V[P_DENS] * un;
   F[1] = F[0] * V[P_VELX];
   F[2] = F[0] * V[P_VELY];
   F[3] = F[0] * V[P_VELZ];
-  F[1 + dir] = F[1 + dir] + fP * V[P_PRES];
+  F[1 + DIR] = F[1 + DIR] + fP * V[P_PRES];
   F[4] = (E + V[P_PRES]) * un;
 }
 
 // returns 0 ok, 1 imaginary sound speed
-__device__ __forceinline__ int hllc(const DevParams& P, int dir, const double* Vm,
+template <int DIR>
+__device__ __forceinline__ int hllc(const DevParams& P, const double* Vm,
                                     const double* Vp, double F[6]) {
-  const double velNL = Vm[P_VELX + dir], velNR = Vp[P_VELX + dir];
+  const double velNL = Vm[P_VELX + DIR], velNR = Vp[P_VELX + DIR];
   const double aL2 = Vm[P_GAMC] * Vm[P_PRES] / Vm[P_DENS];
   const double aR2 = Vp[P_GAMC] * Vp[P_PRES] / Vp[P_DENS];
   if ((Vm[P_DENS] < P.tiny && Vm[P_DENS] > 0.) || (Vp[P_DENS] < P.tiny && Vp[P_DENS] > 0.) ||
@@ -365,11 +380,11 @@ __device__ __forceinline__ int hllc(const DevParams& P, int dir, const double* V
                   0. * 0. + 0. * 0.);
   qStar = qStar / (Vp[P_DENS] * (SR - velNR) - Vm[P_DENS] * (SL - velNL));
   if (SL >= 0.) {
-    prim2flx(dir, Vm, P.fP, F);
+    prim2flx<DIR>(Vm, P.fP, F);
     return 0;
   }
   if (!(qStar < 0. && SR >= 0.) && !(SL < 0. && qStar >= 0.)) {
-    prim2flx(dir, Vp, P.fP, F);
+    prim2flx<DIR>(Vp, P.fP, F);
     return 0;
   }
   const bool left = (SL < 0. && qStar >= 0.);
@@ -380,7 +395,7 @@ __device__ __forceinline__ int hllc(const DevParams& P, int dir, const double* V
   const double numer = S - velN, denom = S - qStar;
   const double pStar = Vm[P_DENS] * (SL - velNL) * (qStar - velNL) + tpL;
   double Fs[6];
-  prim2flx(dir, V, P.fP, Fs);
+  prim2flx<DIR>(V, P.fP, Fs);
   // conserved state of the chosen side (hy_prim2con.F90)
   const double u2 = V[P_VELX] * V[P_VELX] + V[P_VELY] * V[P_VELY] + V[P_VELZ] * V[P_VELZ];
   double Uc[5];
@@ -394,7 +409,7 @@ __device__ __forceinline__ int hllc(const DevParams& P, int dir, const double* V
   Us[0] = dStar;
   Us[4] = Uc[4] * numer / denom + ((pStar * qStar - tp * velN)) / denom;
 #pragma unroll
-  for (int m = 0; m < 3; ++m) Us[1 + m] = (m == dir) ? dStar * qStar : Uc[1 + m] * numer / denom;
+  for (int m = 0; m < 3; ++m) Us[1 + m] = (m == DIR) ? dStar * qStar : Uc[1 + m] * numer / denom;
 #pragma unroll
   for (int f = 0; f < 5; ++f) F[f] = Fs[f] + S * (Us[f] - Uc[f]);
   F[5] = Fs[5];

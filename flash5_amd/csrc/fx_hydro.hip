@@ -106,21 +106,124 @@ __device__ __forceinline__ long axis_stride(const DevParams& P, int a) {
   return a == 0 ? 1L : (a == 1 ? (long)P.nx : (long)P.nx * P.ny);
 }
 
-// sigma of cell c along direction sd (transverse-only PPM call)
+// sigma of cell c along direction SD (the transverse-only PPM call)
+template <int SD>
 __device__ __forceinline__ bool sigma_at(const DevParams& P, const double* __restrict__ U, long c,
-                                         int sd, double sig[8], Eig* eout) {
-  const long s = axis_stride(P, sd);
+                                         double sig[8], Eig* eout) {
+  const long s = axis_stride(P, SD);
   double Vm[8], Vc[8], Vp[8];
   load_prim(P, U, c - s, Vm);
   load_prim(P, U, c, Vc);
   load_prim(P, U, c + s, Vp);
   Eig e;
-  if (!eig_make(Vc, sd, e)) return false;
-  trans_sigma(P, sd, Vm, Vc, Vp, e, sig);
+  if (!eig_make<SD>(Vc, e)) return false;
+  trans_sigma<SD>(P, Vm, Vc, Vp, e, sig);
   if (eout) *eout = e;
   return true;
 }
 
+// Face states of one state cell of direction D (hy_getRiemannState.F90:470-901
+// for that direction, then :952-1327 in 3-D).  Returns false on an
+// imaginary sound speed.
+template <int D, int NDIM>
+__device__ __forceinline__ bool state_cell(const DevParams& P, const double* __restrict__ U,
+                                           const double del[3], double dt, long c, double Wp[8],
+                                           double Wn[8], DevStatus* status) {
+  constexpr int O1 = (D + 1) % 3, O2 = (D + 2) % 3;
+  constexpr bool A1 = O1 < NDIM, A2 = O2 < NDIM;
+  const long sd = axis_stride(P, D);
+  double Vc[8];
+  {
+    double Vm[8], Vp[8], Vmm[8], Vpp[8];
+    load_prim(P, U, c, Vc);
+    load_prim(P, U, c - sd, Vm);
+    load_prim(P, U, c + sd, Vp);
+    load_prim(P, U, c - 2 * sd, Vmm);
+    load_prim(P, U, c + 2 * sd, Vpp);
+    Eig e0;
+    const double flat = P.flat ? flat_coeff(P, U, c, sd, D) : 0.;
+    if (!eig_make<D>(Vc, e0)) return false;
+    if (!ppm_full<D>(P, dt / del[D], del[D], Vmm, Vm, Vc, Vp, Vpp, flat, e0, Wp, Wn)) return false;
+  }
+  // transverse sigma at this cell and 2-D CTU corrections, in ascending
+  // order of the transverse direction (hy_dataReconstOneStep.F90:906-951)
+  double s1[8], s2[8];
+  Eig e1, e2;
+  if constexpr (A1) {
+    if (!sigma_at<O1>(P, U, c, s1, &e1)) return false;
+  }
+  if constexpr (A2) {
+    if (!sigma_at<O2>(P, U, c, s2, &e2)) return false;
+  }
+  auto correct = [&](const double* sg, int o) {
+    const double hdt = 0.5 * dt / del[o];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      Wn[v] = Wn[v] - hdt * sg[v];
+      Wp[v] = Wp[v] - hdt * sg[v];
+    }
+  };
+  if constexpr (A1 && A2) {
+    if constexpr (O1 < O2) {
+      correct(s1, O1);
+      correct(s2, O2);
+    } else {
+      correct(s2, O2);
+      correct(s1, O1);
+    }
+  } else if constexpr (A1) {
+    correct(s1, O1);
+  } else if constexpr (A2) {
+    correct(s2, O2);
+  }
+  if (Wn[P_DENS] < 0. || Wp[P_DENS] < 0. || Wn[P_PRES] < 0. || Wp[P_PRES] < 0.) {
+    atomicAdd(&status->fallbacks, 1ULL);
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      Wn[v] = Vc[v];
+      Wp[v] = Vc[v];
+    }
+  }
+  // 3-D full CTU: term A = sigma of O2 at the +-1 neighbours along O1,
+  // upwinded with O1's eigen system here; term B = sigma of O1 along O2
+  // with O2's eigen system (X: YZ then ZY, Y: ZX then XZ, Z: XY then YX).
+  if constexpr (NDIM == 3) {
+    if (P.ctu3d) {
+      double tA[8], tB[8];
+      {
+        const long sn = axis_stride(P, O1);
+        double sm[8], sp[8];
+        if (!sigma_at<O2>(P, U, c - sn, sm, nullptr)) return false;
+        if (!sigma_at<O2>(P, U, c + sn, sp, nullptr)) return false;
+        trans_sigma<O1>(P, sm, s2, sp, e1, tA);
+      }
+      {
+        const long sn = axis_stride(P, O2);
+        double sm[8], sp[8];
+        if (!sigma_at<O1>(P, U, c - sn, sm, nullptr)) return false;
+        if (!sigma_at<O1>(P, U, c + sn, sp, nullptr)) return false;
+        trans_sigma<O2>(P, sm, s1, sp, e2, tB);
+      }
+      const double fac = dt * dt / (6. * del[O1] * del[O2]);
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        Wp[v] = Wp[v] + (tA[v] + tB[v]) * fac;
+        Wn[v] = Wn[v] + (tA[v] + tB[v]) * fac;
+      }
+      if (Wp[P_DENS] <= 0. || Wn[P_DENS] <= 0. || Wp[P_PRES] <= 0. || Wn[P_PRES] <= 0.) {
+        atomicAdd(&status->fallbacks, 1ULL);
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+          Wn[v] = Vc[v];
+          Wp[v] = Vc[v];
+        }
+      }
+    }
+  }
+  return true;
+}
+
+template <int NDIM>
 __global__ void __launch_bounds__(kThreads) k_states(DevParams P, Launch L) {
   const long perTile = P.scellsPerTile;
   const long total = perTile * L.nTiles;
@@ -129,7 +232,7 @@ __global__ void __launch_bounds__(kThreads) k_states(DevParams P, Launch L) {
     const int t = (int)(gid / perTile);
     long r = gid - (long)t * perTile;
     int d = 0;
-    while (d < P.ndim - 1 && r >= P.bcells[d]) {
+    while (d < NDIM - 1 && r >= P.bcells[d]) {
       r -= P.bcells[d];
       ++d;
     }
@@ -139,119 +242,64 @@ __global__ void __launch_bounds__(kThreads) k_states(DevParams P, Launch L) {
     const int t1 = (int)((r / x0) % x1);
     const int t2 = (int)(r / ((long)x0 * x1));
     int ci[3] = {t0 + P.lo[0], t1 + P.lo[1], t2 + P.lo[2]};
-    ci[d] -= 1;
+    if (d == 0) ci[0] -= 1;
+    else if (d == 1) ci[1] -= 1;
+    else ci[2] -= 1;
     const fx_tile& T = L.tiles[t];
-    const double* __restrict__ U = T.U;
     const long c = cidx(P, ci[0], ci[1], ci[2]);
-    const long sd = axis_stride(P, d);
-    const double dt = L.dt;
-
-    double Vc[8], Vm[8], Vp[8], Vmm[8], Vpp[8];
-    load_prim(P, U, c, Vc);
-    load_prim(P, U, c - sd, Vm);
-    load_prim(P, U, c + sd, Vp);
-    load_prim(P, U, c - 2 * sd, Vmm);
-    load_prim(P, U, c + 2 * sd, Vpp);
-    Eig e0;
     double Wp[8], Wn[8];
-    const double flat = P.flat ? flat_coeff(P, U, c, sd, d) : 0.;
-    bool ok = eig_make(Vc, d, e0);
-    if (ok) ok = ppm_full(P, d, dt / T.del[d], T.del[d], Vmm, Vm, Vc, Vp, Vpp, flat, e0, Wp, Wn);
-    // transverse sigma at this cell and 2-D CTU corrections
-    // (hy_dataReconstOneStep.F90:906-951)
-    double sigo[3][8];
-    Eig eo[3];
-    for (int o = 0; o < P.ndim && ok; ++o) {
-      if (o == d) continue;
-      ok = sigma_at(P, U, c, o, sigo[o], &eo[o]);
-    }
+    bool ok;
+    if (d == 0) ok = state_cell<0, NDIM>(P, T.U, T.del, L.dt, c, Wp, Wn, L.status);
+    else if (d == 1) ok = state_cell<1, NDIM>(P, T.U, T.del, L.dt, c, Wp, Wn, L.status);
+    else ok = state_cell<2, NDIM>(P, T.U, T.del, L.dt, c, Wp, Wn, L.status);
     if (!ok) {
       report_error(L.status, 1, t, ci[0], ci[1], ci[2], d);
       continue;
     }
-    for (int o = 0; o < P.ndim; ++o) {
-      if (o == d) continue;
-      const double hdt = 0.5 * dt / T.del[o];
-#pragma unroll
-      for (int v = 0; v < 8; ++v) {
-        Wn[v] = Wn[v] - hdt * sigo[o][v];
-        Wp[v] = Wp[v] - hdt * sigo[o][v];
-      }
-    }
-    if (Wn[P_DENS] < 0. || Wp[P_DENS] < 0. || Wn[P_PRES] < 0. || Wp[P_PRES] < 0.) {
-      atomicAdd(&L.status->fallbacks, 1ULL);
-#pragma unroll
-      for (int v = 0; v < 8; ++v) {
-        Wn[v] = Vc[v];
-        Wp[v] = Vc[v];
-      }
-    }
-    // 3-D full CTU (hy_getRiemannState.F90:952-1327): two corner terms per
-    // state: sigma of direction sd at the +-1 neighbours along nbr, upwinded
-    // with the eigen system of direction nbr at this cell.
-    if (P.ndim == 3 && P.ctu3d) {
-      const int o1 = (d + 1) % 3, o2 = (d + 2) % 3;
-      // term order: X (ZY over j? no: YZ then ZY), Y (ZX then XZ), Z (XY then YX)
-      int sdA, nbA, sdB, nbB;
-      if (d == 0) { sdA = 2; nbA = 1; sdB = 1; nbB = 2; }
-      else if (d == 1) { sdA = 0; nbA = 2; sdB = 2; nbB = 0; }
-      else { sdA = 1; nbA = 0; sdB = 0; nbB = 1; }
-      (void)o1;
-      (void)o2;
-      double tf[2][8];
-      const int sds[2] = {sdA, sdB}, nbs[2] = {nbA, nbB};
-      for (int q = 0; q < 2 && ok; ++q) {
-        const long sn = axis_stride(P, nbs[q]);
-        double sm[8], sp[8];
-        ok = sigma_at(P, U, c - sn, sds[q], sm, nullptr) && sigma_at(P, U, c + sn, sds[q], sp, nullptr);
-        if (ok) trans_sigma(P, nbs[q], sm, sigo[sds[q]], sp, eo[nbs[q]], tf[q]);
-      }
-      if (!ok) {
-        report_error(L.status, 1, t, ci[0], ci[1], ci[2], d);
-        continue;
-      }
-      const int a = (d + 1) % 3, b = (d + 2) % 3;
-      const double fac = dt * dt / (6. * T.del[a] * T.del[b]);
-      // dt2dydz6 = dt*dt/(6*dy*dz), dt2dzdx6 = dt*dt/(6*dz*dx),
-      // dt2dxdy6 = dt*dt/(6*dx*dy): the (a, b) pairs (y,z), (z,x), (x,y)
-#pragma unroll
-      for (int v = 0; v < 8; ++v) {
-        Wp[v] = Wp[v] + (tf[0][v] + tf[1][v]) * fac;
-        Wn[v] = Wn[v] + (tf[0][v] + tf[1][v]) * fac;
-      }
-      if (Wp[P_DENS] <= 0. || Wn[P_DENS] <= 0. || Wp[P_PRES] <= 0. || Wn[P_PRES] <= 0.) {
-        atomicAdd(&L.status->fallbacks, 1ULL);
-#pragma unroll
-        for (int v = 0; v < 8; ++v) {
-          Wn[v] = Vc[v];
-          Wp[v] = Vc[v];
-        }
-      }
-    }
     // store Wp (comps 0..5) and Wn (6..11): dens, vx, vy, vz, pres, eint
     double* S = L.states + (long)t * perTile * kStateComps + P.boff[d] * kStateComps;
     const long bc = P.bcells[d];
-    const long ri = gid - (long)t * perTile - P.boff[d];
-    S[0 * bc + ri] = Wp[P_DENS];
-    S[1 * bc + ri] = Wp[P_VELX];
-    S[2 * bc + ri] = Wp[P_VELY];
-    S[3 * bc + ri] = Wp[P_VELZ];
-    S[4 * bc + ri] = Wp[P_PRES];
-    S[5 * bc + ri] = Wp[P_EINT];
-    S[6 * bc + ri] = Wn[P_DENS];
-    S[7 * bc + ri] = Wn[P_VELX];
-    S[8 * bc + ri] = Wn[P_VELY];
-    S[9 * bc + ri] = Wn[P_VELZ];
-    S[10 * bc + ri] = Wn[P_PRES];
-    S[11 * bc + ri] = Wn[P_EINT];
+    S[0 * bc + r] = Wp[P_DENS];
+    S[1 * bc + r] = Wp[P_VELX];
+    S[2 * bc + r] = Wp[P_VELY];
+    S[3 * bc + r] = Wp[P_VELZ];
+    S[4 * bc + r] = Wp[P_PRES];
+    S[5 * bc + r] = Wp[P_EINT];
+    S[6 * bc + r] = Wn[P_DENS];
+    S[7 * bc + r] = Wn[P_VELX];
+    S[8 * bc + r] = Wn[P_VELY];
+    S[9 * bc + r] = Wn[P_VELZ];
+    S[10 * bc + r] = Wn[P_PRES];
+    S[11 * bc + r] = Wn[P_EINT];
   }
 }
 
 // band-local linear index of block cell (i,j,k) in direction d's band
 __device__ __forceinline__ long band_index(const DevParams& P, int d, int i, int j, int k) {
-  int ci[3] = {i - P.lo[0], j - P.lo[1], k - P.lo[2]};
-  ci[d] += 1;
-  return ((long)ci[2] * P.bext[d][1] + ci[1]) * P.bext[d][0] + ci[0];
+  int a = i - P.lo[0], b = j - P.lo[1], cc = k - P.lo[2];
+  if (d == 0) a += 1;
+  else if (d == 1) b += 1;
+  else cc += 1;
+  return ((long)cc * P.bext[d][1] + b) * P.bext[d][0] + a;
+}
+
+template <int D>
+__device__ __forceinline__ int face_flux(const DevParams& P, const double* VL, const double* VR,
+                                         double F[8]) {
+  const int ierr = hllc<D>(P, VL, VR, F);
+  if (ierr) return ierr;
+  F[6] = 0.;
+  F[7] = 0.;
+  if (P.aux) {
+    if (F[0] > 0.) {
+      F[7] = F[0] / VL[P_DENS];
+      F[6] = F[0] * VL[P_EINT];
+    } else {
+      F[7] = F[0] / VR[P_DENS];
+      F[6] = F[0] * VR[P_EINT];
+    }
+  }
+  return 0;
 }
 
 __global__ void __launch_bounds__(kThreads) k_fluxes(DevParams P, Launch L) {
@@ -271,13 +319,15 @@ __global__ void __launch_bounds__(kThreads) k_fluxes(DevParams P, Launch L) {
     const int a1 = (int)((r / f0) % f1);
     const int a2 = (int)(r / ((long)f0 * f1));
     const int i = a0 + P.lo[0], j = a1 + P.lo[1], k = a2 + P.lo[2];
-    int im[3] = {i, j, k};
-    im[d] -= 1;
+    int im = i, jm = j, km = k;
+    if (d == 0) im -= 1;
+    else if (d == 1) jm -= 1;
+    else km -= 1;
     const fx_tile& T = L.tiles[t];
     const double* __restrict__ U = T.U;
     const double* S = L.states + (long)t * P.scellsPerTile * kStateComps + P.boff[d] * kStateComps;
     const long bc = P.bcells[d];
-    const long bl = band_index(P, d, im[0], im[1], im[2]);
+    const long bl = band_index(P, d, im, jm, km);
     const long br = band_index(P, d, i, j, k);
     double VL[8], VR[8];
     VL[P_DENS] = S[0 * bc + bl];
@@ -294,27 +344,19 @@ __global__ void __launch_bounds__(kThreads) k_fluxes(DevParams P, Launch L) {
     VR[P_EINT] = S[11 * bc + br];
     // face-state gammas equal the cell's (PPM.F90:733-736 under
     // FLASH_EOS_GAMMA, and the first-order fallback)
-    const long cl = cidx(P, im[0], im[1], im[2]), cr = cidx(P, i, j, k);
+    const long cl = cidx(P, im, jm, km), cr = cidx(P, i, j, k);
     VL[P_GAMC] = U[P.voff[FX_GAMC] + cl];
     VL[P_GAME] = U[P.voff[FX_GAME] + cl];
     VR[P_GAMC] = U[P.voff[FX_GAMC] + cr];
     VR[P_GAME] = U[P.voff[FX_GAME] + cr];
     double F[8];
-    int ierr = hllc(P, d, VL, VR, F);
+    int ierr;
+    if (d == 0) ierr = face_flux<0>(P, VL, VR, F);
+    else if (d == 1) ierr = face_flux<1>(P, VL, VR, F);
+    else ierr = face_flux<2>(P, VL, VR, F);
     if (ierr) {
       report_error(L.status, 2, t, i, j, k, d);
       continue;
-    }
-    F[FX_NFLUX - 2] = 0.;
-    F[FX_NFLUX - 1] = 0.;
-    if (P.aux) {
-      if (F[0] > 0.) {
-        F[7] = F[0] / VL[P_DENS];
-        F[6] = F[0] * VL[P_EINT];
-      } else {
-        F[7] = F[0] / VR[P_DENS];
-        F[6] = F[0] * VR[P_EINT];
-      }
     }
     double* Fo = T.flux[d];
     const long nf = P.nface[d];
@@ -717,7 +759,12 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
   Launch L{g.dtiles, nTiles, g.states, g.status, dt};
   hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
   if (record(0)) return 4;
-  hipLaunchKernelGGL(k_states, dim3(grid_for(P.scellsPerTile * nTiles)), dim3(kThreads), 0, g.stream, P, L);
+  {
+    const dim3 gs(grid_for(P.scellsPerTile * nTiles));
+    if (P.ndim == 3) hipLaunchKernelGGL(k_states<3>, gs, dim3(kThreads), 0, g.stream, P, L);
+    else if (P.ndim == 2) hipLaunchKernelGGL(k_states<2>, gs, dim3(kThreads), 0, g.stream, P, L);
+    else hipLaunchKernelGGL(k_states<1>, gs, dim3(kThreads), 0, g.stream, P, L);
+  }
   if (record(1)) return 4;
   hipLaunchKernelGGL(k_fluxes, dim3(grid_for((P.nface[0] + P.nface[1] + P.nface[2]) * nTiles)),
                      dim3(kThreads), 0, g.stream, P, L);
