@@ -219,18 +219,17 @@ __device__ __forceinline__ void trans_sigma(const DevParams& P, const double* vm
 
 // Full PPM reconstruction of one cell along dir (PPM.F90:378-842).
 // Returns false on an imaginary sound speed in a neighbour's eigen system.
+// PPM interface values, limiting and characteristic tracing from the three
+// projected slopes of cells i-1, i, i+1 (PPM.F90:578-842).  Vm/Vc/Vp are the
+// primitive vectors of i-1, i, i+1; densMM/densPP the densities of i-2, i+2
+// (contact steepening only).
 template <int DIR>
-__device__ __forceinline__ bool ppm_full(const DevParams& P, double dtn, double delta,
-                                         const double* Vmm, const double* Vm, const double* Vc,
-                                         const double* Vp, const double* Vpp, double flatC,
-                                         const Eig& e0, double Wp[8], double Wm[8]) {
-  Eig eP, eN;
-  if (!eig_make<DIR>(Vp, eP)) return false;
-  if (!eig_make<DIR>(Vm, eN)) return false;
-  double dbN[8], db0[8], dbP[8];
-  tvd_slope<DIR>(P, Vmm, Vm, Vc, eN, dbN);
-  tvd_slope<DIR>(P, Vm, Vc, Vp, e0, db0);
-  tvd_slope<DIR>(P, Vc, Vp, Vpp, eP, dbP);
+__device__ __forceinline__ void ppm_trace(const DevParams& P, double dtn, double delta,
+                                          const double* Vm, const double* Vc, const double* Vp,
+                                          double densMM, double densPP, const double* dbN,
+                                          const double* db0, const double* dbP, double flatC,
+                                          const Eig& e0, double Wp[8], double Wm[8]) {
+  double Vmm[1] = {densMM}, Vpp[1] = {densPP};
   const int endVar = P.aux ? P_EINT : P_PRES;
   double vL[8], vR[8];
 #pragma unroll
@@ -245,8 +244,8 @@ __device__ __forceinline__ bool ppm_full(const DevParams& P, double dtn, double 
   if (P.steep) {
     const double temp1 = Vp[P_DENS] - Vm[P_DENS];
     if (fabs(temp1) > P.tiny) {
-      const double d2R = (Vpp[P_DENS] - 2. * Vp[P_DENS] + Vc[P_DENS]) / (6. * delta * delta);
-      const double d2L = (Vc[P_DENS] - 2. * Vm[P_DENS] + Vmm[P_DENS]) / (6. * delta * delta);
+      const double d2R = (Vpp[0] - 2. * Vp[P_DENS] + Vc[P_DENS]) / (6. * delta * delta);
+      const double d2L = (Vc[P_DENS] - 2. * Vm[P_DENS] + Vmm[0]) / (6. * delta * delta);
       double eta = (d2L - d2R) * (delta * delta) / temp1;
       if (d2R * d2L >= 0.) eta = 0.;
       if (0.01 * fmin(Vp[P_DENS], Vm[P_DENS]) - fabs(Vp[P_DENS] - Vm[P_DENS]) >= 0.) eta = 0.;
@@ -340,6 +339,23 @@ __device__ __forceinline__ bool ppm_full(const DevParams& P, double dtn, double 
     Wm[v] = Vc[v] + W6[v] / 12. + sL[v];
     Wp[v] = Vc[v] + W6[v] / 12. + sR[v];
   }
+}
+
+
+template <int DIR>
+__device__ __forceinline__ bool ppm_full(const DevParams& P, double dtn, double delta,
+                                         const double* Vmm, const double* Vm, const double* Vc,
+                                         const double* Vp, const double* Vpp, double flatC,
+                                         const Eig& e0, double Wp[8], double Wm[8]) {
+  Eig eP, eN;
+  if (!eig_make<DIR>(Vp, eP)) return false;
+  if (!eig_make<DIR>(Vm, eN)) return false;
+  double dbN[8], db0[8], dbP[8];
+  tvd_slope<DIR>(P, Vmm, Vm, Vc, eN, dbN);
+  tvd_slope<DIR>(P, Vm, Vc, Vp, e0, db0);
+  tvd_slope<DIR>(P, Vc, Vp, Vpp, eP, dbP);
+  ppm_trace<DIR>(P, dtn, delta, Vm, Vc, Vp, Vmm[P_DENS], Vpp[P_DENS], dbN, db0, dbP, flatC, e0, Wp,
+                 Wm);
   return true;
 }
 

@@ -76,11 +76,12 @@ __device__ __forceinline__ void load_prim(const DevParams& P, const double* __re
   V[P_EINT] = U[P.voff[FX_EINT] + c];
 }
 
-// FlatTilde of one cell along dir (hy_getRiemannState.F90:365-404)
+// FlatTilde of one cell along DIR (hy_getRiemannState.F90:365-404)
+template <int DIR>
 __device__ __forceinline__ double flat_tilde(const DevParams& P, const double* __restrict__ U,
-                                             long c, long s, int dir) {
+                                             long c, long s) {
   const double* pr = U + P.voff[FX_PRES];
-  const double* vn = U + P.voff[FX_VELX + dir];
+  const double* vn = U + P.voff[FX_VELX + DIR];
   const double dp1 = (pr[c + s] - pr[c - s]);
   const double dp2 = (pr[c + 2 * s] - pr[c - 2 * s]);
   const double dv1 = vn[c + s] - vn[c - s];
@@ -92,14 +93,15 @@ __device__ __forceinline__ double flat_tilde(const DevParams& P, const double* _
 }
 
 // FlatCoeff (hy_getRiemannState.F90:408-453)
+template <int DIR>
 __device__ __forceinline__ double flat_coeff(const DevParams& P, const double* __restrict__ U,
-                                             long c, long s, int dir) {
+                                             long c, long s) {
   const double* pr = U + P.voff[FX_PRES];
   const double dp1 = (pr[c + s] - pr[c - s]);
-  const double f0 = flat_tilde(P, U, c, s, dir);
-  if (dp1 < 0.0) return fmax(f0, flat_tilde(P, U, c + s, s, dir));
+  const double f0 = flat_tilde<DIR>(P, U, c, s);
+  if (dp1 < 0.0) return fmax(f0, flat_tilde<DIR>(P, U, c + s, s));
   if (dp1 == 0.) return f0;
-  return fmax(f0, flat_tilde(P, U, c - s, s, dir));
+  return fmax(f0, flat_tilde<DIR>(P, U, c - s, s));
 }
 
 __device__ __forceinline__ long axis_stride(const DevParams& P, int a) {
@@ -141,7 +143,7 @@ __device__ __forceinline__ bool state_cell(const DevParams& P, const double* __r
     load_prim(P, U, c - 2 * sd, Vmm);
     load_prim(P, U, c + 2 * sd, Vpp);
     Eig e0;
-    const double flat = P.flat ? flat_coeff(P, U, c, sd, D) : 0.;
+    const double flat = P.flat ? flat_coeff<D>(P, U, c, sd) : 0.;
     if (!eig_make<D>(Vc, e0)) return false;
     if (!ppm_full<D>(P, dt / del[D], del[D], Vmm, Vm, Vc, Vp, Vpp, flat, e0, Wp, Wn)) return false;
   }
@@ -369,6 +371,8 @@ __device__ __forceinline__ long face_index(const DevParams& P, int d, int i, int
   return ((long)(k - P.lo[2]) * P.fext[d][1] + (j - P.lo[1])) * P.fext[d][0] + (i - P.lo[0]);
 }
 
+#include "fx_march.hpp"
+
 // FROM_STATES: face-averaged pressure from this call's states; otherwise
 // read from the tile's pavg array (hy_updateSolution path).
 template <bool FROM_STATES, bool WRITE_PAVG_ONLY>
@@ -549,6 +553,8 @@ struct Lib {
   size_t statesCap = 0;   // doubles
   double* flux = nullptr; // library-owned flux arrays for tiles without them
   size_t fluxCap = 0;
+  double* pavg = nullptr; // library-owned face-averaged pressure arrays
+  size_t pavgCap = 0;
   fx_tile* dtiles = nullptr;
   size_t dtilesCap = 0;
   unsigned char* dinplace = nullptr;
@@ -565,6 +571,7 @@ struct Lib {
   bool batchValid = false;
   // per-stage timing (fx_hydro_set_timing)
   int timing = 0;
+  int legacy = 0;  // 1: force the three-kernel path (fx_hydro_set_kernel_path)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   float stageMs[3] = {0.f, 0.f, 0.f};
 };
@@ -655,6 +662,24 @@ DevParams make_dev_params(const fx_params& p) {
   return d;
 }
 
+// Column-tile shape of the fused kernel for this block shape, or false
+// when only the three-kernel path applies (1-D, or sizes not tileable).
+bool march_shape(const DevParams& P, int* TX, int* TY, int* NT) {
+  if (P.ndim < 2) return false;
+  const int nx = P.nxb, ny = P.nyb;
+  if (nx % 16 == 0 && ny % 8 == 0) {
+    *TX = 16;
+    *TY = 8;
+  } else if (nx % 8 == 0 && ny % 8 == 0) {
+    *TX = 8;
+    *TY = 8;
+  } else {
+    return false;
+  }
+  *NT = (P.ndim == 3 && *TX == 16) ? 512 : 256;
+  return true;
+}
+
 int grid_for(long items) {
   long blocks = (items + kThreads - 1) / kThreads;
   const long cap = (long)g.numCU * 64;
@@ -715,6 +740,16 @@ int upload_tiles(const fx_tile* tiles, int nTiles, bool needFlux, fx_status* st)
       }
     }
   }
+  if (needFlux) {
+    size_t missingP = 0;
+    for (int t = 0; t < nTiles; ++t)
+      if (!g.htiles[t].pavg) ++missingP;
+    if (missingP) {
+      if (grow(&g.pavg, &g.pavgCap, (size_t)P.icells * nTiles, st)) return 4;
+      for (int t = 0; t < nTiles; ++t)
+        if (!g.htiles[t].pavg) g.htiles[t].pavg = g.pavg + (size_t)P.icells * t;
+    }
+  }
   if (grow(&g.dtiles, &g.dtilesCap, (size_t)nTiles, st)) return 4;
   HIPCALL(hipMemcpyAsync(g.dtiles, g.htiles.data(), sizeof(fx_tile) * nTiles, hipMemcpyHostToDevice,
                          g.stream),
@@ -755,6 +790,23 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
     HIPCALL(hipMemcpyAsync(g.dinplace, inPlace, nTiles, hipMemcpyHostToDevice, g.stream), st);
     dIn = g.dinplace;
   }
+  int TX = 0, TY = 0, NT = 0;
+  if (!g.legacy && march_shape(P, &TX, &TY, &NT)) {
+    MarchLaunch M{g.dtiles, nTiles, P.nxb / TX, (P.ndim > 1 ? P.nyb : 1) / TY, g.status, dt};
+    const dim3 grid((unsigned)(nTiles * M.tilesX * M.tilesY));
+    Launch L{g.dtiles, nTiles, nullptr, g.status, dt};
+    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
+    if (record(0)) return 4;
+    if (P.ndim == 3 && TX == 16) hipLaunchKernelGGL((k_march<3, 16, 8, 512>), grid, dim3(512), 0, g.stream, P, M);
+    else if (P.ndim == 3) hipLaunchKernelGGL((k_march<3, 8, 8, 256>), grid, dim3(256), 0, g.stream, P, M);
+    else if (TX == 16) hipLaunchKernelGGL((k_march<2, 16, 8, 256>), grid, dim3(256), 0, g.stream, P, M);
+    else hipLaunchKernelGGL((k_march<2, 8, 8, 256>), grid, dim3(256), 0, g.stream, P, M);
+    if (record(1)) return 4;
+    if (record(2)) return 4;
+    hipLaunchKernelGGL((k_update<false, false>), dim3(grid_for(P.icells * nTiles)), dim3(kThreads), 0,
+                       g.stream, P, L, dIn);
+    if (record(3)) return 4;
+  } else {
   if (grow(&g.states, &g.statesCap, (size_t)P.scellsPerTile * kStateComps * nTiles, st)) return 4;
   Launch L{g.dtiles, nTiles, g.states, g.status, dt};
   hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
@@ -773,6 +825,7 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
   hipLaunchKernelGGL((k_update<true, false>), dim3(grid_for(P.icells * nTiles)), dim3(kThreads), 0,
                      g.stream, P, L, dIn);
   if (record(3)) return 4;
+  }
   HIPCALL(hipGetLastError(), st);
   int rc = finish(st);
   if (g.timing && !g.async) {
@@ -804,7 +857,7 @@ int fx_hydro_init(const fx_params* p) {
 }
 
 void fx_hydro_finalize(void) {
-  void* bufs[] = {g.states, g.flux, g.dtiles, g.dinplace, g.status, g.partial};
+  void* bufs[] = {g.states, g.flux, g.pavg, g.dtiles, g.dinplace, g.status, g.partial};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : g.ev)
@@ -889,6 +942,10 @@ int fx_hydro_compute_dt(const double* U, long nBlocks, long blockStride, const d
 
 void fx_hydro_set_stream(void* stream) { g.stream = (hipStream_t)stream; }
 void fx_hydro_set_timing(int on) { g.timing = on; }
+void fx_hydro_set_kernel_path(int path) {
+  g.legacy = (path == 1);
+  g.batchValid = false;
+}
 int fx_hydro_stage_times(float* ms, int n) {
   for (int k = 0; k < n && k < 3; ++k) ms[k] = g.stageMs[k];
   return 3;

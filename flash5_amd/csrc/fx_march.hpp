@@ -1,0 +1,486 @@
+// fx_march.hpp -- fused face-state + flux kernel: one workgroup per
+// (block, TX x TY column tile), marching along z through the block.
+//
+// Included inside fx_hydro.hip's anonymous namespace (uses its helpers).
+//
+// Per z-plane k the workgroup computes, each value once (the work of
+// hy_getRiemannState.F90:356-1327 regrouped by quantity instead of by cell):
+//   stage A  transverse fluxes sigma_x, sigma_y of plane k+1 into 3-plane LDS
+//            rings (they feed the 3-D CTU terms of planes k and k+2),
+//            sigma_z of plane k, limited characteristic slopes along x and y
+//            of plane k, and along z (with the z flattening tilde) of plane
+//            k+1 into 3-plane rings
+//   stage B1 x-, y- and z-face states of plane k: PPM with characteristic
+//            tracing, 2-D CTU transverse corrections, positivity fallback
+//   stage B2 3-D full-CTU corner terms added to those states
+//   stage C  x-, y- and z-face fluxes (HLLC and the aux internal-energy
+//            fluxes, hy_getFaceFlux.F90) to the tile's flux arrays and the
+//            face-averaged pressure of each interior cell to pavg.
+// The conservative update then runs as k_update from those arrays.
+// x/y states exist for interior planes only; z states for planes lo-1..hi+1.
+
+// LDS index helpers, tile-local coordinates a = i - ox, b = j - oy
+template <int TX, int TY> struct TileIdx {
+  // R_x: a in [0,TX), b in [-1,TY]   (sigma_x, y-face states)
+  static __device__ __forceinline__ int rx(int a, int b) { return (b + 1) * TX + a; }
+  // R_y: a in [-1,TX], b in [0,TY)   (sigma_y, x-face states)
+  static __device__ __forceinline__ int ry(int a, int b) { return b * (TX + 2) + (a + 1); }
+  // C: a in [-1,TX], b in [-1,TY]    (sigma_z)
+  static __device__ __forceinline__ int cc(int a, int b) { return (b + 1) * (TX + 2) + (a + 1); }
+  // slopes along x: a in [-2,TX+1], b in [0,TY)
+  static __device__ __forceinline__ int lx(int a, int b) { return b * (TX + 4) + (a + 2); }
+  // slopes along y: a in [0,TX), b in [-2,TY+1]
+  static __device__ __forceinline__ int ly(int a, int b) { return (b + 2) * TX + a; }
+  static constexpr int NRX = TX * (TY + 2), NRY = (TX + 2) * TY, NC = (TX + 2) * (TY + 2);
+  static constexpr int NLX = (TX + 4) * TY, NLY = TX * (TY + 4);
+};
+
+// components stored for sigma and slopes: dens, vx, vy, vz, pres, eint
+__device__ __forceinline__ int comp_slot(int q) { return q < 5 ? q : P_EINT; }
+
+struct MarchLaunch {
+  const fx_tile* tiles;
+  int nTiles;
+  int tilesX, tilesY;  // column tiles per block
+  DevStatus* status;
+  double dt;
+};
+
+template <int SD>
+__device__ __forceinline__ bool sigma_cell(const DevParams& P, const double* __restrict__ U, long c,
+                                           double sig[8]) {
+  return sigma_at<SD>(P, U, c, sig, nullptr);
+}
+
+template <int DIR>
+__device__ __forceinline__ bool slope_cell(const DevParams& P, const double* __restrict__ U, long c,
+                                           double db[8]) {
+  const long s = axis_stride(P, DIR);
+  double Vm[8], Vc[8], Vp[8];
+  load_prim(P, U, c - s, Vm);
+  load_prim(P, U, c, Vc);
+  load_prim(P, U, c + s, Vp);
+  Eig e;
+  if (!eig_make<DIR>(Vc, e)) return false;
+  tvd_slope<DIR>(P, Vm, Vc, Vp, e, db);
+  return true;
+}
+
+__device__ __forceinline__ void lds_put6(double* base, int n, int idx, const double v[8]) {
+#pragma unroll
+  for (int q = 0; q < 6; ++q) base[q * n + idx] = v[comp_slot(q)];
+}
+__device__ __forceinline__ void lds_get6(const double* base, int n, int idx, double v[8]) {
+#pragma unroll
+  for (int q = 0; q < 6; ++q) v[comp_slot(q)] = base[q * n + idx];
+  v[P_GAMC] = 0.;
+  v[P_GAME] = 0.;
+}
+
+__device__ __forceinline__ void fallback_first_order(const double* Vc, double Wp[8], double Wn[8],
+                                                     DevStatus* st) {
+  atomicAdd(&st->fallbacks, 1ULL);
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    Wn[v] = Vc[v];
+    Wp[v] = Vc[v];
+  }
+}
+
+__device__ __forceinline__ void sub_sigma(double Wp[8], double Wn[8], const double* sg, double hdt) {
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    Wn[v] = Wn[v] - hdt * sg[v];
+    Wp[v] = Wp[v] - hdt * sg[v];
+  }
+}
+
+__device__ __forceinline__ void add_ctu(double Wp[8], double Wn[8], const double* tA,
+                                        const double* tB, double fac) {
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    Wp[v] = Wp[v] + (tA[v] + tB[v]) * fac;
+    Wn[v] = Wn[v] + (tA[v] + tB[v]) * fac;
+  }
+}
+
+__device__ __forceinline__ bool nonpositive(const double* Wp, const double* Wn) {
+  return Wp[P_DENS] <= 0. || Wn[P_DENS] <= 0. || Wp[P_PRES] <= 0. || Wn[P_PRES] <= 0.;
+}
+__device__ __forceinline__ bool negative(const double* Wp, const double* Wn) {
+  return Wn[P_DENS] < 0. || Wp[P_DENS] < 0. || Wn[P_PRES] < 0. || Wp[P_PRES] < 0.;
+}
+
+// store/load a face-state pair (Wp, Wn) of 6 components each
+__device__ __forceinline__ void put_state(double* base, int n, int idx, const double Wp[8],
+                                          const double Wn[8]) {
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    base[q * n + idx] = Wp[comp_slot(q)];
+    base[(6 + q) * n + idx] = Wn[comp_slot(q)];
+  }
+}
+
+template <int D>
+__device__ __forceinline__ void write_flux(const DevParams& P, double* Fo, long r, const double F[8]) {
+  const long nf = P.nface[D];
+#pragma unroll
+  for (int q = 0; q < FX_NFLUX; ++q) Fo[q * nf + r] = F[q];
+}
+
+__device__ __forceinline__ void get_state(const double* base, int n, int idx, double Wp[8],
+                                          double Wn[8]) {
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    Wp[comp_slot(q)] = base[q * n + idx];
+    Wn[comp_slot(q)] = base[(6 + q) * n + idx];
+  }
+  Wp[P_GAMC] = Wp[P_GAME] = Wn[P_GAMC] = Wn[P_GAME] = 0.;
+}
+
+__device__ __forceinline__ double flat_tilde_vals(double pm2, double pm1, double pp1, double pp2,
+                                                  double vm1, double vp1) {
+  const double dp1 = (pp1 - pm1);
+  const double dp2 = (pp2 - pm2);
+  const double dv1 = vp1 - vm1;
+  const double Sp = (fabs(dp2) > 1.e-15) ? dp1 / dp2 - 0.75 : 0.;
+  double ft = fmax(0.0, fmin(1.0, 10.0 * Sp));
+  if ((fabs(dp1) / fmin(pm1, pp1) < 1. / 3.) || dv1 > 0.) ft = 0.;
+  return ft;
+}
+
+template <int NDIM, int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
+  using I = TileIdx<TX, TY>;
+  constexpr int NR = NDIM == 3 ? 3 : 1;        // ring depth
+  constexpr int NCOL = TX * TY;
+  constexpr int Z = NDIM == 3 ? 1 : 0;
+  __shared__ double SX[NR * 6 * I::NRX];        // sigma_x ring
+  __shared__ double SY[NR * 6 * I::NRY];        // sigma_y ring
+  __shared__ double SZ[Z ? 6 * I::NC : 1];      // sigma_z of plane k
+  __shared__ double LX[6 * I::NLX];             // slopes along x, plane k
+  __shared__ double LY[6 * I::NLY];             // slopes along y, plane k
+  __shared__ double LZ[Z ? 3 * 6 * NCOL : 1];   // slopes along z, ring
+  __shared__ double FZ[Z ? 3 * NCOL : 1];       // flattening tilde along z, ring
+  __shared__ double WX[12 * I::NRY];            // x-face states (Wp, Wn)
+  __shared__ double WY[12 * I::NRX];            // y-face states
+  __shared__ double WZ[Z ? 12 * NCOL : 1];      // z-face states of plane k
+  __shared__ double WZP[Z ? 6 * NCOL : 1];      // z-face Wp of plane k-1
+
+  const int tid = threadIdx.x;
+  const int tilesPerBlock = L.tilesX * L.tilesY;
+  const int t = blockIdx.x / tilesPerBlock;
+  const int tr = blockIdx.x - t * tilesPerBlock;
+  const int ox = P.lo[0] + (tr % L.tilesX) * TX;
+  const int oy = P.lo[1] + (tr / L.tilesX) * TY;
+  const fx_tile& T = L.tiles[t];
+  const double* __restrict__ U = T.U;
+  const double dt = L.dt;
+  const double dx = T.del[0], dy = T.del[1], dz = T.del[2];
+  const long sx = 1, sy = P.nx, sz = (long)P.nx * P.ny;
+  const int klo = P.lo[2], khi = P.hi[2];
+  DevStatus* st = L.status;
+  bool bad = false;  // eigen failure seen by this thread
+
+  auto ring = [](int k) { return NR == 3 ? (k % 3) : 0; };
+
+  // stage-A item: sigma_x / sigma_y of plane kk (w < NRX + NRY)
+  auto sig_xy_item = [&](int w, int kk) {
+    double sg[8];
+    if (w < I::NRX) {
+      const int a = w % TX, b = w / TX - 1;
+      if (!sigma_at<0>(P, U, cidx(P, ox + a, oy + b, kk), sg, nullptr)) { bad = true; return; }
+      lds_put6(SX + ring(kk) * 6 * I::NRX, I::NRX, w, sg);
+    } else {
+      const int w2 = w - I::NRX;
+      const int a = w2 % (TX + 2) - 1, b = w2 / (TX + 2);
+      if (!sigma_at<1>(P, U, cidx(P, ox + a, oy + b, kk), sg, nullptr)) { bad = true; return; }
+      lds_put6(SY + ring(kk) * 6 * I::NRY, I::NRY, w2, sg);
+    }
+  };
+  // stage-A item: z-slope and z flattening tilde of column w at plane m
+  auto zslope_item = [&](int w, int m) {
+    const int a = w % TX, b = w / TX;
+    const long c = cidx(P, ox + a, oy + b, m);
+    double db[8];
+    if (!slope_cell<2>(P, U, c, db)) { bad = true; return; }
+    lds_put6(LZ + ring(m) * 6 * NCOL, NCOL, w, db);
+    const double* pr = U + P.voff[FX_PRES];
+    const double* vz = U + P.voff[FX_VELZ];
+    FZ[ring(m) * NCOL + w] =
+        flat_tilde_vals(pr[c - 2 * sz], pr[c - sz], pr[c + sz], pr[c + 2 * sz], vz[c - sz], vz[c + sz]);
+  };
+
+  const int kBeg = Z ? klo - 1 : 0;
+  const int kEnd = Z ? khi + 1 : 0;
+  if (Z) {
+    // prologue: sigma_x/y of plane kBeg, z slopes of kBeg-1 and kBeg
+    for (int w = tid; w < I::NRX + I::NRY + 2 * NCOL; w += NT) {
+      if (w < I::NRX + I::NRY) sig_xy_item(w, kBeg);
+      else if (w < I::NRX + I::NRY + NCOL) zslope_item(w - I::NRX - I::NRY, kBeg - 1);
+      else zslope_item(w - I::NRX - I::NRY - NCOL, kBeg);
+    }
+  }
+
+  for (int k = kBeg; k <= kEnd; ++k) {
+    const bool interior = !Z || (k >= klo && k <= khi);
+    // ------------------------------ stage A ------------------------------
+    {
+      const int nSig = (!Z || k + 1 <= kEnd) ? I::NRX + I::NRY : 0;
+      const int nSz = (Z && interior) ? I::NC : 0;
+      const int nL = interior ? I::NLX + I::NLY : 0;
+      const int nZs = Z ? NCOL : 0;
+      const int nA = nSig + nSz + nL + nZs;
+      for (int w = tid; w < nA; w += NT) {
+        int r = w;
+        if (r < nSig) { sig_xy_item(r, Z ? k + 1 : 0); continue; }
+        r -= nSig;
+        if (r < nSz) {
+          double v8[8];
+          const int a = r % (TX + 2) - 1, b = r / (TX + 2) - 1;
+          if (!sigma_at<2>(P, U, cidx(P, ox + a, oy + b, k), v8, nullptr)) { bad = true; continue; }
+          lds_put6(SZ, I::NC, r, v8);
+          continue;
+        }
+        r -= nSz;
+        if (r < nL) {
+          double v8[8];
+          if (r < I::NLX) {
+            const int a = r % (TX + 4) - 2, b = r / (TX + 4);
+            if (!slope_cell<0>(P, U, cidx(P, ox + a, oy + b, k), v8)) { bad = true; continue; }
+            lds_put6(LX, I::NLX, r, v8);
+          } else {
+            r -= I::NLX;
+            const int a = r % TX, b = r / TX - 2;
+            if (!slope_cell<1>(P, U, cidx(P, ox + a, oy + b, k), v8)) { bad = true; continue; }
+            lds_put6(LY, I::NLY, r, v8);
+          }
+          continue;
+        }
+        r -= nL;
+        zslope_item(r, k + 1);
+      }
+    }
+    __syncthreads();
+
+    const double* sxk = SX + ring(k) * 6 * I::NRX;
+    const double* syk = SY + ring(k) * 6 * I::NRY;
+    // ------------------------------ stage B1 -----------------------------
+    {
+      const int nX = interior ? I::NRY : 0;
+      const int nY = interior ? I::NRX : 0;
+      const int nZ = Z ? NCOL : 0;
+      for (int w = tid; w < nX + nY + nZ; w += NT) {
+        double Wp[8], Wn[8], Vm[8], Vc[8], Vp[8], dbN[8], db0[8], dbP[8];
+        Eig e0;
+        if (w < nX) {
+          const int a = w % (TX + 2) - 1, b = w / (TX + 2);
+          const long c = cidx(P, ox + a, oy + b, k);
+          load_prim(P, U, c - sx, Vm);
+          load_prim(P, U, c, Vc);
+          load_prim(P, U, c + sx, Vp);
+          const double flat = P.flat ? flat_coeff<0>(P, U, c, sx) : 0.;
+          lds_get6(LX, I::NLX, I::lx(a - 1, b), dbN);
+          lds_get6(LX, I::NLX, I::lx(a, b), db0);
+          lds_get6(LX, I::NLX, I::lx(a + 1, b), dbP);
+          if (!eig_make<0>(Vc, e0)) { bad = true; continue; }
+          const double dmm = P.steep ? U[P.voff[FX_DENS] + c - 2 * sx] : 0.;
+          const double dpp = P.steep ? U[P.voff[FX_DENS] + c + 2 * sx] : 0.;
+          ppm_trace<0>(P, dt / dx, dx, Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
+          double sg[8];
+          lds_get6(syk, I::NRY, I::ry(a, b), sg);
+          sub_sigma(Wp, Wn, sg, 0.5 * dt / dy);
+          if (Z) {
+            lds_get6(SZ, I::NC, I::cc(a, b), sg);
+            sub_sigma(Wp, Wn, sg, 0.5 * dt / dz);
+          }
+          if (negative(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st);
+          put_state(WX, I::NRY, w, Wp, Wn);
+        } else if (w < nX + nY) {
+          const int w2 = w - nX;
+          const int a = w2 % TX, b = w2 / TX - 1;
+          const long c = cidx(P, ox + a, oy + b, k);
+          load_prim(P, U, c - sy, Vm);
+          load_prim(P, U, c, Vc);
+          load_prim(P, U, c + sy, Vp);
+          const double flat = P.flat ? flat_coeff<1>(P, U, c, sy) : 0.;
+          lds_get6(LY, I::NLY, I::ly(a, b - 1), dbN);
+          lds_get6(LY, I::NLY, I::ly(a, b), db0);
+          lds_get6(LY, I::NLY, I::ly(a, b + 1), dbP);
+          if (!eig_make<1>(Vc, e0)) { bad = true; continue; }
+          const double dmm = P.steep ? U[P.voff[FX_DENS] + c - 2 * sy] : 0.;
+          const double dpp = P.steep ? U[P.voff[FX_DENS] + c + 2 * sy] : 0.;
+          ppm_trace<1>(P, dt / dy, dy, Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
+          double sg[8];
+          lds_get6(sxk, I::NRX, I::rx(a, b), sg);
+          sub_sigma(Wp, Wn, sg, 0.5 * dt / dx);
+          if (Z) {
+            lds_get6(SZ, I::NC, I::cc(a, b), sg);
+            sub_sigma(Wp, Wn, sg, 0.5 * dt / dz);
+          }
+          if (negative(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st);
+          put_state(WY, I::NRX, w2, Wp, Wn);
+        } else {
+          const int w2 = w - nX - nY;
+          const int a = w2 % TX, b = w2 / TX;
+          const long c = cidx(P, ox + a, oy + b, k);
+          load_prim(P, U, c - sz, Vm);
+          load_prim(P, U, c, Vc);
+          load_prim(P, U, c + sz, Vp);
+          double flat = 0.;
+          if (P.flat) {
+            const double dp1 = Vp[P_PRES] - Vm[P_PRES];
+            const double f0 = FZ[ring(k) * NCOL + w2];
+            flat = dp1 < 0.0 ? fmax(f0, FZ[ring(k + 1) * NCOL + w2])
+                             : (dp1 == 0. ? f0 : fmax(f0, FZ[ring(k - 1) * NCOL + w2]));
+          }
+          lds_get6(LZ + ring(k - 1) * 6 * NCOL, NCOL, w2, dbN);
+          lds_get6(LZ + ring(k) * 6 * NCOL, NCOL, w2, db0);
+          lds_get6(LZ + ring(k + 1) * 6 * NCOL, NCOL, w2, dbP);
+          if (!eig_make<2>(Vc, e0)) { bad = true; continue; }
+          const double dmm = P.steep ? U[P.voff[FX_DENS] + c - 2 * sz] : 0.;
+          const double dpp = P.steep ? U[P.voff[FX_DENS] + c + 2 * sz] : 0.;
+          ppm_trace<2>(P, dt / dz, dz, Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
+          double sg[8];
+          lds_get6(sxk, I::NRX, I::rx(a, b), sg);
+          sub_sigma(Wp, Wn, sg, 0.5 * dt / dx);
+          lds_get6(syk, I::NRY, I::ry(a, b), sg);
+          sub_sigma(Wp, Wn, sg, 0.5 * dt / dy);
+          if (negative(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st);
+          put_state(WZ, NCOL, w2, Wp, Wn);
+        }
+      }
+    }
+    __syncthreads();
+
+    // ------------------------------ stage B2 -----------------------------
+    if (Z && P.ctu3d) {
+      const int nX = interior ? I::NRY : 0;
+      const int nY = interior ? I::NRX : 0;
+      for (int w = tid; w < nX + nY + NCOL; w += NT) {
+        double Wp[8], Wn[8], Vc[8], sm[8], sc[8], sp[8], tA[8], tB[8];
+        Eig e1, e2;
+        double fac;
+        double* base;
+        int n, idx;
+        if (w < nX) {
+          // X: sigma_z over j with the y eigen system, sigma_y over k with z's
+          const int a = w % (TX + 2) - 1, b = w / (TX + 2);
+          load_prim(P, U, cidx(P, ox + a, oy + b, k), Vc);
+          if (!(eig_make<1>(Vc, e1) && eig_make<2>(Vc, e2))) { bad = true; continue; }
+          lds_get6(SZ, I::NC, I::cc(a, b - 1), sm);
+          lds_get6(SZ, I::NC, I::cc(a, b), sc);
+          lds_get6(SZ, I::NC, I::cc(a, b + 1), sp);
+          trans_sigma<1>(P, sm, sc, sp, e1, tA);
+          lds_get6(SY + ring(k - 1) * 6 * I::NRY, I::NRY, I::ry(a, b), sm);
+          lds_get6(syk, I::NRY, I::ry(a, b), sc);
+          lds_get6(SY + ring(k + 1) * 6 * I::NRY, I::NRY, I::ry(a, b), sp);
+          trans_sigma<2>(P, sm, sc, sp, e2, tB);
+          fac = dt * dt / (6. * dy * dz);
+          base = WX; n = I::NRY; idx = w;
+        } else if (w < nX + nY) {
+          // Y: sigma_x over k with z's eigen system, sigma_z over i with x's
+          const int w2 = w - nX;
+          const int a = w2 % TX, b = w2 / TX - 1;
+          load_prim(P, U, cidx(P, ox + a, oy + b, k), Vc);
+          if (!(eig_make<2>(Vc, e1) && eig_make<0>(Vc, e2))) { bad = true; continue; }
+          lds_get6(SX + ring(k - 1) * 6 * I::NRX, I::NRX, I::rx(a, b), sm);
+          lds_get6(sxk, I::NRX, I::rx(a, b), sc);
+          lds_get6(SX + ring(k + 1) * 6 * I::NRX, I::NRX, I::rx(a, b), sp);
+          trans_sigma<2>(P, sm, sc, sp, e1, tA);
+          lds_get6(SZ, I::NC, I::cc(a - 1, b), sm);
+          lds_get6(SZ, I::NC, I::cc(a, b), sc);
+          lds_get6(SZ, I::NC, I::cc(a + 1, b), sp);
+          trans_sigma<0>(P, sm, sc, sp, e2, tB);
+          fac = dt * dt / (6. * dz * dx);
+          base = WY; n = I::NRX; idx = w2;
+        } else {
+          // Z: sigma_y over i with x's eigen system, sigma_x over j with y's
+          const int w2 = w - nX - nY;
+          const int a = w2 % TX, b = w2 / TX;
+          load_prim(P, U, cidx(P, ox + a, oy + b, k), Vc);
+          if (!(eig_make<0>(Vc, e1) && eig_make<1>(Vc, e2))) { bad = true; continue; }
+          lds_get6(syk, I::NRY, I::ry(a - 1, b), sm);
+          lds_get6(syk, I::NRY, I::ry(a, b), sc);
+          lds_get6(syk, I::NRY, I::ry(a + 1, b), sp);
+          trans_sigma<0>(P, sm, sc, sp, e1, tA);
+          lds_get6(sxk, I::NRX, I::rx(a, b - 1), sm);
+          lds_get6(sxk, I::NRX, I::rx(a, b), sc);
+          lds_get6(sxk, I::NRX, I::rx(a, b + 1), sp);
+          trans_sigma<1>(P, sm, sc, sp, e2, tB);
+          fac = dt * dt / (6. * dx * dy);
+          base = WZ; n = NCOL; idx = w2;
+        }
+        get_state(base, n, idx, Wp, Wn);
+        add_ctu(Wp, Wn, tA, tB, fac);
+        if (nonpositive(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st);
+        put_state(base, n, idx, Wp, Wn);
+      }
+      __syncthreads();
+    }
+
+    // ------------------------------ stage C ------------------------------
+    {
+      const int nFX = interior ? (TX + 1) * TY : 0;
+      const int nFY = interior ? TX * (TY + 1) : 0;
+      const int nC = Z ? NCOL : (interior ? NCOL : 0);
+      for (int w = tid; w < nFX + nFY + nC; w += NT) {
+        double VL[8], VR[8], F[8];
+        if (w < nFX) {
+          const int a = w % (TX + 1), b = w / (TX + 1);
+          lds_get6(WX, I::NRY, I::ry(a - 1, b), VL);
+          lds_get6(WX + 6 * I::NRY, I::NRY, I::ry(a, b), VR);
+          const long c = cidx(P, ox + a, oy + b, k);
+          VL[P_GAMC] = U[P.voff[FX_GAMC] + c - sx];
+          VL[P_GAME] = U[P.voff[FX_GAME] + c - sx];
+          VR[P_GAMC] = U[P.voff[FX_GAMC] + c];
+          VR[P_GAME] = U[P.voff[FX_GAME] + c];
+          if (face_flux<0>(P, VL, VR, F)) { report_error(st, 2, t, ox + a, oy + b, k, 0); continue; }
+          write_flux<0>(P, T.flux[0], face_index(P, 0, ox + a, oy + b, k), F);
+        } else if (w < nFX + nFY) {
+          const int w2 = w - nFX;
+          const int a = w2 % TX, b = w2 / TX;
+          lds_get6(WY, I::NRX, I::rx(a, b - 1), VL);
+          lds_get6(WY + 6 * I::NRX, I::NRX, I::rx(a, b), VR);
+          const long c = cidx(P, ox + a, oy + b, k);
+          VL[P_GAMC] = U[P.voff[FX_GAMC] + c - sy];
+          VL[P_GAME] = U[P.voff[FX_GAME] + c - sy];
+          VR[P_GAMC] = U[P.voff[FX_GAMC] + c];
+          VR[P_GAME] = U[P.voff[FX_GAME] + c];
+          if (face_flux<1>(P, VL, VR, F)) { report_error(st, 2, t, ox + a, oy + b, k, 1); continue; }
+          write_flux<1>(P, T.flux[1], face_index(P, 1, ox + a, oy + b, k), F);
+        } else {
+          // column item: z-face flux at face k, pavg of the cell, carry Wp
+          const int w2 = w - nFX - nFY;
+          const int a = w2 % TX, b = w2 / TX;
+          const long c = cidx(P, ox + a, oy + b, k);
+          if (Z && k >= klo) {
+            lds_get6(WZP, NCOL, w2, VL);
+            lds_get6(WZ + 6 * NCOL, NCOL, w2, VR);
+            VL[P_GAMC] = U[P.voff[FX_GAMC] + c - sz];
+            VL[P_GAME] = U[P.voff[FX_GAME] + c - sz];
+            VR[P_GAMC] = U[P.voff[FX_GAMC] + c];
+            VR[P_GAME] = U[P.voff[FX_GAME] + c];
+            if (face_flux<2>(P, VL, VR, F)) report_error(st, 2, t, ox + a, oy + b, k, 2);
+            else write_flux<2>(P, T.flux[2], face_index(P, 2, ox + a, oy + b, k), F);
+          }
+          if (interior && P.aux) {
+            // face-averaged pressure (hy_getFaceFlux.F90:507-561 and the
+            // y, z analogues)
+            double pres = 0.5 * (WX[10 * I::NRY + I::ry(a, b)] + WX[4 * I::NRY + I::ry(a, b)]);
+            pres = pres * 0.5 + (WY[10 * I::NRX + I::rx(a, b)] + WY[4 * I::NRX + I::rx(a, b)]) * 0.25;
+            if (Z) pres = (pres * 4.0 + (WZ[10 * NCOL + w2] + WZ[4 * NCOL + w2])) / 6.0;
+            const long r = ((long)(k - P.lo[2]) * P.nyb + (oy + b - P.lo[1])) * P.nxb + (ox + a - P.lo[0]);
+            T.pavg[r] = pres;
+          }
+          if (Z) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) WZP[q * NCOL + w2] = WZ[q * NCOL + w2];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (bad) report_error(st, 1, t, ox, oy, -1, -1);
+}

@@ -82,6 +82,7 @@ EXPORTS = {
     "fx_hydro_set_stream": ([C.c_void_p], None),
     "fx_hydro_set_async": ([C.c_int], None),
     "fx_hydro_set_timing": ([C.c_int], None),
+    "fx_hydro_set_kernel_path": ([C.c_int], None),
     "fx_hydro_stage_times": ([C.POINTER(C.c_float), C.c_int], C.c_int),
     "fx_hydro_version": ([], C.c_char_p),
 }
@@ -234,6 +235,10 @@ class Hydro:
         rc = self.lib.fx_hydro_init(C.byref(self._cp))
         if rc:
             raise FxError("fx_hydro_init", rc=rc)
+
+    def set_kernel_path(self, path: int) -> None:
+        """0 = fused column-march kernel (default), 1 = three-kernel path."""
+        self.lib.fx_hydro_set_kernel_path(int(path))
 
     def set_timing(self, on: bool) -> None:
         self.lib.fx_hydro_set_timing(int(on))

@@ -143,6 +143,9 @@ void fx_hydro_set_async(int async_);
  * call's durations in ms (states, fluxes, update) and returns the count. */
 void fx_hydro_set_timing(int on);
 int fx_hydro_stage_times(float* ms, int n);
+/* Kernel path: 0 = fused column-march kernel where the block shape allows
+ * (default), 1 = three-kernel path (states, fluxes, update). */
+void fx_hydro_set_kernel_path(int path);
 /* Library version string. */
 const char* fx_hydro_version(void);
 

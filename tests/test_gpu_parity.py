@@ -42,7 +42,7 @@ def _blocks(hp: HydroParams, nblocks: int, seed: int):
     return U2
 
 
-def _run_both(hp: HydroParams, U: np.ndarray, dt: float, dx: float):
+def _run_both(hp: HydroParams, U: np.ndarray, dt: float, dx: float, path: int = 0):
     deltas = (dx, dx, dx)
     po = _oracle_params(hp)
     Uo = U.copy()
@@ -50,8 +50,12 @@ def _run_both(hp: HydroParams, U: np.ndarray, dt: float, dx: float):
         rc, _, _, st = O.compute_fluxes(po, Uo[b], dt, deltas, update=True, want_flux=False)
         assert rc == 0
     hy = Hydro(hp)
-    Ug = torch.from_numpy(U.copy()).cuda()
-    st = hy.advance_batch(Ug, dt, deltas)
+    hy.set_kernel_path(path)
+    try:
+        Ug = torch.from_numpy(U.copy()).cuda()
+        st = hy.advance_batch(Ug, dt, deltas)
+    finally:
+        hy.set_kernel_path(0)
     assert st.code == 0
     return Uo, Ug.cpu().numpy()
 
@@ -66,14 +70,16 @@ def _check(hp, Uo, Ug, bitwise):
     return norms
 
 
-@pytest.mark.parametrize("ndim,n", [(3, 16), (3, 8), (2, 16), (2, 8), (1, 16)])
-def test_fp64_block_update_matches_oracle(ndim, n):
+@pytest.mark.parametrize("path", [0, 1], ids=["fused", "three-kernel"])
+@pytest.mark.parametrize("ndim,n", [(3, 16), (3, 8), (3, 32), (2, 16), (2, 8), (1, 16)])
+def test_fp64_block_update_matches_oracle(ndim, n, path):
     gamma = 5.0 / 3.0
     hp = HydroParams(ndim=ndim, nxb=n, nyb=n, nzb=n, gamma=gamma)
-    U = _blocks(hp, 6 if ndim == 3 else 12, seed=W.SEED_BASE + 2)
+    nb = {3: 6, 2: 12, 1: 12}[ndim] if n < 32 else 2
+    U = _blocks(hp, nb, seed=W.SEED_BASE + 2)
     dx = 1.0 / n
     dt = W.cfl_dt(torch.from_numpy(U), ndim, 4, (dx,) * 3, 0.8)
-    Uo, Ug = _run_both(hp, U, dt, dx)
+    Uo, Ug = _run_both(hp, U, dt, dx, path)
     _check(hp, Uo, Ug, bitwise=True)
 
 
