@@ -166,3 +166,70 @@ def compute_fluxes_batch(p: Params, U: np.ndarray, dt: float, dx, nthreads: int 
 
 def block_dt(p: Params, U: np.ndarray, dx, cfl: float) -> float:
     return lib().ox_block_dt(C.byref(p), _p(U), (C.c_double * 3)(*dx), cfl)
+
+
+# ---- per-stage entry points (primitive vectors: dens, vx, vy, vz, pres,
+# gamc, game, eint, grav) ----
+def eigen(V, d: int):
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    lam, le, re = np.zeros(5), np.zeros((5, 5)), np.zeros((5, 5))
+    rc = lib().ox_eigen(_p(V), d, _p(lam), _p(le), _p(re))
+    return rc, lam, le, re          # le[wave][var], re[wave][var]
+
+
+def limiter(kind: int, a: float, b: float, beta: float = 1.0) -> float:
+    return lib().ox_limiter(kind, a, b, beta)
+
+
+def tvd_slope(p: Params, VL, V0, VR, d: int):
+    rc, lam, le, re = eigen(V0, d)
+    out = np.zeros(9)
+    lib().ox_tvd_slope(C.byref(p), _p(np.ascontiguousarray(VL, dtype=np.float64)),
+                       _p(np.ascontiguousarray(V0, dtype=np.float64)),
+                       _p(np.ascontiguousarray(VR, dtype=np.float64)), _p(lam), _p(le), _p(out))
+    return out
+
+
+def trans_flux(p: Params, vm1, vc0, vp1, V0, d: int):
+    """sigma of a cell with eigen system of V0 (hy_upwindTransverseFlux)."""
+    rc, lam, le, re = eigen(V0, d)
+    sig = np.zeros(9)
+    f = lambda a: _p(np.ascontiguousarray(a, dtype=np.float64))
+    lib().ox_upwind_trans_flux(C.byref(p), f(vm1), f(vc0), f(vp1), _p(lam), _p(le), _p(re), _p(sig))
+    return sig
+
+
+def ppm(p: Params, d: int, dt: float, delta: float, data, flat: float = 0.0, trans: bool = False):
+    data = np.ascontiguousarray(data, dtype=np.float64).reshape(5, 9)
+    lam, le, re = np.zeros(5), np.zeros((5, 5)), np.zeros((5, 5))
+    Wp, Wm, sig = np.zeros(9), np.zeros(9), np.zeros(9)
+    rc = lib().ox_ppm(C.byref(p), d, dt, delta, _p(data), flat, int(trans), _p(lam), _p(le), _p(re),
+                      _p(Wp), _p(Wm), _p(sig))
+    return rc, Wp, Wm, sig
+
+
+def hllc(p: Params, d: int, VL, VR):
+    F, sp = np.zeros(6), C.c_double(0.0)
+    f = lambda a: _p(np.ascontiguousarray(a, dtype=np.float64))
+    rc = lib().ox_hllc(C.byref(p), d, f(VL), f(VR), _p(F), C.byref(sp))
+    return rc, F, sp.value
+
+
+def exact_flux(p: Params, d: int, VL, VR):
+    F, sp = np.zeros(6), C.c_double(0.0)
+    f = lambda a: _p(np.ascontiguousarray(a, dtype=np.float64))
+    st = Stats()
+    rc = lib().ox_exact_riemann(C.byref(p), d, f(VL), f(VR), _p(F), C.byref(sp), C.byref(st))
+    return rc, F, st
+
+
+def exact_pstar(gamma, rl, ul, pl, rr, ur, pr, guess=-1.0, tol=1e-12, maxit=50):
+    ps, us = C.c_double(0.0), C.c_double(0.0)
+    it = lib().ox_exact_pstar(gamma, rl, ul, pl, rr, ur, pr, guess, tol, maxit,
+                              C.byref(ps), C.byref(us))
+    return it, ps.value, us.value
+
+
+def prim(rho, u, v, w, p, gamma):
+    """Primitive reconstruction vector with gamc = game = gamma, eint from the EOS."""
+    return np.array([rho, u, v, w, p, gamma, gamma, p / ((gamma - 1.0) * rho), 0.0])

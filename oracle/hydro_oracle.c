@@ -546,7 +546,7 @@ int ox_exact_pstar(double g, double rl, double ul, double pl, double rr,
   exact_fk(p, rr, pr, cr, g, &fr, &frd);
   *pstar = p;
   *ustar = 0.5 * (ul + ur) + 0.5 * (fr - fl);        /* Toro eq. 4.9 */
-  return ok ? it : -it;
+  return ok ? it : -maxit;
 }
 
 /* sample the exact solution at S = x/t = 0 (Toro sec. 4.5) */

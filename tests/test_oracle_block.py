@@ -1,0 +1,386 @@
+"""Pinning tests for the oracle's block update (CPU only).
+
+The FLASH-specific assembly (hy_getRiemannState.F90, hy_dataReconstOneStep.F90,
+hy_getFaceFlux.F90, hy_unsplitUpdate.F90, hy_energyFix.F90) has no in-repo
+reference vectors and the reference cannot be built here, so it is pinned by
+properties that a wrong index, sign or dropped term breaks:
+
+* a uniform state stays uniform and its fluxes are the physical flux;
+* conservation: interior sums of mass, momentum and total energy change only
+  by the boundary fluxes;
+* mirror symmetry and axis-rotation symmetry (2-D swap, 3-D cycle);
+* a 3-D (2-D) block whose data vary along one axis reproduces the 1-D block;
+* in 1-D, the block fluxes equal the composition of the pinned stage routines
+  with Colella & Woodward's (1984) flattening (eqs. 4.1-4.5 as FLASH states
+  them, hy_getRiemannState.F90:356-454) and the aux fluxes' upwinding
+  (hy_getFaceFlux.F90:433-442); pavg is the mean of the cell's face pressures;
+* after the update the EOS relations hold and the energy switch picks the
+  total-energy or the auxiliary internal energy (hy_unsplitUpdate.F90:902-915);
+* smooth entropy-wave advection along the diagonal converges at second order
+  or better in 2-D and 3-D at CFL 0.8, which needs every CTU corner term.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+G = 1.4
+RNG = np.random.default_rng(4242)
+
+
+def make_block(p, rho, vel, pres):
+    nunk, nz, ny, nx = O.block_shape(p)
+    U = np.zeros((nunk, nz, ny, nx))
+    U[O.DENS] = rho
+    U[O.PRES] = pres
+    U[O.VELX], U[O.VELY], U[O.VELZ] = vel
+    O.eos(p, U, 1)
+    return U
+
+
+def coords(p):
+    _, nz, ny, nx = O.block_shape(p)
+    ng = p.ng
+    x = (np.arange(nx) - ng + 0.5) / p.nxb
+    y = (np.arange(ny) - ng + 0.5) / p.nyb if p.ndim > 1 else np.zeros(1)
+    z = (np.arange(nz) - ng + 0.5) / p.nzb if p.ndim > 2 else np.zeros(1)
+    return np.meshgrid(z, y, x, indexing="ij")[::-1]          # X, Y, Z with shape (nz,ny,nx)
+
+
+def smooth_block(p, seed, amp=0.2):
+    r = np.random.default_rng(seed)
+    X, Y, Z = coords(p)
+    def field():
+        k = r.integers(1, 3, 3)
+        ph = r.uniform(0, 2 * np.pi, 3)
+        return np.sin(2 * np.pi * k[0] * X + ph[0]) * np.cos(2 * np.pi * k[1] * Y + ph[1]) * \
+            np.sin(2 * np.pi * k[2] * Z + ph[2] + 0.3)
+    rho = 1.0 + amp * field()
+    pres = 1.0 + amp * field()
+    vel = [0.3 * field() for _ in range(3)]
+    return make_block(p, rho, vel, pres)
+
+
+def interior(p, A):
+    g = p.ng
+    sl = [slice(None)] * A.ndim
+    sl[-1] = slice(g, g + p.nxb)
+    if p.ndim > 1:
+        sl[-2] = slice(g, g + p.nyb)
+    if p.ndim > 2:
+        sl[-3] = slice(g, g + p.nzb)
+    return A[tuple(sl)]
+
+
+def step(p, U, dt, dx, **kw):
+    U = U.copy()
+    rc, fl, pavg, st = O.compute_fluxes(p, U, dt, (dx,) * 3, update=True, **kw)
+    assert rc == 0
+    return U, fl, pavg, st
+
+
+def cfl_dt(p, U, dx, cfl=0.8):
+    return O.block_dt(p, U, (dx,) * 3, cfl)
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_uniform_state_is_preserved(ndim):
+    p = O.params(ndim, 8, G)
+    shape = O.block_shape(p)[1:]
+    U = make_block(p, np.full(shape, 1.3), [np.full(shape, v) for v in (0.4, -0.7, 0.2)],
+                   np.full(shape, 2.2))
+    U1, fl, _, st = step(p, U, 0.01, 0.125)
+    for v in (O.DENS, O.VELX, O.VELY, O.VELZ, O.PRES, O.ENER, O.EINT):
+        np.testing.assert_allclose(interior(p, U1[v]), interior(p, U[v]), rtol=2e-15)
+    V = O.prim(1.3, 0.4, -0.7, 0.2, 2.2, G)
+    for d in range(ndim):
+        F = np.array([fl[d][q].ravel()[0] for q in range(6)])
+        E = 0.5 * 1.3 * (0.4 ** 2 + 0.7 ** 2 + 0.2 ** 2) + 2.2 / (G - 1)
+        un = V[1 + d]
+        want = [1.3 * un, 1.3 * un * 0.4, 1.3 * un * -0.7, 1.3 * un * 0.2, (E + 2.2) * un, 2.2]
+        want[1 + d] += 2.2
+        np.testing.assert_allclose(F, want, rtol=1e-14, atol=1e-15)
+        assert np.ptp(fl[d][:6], axis=(1, 2, 3)).max() < 1e-14
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_conservation_by_boundary_fluxes(ndim):
+    n = 8
+    p = O.params(ndim, n, G)
+    dx = 1.0 / n
+    for seed in range(3):
+        U = smooth_block(p, seed)
+        dt = cfl_dt(p, U, dx)
+        U1, fl, _, _ = step(p, U, dt, dx)
+        rho0, rho1 = interior(p, U[O.DENS]), interior(p, U1[O.DENS])
+        cons0 = [rho0] + [rho0 * interior(p, U[v]) for v in (O.VELX, O.VELY, O.VELZ)] + \
+            [rho0 * interior(p, U[O.ENER])]
+        cons1 = [rho1] + [rho1 * interior(p, U1[v]) for v in (O.VELX, O.VELY, O.VELZ)] + \
+            [rho1 * interior(p, U1[O.ENER])]
+        for q in range(5):
+            dsum = cons1[q].sum() - cons0[q].sum()
+            bnd = 0.0
+            for d in range(ndim):
+                F = fl[d][q]
+                ax = 2 - d
+                hi = np.take(F, -1, axis=ax).sum()
+                lo = np.take(F, 0, axis=ax).sum()
+                bnd -= dt / dx * (hi - lo)
+            scale = np.abs(cons0[q]).sum()
+            assert abs(dsum - bnd) <= 1e-13 * scale, (q, dsum, bnd)
+
+
+def _mirror_x(p, U):
+    g, n = p.ng, p.nxb
+    V = U[..., ::-1].copy()
+    V[O.VELX] *= -1
+    return V
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_mirror_symmetry(ndim):
+    n = 8
+    p = O.params(ndim, n, G)
+    dx = 1.0 / n
+    U = smooth_block(p, 11)
+    dt = cfl_dt(p, U, dx)
+    Ua, _, _, _ = step(p, U, dt, dx)
+    Ub, _, _, _ = step(p, _mirror_x(p, U), dt, dx)
+    Ub = _mirror_x(p, Ub)
+    for v in (O.DENS, O.VELX, O.VELY, O.VELZ, O.PRES, O.EINT):
+        a, b = interior(p, Ua[v]), interior(p, Ub[v])
+        np.testing.assert_allclose(b, a, atol=1e-13 * np.abs(a).max())
+
+
+def _rotate(p, U, inverse=False):
+    """3-D: axes cycle x' = y, y' = z, z' = x; 2-D: swap x and y."""
+    V = np.empty_like(U)
+    if p.ndim == 3:
+        ax = (1, 2, 0) if inverse else (2, 0, 1)
+        perm = {O.VELY: O.VELX, O.VELZ: O.VELY, O.VELX: O.VELZ}     # source -> destination
+        if inverse:
+            perm = {b: a for a, b in perm.items()}
+    else:
+        ax = (0, 2, 1)
+        perm = {O.VELX: O.VELY, O.VELY: O.VELX, O.VELZ: O.VELZ}
+    for v in range(U.shape[0]):
+        V[perm.get(v, v)] = np.transpose(U[v], ax)
+    return V
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_rotation_symmetry(ndim):
+    n = 8
+    p = O.params(ndim, n, G)
+    dx = 1.0 / n
+    U = smooth_block(p, 5)
+    dt = cfl_dt(p, U, dx)
+    Ua, _, _, _ = step(p, U, dt, dx)
+    Ub, _, _, _ = step(p, _rotate(p, U), dt, dx)
+    Ub = _rotate(p, Ub, inverse=True)
+    for v in (O.DENS, O.VELX, O.VELY, O.VELZ, O.PRES, O.EINT):
+        a, b = interior(p, Ua[v]), interior(p, Ub[v])
+        np.testing.assert_allclose(b, a, atol=1e-13 * np.abs(a).max())
+
+
+@pytest.mark.parametrize("ndim,axis", [(2, 0), (2, 1), (3, 0), (3, 1), (3, 2)])
+def test_one_dimensional_data_reduces_to_1d_block(ndim, axis):
+    n = 16
+    p1 = O.params(1, n, G)
+    pn = O.params(ndim, n, G)
+    dx = 1.0 / n
+    U1 = smooth_block(p1, 3, amp=0.4)                  # (nunk, 1, 1, nx)
+    line = U1[:, 0, 0, :]
+    _, nz, ny, nx = O.block_shape(pn)
+    Un = np.empty(O.block_shape(pn))
+    vel_axis = [O.VELX, O.VELY, O.VELZ]
+    for v in range(line.shape[0]):
+        src = v
+        if v in vel_axis:                              # 1-D x-velocity becomes the normal one
+            src = vel_axis[(vel_axis.index(v) - axis) % 3]
+        shp = [1, 1, 1]
+        shp[2 - axis] = nx
+        Un[v] = np.broadcast_to(line[src].reshape(shp), (nz, ny, nx))
+    dt = cfl_dt(p1, U1, dx)
+    r1, _, _, _ = step(p1, U1, dt, dx)
+    rn, _, _, _ = step(pn, Un, dt, dx)
+    g = pn.ng
+    for v in (O.DENS, O.VELX, O.VELY, O.VELZ, O.PRES, O.ENER, O.EINT):
+        src = v
+        if v in vel_axis:
+            src = vel_axis[(vel_axis.index(v) - axis) % 3]
+        a = r1[src, 0, 0, g:g + n]
+        idx = [g + 3] * 3
+        idx[2 - axis] = slice(g, g + n)
+        if ndim == 2:
+            idx[0] = 0
+        b = rn[v][tuple(idx)]
+        np.testing.assert_allclose(b, a, rtol=1e-14, atol=1e-15, err_msg=str(v))
+
+
+def _flat_tilde(pres, vel, i):
+    dp1 = pres[i + 1] - pres[i - 1]
+    dp2 = pres[i + 2] - pres[i - 2]
+    ft = min(1.0, max(0.0, 10.0 * (dp1 / dp2 - 0.75))) if abs(dp2) > 1e-15 else 0.0
+    if abs(dp1) / min(pres[i - 1], pres[i + 1]) < 1.0 / 3.0 or vel[i + 1] - vel[i - 1] > 0:
+        ft = 0.0
+    return ft
+
+
+def test_1d_block_is_the_composition_of_its_stages():
+    """Shock-tube-like data (so flattening is active) through the block
+    routine and through ox_ppm + ox_hllc per face."""
+    n = 16
+    p = O.params(1, n, G)
+    g = p.ng
+    nx = n + 2 * g
+    xc = np.arange(nx) - g + 0.5
+    rho = np.where(xc < 7, 1.0, 0.25) + 0.02 * np.sin(xc)
+    pres = np.where(xc < 7, 1.0, 0.15) * (1 + 0.01 * np.cos(xc))
+    velx = np.where(xc < 7, -0.05, -0.6)
+    U = make_block(p, rho[None, None, :], [velx[None, None, :], 0.1 + 0 * rho[None, None, :],
+                                           -0.2 + 0 * rho[None, None, :]], pres[None, None, :])
+    dx = 1.0 / n
+    dt = cfl_dt(p, U, dx)
+    rc, fl, pavg, _ = O.compute_fluxes(p, U.copy(), dt, (dx,) * 3, update=False, want_pavg=True)
+    assert rc == 0
+    V = [np.array([U[O.DENS, 0, 0, i], U[O.VELX, 0, 0, i], U[O.VELY, 0, 0, i], U[O.VELZ, 0, 0, i],
+                   U[O.PRES, 0, 0, i], U[O.GAMC, 0, 0, i], U[O.GAME, 0, 0, i], U[O.EINT, 0, 0, i], 0.0])
+         for i in range(nx)]
+    P, Vx = U[O.PRES, 0, 0], U[O.VELX, 0, 0]
+    ft = {i: _flat_tilde(P, Vx, i) for i in range(g - 2, g + n + 2)}
+    assert max(ft.values()) > 0, "data must switch flattening on"
+    Wp, Wm = {}, {}
+    for i in range(g - 1, g + n + 1):
+        dp1 = P[i + 1] - P[i - 1]
+        fc = ft[i] if dp1 == 0 else max(ft[i], ft[i + 1] if dp1 < 0 else ft[i - 1])
+        rc, wp, wm, _ = O.ppm(p, 0, dt, dx, np.stack(V[i - 2:i + 3]), flat=fc)
+        assert rc == 0 and min(wp[0], wm[0], wp[4], wm[4]) > 0
+        Wp[i], Wm[i] = wp, wm
+    for f in range(n + 1):
+        L, R = Wp[g + f - 1], Wm[g + f]
+        rc, F, _ = O.hllc(p, 0, L, R)
+        up = L if F[0] > 0 else R
+        want = np.concatenate([F, [F[0] * up[7], F[0] / up[0]]])
+        np.testing.assert_array_equal(fl[0][:, 0, 0, f], want)
+    for i in range(n):
+        assert pavg[0, 0, i] == 0.5 * (Wm[g + i][4] + Wp[g + i][4])
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_eos_relations_and_energy_switch_after_update(ndim):
+    n = 8
+    dx = 1.0 / n
+    p0 = O.params(ndim, n, G, eintSwitch=0.0)
+    U = smooth_block(p0, 21)
+    dt = cfl_dt(p0, U, dx)
+    Ua, _, _, _ = step(p0, U, dt, dx)
+    rho, e = interior(p0, Ua[O.DENS]), interior(p0, Ua[O.EINT])
+    ke = 0.5 * sum(interior(p0, Ua[v]) ** 2 for v in (O.VELX, O.VELY, O.VELZ))
+    np.testing.assert_allclose(interior(p0, Ua[O.ENER]), e + ke, rtol=1e-14)
+    np.testing.assert_allclose(interior(p0, Ua[O.PRES]), (G - 1) * rho * e, rtol=1e-14)
+    np.testing.assert_allclose(interior(p0, Ua[O.TEMP]), (G - 1) * e / p0.gasConstant, rtol=1e-14)
+    # always-aux switch: internal energy from the aux equation; it must follow
+    # the step's change of the total-energy one to truncation error, and
+    # converge to it with resolution (a dropped or mis-signed p dV work term
+    # leaves an error of the size of the change itself)
+    p1 = O.params(ndim, n, G, eintSwitch=1e30)
+    Ub, _, _, _ = step(p1, U, dt, dx)
+    eb = interior(p1, Ub[O.EINT])
+    change = np.abs(e - interior(p0, U[O.EINT])).mean()
+    gap8 = np.abs(eb - e).mean()
+    assert 0 < gap8 < 0.25 * change, (gap8, change)
+    np.testing.assert_allclose(interior(p1, Ub[O.DENS]), rho, rtol=1e-14)
+    q0, q1 = O.params(ndim, 2 * n, G), O.params(ndim, 2 * n, G, eintSwitch=1e30)
+    U2 = smooth_block(q0, 21)
+    dt2 = cfl_dt(q0, U2, dx / 2)
+    gap16 = np.abs(interior(q1, step(q1, U2, dt2, dx / 2)[0][O.EINT]) -
+                   interior(q0, step(q0, U2, dt2, dx / 2)[0][O.EINT])).mean()
+    assert gap8 / gap16 > 2.5, (gap8, gap16)
+
+
+def _periodic_fill(p, U):
+    g = p.ng
+    for d in range(p.ndim):
+        ax = 3 - d
+        n = U.shape[ax] - 2 * g
+        lo = [slice(None)] * 4
+        hi = [slice(None)] * 4
+        src_lo = [slice(None)] * 4
+        src_hi = [slice(None)] * 4
+        lo[ax], src_lo[ax] = slice(0, g), slice(n, n + g)
+        hi[ax], src_hi[ax] = slice(n + g, n + 2 * g), slice(g, 2 * g)
+        U[tuple(lo)] = U[tuple(src_lo)]
+        U[tuple(hi)] = U[tuple(src_hi)]
+
+
+def _advect(ndim, n, cfl=0.8, periods=1):
+    """Entropy wave rho = 1 + 0.2 sin(2 pi (x + y + z)) advected by u = (1,1,1)
+    at constant pressure on a periodic block; returns the mean density error
+    after each full period and the largest pressure deviation."""
+    p = O.params(ndim, n, G)
+    X, Y, Z = coords(p)
+    s = X + (Y if ndim > 1 else 0) + (Z if ndim > 2 else 0)
+    rho = 1.0 + 0.2 * np.sin(2 * np.pi * s)
+    vel = [np.ones_like(rho) if d < ndim else np.zeros_like(rho) for d in range(3)]
+    U = make_block(p, rho, vel, np.ones_like(rho))
+    U0 = U.copy()
+    dx = 1.0 / n
+    nsteps = int(np.ceil(1.0 / (cfl * dx / (1.0 + np.sqrt(G)))))
+    dt = 1.0 / nsteps
+    errs = []
+    for _ in range(periods):
+        for _ in range(nsteps):
+            _periodic_fill(p, U)
+            rc, _, _, _ = O.compute_fluxes(p, U, dt, (dx,) * 3, update=True, want_flux=False)
+            assert rc == 0
+        errs.append(np.abs(interior(p, U[O.DENS]) - interior(p, U0[O.DENS])).mean())
+    return errs, np.abs(interior(p, U[O.PRES]) - 1.0).max()
+
+
+@pytest.mark.parametrize("ndim,sizes,order", [(2, (32, 64), 1.5), (3, (16, 32), 1.7)])
+def test_diagonal_advection_converges(ndim, sizes, order):
+    (e1,), p1 = _advect(ndim, sizes[0])
+    (e2,), p2 = _advect(ndim, sizes[1])
+    rate = np.log2(e1 / e2)
+    assert rate >= order, (e1, e2, rate)
+    assert max(p1, p2) < 1e-12            # pressure and velocity stay uniform
+
+
+def test_3d_full_ctu_is_stable_near_cfl_one():
+    """Without the corner terms the unsplit 3-D update is only stable to
+    CFL 1/2 (hy_getRiemannState.F90:955); with them it must stay bounded."""
+    errs, pdev = _advect(3, 16, cfl=0.95, periods=2)
+    assert errs[1] < 0.2 and pdev < 1e-12, errs
+
+
+@pytest.mark.parametrize("vel", [(0.3, 0.2, -0.25), (-0.2, -0.3, 0.15)])
+@pytest.mark.parametrize("face", [0, 1, 2])
+def test_transverse_and_corner_terms_average_the_characteristic(face, vel):
+    """Entropy wave rho = 1 + eps * s1 * s2 (s1, s2 the two coordinates
+    transverse to the faces, in cell units) at constant velocity and pressure.
+    The upwinded differences are exact for bilinear data, so each face state
+    must be the exact time average of rho along the characteristic through the
+    face: s1 s2 - (u2 s1 + u1 s2) dt/2 + u1 u2 dt^2/3 (velocities in cells per
+    unit time).  The first-order terms are the transverse corrections
+    (hy_dataReconstOneStep.F90:920-951), the dt^2/3 term the two corner terms
+    (hy_getRiemannState.F90:952-1327, dt^2/6 each)."""
+    n, eps = 8, 1e-3
+    p = O.params(3, n, G)
+    dx = 1.0 / n
+    X, Y, Z = coords(p)
+    C = [X * n, Y * n, Z * n]                         # cell units
+    t1, t2 = (face + 1) % 3, (face + 2) % 3
+    rho = 1.0 + eps * C[t1] * C[t2]
+    U = make_block(p, rho, [np.full_like(rho, v) for v in vel], np.ones_like(rho))
+    dt = 0.5 * cfl_dt(p, U, dx)
+    rc, fl, _, _ = O.compute_fluxes(p, U.copy(), dt, (dx,) * 3, update=False)
+    assert rc == 0
+    u1, u2 = vel[t1] * dt / dx, vel[t2] * dt / dx       # displacement in cells
+    # face-centre transverse coordinates, cell units, on the face array's grid
+    shp = fl[face].shape[1:]
+    grid = np.meshgrid(*[np.arange(m) + 0.5 for m in shp], indexing="ij")   # (k, j, i)
+    s1, s2 = grid[2 - t1], grid[2 - t2]
+    s_avg = s1 * s2 - (u2 * s1 + u1 * s2) / 2 + u1 * u2 / 3
+    want = vel[face] * (1.0 + eps * s_avg)
+    np.testing.assert_allclose(fl[face][O.F_DENS], want, rtol=0, atol=2e-15)
