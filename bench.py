@@ -102,6 +102,8 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--blocks", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--riemann", choices=["hllc", "exact"], default="hllc",
+                    help="face flux solver (HLLC, or the exact iterative solver)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run plain 1 GiB copies (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -118,7 +120,7 @@ def main() -> None:
     dev = torch.device("cuda:0")
     n, g, gamma = 16, 4, 5.0 / 3.0
     dx = 1.0 / n
-    hp = HydroParams(ndim=3, nxb=n, nyb=n, nzb=n, gamma=gamma)
+    hp = HydroParams(ndim=3, nxb=n, nyb=n, nzb=n, gamma=gamma, RiemannSolver=args.riemann)
     U = W.fourier_blocks(args.blocks, n=n, ng=g, gamma=gamma, device=dev)
     dt = W.cfl_dt(U, 3, g, (dx,) * 3, 0.8)
     hy = Hydro(hp)
@@ -155,7 +157,7 @@ def main() -> None:
     bmz = march_bytes_per_zone(3, n, g)
     achieved = bmz * zones / (march_ms * 1e-3) / 1e9
     fallbacks = int(hy.last_status.fallbacks)
-    workload = f"{args.blocks}x16^3+4g fp64 hllc"
+    workload = f"{args.blocks}x16^3+4g fp64 {args.riemann}"
     if args.copy_probe:
         copy_probe()
 
@@ -182,9 +184,11 @@ def main() -> None:
         "data": "synthetic",
         "config": {"workload": f"{args.blocks} independent 3-D blocks, 16^3 interior + 4 guard "
                                "cells, gamma=5/3, seeded Fourier-mode fields, one unsplit "
-                               "PPM+HLLC step (plain FP64 path)",
+                               f"PPM+{args.riemann.upper()} step (plain FP64 path)",
                    "blocks": args.blocks, "block": [16, 16, 16], "nguard": 4,
-                   "precision": "fp64", "riemann": "HLLC", "dt": dt},
+                   "precision": "fp64", "riemann": args.riemann, "dt": dt,
+                   "newton_iters_per_face": (int(hy.last_status.newton_iters) /
+                                             (3 * args.blocks * 17 * 16 * 16))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic("k_march", workload),

@@ -404,7 +404,11 @@ __device__ __forceinline__ int hllc(const DevParams& P, const double* Vm,
     return 0;
   }
   const bool left = (SL < 0. && qStar >= 0.);
-  const double* V = left ? Vm : Vp;
+  // element-wise select keeps both inputs in registers (a pointer select
+  // between two private arrays would put them in scratch memory)
+  double V[8];
+#pragma unroll
+  for (int v = 0; v < 8; ++v) V[v] = left ? Vm[v] : Vp[v];
   const double S = left ? SL : SR;
   const double velN = left ? velNL : velNR;
   const double tp = left ? tpL : tpR;
@@ -429,6 +433,159 @@ __device__ __forceinline__ int hllc(const DevParams& P, const double* Vm,
 #pragma unroll
   for (int f = 0; f < 5; ++f) F[f] = Fs[f] + S * (Us[f] - Uc[f]);
   F[5] = Fs[5];
+  return 0;
+}
+
+// ---- exact Riemann solver for the gamma-law gas (no counterpart in the
+// reference; it takes hy_HLLC.F90's contract).  Newton iteration on the
+// exact pressure function (Toro, Riemann Solvers and Numerical Methods, 3rd
+// ed., eqs. 4.6-4.9, derivative 4.37), seeded by a caller guess or by the
+// two-shock guess (Toro eq. 9.42), sampled at x/t = 0 (Toro sec. 4.5).  The
+// operation order follows the CPU oracle's restatement.
+__device__ __forceinline__ void exact_fk(double p, double rk, double pk, double ck, double g,
+                                         double& f, double& fd) {
+  if (p <= pk) {
+    const double pratio = p / pk;
+    f = 2. / (g - 1.) * ck * (pow(pratio, (g - 1.) / (2. * g)) - 1.);
+    fd = (1. / (rk * ck)) * pow(pratio, -(g + 1.) / (2. * g));
+  } else {
+    const double ak = 2. / ((g + 1.) * rk);
+    const double bk = (g - 1.) / (g + 1.) * pk;
+    const double qrt = sqrt(ak / (bk + p));
+    f = (p - pk) * qrt;
+    fd = (1. - 0.5 * (p - pk) / (bk + p)) * qrt;
+  }
+}
+
+__device__ __forceinline__ double two_shock_guess(double g, double rl, double ul, double pl,
+                                                  double cl, double rr, double ur, double pr,
+                                                  double cr) {
+  const double ppv = 0.5 * (pl + pr) - 0.125 * (ur - ul) * (rl + rr) * (cl + cr);
+  const double p0 = fmax(1.e-30, ppv);
+  const double gl = sqrt((2. / ((g + 1.) * rl)) / ((g - 1.) / (g + 1.) * pl + p0));
+  const double gr = sqrt((2. / ((g + 1.) * rr)) / ((g - 1.) / (g + 1.) * pr + p0));
+  const double pts = (gl * pl + gr * pr - (ur - ul)) / (gl + gr);
+  return fmax(1.e-30, pts);
+}
+
+// Newton from p0; returns iterations (negative when maxit was reached)
+__device__ __forceinline__ int exact_newton(double g, double rl, double pl, double cl, double rr,
+                                            double pr, double cr, double du, double p0, double tol,
+                                            int maxit, double& pstar) {
+  double p = p0;
+  for (int it = 1; it <= maxit; ++it) {
+    double fl, fld, fr, frd;
+    exact_fk(p, rl, pl, cl, g, fl, fld);
+    exact_fk(p, rr, pr, cr, g, fr, frd);
+    double pn = p - (fl + fr + du) / (fld + frd);
+    if (!(pn > 0.)) pn = 1.e-30;
+    const double change = 2. * fabs((pn - p) / (pn + p));
+    p = pn;
+    if (change <= tol) {
+      pstar = p;
+      return it;
+    }
+  }
+  pstar = p;
+  return -maxit;
+}
+
+struct ExactStats {
+  unsigned long long iters, fallbacks;
+};
+
+// state at x/t = 0 (Toro sec. 4.5); leftside tells which input supplies the
+// tangential velocities and gammas
+__device__ __forceinline__ void exact_sample(double g, double rl, double ul, double pl, double cl,
+                                             double rr, double ur, double pr, double cr, double pm,
+                                             double um, double& r, double& u, double& pp,
+                                             bool& leftside) {
+  if (0. <= um) {
+    leftside = true;
+    if (pm <= pl) {
+      const double shl = ul - cl;
+      if (0. <= shl) { r = rl; u = ul; pp = pl; return; }
+      const double cml = cl * pow(pm / pl, (g - 1.) / (2. * g));
+      const double stl = um - cml;
+      if (0. > stl) { r = rl * pow(pm / pl, 1. / g); u = um; pp = pm; return; }
+      const double c = 2. / (g + 1.) * (cl + (g - 1.) / 2. * (ul - 0.));
+      u = 2. / (g + 1.) * (cl + (g - 1.) / 2. * ul + 0.);
+      r = rl * pow(c / cl, 2. / (g - 1.));
+      pp = pl * pow(c / cl, 2. * g / (g - 1.));
+      return;
+    }
+    const double pml = pm / pl;
+    const double sl = ul - cl * sqrt((g + 1.) / (2. * g) * pml + (g - 1.) / (2. * g));
+    if (0. <= sl) { r = rl; u = ul; pp = pl; return; }
+    r = rl * (pml + (g - 1.) / (g + 1.)) / (pml * (g - 1.) / (g + 1.) + 1.);
+    u = um; pp = pm;
+    return;
+  }
+  leftside = false;
+  if (pm > pr) {
+    const double pmr = pm / pr;
+    const double sr = ur + cr * sqrt((g + 1.) / (2. * g) * pmr + (g - 1.) / (2. * g));
+    if (0. >= sr) { r = rr; u = ur; pp = pr; return; }
+    r = rr * (pmr + (g - 1.) / (g + 1.)) / (pmr * (g - 1.) / (g + 1.) + 1.);
+    u = um; pp = pm;
+    return;
+  }
+  const double shr = ur + cr;
+  if (0. >= shr) { r = rr; u = ur; pp = pr; return; }
+  const double cmr = cr * pow(pm / pr, (g - 1.) / (2. * g));
+  const double str = um + cmr;
+  if (0. <= str) { r = rr * pow(pm / pr, 1. / g); u = um; pp = pm; return; }
+  const double c = 2. / (g + 1.) * (cr - (g - 1.) / 2. * (ur - 0.));
+  u = 2. / (g + 1.) * (-cr + (g - 1.) / 2. * ur + 0.);
+  r = rr * pow(c / cr, 2. / (g - 1.));
+  pp = pr * pow(c / cr, 2. * g / (g - 1.));
+}
+
+// Godunov flux from the exact solution; returns 0 ok, 1 non-positive state,
+// 2 vacuum generated.  guess > 0 seeds Newton; when that start does not
+// converge, Newton restarts from the two-shock guess (counted as fallback).
+template <int DIR>
+__device__ __forceinline__ int exact_flux(const DevParams& P, const double* VL, const double* VR,
+                                          double guess, double F[6], ExactStats& es) {
+  const double g = VL[P_GAMC];
+  if (!(VL[P_DENS] > 0. && VR[P_DENS] > 0. && VL[P_PRES] > 0. && VR[P_PRES] > 0.)) return 1;
+  const double rl = VL[P_DENS], ul = VL[P_VELX + DIR], pl = VL[P_PRES];
+  const double rr = VR[P_DENS], ur = VR[P_VELX + DIR], pr = VR[P_PRES];
+  const double cl = sqrt(g * pl / rl), cr = sqrt(g * pr / rr);
+  if (2. / (g - 1.) * (cl + cr) <= ur - ul) return 2;
+  const double du = ur - ul;
+  double pm;
+  int it;
+  if (guess > 0.) {
+    it = exact_newton(g, rl, pl, cl, rr, pr, cr, du, guess, P.exactTol, P.exactMaxIter, pm);
+    if (it < 0) {
+      es.iters += (unsigned long long)(-it);
+      es.fallbacks += 1;
+      it = exact_newton(g, rl, pl, cl, rr, pr, cr, du,
+                        two_shock_guess(g, rl, ul, pl, cl, rr, ur, pr, cr), P.exactTol,
+                        P.exactMaxIter, pm);
+    }
+  } else {
+    it = exact_newton(g, rl, pl, cl, rr, pr, cr, du,
+                      two_shock_guess(g, rl, ul, pl, cl, rr, ur, pr, cr), P.exactTol,
+                      P.exactMaxIter, pm);
+    if (it < 0) es.fallbacks += 1;
+  }
+  es.iters += (unsigned long long)(it < 0 ? -it : it);
+  double fl, fld, fr, frd;
+  exact_fk(pm, rl, pl, cl, g, fl, fld);
+  exact_fk(pm, rr, pr, cr, g, fr, frd);
+  const double um = 0.5 * (ul + ur) + 0.5 * (fr - fl);
+  double r, u, pp;
+  bool left;
+  exact_sample(g, rl, ul, pl, cl, rr, ur, pr, cr, pm, um, r, u, pp, left);
+  double W[8];
+#pragma unroll
+  for (int v = 0; v < 8; ++v) W[v] = left ? VL[v] : VR[v];
+  W[P_DENS] = r;
+  W[P_VELX + DIR] = u;
+  W[P_PRES] = pp;
+  prim2flx<DIR>(W, P.fP, F);
   return 0;
 }
 

@@ -60,6 +60,11 @@ __device__ void report_error(DevStatus* s, int code, int tile, int i, int j, int
   }
 }
 
+__device__ __forceinline__ void flush_exact(DevStatus* s, const ExactStats& es) {
+  if (es.iters) atomicAdd(&s->newtonIters, es.iters);
+  if (es.fallbacks) atomicAdd(&s->newtonFallbacks, es.fallbacks);
+}
+
 __device__ __forceinline__ long cidx(const DevParams& P, int i, int j, int k) {
   return ((long)k * P.ny + j) * P.nx + i;
 }
@@ -285,10 +290,10 @@ __device__ __forceinline__ long band_index(const DevParams& P, int d, int i, int
   return ((long)cc * P.bext[d][1] + b) * P.bext[d][0] + a;
 }
 
-template <int D>
+template <int D, int RS>
 __device__ __forceinline__ int face_flux(const DevParams& P, const double* VL, const double* VR,
-                                         double F[8]) {
-  const int ierr = hllc<D>(P, VL, VR, F);
+                                         double F[8], ExactStats& es) {
+  const int ierr = RS == FX_EXACT ? exact_flux<D>(P, VL, VR, -1.0, F, es) : hllc<D>(P, VL, VR, F);
   if (ierr) return ierr;
   F[6] = 0.;
   F[7] = 0.;
@@ -304,9 +309,11 @@ __device__ __forceinline__ int face_flux(const DevParams& P, const double* VL, c
   return 0;
 }
 
+template <int RS>
 __global__ void __launch_bounds__(kThreads) k_fluxes(DevParams P, Launch L) {
   const long perTile = P.nface[0] + P.nface[1] + P.nface[2];
   const long total = perTile * L.nTiles;
+  ExactStats es{0ULL, 0ULL};
   for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
        gid += (long)gridDim.x * blockDim.x) {
     const int t = (int)(gid / perTile);
@@ -353,9 +360,9 @@ __global__ void __launch_bounds__(kThreads) k_fluxes(DevParams P, Launch L) {
     VR[P_GAME] = U[P.voff[FX_GAME] + cr];
     double F[8];
     int ierr;
-    if (d == 0) ierr = face_flux<0>(P, VL, VR, F);
-    else if (d == 1) ierr = face_flux<1>(P, VL, VR, F);
-    else ierr = face_flux<2>(P, VL, VR, F);
+    if (d == 0) ierr = face_flux<0, RS>(P, VL, VR, F, es);
+    else if (d == 1) ierr = face_flux<1, RS>(P, VL, VR, F, es);
+    else ierr = face_flux<2, RS>(P, VL, VR, F, es);
     if (ierr) {
       report_error(L.status, 2, t, i, j, k, d);
       continue;
@@ -365,6 +372,7 @@ __global__ void __launch_bounds__(kThreads) k_fluxes(DevParams P, Launch L) {
 #pragma unroll
     for (int q = 0; q < FX_NFLUX; ++q) Fo[q * nf + r] = F[q];
   }
+  flush_exact(L.status, es);
 }
 
 __device__ __forceinline__ long face_index(const DevParams& P, int d, int i, int j, int k) {
@@ -543,6 +551,56 @@ __global__ void k_reset_status(DevStatus* s) {
 // ---------------------------------------------------------------------
 // host state
 // ---------------------------------------------------------------------
+// ---- flux register (fx_fluxreg_put / fx_fluxreg_conserve) ----
+struct RegLaunch {
+  const fx_tile* tiles;
+  const fx_fluxreg_face* faces;
+  int nFaces;
+  double* reg;
+};
+
+__device__ __forceinline__ long reg_face(const DevParams& P, int d, int a, int b, int c) {
+  return ((long)c * P.fext[d][1] + b) * P.fext[d][0] + a;
+}
+
+// reg = sum over the covering fine faces of coef * F, transverse offsets
+// ascending with the higher axis outermost (AMReX fluxreg_fineadd order)
+__global__ void __launch_bounds__(kThreads) k_fluxreg_put(DevParams P, RegLaunch R, double coef) {
+  const long total = (long)R.nFaces * FX_NFLUX;
+  for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+       gid += (long)gridDim.x * blockDim.x) {
+    const int q = (int)(gid / R.nFaces);
+    const int e = (int)(gid - (long)q * R.nFaces);
+    const fx_fluxreg_face f = R.faces[e];
+    const int d = f.dir;
+    const double* F = R.tiles[f.fine_tile].flux[d] + (long)q * P.nface[d];
+    const int inner = d == 0 ? 1 : 0, outer = d == 2 ? 1 : 2;
+    const int nIn = P.ndim > 1 ? 2 : 1, nOut = P.ndim > 2 ? 2 : 1;
+    double acc = 0.0;
+    for (int o = 0; o < nOut; ++o)
+      for (int in = 0; in < nIn; ++in) {
+        int idx[3] = {f.fi, f.fj, f.fk};
+        idx[inner] += in;
+        idx[outer] += o;
+        acc = acc + coef * F[reg_face(P, d, idx[0], idx[1], idx[2])];
+      }
+    R.reg[gid] = acc;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_fluxreg_conserve(DevParams P, RegLaunch R) {
+  const long total = (long)R.nFaces * FX_NFLUX;
+  for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+       gid += (long)gridDim.x * blockDim.x) {
+    const int q = (int)(gid / R.nFaces);
+    const int e = (int)(gid - (long)q * R.nFaces);
+    const fx_fluxreg_face f = R.faces[e];
+    const int d = f.dir;
+    double* F = R.tiles[f.coarse_tile].flux[d] + (long)q * P.nface[d];
+    F[reg_face(P, d, f.ci, f.cj, f.ck)] = R.reg[gid] * 1.0;
+  }
+}
+
 struct Lib {
   bool init = false;
   fx_params prm{};
@@ -557,6 +615,10 @@ struct Lib {
   size_t pavgCap = 0;
   fx_tile* dtiles = nullptr;
   size_t dtilesCap = 0;
+  fx_tile* dregTiles = nullptr;        // tile list of a register call
+  size_t dregTilesCap = 0;
+  fx_fluxreg_face* dfaces = nullptr;   // face list of a register call
+  size_t dfacesCap = 0;
   unsigned char* dinplace = nullptr;
   size_t dinplaceCap = 0;
   DevStatus* status = nullptr;
@@ -696,7 +758,7 @@ int validate(const fx_params* p) {
   for (int v = 0; v < FX_NVAR_USED; ++v)
     if (p->var[v] < 0 || p->var[v] >= p->nunk) return 3;
   if (!(p->gamma > 1.0)) return 3;
-  if (p->riemann != FX_HLLC) return 3;  // other solvers: not in this build yet
+  if (p->riemann != FX_HLLC && p->riemann != FX_EXACT) return 3;  // HLLC or the exact solver
   return 0;
 }
 
@@ -757,6 +819,14 @@ int upload_tiles(const fx_tile* tiles, int nTiles, bool needFlux, fx_status* st)
   return 0;
 }
 
+template <int RS>
+void launch_march(const DevParams& P, const MarchLaunch& M, int TX, dim3 grid) {
+  if (P.ndim == 3 && TX == 16) hipLaunchKernelGGL((k_march<3, 16, 8, 512, RS>), grid, dim3(512), 0, g.stream, P, M);
+  else if (P.ndim == 3) hipLaunchKernelGGL((k_march<3, 8, 8, 256, RS>), grid, dim3(256), 0, g.stream, P, M);
+  else if (TX == 16) hipLaunchKernelGGL((k_march<2, 16, 8, 256, RS>), grid, dim3(256), 0, g.stream, P, M);
+  else hipLaunchKernelGGL((k_march<2, 8, 8, 256, RS>), grid, dim3(256), 0, g.stream, P, M);
+}
+
 int record(int n) {
   if (!g.timing) return 0;
   if (!g.ev[n]) {
@@ -797,10 +867,8 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
     Launch L{g.dtiles, nTiles, nullptr, g.status, dt};
     hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
     if (record(0)) return 4;
-    if (P.ndim == 3 && TX == 16) hipLaunchKernelGGL((k_march<3, 16, 8, 512>), grid, dim3(512), 0, g.stream, P, M);
-    else if (P.ndim == 3) hipLaunchKernelGGL((k_march<3, 8, 8, 256>), grid, dim3(256), 0, g.stream, P, M);
-    else if (TX == 16) hipLaunchKernelGGL((k_march<2, 16, 8, 256>), grid, dim3(256), 0, g.stream, P, M);
-    else hipLaunchKernelGGL((k_march<2, 8, 8, 256>), grid, dim3(256), 0, g.stream, P, M);
+    if (P.riemann == FX_EXACT) launch_march<FX_EXACT>(P, M, TX, grid);
+    else launch_march<FX_HLLC>(P, M, TX, grid);
     if (record(1)) return 4;
     if (record(2)) return 4;
     hipLaunchKernelGGL((k_update<false, false>), dim3(grid_for(P.icells * nTiles)), dim3(kThreads), 0,
@@ -818,8 +886,11 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
     else hipLaunchKernelGGL(k_states<1>, gs, dim3(kThreads), 0, g.stream, P, L);
   }
   if (record(1)) return 4;
-  hipLaunchKernelGGL(k_fluxes, dim3(grid_for((P.nface[0] + P.nface[1] + P.nface[2]) * nTiles)),
-                     dim3(kThreads), 0, g.stream, P, L);
+  {
+    const dim3 gf(grid_for((P.nface[0] + P.nface[1] + P.nface[2]) * nTiles));
+    if (P.riemann == FX_EXACT) hipLaunchKernelGGL(k_fluxes<FX_EXACT>, gf, dim3(kThreads), 0, g.stream, P, L);
+    else hipLaunchKernelGGL(k_fluxes<FX_HLLC>, gf, dim3(kThreads), 0, g.stream, P, L);
+  }
   if (record(2)) return 4;
   (void)anyFluxOnly;
   hipLaunchKernelGGL((k_update<true, false>), dim3(grid_for(P.icells * nTiles)), dim3(kThreads), 0,
@@ -833,6 +904,62 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
       if (hipEventElapsedTime(&g.stageMs[k], g.ev[k], g.ev[k + 1]) != hipSuccess) g.stageMs[k] = -1.f;
   }
   return rc;
+}
+
+// Check every register face against the tile list and flux extents before
+// anything is launched; the kernels index without further checks.
+int check_reg_faces(const fx_tile* tiles, int nTiles, const fx_fluxreg_face* faces, int nFaces,
+                    bool fine) {
+  const DevParams& P = g.dp;
+  for (int e = 0; e < nFaces; ++e) {
+    const fx_fluxreg_face& f = faces[e];
+    const int t = fine ? f.fine_tile : f.coarse_tile;
+    if (t < 0 || t >= nTiles || f.dir < 0 || f.dir >= P.ndim) return 3;
+    if (!tiles[t].flux[f.dir]) return 3;
+    const int* ext = P.fext[f.dir];
+    int idx[3] = {fine ? f.fi : f.ci, fine ? f.fj : f.cj, fine ? f.fk : f.ck};
+    int span[3] = {1, 1, 1};
+    if (fine)
+      for (int a = 0; a < P.ndim; ++a)
+        if (a != f.dir) span[a] = 2;
+    for (int a = 0; a < 3; ++a)
+      if (idx[a] < 0 || idx[a] + span[a] > ext[a]) return 3;
+  }
+  return 0;
+}
+
+int run_fluxreg(const fx_tile* tiles, int nTiles, const fx_fluxreg_face* faces, int nFaces,
+                double* reg, bool put, fx_status* st) {
+  if (st) memset(st, 0, sizeof *st);
+  if (!g.init) {
+    if (st) st->code = 3;
+    return 3;
+  }
+  if (nFaces <= 0) return 0;
+  if (!tiles || !faces || !reg || nTiles <= 0 || check_reg_faces(tiles, nTiles, faces, nFaces, put)) {
+    if (st) st->code = 3;
+    return 3;
+  }
+  if (grow(&g.dregTiles, &g.dregTilesCap, (size_t)nTiles, st)) return 4;
+  if (grow(&g.dfaces, &g.dfacesCap, (size_t)nFaces, st)) return 4;
+  HIPCALL(hipMemcpyAsync(g.dregTiles, tiles, sizeof(fx_tile) * nTiles, hipMemcpyHostToDevice, g.stream),
+          st);
+  HIPCALL(hipMemcpyAsync(g.dfaces, faces, sizeof(fx_fluxreg_face) * nFaces, hipMemcpyHostToDevice,
+                         g.stream),
+          st);
+  const DevParams& P = g.dp;
+  RegLaunch R{g.dregTiles, g.dfaces, nFaces, reg};
+  const dim3 grid(grid_for((long)nFaces * FX_NFLUX));
+  if (put) {
+    const double coef = P.ndim == 3 ? 0.25 : (P.ndim == 2 ? 0.5 : 1.0);
+    hipLaunchKernelGGL(k_fluxreg_put, grid, dim3(kThreads), 0, g.stream, P, R, coef);
+  } else {
+    hipLaunchKernelGGL(k_fluxreg_conserve, grid, dim3(kThreads), 0, g.stream, P, R);
+  }
+  HIPCALL(hipGetLastError(), st);
+  if (g.async) return 0;
+  HIPCALL(hipStreamSynchronize(g.stream), st);
+  return 0;
 }
 
 }  // namespace
@@ -857,7 +984,8 @@ int fx_hydro_init(const fx_params* p) {
 }
 
 void fx_hydro_finalize(void) {
-  void* bufs[] = {g.states, g.flux, g.pavg, g.dtiles, g.dinplace, g.status, g.partial};
+  void* bufs[] = {g.states, g.flux, g.pavg, g.dtiles, g.dinplace, g.status, g.partial,
+                  g.dregTiles, g.dfaces};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : g.ev)
@@ -938,6 +1066,16 @@ int fx_hydro_compute_dt(const double* U, long nBlocks, long blockStride, const d
   for (double v : h) m = v > m ? v : m;
   *dtOut = m != 0. ? cfl / m : 1.e300;
   return 0;
+}
+
+int fx_fluxreg_put(const fx_tile* fine, int nFine, const fx_fluxreg_face* faces, int nFaces,
+                   double* reg, fx_status* st) {
+  return run_fluxreg(fine, nFine, faces, nFaces, reg, true, st);
+}
+
+int fx_fluxreg_conserve(const fx_tile* coarse, int nCoarse, const fx_fluxreg_face* faces,
+                        int nFaces, const double* reg, fx_status* st) {
+  return run_fluxreg(coarse, nCoarse, faces, nFaces, const_cast<double*>(reg), false, st);
 }
 
 void fx_hydro_set_stream(void* stream) { g.stream = (hipStream_t)stream; }

@@ -149,7 +149,7 @@ __device__ __forceinline__ double flat_tilde_vals(double pm2, double pm1, double
   return ft;
 }
 
-template <int NDIM, int TX, int TY, int NT>
+template <int NDIM, int TX, int TY, int NT, int RS>
 __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
   using I = TileIdx<TX, TY>;
   constexpr int NR = NDIM == 3 ? 3 : 1;        // ring depth
@@ -173,6 +173,11 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
   const int tr = blockIdx.x - t * tilesPerBlock;
   const int ox = P.lo[0] + (tr % L.tilesX) * TX;
   const int oy = P.lo[1] + (tr / L.tilesX) * TY;
+  // A tile owns its high x (y) face row only at the block's high side;
+  // otherwise the neighbouring tile computes that row (and the states
+  // behind it) as its low row.
+  const int ex = (ox + TX > P.hi[0]) ? 1 : 0;
+  const int ey = (oy + TY > P.hi[1]) ? 1 : 0;
   const fx_tile& T = L.tiles[t];
   const double* __restrict__ U = T.U;
   const double dt = L.dt;
@@ -181,6 +186,7 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
   const int klo = P.lo[2], khi = P.hi[2];
   DevStatus* st = L.status;
   bool bad = false;  // eigen failure seen by this thread
+  ExactStats es{0ULL, 0ULL};
 
   auto ring = [](int k) { return NR == 3 ? (k % 3) : 0; };
 
@@ -267,14 +273,14 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
     const double* syk = SY + ring(k) * 6 * I::NRY;
     // ------------------------------ stage B1 -----------------------------
     {
-      const int nX = interior ? I::NRY : 0;
-      const int nY = interior ? I::NRX : 0;
+      const int nX = interior ? (TX + 1 + ex) * TY : 0;
+      const int nY = interior ? TX * (TY + 1 + ey) : 0;
       const int nZ = Z ? NCOL : 0;
       for (int w = tid; w < nX + nY + nZ; w += NT) {
         double Wp[8], Wn[8], Vm[8], Vc[8], Vp[8], dbN[8], db0[8], dbP[8];
         Eig e0;
         if (w < nX) {
-          const int a = w % (TX + 2) - 1, b = w / (TX + 2);
+          const int a = w % (TX + 1 + ex) - 1, b = w / (TX + 1 + ex);
           const long c = cidx(P, ox + a, oy + b, k);
           load_prim(P, U, c - sx, Vm);
           load_prim(P, U, c, Vc);
@@ -295,7 +301,7 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
             sub_sigma(Wp, Wn, sg, 0.5 * dt / dz);
           }
           if (negative(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st);
-          put_state(WX, I::NRY, w, Wp, Wn);
+          put_state(WX, I::NRY, I::ry(a, b), Wp, Wn);
         } else if (w < nX + nY) {
           const int w2 = w - nX;
           const int a = w2 % TX, b = w2 / TX - 1;
@@ -319,7 +325,7 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
             sub_sigma(Wp, Wn, sg, 0.5 * dt / dz);
           }
           if (negative(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st);
-          put_state(WY, I::NRX, w2, Wp, Wn);
+          put_state(WY, I::NRX, I::rx(a, b), Wp, Wn);
         } else {
           const int w2 = w - nX - nY;
           const int a = w2 % TX, b = w2 / TX;
@@ -355,8 +361,8 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
 
     // ------------------------------ stage B2 -----------------------------
     if (Z && P.ctu3d) {
-      const int nX = interior ? I::NRY : 0;
-      const int nY = interior ? I::NRX : 0;
+      const int nX = interior ? (TX + 1 + ex) * TY : 0;
+      const int nY = interior ? TX * (TY + 1 + ey) : 0;
       for (int w = tid; w < nX + nY + NCOL; w += NT) {
         double Wp[8], Wn[8], Vc[8], sm[8], sc[8], sp[8], tA[8], tB[8];
         Eig e1, e2;
@@ -365,7 +371,7 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
         int n, idx;
         if (w < nX) {
           // X: sigma_z over j with the y eigen system, sigma_y over k with z's
-          const int a = w % (TX + 2) - 1, b = w / (TX + 2);
+          const int a = w % (TX + 1 + ex) - 1, b = w / (TX + 1 + ex);
           load_prim(P, U, cidx(P, ox + a, oy + b, k), Vc);
           if (!(eig_make<1>(Vc, e1) && eig_make<2>(Vc, e2))) { bad = true; continue; }
           lds_get6(SZ, I::NC, I::cc(a, b - 1), sm);
@@ -377,7 +383,7 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
           lds_get6(SY + ring(k + 1) * 6 * I::NRY, I::NRY, I::ry(a, b), sp);
           trans_sigma<2>(P, sm, sc, sp, e2, tB);
           fac = dt * dt / (6. * dy * dz);
-          base = WX; n = I::NRY; idx = w;
+          base = WX; n = I::NRY; idx = I::ry(a, b);
         } else if (w < nX + nY) {
           // Y: sigma_x over k with z's eigen system, sigma_z over i with x's
           const int w2 = w - nX;
@@ -393,7 +399,7 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
           lds_get6(SZ, I::NC, I::cc(a + 1, b), sp);
           trans_sigma<0>(P, sm, sc, sp, e2, tB);
           fac = dt * dt / (6. * dz * dx);
-          base = WY; n = I::NRX; idx = w2;
+          base = WY; n = I::NRX; idx = I::rx(a, b);
         } else {
           // Z: sigma_y over i with x's eigen system, sigma_x over j with y's
           const int w2 = w - nX - nY;
@@ -421,13 +427,13 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
 
     // ------------------------------ stage C ------------------------------
     {
-      const int nFX = interior ? (TX + 1) * TY : 0;
-      const int nFY = interior ? TX * (TY + 1) : 0;
+      const int nFX = interior ? (TX + ex) * TY : 0;
+      const int nFY = interior ? TX * (TY + ey) : 0;
       const int nC = Z ? NCOL : (interior ? NCOL : 0);
       for (int w = tid; w < nFX + nFY + nC; w += NT) {
         double VL[8], VR[8], F[8];
         if (w < nFX) {
-          const int a = w % (TX + 1), b = w / (TX + 1);
+          const int a = w % (TX + ex), b = w / (TX + ex);
           lds_get6(WX, I::NRY, I::ry(a - 1, b), VL);
           lds_get6(WX + 6 * I::NRY, I::NRY, I::ry(a, b), VR);
           const long c = cidx(P, ox + a, oy + b, k);
@@ -435,7 +441,7 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
           VL[P_GAME] = U[P.voff[FX_GAME] + c - sx];
           VR[P_GAMC] = U[P.voff[FX_GAMC] + c];
           VR[P_GAME] = U[P.voff[FX_GAME] + c];
-          if (face_flux<0>(P, VL, VR, F)) { report_error(st, 2, t, ox + a, oy + b, k, 0); continue; }
+          if (face_flux<0, RS>(P, VL, VR, F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 0); continue; }
           write_flux<0>(P, T.flux[0], face_index(P, 0, ox + a, oy + b, k), F);
         } else if (w < nFX + nFY) {
           const int w2 = w - nFX;
@@ -447,7 +453,7 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
           VL[P_GAME] = U[P.voff[FX_GAME] + c - sy];
           VR[P_GAMC] = U[P.voff[FX_GAMC] + c];
           VR[P_GAME] = U[P.voff[FX_GAME] + c];
-          if (face_flux<1>(P, VL, VR, F)) { report_error(st, 2, t, ox + a, oy + b, k, 1); continue; }
+          if (face_flux<1, RS>(P, VL, VR, F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 1); continue; }
           write_flux<1>(P, T.flux[1], face_index(P, 1, ox + a, oy + b, k), F);
         } else {
           // column item: z-face flux at face k, pavg of the cell, carry Wp
@@ -461,7 +467,7 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
             VL[P_GAME] = U[P.voff[FX_GAME] + c - sz];
             VR[P_GAMC] = U[P.voff[FX_GAMC] + c];
             VR[P_GAME] = U[P.voff[FX_GAME] + c];
-            if (face_flux<2>(P, VL, VR, F)) report_error(st, 2, t, ox + a, oy + b, k, 2);
+            if (face_flux<2, RS>(P, VL, VR, F, es)) report_error(st, 2, t, ox + a, oy + b, k, 2);
             else write_flux<2>(P, T.flux[2], face_index(P, 2, ox + a, oy + b, k), F);
           }
           if (interior && P.aux) {
@@ -483,4 +489,5 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
     __syncthreads();
   }
   if (bad) report_error(st, 1, t, ox, oy, -1, -1);
+  flush_exact(st, es);
 }

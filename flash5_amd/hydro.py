@@ -68,6 +68,13 @@ class FxStatus(C.Structure):
                 ("newton_fallbacks", C.c_longlong), ("min_cell_cfl", C.c_double)]
 
 
+class FxFluxRegFace(C.Structure):
+    """fx_fluxreg_face: one copy of a coarse face on a fine-coarse boundary."""
+    _fields_ = [("coarse_tile", C.c_int), ("fine_tile", C.c_int), ("dir", C.c_int),
+                ("ci", C.c_int), ("cj", C.c_int), ("ck", C.c_int),
+                ("fi", C.c_int), ("fj", C.c_int), ("fk", C.c_int)]
+
+
 # Exported symbols and their signatures (include/fx_hydro.h).
 EXPORTS = {
     "fx_hydro_init": ([C.POINTER(FxParams)], C.c_int),
@@ -79,6 +86,10 @@ EXPORTS = {
     "fx_hydro_advance_batch": ([_DP, C.c_long, C.c_long, C.c_double * 3, C.c_double,
                                 C.POINTER(FxStatus)], C.c_int),
     "fx_hydro_compute_dt": ([_DP, C.c_long, C.c_long, C.c_double * 3, C.c_double, _DP], C.c_int),
+    "fx_fluxreg_put": ([C.POINTER(FxTile), C.c_int, C.POINTER(FxFluxRegFace), C.c_int, _DP,
+                        C.POINTER(FxStatus)], C.c_int),
+    "fx_fluxreg_conserve": ([C.POINTER(FxTile), C.c_int, C.POINTER(FxFluxRegFace), C.c_int, _DP,
+                             C.POINTER(FxStatus)], C.c_int),
     "fx_hydro_set_stream": ([C.c_void_p], None),
     "fx_hydro_set_async": ([C.c_int], None),
     "fx_hydro_set_timing": ([C.c_int], None),
@@ -286,6 +297,38 @@ class Hydro:
         self.last_status = st
         if rc:
             raise FxError("fx_hydro_update_solution", st, rc)
+        return st
+
+    @staticmethod
+    def _faces(faces):
+        arr = (FxFluxRegFace * max(1, len(faces)))()
+        for e, f in enumerate(faces):
+            arr[e] = FxFluxRegFace(*[int(x) for x in f])
+        return arr
+
+    def fluxreg_put(self, fine_tiles: Sequence[Tile], faces, reg) -> FxStatus:
+        """Grid_putFluxData for a fine level: reg (device, FX_NFLUX x nFaces)
+        := coef * sum of the covering fine face fluxes.  faces: sequences
+        (coarse_tile, fine_tile, dir, ci, cj, ck, fi, fj, fk), e.g. from
+        flash5_amd.amr.BlockMesh.register_faces."""
+        self._reinit()
+        st = FxStatus()
+        rc = self.lib.fx_fluxreg_put(self._tiles(fine_tiles), len(fine_tiles), self._faces(faces),
+                                     len(faces), _ptr(reg), C.byref(st))
+        self.last_status = st
+        if rc:
+            raise FxError("fx_fluxreg_put", st, rc)
+        return st
+
+    def fluxreg_conserve(self, coarse_tiles: Sequence[Tile], faces, reg) -> FxStatus:
+        """Grid_conserveFluxes for a coarse level: listed coarse face fluxes := reg."""
+        self._reinit()
+        st = FxStatus()
+        rc = self.lib.fx_fluxreg_conserve(self._tiles(coarse_tiles), len(coarse_tiles),
+                                          self._faces(faces), len(faces), _ptr(reg), C.byref(st))
+        self.last_status = st
+        if rc:
+            raise FxError("fx_fluxreg_conserve", st, rc)
         return st
 
     def advance_batch(self, U, dt: float, deltas: Sequence[float]) -> FxStatus:

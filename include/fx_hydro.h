@@ -134,6 +134,33 @@ int fx_hydro_advance_batch(double* U, long nBlocks, long blockStride,
 int fx_hydro_compute_dt(const double* U, long nBlocks, long blockStride,
                         const double del[3], double cfl, double* dtOut);
 
+/* ---- flux-register correction (Cartesian, refinement ratio 2) ----
+ * One entry per copy of a coarse face on a fine-coarse boundary: a face
+ * shared by two coarse blocks is listed once for each block's flux array.
+ * Indices are 0-based positions in the tiles' interior-face flux arrays
+ * (extent nxb+1 along the face normal).  The 2^(NDIM-1) fine faces covering
+ * the coarse face start at (fi, fj, fk) and step +1 along the transverse axes
+ * of the same fine tile. */
+typedef struct {
+  int coarse_tile, fine_tile;  /* indices into the coarse / fine tile lists */
+  int dir;                     /* face normal, 0 .. ndim-1 */
+  int ci, cj, ck;              /* face in the coarse tile's flux[dir] array */
+  int fi, fj, fk;              /* first covering face in the fine tile's flux[dir] */
+} fx_fluxreg_face;
+
+/* Grid_putFluxData(level) for the fine level: reg := 0, then
+ * reg += coef * each covering fine flux, coef = 1, 1/2, 1/4 in 1-, 2-, 3-D
+ * (Grid_putFluxData.F90:75 -> Grid_addFineToFluxRegister.F90:128-171,
+ * AMReX fluxregister fineadd).  reg is a device array reg[q*nFaces + e] of
+ * FX_NFLUX*nFaces doubles; the fine tiles' flux arrays are read. */
+int fx_fluxreg_put(const fx_tile* fine, int nFine, const fx_fluxreg_face* faces,
+                   int nFaces, double* reg, fx_status* st);
+/* Grid_conserveFluxes(ALLDIR, level) for the coarse level: each listed
+ * coarse face flux := reg (Grid_conserveFluxes.F90:95-108, AMReX
+ * fluxregister overwrite).  Writes the coarse tiles' flux arrays. */
+int fx_fluxreg_conserve(const fx_tile* coarse, int nCoarse, const fx_fluxreg_face* faces,
+                        int nFaces, const double* reg, fx_status* st);
+
 /* Work on the given HIP stream (hipStream_t as void*); NULL = default. */
 void fx_hydro_set_stream(void* stream);
 /* 1: entry points return without synchronising (needed for graph capture). */

@@ -88,6 +88,9 @@ def lib():
                                        C.POINTER(Stats)]
         L.ox_exact_pstar.argtypes = [C.c_double] * 8 + [C.c_double, C.c_int, _DP, _DP]
         L.ox_exact_pstar.restype = C.c_int
+        L.ox_fluxreg_put.argtypes = [C.POINTER(Params), C.POINTER(_DP), C.c_void_p, C.c_int, _DP]
+        L.ox_fluxreg_conserve.argtypes = [C.POINTER(Params), C.POINTER(_DP), C.c_void_p, C.c_int,
+                                          _DP]
         _lib = L
     return _lib
 
@@ -233,3 +236,36 @@ def exact_pstar(gamma, rl, ul, pl, rr, ur, pr, guess=-1.0, tol=1e-12, maxit=50):
 def prim(rho, u, v, w, p, gamma):
     """Primitive reconstruction vector with gamc = game = gamma, eint from the EOS."""
     return np.array([rho, u, v, w, p, gamma, gamma, p / ((gamma - 1.0) * rho), 0.0])
+
+
+# ---- flux register (faces: sequence of 9-tuples (coarse_tile, fine_tile, dir,
+# ci, cj, ck, fi, fj, fk)); fluxes: per tile a list of 3 arrays or None ----
+def _faces_c(faces):
+    arr = (C.c_int * (9 * max(1, len(faces))))()
+    for e, f in enumerate(faces):
+        for k in range(9):
+            arr[9 * e + k] = int(f[k])
+    return arr
+
+
+def _flux_ptrs(fluxes):
+    ptrs = (_DP * (3 * max(1, len(fluxes))))()
+    for t, fl in enumerate(fluxes):
+        for d in range(3):
+            ptrs[3 * t + d] = _p(fl[d]) if fl[d] is not None else _DP()
+    return ptrs
+
+
+def fluxreg_put(p: Params, fine_fluxes, faces):
+    """Grid_putFluxData on a level: register values, shape (NFLUX, nFaces)."""
+    reg = np.zeros((NFLUX, max(1, len(faces))))
+    lib().ox_fluxreg_put(C.byref(p), _flux_ptrs(fine_fluxes), C.cast(_faces_c(faces), C.c_void_p),
+                         len(faces), _p(reg))
+    return reg
+
+
+def fluxreg_conserve(p: Params, coarse_fluxes, faces, reg) -> None:
+    """Grid_conserveFluxes on a level: overwrite coarse fluxes in place."""
+    reg = np.ascontiguousarray(reg, dtype=np.float64)
+    lib().ox_fluxreg_conserve(C.byref(p), _flux_ptrs(coarse_fluxes),
+                              C.cast(_faces_c(faces), C.c_void_p), len(faces), _p(reg))

@@ -1139,3 +1139,51 @@ double ox_block_dt(const ox_params* p, const double* U, const double del[3], dou
       }
   return dt_temp != 0.0 ? tempCfl / dt_temp : 1.e300;
 }
+
+/* ------------------------------------------------------------------ */
+/* flux register: Grid_addFineToFluxRegister.F90:128-171 (coef 1, 1/2,   */
+/* 1/4 by NDIM; ratio 2), Grid_conserveFluxes.F90:95-108; arithmetic of  */
+/* AMReX fluxregister fineadd / overwrite as restated in the header.     */
+/* fineFlux/coarseFlux: 3 pointers per tile (tile*3 + dir).              */
+/* ------------------------------------------------------------------ */
+static long reg_face_index(const ox_params* p, int d, int a, int b, int c) {
+  fgeom_t f = fgeom_of(p, d);
+  return ((long)c * f.fy + b) * f.fx + a;
+}
+
+void ox_fluxreg_put(const ox_params* p, double* const* fineFlux, const ox_fluxreg_face* faces,
+                    int nFaces, double* reg) {
+  const double coef = p->ndim == 3 ? 0.25 : (p->ndim == 2 ? 0.5 : 1.0);
+  for (int e = 0; e < nFaces; ++e) {
+    const ox_fluxreg_face* fc = &faces[e];
+    const int d = fc->dir;
+    fgeom_t fg = fgeom_of(p, d);
+    const double* F = fineFlux[fc->fine_tile * 3 + d];
+    /* transverse offset axes: outer = higher axis, inner = lower axis */
+    int inner = d == 0 ? 1 : 0, outer = d == 2 ? 1 : 2;
+    int nIn = p->ndim > 1 ? 2 : 1, nOut = p->ndim > 2 ? 2 : 1;
+    for (int q = 0; q < OX_NFLUX; ++q) {
+      double acc = 0.0;
+      for (int o = 0; o < nOut; ++o)
+        for (int in = 0; in < nIn; ++in) {
+          int idx[3] = {fc->fi, fc->fj, fc->fk};
+          idx[inner] += in;
+          idx[outer] += o;
+          acc = acc + coef * F[(long)q * fg.nf + reg_face_index(p, d, idx[0], idx[1], idx[2])];
+        }
+      reg[(long)q * nFaces + e] = acc;
+    }
+  }
+}
+
+void ox_fluxreg_conserve(const ox_params* p, double* const* coarseFlux,
+                         const ox_fluxreg_face* faces, int nFaces, const double* reg) {
+  for (int e = 0; e < nFaces; ++e) {
+    const ox_fluxreg_face* fc = &faces[e];
+    const int d = fc->dir;
+    fgeom_t fg = fgeom_of(p, d);
+    double* F = coarseFlux[fc->coarse_tile * 3 + d];
+    long f = reg_face_index(p, d, fc->ci, fc->cj, fc->ck);
+    for (int q = 0; q < OX_NFLUX; ++q) F[(long)q * fg.nf + f] = reg[(long)q * nFaces + e] * 1.0;
+  }
+}

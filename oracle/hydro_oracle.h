@@ -147,6 +147,27 @@ int ox_exact_pstar(double gamma, double rl, double ul, double pl, double rr,
                    double ur, double pr, double guess, double tol, int maxit,
                    double* pstar, double* ustar);
 
+/* ---- flux register (Cartesian, refinement ratio 2) ----
+ * Grid_putFluxData -> Grid_addFineToFluxRegister(zeroFullRegister)
+ * (Grid/GridMain/AMR/Amrex/Grid_addFineToFluxRegister.F90:128-171) and
+ * Grid_conserveFluxes (Grid_conserveFluxes.F90:95-108).  The register
+ * arithmetic lives in AMReX (amrex_fluxregister fineadd / overwrite, not
+ * vendored): the register is zeroed, each covering fine face flux is added
+ * times coef = 1, 1/2, 1/4 (1-D, 2-D, 3-D), the fine faces taken in order of
+ * ascending transverse offset with the higher axis outermost; overwrite
+ * copies the register into the coarse flux array.  One entry per copy of a
+ * coarse face (a face shared by two coarse blocks is listed for each). */
+typedef struct {
+  int coarse_tile, fine_tile, dir;
+  int ci, cj, ck;   /* face in the coarse tile's flux[dir] array */
+  int fi, fj, fk;   /* first covering face in the fine tile's flux[dir] array */
+} ox_fluxreg_face;
+/* reg[q * nFaces + e]; fine[t][d] is tile t's flux array for direction d */
+void ox_fluxreg_put(const ox_params* p, double* const* fineFlux, const ox_fluxreg_face* faces,
+                    int nFaces, double* reg);
+void ox_fluxreg_conserve(const ox_params* p, double* const* coarseFlux,
+                         const ox_fluxreg_face* faces, int nFaces, const double* reg);
+
 #ifdef __cplusplus
 }
 #endif
