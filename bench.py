@@ -104,6 +104,8 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--riemann", choices=["hllc", "exact"], default="hllc",
                     help="face flux solver (HLLC, or the exact iterative solver)")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="fp64, or FP32 reconstruction/fluxes with the FP64 update")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run plain 1 GiB copies (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -115,12 +117,14 @@ def main() -> None:
 
     import torch
     from flash5_amd import workloads as W
-    from flash5_amd.hydro import Hydro, HydroParams
+    from flash5_amd.hydro import PREC_FP64, PREC_MIXED, Hydro, HydroParams
 
     dev = torch.device("cuda:0")
     n, g, gamma = 16, 4, 5.0 / 3.0
     dx = 1.0 / n
-    hp = HydroParams(ndim=3, nxb=n, nyb=n, nzb=n, gamma=gamma, RiemannSolver=args.riemann)
+    prec = {"fp64": PREC_FP64, "mixed": PREC_MIXED}[args.precision]
+    hp = HydroParams(ndim=3, nxb=n, nyb=n, nzb=n, gamma=gamma, RiemannSolver=args.riemann,
+                     precision=prec)
     U = W.fourier_blocks(args.blocks, n=n, ng=g, gamma=gamma, device=dev)
     dt = W.cfl_dt(U, 3, g, (dx,) * 3, 0.8)
     hy = Hydro(hp)
@@ -157,7 +161,7 @@ def main() -> None:
     bmz = march_bytes_per_zone(3, n, g)
     achieved = bmz * zones / (march_ms * 1e-3) / 1e9
     fallbacks = int(hy.last_status.fallbacks)
-    workload = f"{args.blocks}x16^3+4g fp64 {args.riemann}"
+    workload = f"{args.blocks}x16^3+4g {args.precision} {args.riemann}"
     if args.copy_probe:
         copy_probe()
 
@@ -180,13 +184,13 @@ def main() -> None:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"{args.blocks} independent 3-D blocks, 16^3 interior + 4 guard "
                                "cells, gamma=5/3, seeded Fourier-mode fields, one unsplit "
-                               f"PPM+{args.riemann.upper()} step (plain FP64 path)",
+                               f"PPM+{args.riemann.upper()} step ({args.precision})",
                    "blocks": args.blocks, "block": [16, 16, 16], "nguard": 4,
-                   "precision": "fp64", "riemann": args.riemann, "dt": dt,
+                   "precision": args.precision, "riemann": args.riemann, "dt": dt,
                    "newton_iters_per_face": (int(hy.last_status.newton_iters) /
                                              (3 * args.blocks * 17 * 16 * 16))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

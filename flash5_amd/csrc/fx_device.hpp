@@ -39,7 +39,7 @@ struct DevParams {
   double gamma;
   int limiter;
   double beta;
-  int charLim, flat, steep, ctu3d, aux, riemann;
+  int charLim, flat, steep, ctu3d, aux, riemann, precision;
   double fP, smlrho, smallE, eintSwitch, tiny;
   double gammam1, gam1inv, ggprodinv, abar;
   int exactMaxIter;
@@ -53,28 +53,34 @@ struct DevParams {
   long icells;               // interior cells per tile
 };
 
-struct Eig {
-  double un, cf, ahinv, lfp, a2inv, rp, rho;
+// Every routine below is templated on the real type R of the reconstruction:
+// double in the FP64 modes, float in the mixed mode.  Literal constants are
+// written R(c) so that the double instantiation performs exactly the FP64
+// operations of the reference order.
+template <typename R> struct EigT {
+  R un, cf, ahinv, lfp, a2inv, rp, rho;
 };
+using Eig = EigT<double>;
 
-template <int DIR>
-__device__ __forceinline__ bool eig_make(const double* V, Eig& e) {
+template <int DIR, typename R>
+__device__ __forceinline__ bool eig_make(const R* V, EigT<R>& e) {
   e.rho = V[P_DENS];
   e.un = V[P_VELX + DIR];
-  const double a2 = V[P_GAMC] * V[P_PRES] / V[P_DENS];
-  if (!(a2 > 0.0)) return false;
+  const R a2 = V[P_GAMC] * V[P_PRES] / V[P_DENS];
+  if (!(a2 > R(0.0))) return false;
   e.cf = sqrt(a2);
-  const double dinv = 1.0 / V[P_DENS];
-  const double a2e = e.cf * e.cf;
-  e.a2inv = 1.0 / a2e;
-  e.ahinv = 0.5 / e.cf;
-  const double Na = 0.5 * e.a2inv;
+  const R dinv = R(1.0) / V[P_DENS];
+  const R a2e = e.cf * e.cf;
+  e.a2inv = R(1.0) / a2e;
+  e.ahinv = R(0.5) / e.cf;
+  const R Na = R(0.5) * e.a2inv;
   e.lfp = dinv * Na;
   e.rp = V[P_DENS] * a2e;
   return true;
 }
 
-__device__ __forceinline__ void eig_lambda(const Eig& e, double lam[5]) {
+template <typename R>
+__device__ __forceinline__ void eig_lambda(const EigT<R>& e, R lam[5]) {
   lam[0] = e.un - e.cf;
   lam[1] = e.un;
   lam[2] = e.un;
@@ -88,8 +94,8 @@ template <int DIR> struct Vel {
 };
 
 // L_n . d over the five primitive slots
-template <int DIR, int N>
-__device__ __forceinline__ double eig_left1(const Eig& e, const double* d) {
+template <int DIR, int N, typename R>
+__device__ __forceinline__ R eig_left1(const EigT<R>& e, const R* d) {
   constexpr int v1 = Vel<DIR>::v1, v2 = Vel<DIR>::v2, v3 = Vel<DIR>::v3;
   if constexpr (N == 0) return (-e.ahinv) * d[v1] + e.lfp * d[P_PRES];
   else if constexpr (N == 1) return d[v2];
@@ -98,8 +104,8 @@ __device__ __forceinline__ double eig_left1(const Eig& e, const double* d) {
   else return e.ahinv * d[v1] + e.lfp * d[P_PRES];
 }
 
-template <int DIR>
-__device__ __forceinline__ void eig_left(const Eig& e, const double* d, double w[5]) {
+template <int DIR, typename R>
+__device__ __forceinline__ void eig_left(const EigT<R>& e, const R* d, R w[5]) {
   w[0] = eig_left1<DIR, 0>(e, d);
   w[1] = eig_left1<DIR, 1>(e, d);
   w[2] = eig_left1<DIR, 2>(e, d);
@@ -108,8 +114,8 @@ __device__ __forceinline__ void eig_left(const Eig& e, const double* d, double w
 }
 
 // out[0..4] = sum_n R_n * w[n]
-template <int DIR>
-__device__ __forceinline__ void eig_right(const Eig& e, const double w[5], double* out) {
+template <int DIR, typename R>
+__device__ __forceinline__ void eig_right(const EigT<R>& e, const R w[5], R* out) {
   constexpr int v1 = Vel<DIR>::v1, v2 = Vel<DIR>::v2, v3 = Vel<DIR>::v3;
   out[P_DENS] = e.rho * w[0] + w[2] + e.rho * w[4];
   out[v1] = (-e.cf) * w[0] + e.cf * w[4];
@@ -118,28 +124,29 @@ __device__ __forceinline__ void eig_right(const Eig& e, const double w[5], doubl
   out[P_PRES] = e.rp * w[0] + e.rp * w[4];
 }
 
-__device__ __forceinline__ double sgn1(double a) { return copysign(1.0, a); }
+template <typename R> __device__ __forceinline__ R sgn1(R a) { return copysign(R(1.0), a); }
 
-__device__ __forceinline__ double limit(int kind, double a, double b, double beta) {
+template <typename R>
+__device__ __forceinline__ R limit(int kind, R a, R b, R beta) {
   switch (kind) {
-    case FX_MINMOD: return .5 * (sgn1(a) + sgn1(b)) * fmin(fabs(a), fabs(b));
-    case FX_VANLEER: return (a * b <= 0.) ? 0. : 2. * a * b / (a + b);
+    case FX_MINMOD: return R(.5) * (sgn1(a) + sgn1(b)) * fmin(fabs(a), fabs(b));
+    case FX_VANLEER: return (a * b <= R(0.)) ? R(0.) : R(2.) * a * b / (a + b);
     case FX_LIMITED:
-      if (b > 0.) return fmax(fmax(0., fmin(beta * a, b)), fmin(a, beta * b));
-      return fmin(fmin(0., fmax(beta * a, b)), fmax(a, beta * b));
+      if (b > R(0.)) return fmax(fmax(R(0.), fmin(beta * a, b)), fmin(a, beta * b));
+      return fmin(fmin(R(0.), fmax(beta * a, b)), fmax(a, beta * b));
     default:  // FX_MC
-      return (sgn1(a) + sgn1(b)) * fmin(fmin(fabs(a), .25 * fabs(a + b)), fabs(b));
+      return (sgn1(a) + sgn1(b)) * fmin(fmin(fabs(a), R(.25) * fabs(a + b)), fabs(b));
   }
 }
 
 // Limited slope of the cell whose eigen system is e (hy_TVDslope), projected
 // back to primitive slopes when characteristic limiting is on
 // (PPM.F90:523-570).  out[0..7].
-template <int DIR>
-__device__ __forceinline__ void tvd_slope(const DevParams& P, const double* VL,
-                                          const double* V0, const double* VR, const Eig& e,
-                                          double out[8]) {
-  double dL[8], dR[8];
+template <int DIR, typename R>
+__device__ __forceinline__ void tvd_slope(const DevParams& P, const R* VL, const R* V0,
+                                          const R* VR, const EigT<R>& e, R out[8]) {
+  const R beta = R(P.beta);
+  R dL[8], dR[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     dL[k] = V0[k] - VL[k];
@@ -147,155 +154,154 @@ __device__ __forceinline__ void tvd_slope(const DevParams& P, const double* VL,
   }
   if (P.limiter == FX_HYBRID) {
     if (P.charLim) {
-      double a[5], b[5], w[5];
+      R a[5], b[5], w[5];
       eig_left<DIR>(e, dL, a);
       eig_left<DIR>(e, dR, b);
 #pragma unroll
       for (int n = 0; n < 5; ++n)
-        w[n] = (n == 2) ? limit(FX_MC, a[n], b[n], P.beta) : limit(FX_MINMOD, a[n], b[n], P.beta);
+        w[n] = (n == 2) ? limit(FX_MC, a[n], b[n], beta) : limit(FX_MINMOD, a[n], b[n], beta);
       eig_right<DIR>(e, w, out);
-      for (int k = P_GAMC; k <= P_EINT; ++k) out[k] = limit(FX_MC, dL[k], dR[k], P.beta);
+      for (int k = P_GAMC; k <= P_EINT; ++k) out[k] = limit(FX_MC, dL[k], dR[k], beta);
     } else {
-      out[P_DENS] = limit(FX_MC, dL[P_DENS], dR[P_DENS], P.beta);
-      for (int k = P_VELX; k <= P_PRES; ++k) out[k] = limit(FX_MINMOD, dL[k], dR[k], P.beta);
-      out[P_GAMC] = limit(FX_MC, dL[P_GAMC], dR[P_GAMC], P.beta);
-      out[P_GAME] = limit(FX_MC, dL[P_GAME], dR[P_GAME], P.beta);
-      out[P_EINT] = 0.;
+      out[P_DENS] = limit(FX_MC, dL[P_DENS], dR[P_DENS], beta);
+      for (int k = P_VELX; k <= P_PRES; ++k) out[k] = limit(FX_MINMOD, dL[k], dR[k], beta);
+      out[P_GAMC] = limit(FX_MC, dL[P_GAMC], dR[P_GAMC], beta);
+      out[P_GAME] = limit(FX_MC, dL[P_GAME], dR[P_GAME], beta);
+      out[P_EINT] = R(0.);
     }
     return;
   }
   if (P.charLim) {
-    double a[5], b[5], w[5];
+    R a[5], b[5], w[5];
     eig_left<DIR>(e, dL, a);
     eig_left<DIR>(e, dR, b);
 #pragma unroll
-    for (int n = 0; n < 5; ++n) w[n] = limit(P.limiter, a[n], b[n], P.beta);
+    for (int n = 0; n < 5; ++n) w[n] = limit(P.limiter, a[n], b[n], beta);
     eig_right<DIR>(e, w, out);
-    for (int k = P_GAMC; k <= P_EINT; ++k) out[k] = limit(P.limiter, dL[k], dR[k], P.beta);
+    for (int k = P_GAMC; k <= P_EINT; ++k) out[k] = limit(P.limiter, dL[k], dR[k], beta);
   } else {
-    for (int k = 0; k < 8; ++k) out[k] = limit(P.limiter, dL[k], dR[k], P.beta);
+    for (int k = 0; k < 8; ++k) out[k] = limit(P.limiter, dL[k], dR[k], beta);
   }
 }
 
 // Transverse flux sigma of one cell (hy_upwindTransverseFlux, order 1).
-template <int DIR>
-__device__ __forceinline__ void trans_sigma(const DevParams& P, const double* vm1,
-                                            const double* vc0, const double* vp1, const Eig& e,
-                                            double sig[8]) {
-  double lam[5];
+template <int DIR, typename R>
+__device__ __forceinline__ void trans_sigma(const DevParams& P, const R* vm1, const R* vc0,
+                                            const R* vp1, const EigT<R>& e, R sig[8]) {
+  R lam[5];
   eig_lambda(e, lam);
   constexpr int v1 = Vel<DIR>::v1, v2 = Vel<DIR>::v2, v3 = Vel<DIR>::v3;
 #pragma unroll
-  for (int v = 0; v < 8; ++v) sig[v] = 0.;
-  double dm[5], dp[5];
+  for (int v = 0; v < 8; ++v) sig[v] = R(0.);
+  R dm[5], dp[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     dm[v] = vc0[v] - vm1[v];
     dp[v] = vp1[v] - vc0[v];
   }
   // upwinded characteristic amplitude of wave N
-#define FX_TS_DB(N) (lam[N] > 0. ? eig_left1<DIR, N>(e, dm) : eig_left1<DIR, N>(e, dp))
+#define FX_TS_DB(N) (lam[N] > R(0.) ? eig_left1<DIR, N>(e, dm) : eig_left1<DIR, N>(e, dp))
   {
-    const double db = FX_TS_DB(0);
+    const R db = FX_TS_DB(0);
     sig[P_DENS] = sig[P_DENS] + lam[0] * e.rho * db;
     sig[v1] = sig[v1] + lam[0] * (-e.cf) * db;
     sig[P_PRES] = sig[P_PRES] + lam[0] * e.rp * db;
   }
-  { const double db = FX_TS_DB(1); sig[v2] = sig[v2] + lam[1] * db; }
-  { const double db = FX_TS_DB(2); sig[P_DENS] = sig[P_DENS] + lam[2] * db; }
-  { const double db = FX_TS_DB(3); sig[v3] = sig[v3] + lam[3] * db; }
+  { const R db = FX_TS_DB(1); sig[v2] = sig[v2] + lam[1] * db; }
+  { const R db = FX_TS_DB(2); sig[P_DENS] = sig[P_DENS] + lam[2] * db; }
+  { const R db = FX_TS_DB(3); sig[v3] = sig[v3] + lam[3] * db; }
   {
-    const double db = FX_TS_DB(4);
+    const R db = FX_TS_DB(4);
     sig[P_DENS] = sig[P_DENS] + lam[4] * e.rho * db;
     sig[v1] = sig[v1] + lam[4] * e.cf * db;
     sig[P_PRES] = sig[P_PRES] + lam[4] * e.rp * db;
   }
 #undef FX_TS_DB
   if (P.aux) {
-    const double d = (lam[2] > 0.) ? vc0[P_EINT] - vm1[P_EINT] : vp1[P_EINT] - vc0[P_EINT];
+    const R d = (lam[2] > R(0.)) ? vc0[P_EINT] - vm1[P_EINT] : vp1[P_EINT] - vc0[P_EINT];
     sig[P_EINT] = lam[2] * d;
   }
 }
 
-// Full PPM reconstruction of one cell along dir (PPM.F90:378-842).
-// Returns false on an imaginary sound speed in a neighbour's eigen system.
 // PPM interface values, limiting and characteristic tracing from the three
 // projected slopes of cells i-1, i, i+1 (PPM.F90:578-842).  Vm/Vc/Vp are the
 // primitive vectors of i-1, i, i+1; densMM/densPP the densities of i-2, i+2
 // (contact steepening only).
-template <int DIR>
-__device__ __forceinline__ void ppm_trace(const DevParams& P, double dtn, double delta,
-                                          const double* Vm, const double* Vc, const double* Vp,
-                                          double densMM, double densPP, const double* dbN,
-                                          const double* db0, const double* dbP, double flatC,
-                                          const Eig& e0, double Wp[8], double Wm[8]) {
-  double Vmm[1] = {densMM}, Vpp[1] = {densPP};
+template <int DIR, typename R>
+__device__ __forceinline__ void ppm_trace(const DevParams& P, R dtn, R delta, const R* Vm,
+                                          const R* Vc, const R* Vp, R densMM, R densPP,
+                                          const R* dbN, const R* db0, const R* dbP, R flatC,
+                                          const EigT<R>& e0, R Wp[8], R Wm[8]) {
+  R Vmm[1] = {densMM}, Vpp[1] = {densPP};
   const int endVar = P.aux ? P_EINT : P_PRES;
-  double vL[8], vR[8];
+  R vL[8], vR[8];
 #pragma unroll
   for (int v = 0; v < 8; ++v) {
-    vL[v] = 0.;
-    vR[v] = 0.;
+    vL[v] = R(0.);
+    vR[v] = R(0.);
     if (v <= endVar) {
-      vL[v] = 0.5 * (Vc[v] + Vm[v]) - (db0[v] - dbN[v]) / 6.;
-      vR[v] = 0.5 * (Vc[v] + Vp[v]) - (dbP[v] - db0[v]) / 6.;
+      vL[v] = R(0.5) * (Vc[v] + Vm[v]) - (db0[v] - dbN[v]) / R(6.);
+      vR[v] = R(0.5) * (Vc[v] + Vp[v]) - (dbP[v] - db0[v]) / R(6.);
     }
   }
   if (P.steep) {
-    const double temp1 = Vp[P_DENS] - Vm[P_DENS];
-    if (fabs(temp1) > P.tiny) {
-      const double d2R = (Vpp[0] - 2. * Vp[P_DENS] + Vc[P_DENS]) / (6. * delta * delta);
-      const double d2L = (Vc[P_DENS] - 2. * Vm[P_DENS] + Vmm[0]) / (6. * delta * delta);
-      double eta = (d2L - d2R) * (delta * delta) / temp1;
-      if (d2R * d2L >= 0.) eta = 0.;
-      if (0.01 * fmin(Vp[P_DENS], Vm[P_DENS]) - fabs(Vp[P_DENS] - Vm[P_DENS]) >= 0.) eta = 0.;
-      eta = fmax(0., fmin(1., 20.0 * (eta - 0.05)));
-      const double t2 = fabs(Vp[P_PRES] - Vm[P_PRES]) / fmin(Vp[P_PRES], Vm[P_PRES]);
-      const double t3 = fabs(Vp[P_DENS] - Vm[P_DENS]) / fmin(Vp[P_DENS], Vm[P_DENS]);
-      if (Vc[P_GAME] * 0.1 * t3 - t2 < 0.0) eta = 0.;
-      vL[P_DENS] = vL[P_DENS] * (1. - eta) + (Vm[P_DENS] + 0.5 * dbN[P_DENS]) * eta;
-      vR[P_DENS] = vR[P_DENS] * (1. - eta) + (Vp[P_DENS] - 0.5 * dbP[P_DENS]) * eta;
+    const R temp1 = Vp[P_DENS] - Vm[P_DENS];
+    if (fabs(temp1) > R(P.tiny)) {
+      const R d2R = (Vpp[0] - R(2.) * Vp[P_DENS] + Vc[P_DENS]) / (R(6.) * delta * delta);
+      const R d2L = (Vc[P_DENS] - R(2.) * Vm[P_DENS] + Vmm[0]) / (R(6.) * delta * delta);
+      R eta = (d2L - d2R) * (delta * delta) / temp1;
+      if (d2R * d2L >= R(0.)) eta = R(0.);
+      if (R(0.01) * fmin(Vp[P_DENS], Vm[P_DENS]) - fabs(Vp[P_DENS] - Vm[P_DENS]) >= R(0.)) eta = R(0.);
+      eta = fmax(R(0.), fmin(R(1.), R(20.0) * (eta - R(0.05))));
+      const R t2 = fabs(Vp[P_PRES] - Vm[P_PRES]) / fmin(Vp[P_PRES], Vm[P_PRES]);
+      const R t3 = fabs(Vp[P_DENS] - Vm[P_DENS]) / fmin(Vp[P_DENS], Vm[P_DENS]);
+      if (Vc[P_GAME] * R(0.1) * t3 - t2 < R(0.0)) eta = R(0.);
+      vL[P_DENS] = vL[P_DENS] * (R(1.) - eta) + (Vm[P_DENS] + R(0.5) * dbN[P_DENS]) * eta;
+      vR[P_DENS] = vR[P_DENS] * (R(1.) - eta) + (Vp[P_DENS] - R(0.5) * dbP[P_DENS]) * eta;
     }
   }
-  double dW[8], W6[8];
+  R dW[8], W6[8];
 #pragma unroll
   for (int n = 0; n < 8; ++n) {
-    dW[n] = 0.;
-    W6[n] = 0.;
+    dW[n] = R(0.);
+    W6[n] = R(0.);
     if (n > endVar) continue;
     if (P.flat) {
-      vL[n] = flatC * Vc[n] + (1.0 - flatC) * vL[n];
-      vR[n] = flatC * Vc[n] + (1.0 - flatC) * vR[n];
+      vL[n] = flatC * Vc[n] + (R(1.0) - flatC) * vL[n];
+      vR[n] = flatC * Vc[n] + (R(1.0) - flatC) * vR[n];
     }
-    if ((vR[n] - Vc[n]) * (Vc[n] - vL[n]) <= 0.) {
+    if ((vR[n] - Vc[n]) * (Vc[n] - vL[n]) <= R(0.)) {
       vL[n] = Vc[n];
       vR[n] = Vc[n];
     }
-    double dd = vR[n] - vL[n];
-    if (6. * (vR[n] - vL[n]) * (Vc[n] - 0.5 * (vL[n] + vR[n])) > dd * dd) vL[n] = 3. * Vc[n] - 2. * vR[n];
+    R dd = vR[n] - vL[n];
+    if (R(6.) * (vR[n] - vL[n]) * (Vc[n] - R(0.5) * (vL[n] + vR[n])) > dd * dd)
+      vL[n] = R(3.) * Vc[n] - R(2.) * vR[n];
     dd = vR[n] - vL[n];
-    if (6. * (vR[n] - vL[n]) * (Vc[n] - 0.5 * (vL[n] + vR[n])) < -(dd * dd)) vR[n] = 3. * Vc[n] - 2. * vL[n];
+    if (R(6.) * (vR[n] - vL[n]) * (Vc[n] - R(0.5) * (vL[n] + vR[n])) < -(dd * dd))
+      vR[n] = R(3.) * Vc[n] - R(2.) * vL[n];
     dW[n] = vR[n] - vL[n];
-    W6[n] = 6. * (Vc[n] - 0.5 * (vR[n] + vL[n]));
+    W6[n] = R(6.) * (Vc[n] - R(0.5) * (vR[n] + vL[n]));
   }
-  double lam[5];
+  R lam[5];
   eig_lambda(e0, lam);
-  const double hdtn = 0.5 * dtn;
+  const R hdtn = R(0.5) * dtn;
+  const R four3 = R(4.) / R(3.);
 #pragma unroll
   for (int v = 0; v < 8; ++v) {
-    Wp[v] = 0.;
-    Wm[v] = 0.;
+    Wp[v] = R(0.);
+    Wm[v] = R(0.);
   }
   {
-    const double lmax = fmax(lam[4], 0.), lmin = -fmin(lam[0], 0.);
-    const double emax = fmax(lam[2], 0.), emin = -fmin(lam[2], 0.);
+    const R lmax = fmax(lam[4], R(0.)), lmin = -fmin(lam[0], R(0.));
+    const R emax = fmax(lam[2], R(0.)), emin = -fmin(lam[2], R(0.));
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
       if (v > endVar) continue;
-      const double a = v <= P_PRES ? lmax : emax;
-      const double b = v <= P_PRES ? lmin : emin;
-      Wp[v] = vR[v] - a * hdtn * (dW[v] - (1.0 - a * hdtn * 4. / 3.) * W6[v]);
-      Wm[v] = vL[v] + b * hdtn * (dW[v] + (1.0 - b * hdtn * 4. / 3.) * W6[v]);
+      const R a = v <= P_PRES ? lmax : emax;
+      const R b = v <= P_PRES ? lmin : emin;
+      Wp[v] = vR[v] - a * hdtn * (dW[v] - (R(1.0) - a * hdtn * R(4.) / R(3.)) * W6[v]);
+      Wm[v] = vL[v] + b * hdtn * (dW[v] + (R(1.0) - b * hdtn * R(4.) / R(3.)) * W6[v]);
     }
   }
   Wp[P_GAMC] = Vc[P_GAMC];
@@ -304,53 +310,51 @@ __device__ __forceinline__ void ppm_trace(const DevParams& P, double dtn, double
   Wm[P_GAME] = Vc[P_GAME];
   // characteristic tracing (PPM.F90:751-800), HLL-type branch for all
   // solvers but Roe
-  double sL[5] = {0., 0., 0., 0., 0.}, sR[5] = {0., 0., 0., 0., 0.};
-  double mW6[5];
+  R sL[5] = {R(0.), R(0.), R(0.), R(0.), R(0.)}, sR[5] = {R(0.), R(0.), R(0.), R(0.), R(0.)};
+  R mW6[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) mW6[v] = -W6[v];
   constexpr int v1 = Vel<DIR>::v1, v2 = Vel<DIR>::v2, v3 = Vel<DIR>::v3;
   const bool roe = (P.riemann == FX_ROE);
-  double cA[5], cB[5];
+  R cA[5], cB[5];
   eig_left<DIR>(e0, dW, cA);
   eig_left<DIR>(e0, mW6, cB);
-  // contribution of wave N through its right-eigenvector entry R at slot V
-#define FX_TRACE(N, V, R)                                                          \
-  {                                                                                \
-    const double x = dtn * lam[N];                                                 \
-    if (!roe || lam[N] < 0.)                                                       \
-      sL[V] = sL[V] + (.5 * (-1. - x) * (R) * cA[N] +                              \
-                       .25 * (1. + 2. * dtn * lam[N] + 4. / 3. * (x * x)) * (R) * cB[N]); \
-    if (!roe || lam[N] > 0.)                                                       \
-      sR[V] = sR[V] + (.5 * (1. - x) * (R) * cA[N] +                               \
-                       .25 * (1. - 2. * dtn * lam[N] + 4. / 3. * (x * x)) * (R) * cB[N]); \
+  // contribution of wave N through its right-eigenvector entry RV at slot V
+#define FX_TRACE(N, V, RV)                                                              \
+  {                                                                                     \
+    const R x = dtn * lam[N];                                                           \
+    if (!roe || lam[N] < R(0.))                                                         \
+      sL[V] = sL[V] + (R(.5) * (R(-1.) - x) * (RV) * cA[N] +                            \
+                       R(.25) * (R(1.) + R(2.) * dtn * lam[N] + four3 * (x * x)) * (RV) * cB[N]); \
+    if (!roe || lam[N] > R(0.))                                                         \
+      sR[V] = sR[V] + (R(.5) * (R(1.) - x) * (RV) * cA[N] +                             \
+                       R(.25) * (R(1.) - R(2.) * dtn * lam[N] + four3 * (x * x)) * (RV) * cB[N]); \
   }
   FX_TRACE(0, P_DENS, e0.rho)
   FX_TRACE(0, v1, -e0.cf)
   FX_TRACE(0, P_PRES, e0.rp)
-  FX_TRACE(1, v2, 1.)
-  FX_TRACE(2, P_DENS, 1.)
-  FX_TRACE(3, v3, 1.)
+  FX_TRACE(1, v2, R(1.))
+  FX_TRACE(2, P_DENS, R(1.))
+  FX_TRACE(3, v3, R(1.))
   FX_TRACE(4, P_DENS, e0.rho)
   FX_TRACE(4, v1, e0.cf)
   FX_TRACE(4, P_PRES, e0.rp)
 #undef FX_TRACE
 #pragma unroll
   for (int v = 0; v <= P_PRES; ++v) {
-    Wm[v] = Vc[v] + W6[v] / 12. + sL[v];
-    Wp[v] = Vc[v] + W6[v] / 12. + sR[v];
+    Wm[v] = Vc[v] + W6[v] / R(12.) + sL[v];
+    Wp[v] = Vc[v] + W6[v] / R(12.) + sR[v];
   }
 }
 
-
-template <int DIR>
-__device__ __forceinline__ bool ppm_full(const DevParams& P, double dtn, double delta,
-                                         const double* Vmm, const double* Vm, const double* Vc,
-                                         const double* Vp, const double* Vpp, double flatC,
-                                         const Eig& e0, double Wp[8], double Wm[8]) {
-  Eig eP, eN;
+template <int DIR, typename R>
+__device__ __forceinline__ bool ppm_full(const DevParams& P, R dtn, R delta, const R* Vmm,
+                                         const R* Vm, const R* Vc, const R* Vp, const R* Vpp,
+                                         R flatC, const EigT<R>& e0, R Wp[8], R Wm[8]) {
+  EigT<R> eP, eN;
   if (!eig_make<DIR>(Vp, eP)) return false;
   if (!eig_make<DIR>(Vm, eN)) return false;
-  double dbN[8], db0[8], dbP[8];
+  R dbN[8], db0[8], dbP[8];
   tvd_slope<DIR>(P, Vmm, Vm, Vc, eN, dbN);
   tvd_slope<DIR>(P, Vm, Vc, Vp, e0, db0);
   tvd_slope<DIR>(P, Vc, Vp, Vpp, eP, dbP);
@@ -360,11 +364,11 @@ __device__ __forceinline__ bool ppm_full(const DevParams& P, double dtn, double 
 }
 
 // ---- HLLC (hy_HLLC.F90:53-324) ----
-template <int DIR>
-__device__ __forceinline__ void prim2flx(const double* V, double fP, double F[6]) {
-  const double u2 = V[P_VELX] * V[P_VELX] + V[P_VELY] * V[P_VELY] + V[P_VELZ] * V[P_VELZ];
-  const double E = 0.5 * V[P_DENS] * u2 + V[P_PRES] / (V[P_GAME] - 1.);
-  const double un = V[P_VELX + DIR];
+template <int DIR, typename R>
+__device__ __forceinline__ void prim2flx(const R* V, R fP, R F[6]) {
+  const R u2 = V[P_VELX] * V[P_VELX] + V[P_VELY] * V[P_VELY] + V[P_VELZ] * V[P_VELZ];
+  const R E = R(0.5) * V[P_DENS] * u2 + V[P_PRES] / (V[P_GAME] - R(1.));
+  const R un = V[P_VELX + DIR];
   F[5] = V[P_PRES];
   F[0] = V[P_DENS] * un;
   F[1] = F[0] * V[P_VELX];
@@ -375,57 +379,57 @@ __device__ __forceinline__ void prim2flx(const double* V, double fP, double F[6]
 }
 
 // returns 0 ok, 1 imaginary sound speed
-template <int DIR>
-__device__ __forceinline__ int hllc(const DevParams& P, const double* Vm,
-                                    const double* Vp, double F[6]) {
-  const double velNL = Vm[P_VELX + DIR], velNR = Vp[P_VELX + DIR];
-  const double aL2 = Vm[P_GAMC] * Vm[P_PRES] / Vm[P_DENS];
-  const double aR2 = Vp[P_GAMC] * Vp[P_PRES] / Vp[P_DENS];
-  if ((Vm[P_DENS] < P.tiny && Vm[P_DENS] > 0.) || (Vp[P_DENS] < P.tiny && Vp[P_DENS] > 0.) ||
-      (Vm[P_PRES] < P.tiny && Vm[P_PRES] > 0.) || (Vp[P_PRES] < P.tiny && Vp[P_PRES] > 0.)) {
+template <int DIR, typename R>
+__device__ __forceinline__ int hllc(const DevParams& P, const R* Vm, const R* Vp, R F[6]) {
+  const R tiny = R(P.tiny), fP = R(P.fP);
+  const R velNL = Vm[P_VELX + DIR], velNR = Vp[P_VELX + DIR];
+  const R aL2 = Vm[P_GAMC] * Vm[P_PRES] / Vm[P_DENS];
+  const R aR2 = Vp[P_GAMC] * Vp[P_PRES] / Vp[P_DENS];
+  if ((Vm[P_DENS] < tiny && Vm[P_DENS] > R(0.)) || (Vp[P_DENS] < tiny && Vp[P_DENS] > R(0.)) ||
+      (Vm[P_PRES] < tiny && Vm[P_PRES] > R(0.)) || (Vp[P_PRES] < tiny && Vp[P_PRES] > R(0.))) {
 #pragma unroll
-    for (int f = 0; f < 6; ++f) F[f] = 0.;
+    for (int f = 0; f < 6; ++f) F[f] = R(0.);
     return 0;
   }
-  if (aL2 < 0. || aR2 < 0.) return 1;
-  const double cfL = sqrt(aL2), cfR = sqrt(aR2);
-  const double SL = fmin(velNL - cfL, velNR - cfR);
-  const double SR = fmax(velNL + cfL, velNR + cfR);
-  const double tpL = Vm[P_PRES], tpR = Vp[P_PRES];
-  double qStar = (Vp[P_DENS] * velNR * (SR - velNR) - Vm[P_DENS] * velNL * (SL - velNL) + tpL - tpR -
-                  0. * 0. + 0. * 0.);
+  if (aL2 < R(0.) || aR2 < R(0.)) return 1;
+  const R cfL = sqrt(aL2), cfR = sqrt(aR2);
+  const R SL = fmin(velNL - cfL, velNR - cfR);
+  const R SR = fmax(velNL + cfL, velNR + cfR);
+  const R tpL = Vm[P_PRES], tpR = Vp[P_PRES];
+  R qStar = (Vp[P_DENS] * velNR * (SR - velNR) - Vm[P_DENS] * velNL * (SL - velNL) + tpL - tpR -
+             R(0.) * R(0.) + R(0.) * R(0.));
   qStar = qStar / (Vp[P_DENS] * (SR - velNR) - Vm[P_DENS] * (SL - velNL));
-  if (SL >= 0.) {
-    prim2flx<DIR>(Vm, P.fP, F);
+  if (SL >= R(0.)) {
+    prim2flx<DIR>(Vm, fP, F);
     return 0;
   }
-  if (!(qStar < 0. && SR >= 0.) && !(SL < 0. && qStar >= 0.)) {
-    prim2flx<DIR>(Vp, P.fP, F);
+  if (!(qStar < R(0.) && SR >= R(0.)) && !(SL < R(0.) && qStar >= R(0.))) {
+    prim2flx<DIR>(Vp, fP, F);
     return 0;
   }
-  const bool left = (SL < 0. && qStar >= 0.);
+  const bool left = (SL < R(0.) && qStar >= R(0.));
   // element-wise select keeps both inputs in registers (a pointer select
   // between two private arrays would put them in scratch memory)
-  double V[8];
+  R V[8];
 #pragma unroll
   for (int v = 0; v < 8; ++v) V[v] = left ? Vm[v] : Vp[v];
-  const double S = left ? SL : SR;
-  const double velN = left ? velNL : velNR;
-  const double tp = left ? tpL : tpR;
-  const double numer = S - velN, denom = S - qStar;
-  const double pStar = Vm[P_DENS] * (SL - velNL) * (qStar - velNL) + tpL;
-  double Fs[6];
-  prim2flx<DIR>(V, P.fP, Fs);
+  const R S = left ? SL : SR;
+  const R velN = left ? velNL : velNR;
+  const R tp = left ? tpL : tpR;
+  const R numer = S - velN, denom = S - qStar;
+  const R pStar = Vm[P_DENS] * (SL - velNL) * (qStar - velNL) + tpL;
+  R Fs[6];
+  prim2flx<DIR>(V, fP, Fs);
   // conserved state of the chosen side (hy_prim2con.F90)
-  const double u2 = V[P_VELX] * V[P_VELX] + V[P_VELY] * V[P_VELY] + V[P_VELZ] * V[P_VELZ];
-  double Uc[5];
+  const R u2 = V[P_VELX] * V[P_VELX] + V[P_VELY] * V[P_VELY] + V[P_VELZ] * V[P_VELZ];
+  R Uc[5];
   Uc[0] = V[P_DENS];
   Uc[1] = V[P_DENS] * V[P_VELX];
   Uc[2] = V[P_DENS] * V[P_VELY];
   Uc[3] = V[P_DENS] * V[P_VELZ];
-  Uc[4] = 0.5 * V[P_DENS] * u2 + V[P_PRES] / (V[P_GAME] - 1.) + 0.5 * 0.;
-  double Us[5];
-  const double dStar = Uc[0] * numer / denom;
+  Uc[4] = R(0.5) * V[P_DENS] * u2 + V[P_PRES] / (V[P_GAME] - R(1.)) + R(0.5) * R(0.);
+  R Us[5];
+  const R dStar = Uc[0] * numer / denom;
   Us[0] = dStar;
   Us[4] = Uc[4] * numer / denom + ((pStar * qStar - tp * velN)) / denom;
 #pragma unroll

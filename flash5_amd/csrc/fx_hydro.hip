@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "fx_device.hpp"
+#include "fx_mlp.hpp"
 #include "fx_hydro.h"
 
 using namespace fx;
@@ -69,44 +70,47 @@ __device__ __forceinline__ long cidx(const DevParams& P, int i, int j, int k) {
   return ((long)k * P.ny + j) * P.nx + i;
 }
 
+// primitive vector of cell c in the reconstruction type R (a conversion
+// from the FP64 cell data when R is float)
+template <typename R>
 __device__ __forceinline__ void load_prim(const DevParams& P, const double* __restrict__ U, long c,
-                                          double V[8]) {
-  V[P_DENS] = U[P.voff[FX_DENS] + c];
-  V[P_VELX] = U[P.voff[FX_VELX] + c];
-  V[P_VELY] = U[P.voff[FX_VELY] + c];
-  V[P_VELZ] = U[P.voff[FX_VELZ] + c];
-  V[P_PRES] = U[P.voff[FX_PRES] + c];
-  V[P_GAMC] = U[P.voff[FX_GAMC] + c];
-  V[P_GAME] = U[P.voff[FX_GAME] + c];
-  V[P_EINT] = U[P.voff[FX_EINT] + c];
+                                          R V[8]) {
+  V[P_DENS] = R(U[P.voff[FX_DENS] + c]);
+  V[P_VELX] = R(U[P.voff[FX_VELX] + c]);
+  V[P_VELY] = R(U[P.voff[FX_VELY] + c]);
+  V[P_VELZ] = R(U[P.voff[FX_VELZ] + c]);
+  V[P_PRES] = R(U[P.voff[FX_PRES] + c]);
+  V[P_GAMC] = R(U[P.voff[FX_GAMC] + c]);
+  V[P_GAME] = R(U[P.voff[FX_GAME] + c]);
+  V[P_EINT] = R(U[P.voff[FX_EINT] + c]);
 }
 
 // FlatTilde of one cell along DIR (hy_getRiemannState.F90:365-404)
-template <int DIR>
-__device__ __forceinline__ double flat_tilde(const DevParams& P, const double* __restrict__ U,
-                                             long c, long s) {
+template <int DIR, typename R = double>
+__device__ __forceinline__ R flat_tilde(const DevParams& P, const double* __restrict__ U, long c,
+                                        long s) {
   const double* pr = U + P.voff[FX_PRES];
   const double* vn = U + P.voff[FX_VELX + DIR];
-  const double dp1 = (pr[c + s] - pr[c - s]);
-  const double dp2 = (pr[c + 2 * s] - pr[c - 2 * s]);
-  const double dv1 = vn[c + s] - vn[c - s];
-  const double presL = pr[c - s], presR = pr[c + s];
-  const double Sp = (fabs(dp2) > 1.e-15) ? dp1 / dp2 - 0.75 : 0.;
-  double ft = fmax(0.0, fmin(1.0, 10.0 * Sp));
-  if ((fabs(dp1) / fmin(presL, presR) < 1. / 3.) || dv1 > 0.) ft = 0.;
+  const R dp1 = (R(pr[c + s]) - R(pr[c - s]));
+  const R dp2 = (R(pr[c + 2 * s]) - R(pr[c - 2 * s]));
+  const R dv1 = R(vn[c + s]) - R(vn[c - s]);
+  const R presL = R(pr[c - s]), presR = R(pr[c + s]);
+  const R Sp = (fabs(dp2) > R(1.e-15)) ? dp1 / dp2 - R(0.75) : R(0.);
+  R ft = fmax(R(0.0), fmin(R(1.0), R(10.0) * Sp));
+  if ((fabs(dp1) / fmin(presL, presR) < R(1.) / R(3.)) || dv1 > R(0.)) ft = R(0.);
   return ft;
 }
 
 // FlatCoeff (hy_getRiemannState.F90:408-453)
-template <int DIR>
-__device__ __forceinline__ double flat_coeff(const DevParams& P, const double* __restrict__ U,
-                                             long c, long s) {
+template <int DIR, typename R = double>
+__device__ __forceinline__ R flat_coeff(const DevParams& P, const double* __restrict__ U, long c,
+                                        long s) {
   const double* pr = U + P.voff[FX_PRES];
-  const double dp1 = (pr[c + s] - pr[c - s]);
-  const double f0 = flat_tilde<DIR>(P, U, c, s);
-  if (dp1 < 0.0) return fmax(f0, flat_tilde<DIR>(P, U, c + s, s));
-  if (dp1 == 0.) return f0;
-  return fmax(f0, flat_tilde<DIR>(P, U, c - s, s));
+  const R dp1 = (R(pr[c + s]) - R(pr[c - s]));
+  const R f0 = flat_tilde<DIR, R>(P, U, c, s);
+  if (dp1 < R(0.0)) return fmax(f0, flat_tilde<DIR, R>(P, U, c + s, s));
+  if (dp1 == R(0.)) return f0;
+  return fmax(f0, flat_tilde<DIR, R>(P, U, c - s, s));
 }
 
 __device__ __forceinline__ long axis_stride(const DevParams& P, int a) {
@@ -114,15 +118,17 @@ __device__ __forceinline__ long axis_stride(const DevParams& P, int a) {
 }
 
 // sigma of cell c along direction SD (the transverse-only PPM call)
-template <int SD>
+template <class T> struct NoDeduce { using type = T; };
+
+template <int SD, typename R>
 __device__ __forceinline__ bool sigma_at(const DevParams& P, const double* __restrict__ U, long c,
-                                         double sig[8], Eig* eout) {
+                                         R sig[8], typename NoDeduce<EigT<R>>::type* eout) {
   const long s = axis_stride(P, SD);
-  double Vm[8], Vc[8], Vp[8];
+  R Vm[8], Vc[8], Vp[8];
   load_prim(P, U, c - s, Vm);
   load_prim(P, U, c, Vc);
   load_prim(P, U, c + s, Vp);
-  Eig e;
+  EigT<R> e;
   if (!eig_make<SD>(Vc, e)) return false;
   trans_sigma<SD>(P, Vm, Vc, Vp, e, sig);
   if (eout) *eout = e;
@@ -290,20 +296,45 @@ __device__ __forceinline__ long band_index(const DevParams& P, int d, int i, int
   return ((long)cc * P.bext[d][1] + b) * P.bext[d][0] + a;
 }
 
-template <int D, int RS>
-__device__ __forceinline__ int face_flux(const DevParams& P, const double* VL, const double* VR,
-                                         double F[8], ExactStats& es) {
-  const int ierr = RS == FX_EXACT ? exact_flux<D>(P, VL, VR, -1.0, F, es) : hllc<D>(P, VL, VR, F);
-  if (ierr) return ierr;
+// Face flux densities F[0..7] (hy_getFaceFlux.F90:324-442): the Riemann
+// solver RS on the face states VL, VR, then the aux internal-energy fluxes.
+// HLLC runs in the reconstruction type R; the exact solver always runs its
+// Newton iteration and convergence test in FP64.
+template <int D, int RS, typename R>
+__device__ __forceinline__ int face_flux(const DevParams& P, const R* VL, const R* VR, double F[8],
+                                         ExactStats& es) {
+  R f0;
+  if constexpr (RS == FX_EXACT) {
+    double L[8], Rt[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      L[v] = double(VL[v]);
+      Rt[v] = double(VR[v]);
+    }
+    const int ierr = exact_flux<D>(P, L, Rt, -1.0, F, es);
+    if (ierr) return ierr;
+    f0 = R(F[0]);
+  } else if constexpr (sizeof(R) == sizeof(double)) {
+    const int ierr = hllc<D>(P, VL, VR, F);
+    if (ierr) return ierr;
+    f0 = F[0];
+  } else {
+    R Fr[6];
+    const int ierr = hllc<D>(P, VL, VR, Fr);
+    if (ierr) return ierr;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) F[q] = double(Fr[q]);
+    f0 = Fr[0];
+  }
   F[6] = 0.;
   F[7] = 0.;
   if (P.aux) {
-    if (F[0] > 0.) {
-      F[7] = F[0] / VL[P_DENS];
-      F[6] = F[0] * VL[P_EINT];
+    if (f0 > R(0.)) {
+      F[7] = double(f0 / VL[P_DENS]);
+      F[6] = double(f0 * VL[P_EINT]);
     } else {
-      F[7] = F[0] / VR[P_DENS];
-      F[6] = F[0] * VR[P_EINT];
+      F[7] = double(f0 / VR[P_DENS]);
+      F[6] = double(f0 * VR[P_EINT]);
     }
   }
   return 0;
@@ -380,6 +411,7 @@ __device__ __forceinline__ long face_index(const DevParams& P, int d, int i, int
 }
 
 #include "fx_march.hpp"
+#include "fx_learned.hpp"
 
 // FROM_STATES: face-averaged pressure from this call's states; otherwise
 // read from the tile's pavg array (hy_updateSolution path).
@@ -619,6 +651,11 @@ struct Lib {
   size_t dregTilesCap = 0;
   fx_fluxreg_face* dfaces = nullptr;   // face list of a register call
   size_t dfacesCap = 0;
+  MlpPacked* mlpPacked = nullptr;      // BF16 fragments for fx_mlp_* / fx_riemann_batch
+  MlpPacked* seedPacked = nullptr;     // network seeding the exact solver in hydro steps
+  bool seedReady = false;
+  float* seedY = nullptr;              // network outputs per interface / face
+  size_t seedYCap = 0;
   unsigned char* dinplace = nullptr;
   size_t dinplaceCap = 0;
   DevStatus* status = nullptr;
@@ -690,6 +727,7 @@ DevParams make_dev_params(const fx_params& p) {
   d.ctu3d = p.use3dFullCTU;
   d.aux = p.useAuxEint;
   d.riemann = p.riemann;
+  d.precision = p.precision;
   d.fP = p.fP;
   d.smlrho = p.smlrho;
   d.smallE = p.smallE;
@@ -759,6 +797,8 @@ int validate(const fx_params* p) {
     if (p->var[v] < 0 || p->var[v] >= p->nunk) return 3;
   if (!(p->gamma > 1.0)) return 3;
   if (p->riemann != FX_HLLC && p->riemann != FX_EXACT) return 3;  // HLLC or the exact solver
+  if (p->precision != FX_PREC_FP64 && p->precision != FX_PREC_MIXED) return 3;
+  if (p->learnedSeed && p->riemann != FX_EXACT) return 3;
   return 0;
 }
 
@@ -819,12 +859,20 @@ int upload_tiles(const fx_tile* tiles, int nTiles, bool needFlux, fx_status* st)
   return 0;
 }
 
+template <int RS, typename R>
+void launch_march_r(const DevParams& P, const MarchLaunch& M, int TX, dim3 grid) {
+  if (P.ndim == 3 && TX == 16) hipLaunchKernelGGL((k_march<3, 16, 8, 512, RS, R>), grid, dim3(512), 0, g.stream, P, M);
+  else if (P.ndim == 3) hipLaunchKernelGGL((k_march<3, 8, 8, 256, RS, R>), grid, dim3(256), 0, g.stream, P, M);
+  else if (TX == 16) hipLaunchKernelGGL((k_march<2, 16, 8, 256, RS, R>), grid, dim3(256), 0, g.stream, P, M);
+  else hipLaunchKernelGGL((k_march<2, 8, 8, 256, RS, R>), grid, dim3(256), 0, g.stream, P, M);
+}
+
+// FX_PREC_MIXED reconstructs and solves HLLC in FP32; the update kernel and
+// the exact solver's Newton iteration stay FP64
 template <int RS>
 void launch_march(const DevParams& P, const MarchLaunch& M, int TX, dim3 grid) {
-  if (P.ndim == 3 && TX == 16) hipLaunchKernelGGL((k_march<3, 16, 8, 512, RS>), grid, dim3(512), 0, g.stream, P, M);
-  else if (P.ndim == 3) hipLaunchKernelGGL((k_march<3, 8, 8, 256, RS>), grid, dim3(256), 0, g.stream, P, M);
-  else if (TX == 16) hipLaunchKernelGGL((k_march<2, 16, 8, 256, RS>), grid, dim3(256), 0, g.stream, P, M);
-  else hipLaunchKernelGGL((k_march<2, 8, 8, 256, RS>), grid, dim3(256), 0, g.stream, P, M);
+  if (P.precision == FX_PREC_MIXED) launch_march_r<RS, float>(P, M, TX, grid);
+  else launch_march_r<RS, double>(P, M, TX, grid);
 }
 
 int record(int n) {
@@ -962,6 +1010,39 @@ int run_fluxreg(const fx_tile* tiles, int nTiles, const fx_fluxreg_face* faces, 
   return 0;
 }
 
+// ---- learned seed: host helpers ----
+int ensure_status(fx_status* st) {
+  if (g.status) return 0;
+  HIPCALL(hipMalloc((void**)&g.status, sizeof(DevStatus)), st);
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    g.numCU = prop.multiProcessorCount;
+  return 0;
+}
+
+int pack_mlp(const float* params, MlpPacked** dst, fx_status* st) {
+  if (!*dst) HIPCALL(hipMalloc((void**)dst, sizeof(MlpPacked)), st);
+  hipLaunchKernelGGL(k_mlp_pack, dim3(2), dim3(256), 0, g.stream, params, *dst);
+  HIPCALL(hipGetLastError(), st);
+  return 0;
+}
+
+dim3 wave_grid(long n) {
+  const long chunks = (n + 63) / 64;
+  long blocks = (chunks + kLearnWaves - 1) / kLearnWaves;
+  const long cap = (long)g.numCU * 8;
+  if (blocks > cap) blocks = cap;
+  return dim3((unsigned)(blocks < 1 ? 1 : blocks));
+}
+
+int sync_or_async(fx_status* st) {
+  HIPCALL(hipGetLastError(), st);
+  if (g.async) return 0;
+  HIPCALL(hipStreamSynchronize(g.stream), st);
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -985,7 +1066,7 @@ int fx_hydro_init(const fx_params* p) {
 
 void fx_hydro_finalize(void) {
   void* bufs[] = {g.states, g.flux, g.pavg, g.dtiles, g.dinplace, g.status, g.partial,
-                  g.dregTiles, g.dfaces};
+                  g.dregTiles, g.dfaces, g.mlpPacked, g.seedPacked, g.seedY};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : g.ev)
@@ -1076,6 +1157,90 @@ int fx_fluxreg_put(const fx_tile* fine, int nFine, const fx_fluxreg_face* faces,
 int fx_fluxreg_conserve(const fx_tile* coarse, int nCoarse, const fx_fluxreg_face* faces,
                         int nFaces, const double* reg, fx_status* st) {
   return run_fluxreg(coarse, nCoarse, faces, nFaces, const_cast<double*>(reg), false, st);
+}
+
+int fx_mlp_features(const double* states, long n, double gamma, float* x) {
+  if (n <= 0) return 0;
+  if (!states || !x || !(gamma > 1.0)) return 3;
+  if (ensure_status(nullptr)) return 4;
+  hipLaunchKernelGGL(k_mlp_features, dim3(grid_for(n)), dim3(kThreads), 0, g.stream, states, n,
+                     gamma, x);
+  return sync_or_async(nullptr);
+}
+
+int fx_mlp_forward(const float* params, const float* x, long n, float* y) {
+  if (n <= 0) return 0;
+  if (!params || !x || !y) return 3;
+  if (ensure_status(nullptr) || pack_mlp(params, &g.mlpPacked, nullptr)) return 4;
+  hipLaunchKernelGGL(k_mlp_forward, wave_grid(n), dim3(kLearnThreads), 0, g.stream, g.mlpPacked,
+                     x, n, y);
+  return sync_or_async(nullptr);
+}
+
+int fx_mlp_grad(const float* params, const float* x, const float* t, long n, float scale,
+                float* grad, float* loss) {
+  if (n <= 0) return 0;
+  if (!params || !x || !t || !grad || !loss) return 3;
+  if (ensure_status(nullptr) || pack_mlp(params, &g.mlpPacked, nullptr)) return 4;
+  hipLaunchKernelGGL(k_mlp_grad, wave_grid(n), dim3(kLearnThreads), 0, g.stream, g.mlpPacked, x,
+                     t, n, scale, grad, loss);
+  return sync_or_async(nullptr);
+}
+
+int fx_mlp_adam(float* params, const float* grad, float* m, float* v, int step, float lr) {
+  if (!params || !grad || !m || !v || step < 1) return 3;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+  hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_NPARAM + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                     g.stream, params, grad, m, v, lr, b1, b2, c1, c2, 1e-8f);
+  return sync_or_async(nullptr);
+}
+
+int fx_riemann_batch(const double* states, long n, double gamma, const float* params, int maxit,
+                     double tol, double* pstar, int* iters, fx_status* st) {
+  if (st) memset(st, 0, sizeof *st);
+  if (n <= 0) return 0;
+  if (!states || !pstar || !iters || !(gamma > 1.0) || maxit < 1 || !(tol > 0.0)) {
+    if (st) st->code = 3;
+    return 3;
+  }
+  if (ensure_status(st)) return 4;
+  hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
+  if (params) {
+    if (pack_mlp(params, &g.mlpPacked, st)) return 4;
+    if (grow(&g.seedY, &g.seedYCap, (size_t)n, st)) return 4;
+    if (record(0)) return 4;
+    hipLaunchKernelGGL(k_mlp_seed_states, wave_grid(n), dim3(kLearnThreads), 0, g.stream, states,
+                       n, gamma, g.mlpPacked, g.seedY);
+    if (record(1)) return 4;
+    hipLaunchKernelGGL(k_riemann_batch<true>, dim3(grid_for(n)), dim3(kThreads), 0, g.stream,
+                       states, n, gamma, g.seedY, tol, maxit, pstar, iters, g.status);
+    if (record(2)) return 4;
+  } else {
+    if (record(0) || record(1)) return 4;
+    hipLaunchKernelGGL(k_riemann_batch<false>, dim3(grid_for(n)), dim3(kThreads), 0, g.stream,
+                       states, n, gamma, (const float*)nullptr, tol, maxit, pstar, iters, g.status);
+    if (record(2)) return 4;
+  }
+  if (record(3)) return 4;
+  HIPCALL(hipGetLastError(), st);
+  int rc = finish(st);
+  if (g.timing && !g.async)
+    for (int k = 0; k < 3; ++k)
+      if (hipEventElapsedTime(&g.stageMs[k], g.ev[k], g.ev[k + 1]) != hipSuccess) g.stageMs[k] = -1.f;
+  return rc;
+}
+
+int fx_hydro_set_seed_params(const float* params) {
+  if (ensure_status(nullptr)) return 4;
+  if (!params) {
+    g.seedReady = false;
+    return 0;
+  }
+  if (pack_mlp(params, &g.seedPacked, nullptr)) return 4;
+  if (hipStreamSynchronize(g.stream) != hipSuccess) return 4;
+  g.seedReady = true;
+  return 0;
 }
 
 void fx_hydro_set_stream(void* stream) { g.stream = (hipStream_t)stream; }

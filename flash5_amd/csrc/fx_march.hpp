@@ -46,38 +46,41 @@ struct MarchLaunch {
   double dt;
 };
 
-template <int SD>
+template <int SD, typename R>
 __device__ __forceinline__ bool sigma_cell(const DevParams& P, const double* __restrict__ U, long c,
-                                           double sig[8]) {
+                                           R sig[8]) {
   return sigma_at<SD>(P, U, c, sig, nullptr);
 }
 
-template <int DIR>
+template <int DIR, typename R>
 __device__ __forceinline__ bool slope_cell(const DevParams& P, const double* __restrict__ U, long c,
-                                           double db[8]) {
+                                           R db[8]) {
   const long s = axis_stride(P, DIR);
-  double Vm[8], Vc[8], Vp[8];
+  R Vm[8], Vc[8], Vp[8];
   load_prim(P, U, c - s, Vm);
   load_prim(P, U, c, Vc);
   load_prim(P, U, c + s, Vp);
-  Eig e;
+  EigT<R> e;
   if (!eig_make<DIR>(Vc, e)) return false;
   tvd_slope<DIR>(P, Vm, Vc, Vp, e, db);
   return true;
 }
 
-__device__ __forceinline__ void lds_put6(double* base, int n, int idx, const double v[8]) {
+template <typename R>
+__device__ __forceinline__ void lds_put6(R* base, int n, int idx, const R v[8]) {
 #pragma unroll
   for (int q = 0; q < 6; ++q) base[q * n + idx] = v[comp_slot(q)];
 }
-__device__ __forceinline__ void lds_get6(const double* base, int n, int idx, double v[8]) {
+template <typename R>
+__device__ __forceinline__ void lds_get6(const R* base, int n, int idx, R v[8]) {
 #pragma unroll
   for (int q = 0; q < 6; ++q) v[comp_slot(q)] = base[q * n + idx];
-  v[P_GAMC] = 0.;
-  v[P_GAME] = 0.;
+  v[P_GAMC] = R(0.);
+  v[P_GAME] = R(0.);
 }
 
-__device__ __forceinline__ void fallback_first_order(const double* Vc, double Wp[8], double Wn[8],
+template <typename R>
+__device__ __forceinline__ void fallback_first_order(const R* Vc, R Wp[8], R Wn[8],
                                                      DevStatus* st) {
   atomicAdd(&st->fallbacks, 1ULL);
 #pragma unroll
@@ -87,7 +90,8 @@ __device__ __forceinline__ void fallback_first_order(const double* Vc, double Wp
   }
 }
 
-__device__ __forceinline__ void sub_sigma(double Wp[8], double Wn[8], const double* sg, double hdt) {
+template <typename R>
+__device__ __forceinline__ void sub_sigma(R Wp[8], R Wn[8], const R* sg, R hdt) {
 #pragma unroll
   for (int v = 0; v < 8; ++v) {
     Wn[v] = Wn[v] - hdt * sg[v];
@@ -95,8 +99,8 @@ __device__ __forceinline__ void sub_sigma(double Wp[8], double Wn[8], const doub
   }
 }
 
-__device__ __forceinline__ void add_ctu(double Wp[8], double Wn[8], const double* tA,
-                                        const double* tB, double fac) {
+template <typename R>
+__device__ __forceinline__ void add_ctu(R Wp[8], R Wn[8], const R* tA, const R* tB, R fac) {
 #pragma unroll
   for (int v = 0; v < 8; ++v) {
     Wp[v] = Wp[v] + (tA[v] + tB[v]) * fac;
@@ -104,16 +108,18 @@ __device__ __forceinline__ void add_ctu(double Wp[8], double Wn[8], const double
   }
 }
 
-__device__ __forceinline__ bool nonpositive(const double* Wp, const double* Wn) {
-  return Wp[P_DENS] <= 0. || Wn[P_DENS] <= 0. || Wp[P_PRES] <= 0. || Wn[P_PRES] <= 0.;
+template <typename R>
+__device__ __forceinline__ bool nonpositive(const R* Wp, const R* Wn) {
+  return Wp[P_DENS] <= R(0.) || Wn[P_DENS] <= R(0.) || Wp[P_PRES] <= R(0.) || Wn[P_PRES] <= R(0.);
 }
-__device__ __forceinline__ bool negative(const double* Wp, const double* Wn) {
-  return Wn[P_DENS] < 0. || Wp[P_DENS] < 0. || Wn[P_PRES] < 0. || Wp[P_PRES] < 0.;
+template <typename R>
+__device__ __forceinline__ bool negative(const R* Wp, const R* Wn) {
+  return Wn[P_DENS] < R(0.) || Wp[P_DENS] < R(0.) || Wn[P_PRES] < R(0.) || Wp[P_PRES] < R(0.);
 }
 
 // store/load a face-state pair (Wp, Wn) of 6 components each
-__device__ __forceinline__ void put_state(double* base, int n, int idx, const double Wp[8],
-                                          const double Wn[8]) {
+template <typename R>
+__device__ __forceinline__ void put_state(R* base, int n, int idx, const R Wp[8], const R Wn[8]) {
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
     base[q * n + idx] = Wp[comp_slot(q)];
@@ -128,44 +134,44 @@ __device__ __forceinline__ void write_flux(const DevParams& P, double* Fo, long 
   for (int q = 0; q < FX_NFLUX; ++q) Fo[q * nf + r] = F[q];
 }
 
-__device__ __forceinline__ void get_state(const double* base, int n, int idx, double Wp[8],
-                                          double Wn[8]) {
+template <typename R>
+__device__ __forceinline__ void get_state(const R* base, int n, int idx, R Wp[8], R Wn[8]) {
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
     Wp[comp_slot(q)] = base[q * n + idx];
     Wn[comp_slot(q)] = base[(6 + q) * n + idx];
   }
-  Wp[P_GAMC] = Wp[P_GAME] = Wn[P_GAMC] = Wn[P_GAME] = 0.;
+  Wp[P_GAMC] = Wp[P_GAME] = Wn[P_GAMC] = Wn[P_GAME] = R(0.);
 }
 
-__device__ __forceinline__ double flat_tilde_vals(double pm2, double pm1, double pp1, double pp2,
-                                                  double vm1, double vp1) {
-  const double dp1 = (pp1 - pm1);
-  const double dp2 = (pp2 - pm2);
-  const double dv1 = vp1 - vm1;
-  const double Sp = (fabs(dp2) > 1.e-15) ? dp1 / dp2 - 0.75 : 0.;
-  double ft = fmax(0.0, fmin(1.0, 10.0 * Sp));
-  if ((fabs(dp1) / fmin(pm1, pp1) < 1. / 3.) || dv1 > 0.) ft = 0.;
+template <typename R>
+__device__ __forceinline__ R flat_tilde_vals(R pm2, R pm1, R pp1, R pp2, R vm1, R vp1) {
+  const R dp1 = (pp1 - pm1);
+  const R dp2 = (pp2 - pm2);
+  const R dv1 = vp1 - vm1;
+  const R Sp = (fabs(dp2) > R(1.e-15)) ? dp1 / dp2 - R(0.75) : R(0.);
+  R ft = fmax(R(0.0), fmin(R(1.0), R(10.0) * Sp));
+  if ((fabs(dp1) / fmin(pm1, pp1) < R(1.) / R(3.)) || dv1 > R(0.)) ft = R(0.);
   return ft;
 }
 
-template <int NDIM, int TX, int TY, int NT, int RS>
+template <int NDIM, int TX, int TY, int NT, int RS, typename R>
 __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
   using I = TileIdx<TX, TY>;
   constexpr int NR = NDIM == 3 ? 3 : 1;        // ring depth
   constexpr int NCOL = TX * TY;
   constexpr int Z = NDIM == 3 ? 1 : 0;
-  __shared__ double SX[NR * 6 * I::NRX];        // sigma_x ring
-  __shared__ double SY[NR * 6 * I::NRY];        // sigma_y ring
-  __shared__ double SZ[Z ? 6 * I::NC : 1];      // sigma_z of plane k
-  __shared__ double LX[6 * I::NLX];             // slopes along x, plane k
-  __shared__ double LY[6 * I::NLY];             // slopes along y, plane k
-  __shared__ double LZ[Z ? 3 * 6 * NCOL : 1];   // slopes along z, ring
-  __shared__ double FZ[Z ? 3 * NCOL : 1];       // flattening tilde along z, ring
-  __shared__ double WX[12 * I::NRY];            // x-face states (Wp, Wn)
-  __shared__ double WY[12 * I::NRX];            // y-face states
-  __shared__ double WZ[Z ? 12 * NCOL : 1];      // z-face states of plane k
-  __shared__ double WZP[Z ? 6 * NCOL : 1];      // z-face Wp of plane k-1
+  __shared__ R SX[NR * 6 * I::NRX];        // sigma_x ring
+  __shared__ R SY[NR * 6 * I::NRY];        // sigma_y ring
+  __shared__ R SZ[Z ? 6 * I::NC : 1];      // sigma_z of plane k
+  __shared__ R LX[6 * I::NLX];             // slopes along x, plane k
+  __shared__ R LY[6 * I::NLY];             // slopes along y, plane k
+  __shared__ R LZ[Z ? 3 * 6 * NCOL : 1];   // slopes along z, ring
+  __shared__ R FZ[Z ? 3 * NCOL : 1];       // flattening tilde along z, ring
+  __shared__ R WX[12 * I::NRY];            // x-face states (Wp, Wn)
+  __shared__ R WY[12 * I::NRX];            // y-face states
+  __shared__ R WZ[Z ? 12 * NCOL : 1];      // z-face states of plane k
+  __shared__ R WZP[Z ? 6 * NCOL : 1];      // z-face Wp of plane k-1
 
   const int tid = threadIdx.x;
   const int tilesPerBlock = L.tilesX * L.tilesY;
@@ -192,7 +198,7 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
 
   // stage-A item: sigma_x / sigma_y of plane kk (w < NRX + NRY)
   auto sig_xy_item = [&](int w, int kk) {
-    double sg[8];
+    R sg[8];
     if (w < I::NRX) {
       const int a = w % TX, b = w / TX - 1;
       if (!sigma_at<0>(P, U, cidx(P, ox + a, oy + b, kk), sg, nullptr)) { bad = true; return; }
@@ -208,13 +214,13 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
   auto zslope_item = [&](int w, int m) {
     const int a = w % TX, b = w / TX;
     const long c = cidx(P, ox + a, oy + b, m);
-    double db[8];
+    R db[8];
     if (!slope_cell<2>(P, U, c, db)) { bad = true; return; }
     lds_put6(LZ + ring(m) * 6 * NCOL, NCOL, w, db);
     const double* pr = U + P.voff[FX_PRES];
     const double* vz = U + P.voff[FX_VELZ];
-    FZ[ring(m) * NCOL + w] =
-        flat_tilde_vals(pr[c - 2 * sz], pr[c - sz], pr[c + sz], pr[c + 2 * sz], vz[c - sz], vz[c + sz]);
+    FZ[ring(m) * NCOL + w] = flat_tilde_vals(R(pr[c - 2 * sz]), R(pr[c - sz]), R(pr[c + sz]),
+                                             R(pr[c + 2 * sz]), R(vz[c - sz]), R(vz[c + sz]));
   };
 
   const int kBeg = Z ? klo - 1 : 0;
@@ -242,7 +248,7 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
         if (r < nSig) { sig_xy_item(r, Z ? k + 1 : 0); continue; }
         r -= nSig;
         if (r < nSz) {
-          double v8[8];
+          R v8[8];
           const int a = r % (TX + 2) - 1, b = r / (TX + 2) - 1;
           if (!sigma_at<2>(P, U, cidx(P, ox + a, oy + b, k), v8, nullptr)) { bad = true; continue; }
           lds_put6(SZ, I::NC, r, v8);
@@ -250,7 +256,7 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
         }
         r -= nSz;
         if (r < nL) {
-          double v8[8];
+          R v8[8];
           if (r < I::NLX) {
             const int a = r % (TX + 4) - 2, b = r / (TX + 4);
             if (!slope_cell<0>(P, U, cidx(P, ox + a, oy + b, k), v8)) { bad = true; continue; }
@@ -269,36 +275,36 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
     }
     __syncthreads();
 
-    const double* sxk = SX + ring(k) * 6 * I::NRX;
-    const double* syk = SY + ring(k) * 6 * I::NRY;
+    const R* sxk = SX + ring(k) * 6 * I::NRX;
+    const R* syk = SY + ring(k) * 6 * I::NRY;
     // ------------------------------ stage B1 -----------------------------
     {
       const int nX = interior ? (TX + 1 + ex) * TY : 0;
       const int nY = interior ? TX * (TY + 1 + ey) : 0;
       const int nZ = Z ? NCOL : 0;
       for (int w = tid; w < nX + nY + nZ; w += NT) {
-        double Wp[8], Wn[8], Vm[8], Vc[8], Vp[8], dbN[8], db0[8], dbP[8];
-        Eig e0;
+        R Wp[8], Wn[8], Vm[8], Vc[8], Vp[8], dbN[8], db0[8], dbP[8];
+        EigT<R> e0;
         if (w < nX) {
           const int a = w % (TX + 1 + ex) - 1, b = w / (TX + 1 + ex);
           const long c = cidx(P, ox + a, oy + b, k);
           load_prim(P, U, c - sx, Vm);
           load_prim(P, U, c, Vc);
           load_prim(P, U, c + sx, Vp);
-          const double flat = P.flat ? flat_coeff<0>(P, U, c, sx) : 0.;
+          const R flat = P.flat ? flat_coeff<0, R>(P, U, c, sx) : R(0.);
           lds_get6(LX, I::NLX, I::lx(a - 1, b), dbN);
           lds_get6(LX, I::NLX, I::lx(a, b), db0);
           lds_get6(LX, I::NLX, I::lx(a + 1, b), dbP);
           if (!eig_make<0>(Vc, e0)) { bad = true; continue; }
-          const double dmm = P.steep ? U[P.voff[FX_DENS] + c - 2 * sx] : 0.;
-          const double dpp = P.steep ? U[P.voff[FX_DENS] + c + 2 * sx] : 0.;
-          ppm_trace<0>(P, dt / dx, dx, Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
-          double sg[8];
+          const R dmm = P.steep ? R(U[P.voff[FX_DENS] + c - 2 * sx]) : R(0.);
+          const R dpp = P.steep ? R(U[P.voff[FX_DENS] + c + 2 * sx]) : R(0.);
+          ppm_trace<0>(P, R(dt / dx), R(dx), Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
+          R sg[8];
           lds_get6(syk, I::NRY, I::ry(a, b), sg);
-          sub_sigma(Wp, Wn, sg, 0.5 * dt / dy);
+          sub_sigma(Wp, Wn, sg, R(0.5 * dt / dy));
           if (Z) {
             lds_get6(SZ, I::NC, I::cc(a, b), sg);
-            sub_sigma(Wp, Wn, sg, 0.5 * dt / dz);
+            sub_sigma(Wp, Wn, sg, R(0.5 * dt / dz));
           }
           if (negative(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st);
           put_state(WX, I::NRY, I::ry(a, b), Wp, Wn);
@@ -309,20 +315,20 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
           load_prim(P, U, c - sy, Vm);
           load_prim(P, U, c, Vc);
           load_prim(P, U, c + sy, Vp);
-          const double flat = P.flat ? flat_coeff<1>(P, U, c, sy) : 0.;
+          const R flat = P.flat ? flat_coeff<1, R>(P, U, c, sy) : R(0.);
           lds_get6(LY, I::NLY, I::ly(a, b - 1), dbN);
           lds_get6(LY, I::NLY, I::ly(a, b), db0);
           lds_get6(LY, I::NLY, I::ly(a, b + 1), dbP);
           if (!eig_make<1>(Vc, e0)) { bad = true; continue; }
-          const double dmm = P.steep ? U[P.voff[FX_DENS] + c - 2 * sy] : 0.;
-          const double dpp = P.steep ? U[P.voff[FX_DENS] + c + 2 * sy] : 0.;
-          ppm_trace<1>(P, dt / dy, dy, Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
-          double sg[8];
+          const R dmm = P.steep ? R(U[P.voff[FX_DENS] + c - 2 * sy]) : R(0.);
+          const R dpp = P.steep ? R(U[P.voff[FX_DENS] + c + 2 * sy]) : R(0.);
+          ppm_trace<1>(P, R(dt / dy), R(dy), Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
+          R sg[8];
           lds_get6(sxk, I::NRX, I::rx(a, b), sg);
-          sub_sigma(Wp, Wn, sg, 0.5 * dt / dx);
+          sub_sigma(Wp, Wn, sg, R(0.5 * dt / dx));
           if (Z) {
             lds_get6(SZ, I::NC, I::cc(a, b), sg);
-            sub_sigma(Wp, Wn, sg, 0.5 * dt / dz);
+            sub_sigma(Wp, Wn, sg, R(0.5 * dt / dz));
           }
           if (negative(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st);
           put_state(WY, I::NRX, I::rx(a, b), Wp, Wn);
@@ -333,25 +339,25 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
           load_prim(P, U, c - sz, Vm);
           load_prim(P, U, c, Vc);
           load_prim(P, U, c + sz, Vp);
-          double flat = 0.;
+          R flat = R(0.);
           if (P.flat) {
-            const double dp1 = Vp[P_PRES] - Vm[P_PRES];
-            const double f0 = FZ[ring(k) * NCOL + w2];
-            flat = dp1 < 0.0 ? fmax(f0, FZ[ring(k + 1) * NCOL + w2])
-                             : (dp1 == 0. ? f0 : fmax(f0, FZ[ring(k - 1) * NCOL + w2]));
+            const R dp1 = Vp[P_PRES] - Vm[P_PRES];
+            const R f0 = FZ[ring(k) * NCOL + w2];
+            flat = dp1 < R(0.0) ? fmax(f0, FZ[ring(k + 1) * NCOL + w2])
+                                : (dp1 == R(0.) ? f0 : fmax(f0, FZ[ring(k - 1) * NCOL + w2]));
           }
           lds_get6(LZ + ring(k - 1) * 6 * NCOL, NCOL, w2, dbN);
           lds_get6(LZ + ring(k) * 6 * NCOL, NCOL, w2, db0);
           lds_get6(LZ + ring(k + 1) * 6 * NCOL, NCOL, w2, dbP);
           if (!eig_make<2>(Vc, e0)) { bad = true; continue; }
-          const double dmm = P.steep ? U[P.voff[FX_DENS] + c - 2 * sz] : 0.;
-          const double dpp = P.steep ? U[P.voff[FX_DENS] + c + 2 * sz] : 0.;
-          ppm_trace<2>(P, dt / dz, dz, Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
-          double sg[8];
+          const R dmm = P.steep ? R(U[P.voff[FX_DENS] + c - 2 * sz]) : R(0.);
+          const R dpp = P.steep ? R(U[P.voff[FX_DENS] + c + 2 * sz]) : R(0.);
+          ppm_trace<2>(P, R(dt / dz), R(dz), Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
+          R sg[8];
           lds_get6(sxk, I::NRX, I::rx(a, b), sg);
-          sub_sigma(Wp, Wn, sg, 0.5 * dt / dx);
+          sub_sigma(Wp, Wn, sg, R(0.5 * dt / dx));
           lds_get6(syk, I::NRY, I::ry(a, b), sg);
-          sub_sigma(Wp, Wn, sg, 0.5 * dt / dy);
+          sub_sigma(Wp, Wn, sg, R(0.5 * dt / dy));
           if (negative(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st);
           put_state(WZ, NCOL, w2, Wp, Wn);
         }
@@ -364,10 +370,10 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
       const int nX = interior ? (TX + 1 + ex) * TY : 0;
       const int nY = interior ? TX * (TY + 1 + ey) : 0;
       for (int w = tid; w < nX + nY + NCOL; w += NT) {
-        double Wp[8], Wn[8], Vc[8], sm[8], sc[8], sp[8], tA[8], tB[8];
-        Eig e1, e2;
-        double fac;
-        double* base;
+        R Wp[8], Wn[8], Vc[8], sm[8], sc[8], sp[8], tA[8], tB[8];
+        EigT<R> e1, e2;
+        R fac;
+        R* base;
         int n, idx;
         if (w < nX) {
           // X: sigma_z over j with the y eigen system, sigma_y over k with z's
@@ -382,7 +388,7 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
           lds_get6(syk, I::NRY, I::ry(a, b), sc);
           lds_get6(SY + ring(k + 1) * 6 * I::NRY, I::NRY, I::ry(a, b), sp);
           trans_sigma<2>(P, sm, sc, sp, e2, tB);
-          fac = dt * dt / (6. * dy * dz);
+          fac = R(dt * dt / (6. * dy * dz));
           base = WX; n = I::NRY; idx = I::ry(a, b);
         } else if (w < nX + nY) {
           // Y: sigma_x over k with z's eigen system, sigma_z over i with x's
@@ -398,7 +404,7 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
           lds_get6(SZ, I::NC, I::cc(a, b), sc);
           lds_get6(SZ, I::NC, I::cc(a + 1, b), sp);
           trans_sigma<0>(P, sm, sc, sp, e2, tB);
-          fac = dt * dt / (6. * dz * dx);
+          fac = R(dt * dt / (6. * dz * dx));
           base = WY; n = I::NRX; idx = I::rx(a, b);
         } else {
           // Z: sigma_y over i with x's eigen system, sigma_x over j with y's
@@ -414,7 +420,7 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
           lds_get6(sxk, I::NRX, I::rx(a, b), sc);
           lds_get6(sxk, I::NRX, I::rx(a, b + 1), sp);
           trans_sigma<1>(P, sm, sc, sp, e2, tB);
-          fac = dt * dt / (6. * dx * dy);
+          fac = R(dt * dt / (6. * dx * dy));
           base = WZ; n = NCOL; idx = w2;
         }
         get_state(base, n, idx, Wp, Wn);
@@ -431,16 +437,17 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
       const int nFY = interior ? TX * (TY + ey) : 0;
       const int nC = Z ? NCOL : (interior ? NCOL : 0);
       for (int w = tid; w < nFX + nFY + nC; w += NT) {
-        double VL[8], VR[8], F[8];
+        R VL[8], VR[8];
+        double F[8];
         if (w < nFX) {
           const int a = w % (TX + ex), b = w / (TX + ex);
           lds_get6(WX, I::NRY, I::ry(a - 1, b), VL);
           lds_get6(WX + 6 * I::NRY, I::NRY, I::ry(a, b), VR);
           const long c = cidx(P, ox + a, oy + b, k);
-          VL[P_GAMC] = U[P.voff[FX_GAMC] + c - sx];
-          VL[P_GAME] = U[P.voff[FX_GAME] + c - sx];
-          VR[P_GAMC] = U[P.voff[FX_GAMC] + c];
-          VR[P_GAME] = U[P.voff[FX_GAME] + c];
+          VL[P_GAMC] = R(U[P.voff[FX_GAMC] + c - sx]);
+          VL[P_GAME] = R(U[P.voff[FX_GAME] + c - sx]);
+          VR[P_GAMC] = R(U[P.voff[FX_GAMC] + c]);
+          VR[P_GAME] = R(U[P.voff[FX_GAME] + c]);
           if (face_flux<0, RS>(P, VL, VR, F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 0); continue; }
           write_flux<0>(P, T.flux[0], face_index(P, 0, ox + a, oy + b, k), F);
         } else if (w < nFX + nFY) {
@@ -449,10 +456,10 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
           lds_get6(WY, I::NRX, I::rx(a, b - 1), VL);
           lds_get6(WY + 6 * I::NRX, I::NRX, I::rx(a, b), VR);
           const long c = cidx(P, ox + a, oy + b, k);
-          VL[P_GAMC] = U[P.voff[FX_GAMC] + c - sy];
-          VL[P_GAME] = U[P.voff[FX_GAME] + c - sy];
-          VR[P_GAMC] = U[P.voff[FX_GAMC] + c];
-          VR[P_GAME] = U[P.voff[FX_GAME] + c];
+          VL[P_GAMC] = R(U[P.voff[FX_GAMC] + c - sy]);
+          VL[P_GAME] = R(U[P.voff[FX_GAME] + c - sy]);
+          VR[P_GAMC] = R(U[P.voff[FX_GAMC] + c]);
+          VR[P_GAME] = R(U[P.voff[FX_GAME] + c]);
           if (face_flux<1, RS>(P, VL, VR, F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 1); continue; }
           write_flux<1>(P, T.flux[1], face_index(P, 1, ox + a, oy + b, k), F);
         } else {
@@ -463,21 +470,21 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
           if (Z && k >= klo) {
             lds_get6(WZP, NCOL, w2, VL);
             lds_get6(WZ + 6 * NCOL, NCOL, w2, VR);
-            VL[P_GAMC] = U[P.voff[FX_GAMC] + c - sz];
-            VL[P_GAME] = U[P.voff[FX_GAME] + c - sz];
-            VR[P_GAMC] = U[P.voff[FX_GAMC] + c];
-            VR[P_GAME] = U[P.voff[FX_GAME] + c];
+            VL[P_GAMC] = R(U[P.voff[FX_GAMC] + c - sz]);
+            VL[P_GAME] = R(U[P.voff[FX_GAME] + c - sz]);
+            VR[P_GAMC] = R(U[P.voff[FX_GAMC] + c]);
+            VR[P_GAME] = R(U[P.voff[FX_GAME] + c]);
             if (face_flux<2, RS>(P, VL, VR, F, es)) report_error(st, 2, t, ox + a, oy + b, k, 2);
             else write_flux<2>(P, T.flux[2], face_index(P, 2, ox + a, oy + b, k), F);
           }
           if (interior && P.aux) {
             // face-averaged pressure (hy_getFaceFlux.F90:507-561 and the
             // y, z analogues)
-            double pres = 0.5 * (WX[10 * I::NRY + I::ry(a, b)] + WX[4 * I::NRY + I::ry(a, b)]);
-            pres = pres * 0.5 + (WY[10 * I::NRX + I::rx(a, b)] + WY[4 * I::NRX + I::rx(a, b)]) * 0.25;
-            if (Z) pres = (pres * 4.0 + (WZ[10 * NCOL + w2] + WZ[4 * NCOL + w2])) / 6.0;
+            R pres = R(0.5) * (WX[10 * I::NRY + I::ry(a, b)] + WX[4 * I::NRY + I::ry(a, b)]);
+            pres = pres * R(0.5) + (WY[10 * I::NRX + I::rx(a, b)] + WY[4 * I::NRX + I::rx(a, b)]) * R(0.25);
+            if (Z) pres = (pres * R(4.0) + (WZ[10 * NCOL + w2] + WZ[4 * NCOL + w2])) / R(6.0);
             const long r = ((long)(k - P.lo[2]) * P.nyb + (oy + b - P.lo[1])) * P.nxb + (ox + a - P.lo[0]);
-            T.pavg[r] = pres;
+            T.pavg[r] = double(pres);
           }
           if (Z) {
 #pragma unroll

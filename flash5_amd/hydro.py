@@ -90,6 +90,16 @@ EXPORTS = {
                         C.POINTER(FxStatus)], C.c_int),
     "fx_fluxreg_conserve": ([C.POINTER(FxTile), C.c_int, C.POINTER(FxFluxRegFace), C.c_int, _DP,
                              C.POINTER(FxStatus)], C.c_int),
+    "fx_mlp_features": ([_DP, C.c_long, C.c_double, C.POINTER(C.c_float)], C.c_int),
+    "fx_mlp_forward": ([C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_long,
+                        C.POINTER(C.c_float)], C.c_int),
+    "fx_mlp_grad": ([C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_long,
+                     C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)], C.c_int),
+    "fx_mlp_adam": ([C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                     C.POINTER(C.c_float), C.c_int, C.c_float], C.c_int),
+    "fx_riemann_batch": ([_DP, C.c_long, C.c_double, C.POINTER(C.c_float), C.c_int, C.c_double,
+                          _DP, C.POINTER(C.c_int), C.POINTER(FxStatus)], C.c_int),
+    "fx_hydro_set_seed_params": ([C.POINTER(C.c_float)], C.c_int),
     "fx_hydro_set_stream": ([C.c_void_p], None),
     "fx_hydro_set_async": ([C.c_int], None),
     "fx_hydro_set_timing": ([C.c_int], None),

@@ -1,0 +1,153 @@
+"""Learned Newton seed for the exact Riemann solver (no counterpart in the
+reference): sampling of Riemann problems, training of the 6->64->64->1 seed
+network with the library's own BF16-MFMA forward/backward kernels, and the
+batched exact solver with or without the seed.
+
+All compute runs in libfx_hydro.so on the GPU; torch only holds device
+buffers and draws random numbers.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+from .hydro import FxError, FxStatus, load_library
+
+NPARAM = 4673
+OFF = {"W1": 0, "b1": 384, "W2": 448, "b2": 4544, "w3": 4608, "b3": 4672}
+
+
+def _fp(t):
+    import torch
+    assert t.is_cuda and t.is_contiguous() and t.dtype == torch.float32
+    return C.cast(C.c_void_p(t.data_ptr()), C.POINTER(C.c_float))
+
+
+def _dp(t):
+    import torch
+    assert t.is_cuda and t.is_contiguous() and t.dtype == torch.float64
+    return C.cast(C.c_void_p(t.data_ptr()), C.POINTER(C.c_double))
+
+
+def _ip(t):
+    import torch
+    assert t.is_cuda and t.is_contiguous() and t.dtype == torch.int32
+    return C.cast(C.c_void_p(t.data_ptr()), C.POINTER(C.c_int))
+
+
+def sample_interfaces(n: int, seed: int, gamma: float = 1.4, device="cuda"):
+    """Riemann problems of the learned-seed study: rho and p log-uniform on
+    [1e-3, 1e3], normal velocities uniform in [-3, 3] times the local sound
+    speed.  Returns a (6, n) float64 tensor rhoL, uL, pL, rhoR, uR, pR."""
+    import torch
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    u = torch.rand((6, n), generator=gen, device=device, dtype=torch.float64)
+    S = torch.empty((6, n), device=device, dtype=torch.float64)
+    for side in (0, 3):
+        S[side] = 10.0 ** (6.0 * u[side] - 3.0)                 # rho
+        S[side + 2] = 10.0 ** (6.0 * u[side + 2] - 3.0)         # p
+        c = torch.sqrt(gamma * S[side + 2] / S[side])
+        S[side + 1] = (6.0 * u[side + 1] - 3.0) * c             # u
+    return S.contiguous()
+
+
+def init_params(seed: int = 0, device="cuda"):
+    """Glorot-uniform weights, zero biases, FP32 master copy."""
+    import torch
+    gen = torch.Generator(device="cpu")
+    gen.manual_seed(seed)
+    p = torch.zeros(NPARAM, dtype=torch.float32)
+    def glorot(off, fan_out, fan_in):
+        a = math.sqrt(6.0 / (fan_in + fan_out))
+        p[off:off + fan_in * fan_out] = (torch.rand(fan_in * fan_out, generator=gen) * 2 - 1) * a
+    glorot(OFF["W1"], 64, 6)
+    glorot(OFF["W2"], 64, 64)
+    glorot(OFF["w3"], 1, 64)
+    return p.to(device)
+
+
+class SeedNet:
+    """Seed network and its training on the GPU."""
+
+    def __init__(self, params=None, seed: int = 0):
+        import torch
+        self.lib = load_library()
+        self.params = params if params is not None else init_params(seed)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.step_count = 0
+
+    def features(self, S, gamma: float):
+        import torch
+        n = S.shape[1]
+        x = torch.empty((n, 6), dtype=torch.float32, device=S.device)
+        rc = self.lib.fx_mlp_features(_dp(S), n, gamma, _fp(x))
+        if rc:
+            raise FxError("fx_mlp_features", rc=rc)
+        return x
+
+    def forward(self, x):
+        import torch
+        n = x.shape[0]
+        y = torch.empty(n, dtype=torch.float32, device=x.device)
+        rc = self.lib.fx_mlp_forward(_fp(self.params), _fp(x), n, _fp(y))
+        if rc:
+            raise FxError("fx_mlp_forward", rc=rc)
+        return y
+
+    def grad(self, x, t):
+        """Mean squared error over the batch and its gradient."""
+        import torch
+        n = x.shape[0]
+        g = torch.zeros(NPARAM, dtype=torch.float32, device=x.device)
+        loss = torch.zeros(1, dtype=torch.float32, device=x.device)
+        rc = self.lib.fx_mlp_grad(_fp(self.params), _fp(x), _fp(t), n, 1.0 / n, _fp(g), _fp(loss))
+        if rc:
+            raise FxError("fx_mlp_grad", rc=rc)
+        return float(loss.item()), g
+
+    def adam(self, g, lr: float):
+        self.step_count += 1
+        rc = self.lib.fx_mlp_adam(_fp(self.params), _fp(g), _fp(self.m), _fp(self.v),
+                                  self.step_count, lr)
+        if rc:
+            raise FxError("fx_mlp_adam", rc=rc)
+
+    def train(self, x, t, steps: int, batch: int = 4096, lr: float = 2e-3, seed: int = 1,
+              log_every: int = 0):
+        """Adam on random minibatches of (x, t); cosine-decayed learning rate."""
+        import torch
+        gen = torch.Generator(device=x.device)
+        gen.manual_seed(seed)
+        n = x.shape[0]
+        hist = []
+        for k in range(steps):
+            idx = torch.randint(0, n, (batch,), generator=gen, device=x.device)
+            xb, tb = x[idx].contiguous(), t[idx].contiguous()
+            loss, g = self.grad(xb, tb)
+            self.adam(g, lr * 0.5 * (1 + math.cos(math.pi * k / steps)))
+            if log_every and (k % log_every == 0 or k == steps - 1):
+                hist.append((k, loss))
+        return hist
+
+
+def solve(S, gamma: float, params=None, maxit: int = 50, tol: float = 1e-12):
+    """Batched exact Riemann problems: (pstar float64, iters int32, status)."""
+    import torch
+    lib = load_library()
+    n = S.shape[1]
+    ps = torch.empty(n, dtype=torch.float64, device=S.device)
+    it = torch.empty(n, dtype=torch.int32, device=S.device)
+    st = FxStatus()
+    rc = lib.fx_riemann_batch(_dp(S), n, gamma, _fp(params) if params is not None else None,
+                              maxit, tol, _dp(ps), _ip(it), C.byref(st))
+    if rc:
+        raise FxError("fx_riemann_batch", st, rc)
+    return ps, it, st
+
+
+def targets(S, pstar):
+    """Training target ln(p*/P0), P0 = sqrt(pL pR)."""
+    import torch
+    return (torch.log(pstar) - 0.5 * (torch.log(S[2]) + torch.log(S[5]))).float().contiguous()

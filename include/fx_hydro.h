@@ -51,8 +51,10 @@ enum { FX_MINMOD = 1, FX_MC = 2, FX_HYBRID = 3, FX_VANLEER = 4, FX_LIMITED = 5 }
 enum { FX_ROE = 1, FX_HLL = 3, FX_HLLC = 4, FX_EXACT = 20 };
 /* precision modes */
 enum { FX_PREC_FP64 = 0,      /* plain FP64, IEEE ops without contraction */
-       FX_PREC_FP64_FAST = 1, /* FP64 with FMA contraction */
-       FX_PREC_MIXED = 2 };   /* FP32 reconstruction and fluxes, FP64 update */
+       FX_PREC_FP64_FAST = 1, /* reserved (rejected by fx_hydro_init) */
+       FX_PREC_MIXED = 2 };   /* FP32 reconstruction and HLLC fluxes; FP64
+                                 update, e_int recovery, EOS and exact-solver
+                                 Newton iteration; 1-D blocks run FP64 */
 
 /* Runtime knobs read by Hydro_init (Hydro_init.F90:60-420) and the EOS
  * (Eos_init.F90:66-73, eos_initGamma.F90:42). */
@@ -160,6 +162,34 @@ int fx_fluxreg_put(const fx_tile* fine, int nFine, const fx_fluxreg_face* faces,
  * fluxregister overwrite).  Writes the coarse tiles' flux arrays. */
 int fx_fluxreg_conserve(const fx_tile* coarse, int nCoarse, const fx_fluxreg_face* faces,
                         int nFaces, const double* reg, fx_status* st);
+
+/* ---- learned Newton seed of the exact solver (no counterpart in the
+ * reference; for callers that train or use the seed network) ----
+ * Network: 6 -> 64 -> 64 -> 1, tanh, BF16 weights on MFMA with FP32
+ * accumulation.  params: FX_MLP_NPARAM FP32 values on the device, in the
+ * order W1[64][6], b1[64], W2[64][64] (out x in), b2[64], w3[64], b3.
+ * Features of an interface with P0 = sqrt(pL pR), R0 = sqrt(rhoL rhoR),
+ * C0 = sqrt(gamma P0/R0), ubar = (uL+uR)/2: (ln rhoL/R0, ln pL/P0,
+ * (uL-ubar)/C0, ln rhoR/R0, ln pR/P0, (uR-ubar)/C0); target ln(pstar/P0).
+ * states: device SoA [6][n] = rhoL, uL, pL, rhoR, uR, pR. */
+enum { FX_MLP_NPARAM = 4673 };
+int fx_mlp_features(const double* states, long n, double gamma, float* x /* [n][6] */);
+int fx_mlp_forward(const float* params, const float* x, long n, float* y);
+/* grad += d/dparams of scale*sum_n (y_n - t_n)^2; *loss += the same sum */
+int fx_mlp_grad(const float* params, const float* x, const float* t, long n, float scale,
+                float* grad, float* loss);
+/* one Adam step (beta 0.9, 0.999, eps 1e-8) on FP32 master parameters */
+int fx_mlp_adam(float* params, const float* grad, float* m, float* v, int step, float lr);
+/* Batched exact Riemann problems: pstar by FP64 Newton from the two-shock guess
+ * (params == NULL) or from the network's seed; a seeded start that does not
+ * converge restarts from the two-shock guess (counted in
+ * st->newton_fallbacks).  iters[i]: Newton iterations spent on interface i,
+ * negative when even the restart did not converge. */
+int fx_riemann_batch(const double* states, long n, double gamma, const float* params,
+                     int maxit, double tol, double* pstar, int* iters, fx_status* st);
+/* Network used when fx_params.learnedSeed = 1 and riemann = FX_EXACT
+ * (NULL clears it). */
+int fx_hydro_set_seed_params(const float* params);
 
 /* Work on the given HIP stream (hipStream_t as void*); NULL = default. */
 void fx_hydro_set_stream(void* stream);

@@ -1,0 +1,85 @@
+"""GPU kernels of the learned Newton seed: BF16-MFMA forward and backward
+against the float64 oracle (oracle/mlp.py, BF16 rounding emulated), the
+batched exact solver against the C oracle, and the seed's effect on the
+Newton iteration count."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from oracle import mlp as M
+from flash5_amd import learned as L
+
+pytestmark = pytest.mark.gpu
+G = 1.4
+
+
+def _params(seed):
+    p = L.init_params(seed).cpu()
+    gen = torch.Generator().manual_seed(seed + 100)
+    p[L.OFF["b1"]:L.OFF["W2"]] = torch.randn(64, generator=gen) * 0.1
+    p[L.OFF["b2"]:L.OFF["w3"]] = torch.randn(64, generator=gen) * 0.1
+    p[L.OFF["b3"]] = 0.3
+    return p.cuda()
+
+
+def test_mfma_forward_matches_oracle():
+    net = L.SeedNet(params=_params(1))
+    S = L.sample_interfaces(1000, seed=11)            # not a multiple of 64
+    x = net.features(S, G)
+    y = net.forward(x).cpu().numpy()
+    yo, _ = M.forward(net.params.cpu().numpy(), x.cpu().numpy())
+    assert np.isfinite(y).all()
+    np.testing.assert_allclose(y, yo, rtol=2e-3, atol=2e-3)
+
+
+def test_mfma_gradient_matches_oracle():
+    net = L.SeedNet(params=_params(2))
+    S = L.sample_interfaces(700, seed=12)
+    x = net.features(S, G)
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    t = torch.randn(700, generator=gen, device="cuda", dtype=torch.float32)
+    loss, g = net.grad(x, t)
+    go, lo = M.grad(net.params.cpu().numpy(), x.cpu().numpy(), t.cpu().numpy(), 1.0 / 700)
+    g = g.cpu().numpy()
+    assert loss == pytest.approx(lo, rel=1e-3)
+    for name, (a, b) in {"W1": (0, 384), "b1": (384, 448), "W2": (448, 4544),
+                         "b2": (4544, 4608), "w3": (4608, 4672), "b3": (4672, 4673)}.items():
+        err = np.linalg.norm(g[a:b] - go[a:b]) / max(np.linalg.norm(go[a:b]), 1e-30)
+        assert err < 1e-2, (name, err)
+
+
+def test_riemann_batch_matches_oracle():
+    S = L.sample_interfaces(4096, seed=13)
+    ps, it, st = L.solve(S, G)
+    assert st.code == 0 and st.newton_fallbacks == 0
+    Sh = S.cpu().numpy()
+    ps = ps.cpu().numpy()
+    it = it.cpu().numpy()
+    for i in range(0, 4096, 7):
+        k, po, _ = O.exact_pstar(G, *Sh[:, i], tol=1e-12, maxit=50)
+        assert ps[i] == pytest.approx(po, rel=4e-12), i
+        assert abs(it[i] - k) <= 1, (i, it[i], k)
+    assert int(st.newton_iters) == int(it.sum())
+
+
+def test_learned_seed_cuts_newton_iterations():
+    """Train briefly on solver-generated states; on held-out interfaces the
+    seeded Newton must converge to the same p* in fewer iterations."""
+    Str = L.sample_interfaces(1 << 16, seed=21)
+    ps, _, _ = L.solve(Str, G)
+    net = L.SeedNet(seed=4)
+    x = net.features(Str, G)
+    t = L.targets(Str, ps)
+    hist = net.train(x, t, steps=400, batch=4096, lr=3e-3, log_every=100)
+    assert hist[-1][1] < 0.5 * hist[0][1], hist
+    Ste = L.sample_interfaces(1 << 14, seed=22)
+    p0, i0, s0 = L.solve(Ste, G)
+    p1, i1, s1 = L.solve(Ste, G, params=net.params)
+    assert s0.code == 0 and s1.code == 0
+    rel = ((p1 - p0).abs() / p0).max().item()
+    assert rel < 4e-12, rel
+    m0 = i0.abs().float().mean().item()
+    m1 = i1.abs().float().mean().item()
+    assert m1 < m0, (m0, m1)
+    assert s1.newton_fallbacks <= 0.01 * Ste.shape[1]
