@@ -63,6 +63,33 @@ def copy_probe(nbytes: int = 1 << 30, reps: int = 3) -> None:
     print(f"copy probe: {reps} copies of {nbytes} bytes", file=sys.stderr)
 
 
+def train_seed(hp, U, dt, dx, gamma, n_train: int = 1 << 22, steps: int = 1500):
+    """Untimed: one exact-solver step on a copy of the batch, 2^22 of its face
+    states solved for ln(p*/P0), the seed network trained on them with the
+    library's MFMA kernels."""
+    import dataclasses
+    import torch
+    from flash5_amd import learned as Lr
+    from flash5_amd.hydro import Hydro
+    t0 = time.perf_counter()
+    hy0 = Hydro(dataclasses.replace(hp, learnedSeed=False))
+    Uc = U.clone()
+    hy0.advance_batch(Uc, dt, (dx,) * 3)
+    S = hy0.face_states()
+    del Uc
+    gen = torch.Generator(device=S.device).manual_seed(7)
+    idx = torch.randperm(S.shape[1], generator=gen, device=S.device)[:n_train]
+    S = S[:, idx].contiguous()
+    ps, _, _ = Lr.solve(S, gamma)
+    net = Lr.SeedNet(seed=3)
+    hist = net.train(net.features(S, gamma), Lr.targets(S, ps, gamma), steps=steps, batch=8192,
+                     lr=3e-3, log_every=steps // 4)
+    torch.cuda.synchronize()
+    return {"params": net.params, "train_states": int(S.shape[1]), "steps": steps,
+            "loss_first_last": [hist[0][1], hist[-1][1]],
+            "train_s": time.perf_counter() - t0}
+
+
 def cpu_baseline(U_gpu, dt, dx, target_s: float = 12.0):
     """Time the FP64 oracle (OpenMP over blocks) on a bounded sample of the
     batch: as many whole rounds of the sample as fit in about target_s, the
@@ -106,6 +133,9 @@ def main() -> None:
                     help="face flux solver (HLLC, or the exact iterative solver)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="fp64, or FP32 reconstruction/fluxes with the FP64 update")
+    ap.add_argument("--learned", action="store_true",
+                    help="seed the exact solver's Newton iteration with the network, trained "
+                         "(untimed) on face states of this workload")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run plain 1 GiB copies (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -127,7 +157,16 @@ def main() -> None:
                      precision=prec)
     U = W.fourier_blocks(args.blocks, n=n, ng=g, gamma=gamma, device=dev)
     dt = W.cfl_dt(U, 3, g, (dx,) * 3, 0.8)
+    train_info = None
+    if args.learned:
+        if args.riemann != "exact":
+            print("bench.py: --learned needs --riemann exact", file=sys.stderr)
+            sys.exit(2)
+        train_info = train_seed(hp, U, dt, dx, gamma)
+        hp.learnedSeed = True
     hy = Hydro(hp)
+    if args.learned:
+        hy.set_seed_params(train_info.pop("params"))
     stream = torch.cuda.current_stream(dev)
     hy.set_stream(stream)
     for _ in range(args.warmup):
@@ -161,7 +200,8 @@ def main() -> None:
     bmz = march_bytes_per_zone(3, n, g)
     achieved = bmz * zones / (march_ms * 1e-3) / 1e9
     fallbacks = int(hy.last_status.fallbacks)
-    workload = f"{args.blocks}x16^3+4g {args.precision} {args.riemann}"
+    workload = (f"{args.blocks}x16^3+4g {args.precision} {args.riemann}"
+                + (" learned" if args.learned else ""))
     if args.copy_probe:
         copy_probe()
 
@@ -188,17 +228,21 @@ def main() -> None:
         "data": "synthetic",
         "config": {"workload": f"{args.blocks} independent 3-D blocks, 16^3 interior + 4 guard "
                                "cells, gamma=5/3, seeded Fourier-mode fields, one unsplit "
-                               f"PPM+{args.riemann.upper()} step ({args.precision})",
+                               f"PPM+{args.riemann.upper()} step ({args.precision}"
+                               + (", learned Newton seed)" if args.learned else ")"),
                    "blocks": args.blocks, "block": [16, 16, 16], "nguard": 4,
                    "precision": args.precision, "riemann": args.riemann, "dt": dt,
                    "newton_iters_per_face": (int(hy.last_status.newton_iters) /
-                                             (3 * args.blocks * 17 * 16 * 16))},
+                                             (3 * args.blocks * 17 * 16 * 16)),
+                   "newton_fallbacks": int(hy.last_status.newton_fallbacks),
+                   "learned_seed": train_info},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic("k_march", workload),
                      "kernel": "k_march (face states + fluxes, fused column march)",
                      "kernel_ms": march_ms, "bytes_per_zone": bmz, "zones_per_launch": zones,
-                     "stage_ms": {"k_march": stage_ms[0], "k_update": stage_ms[2]},
+                     "stage_ms": {"k_march": stage_ms[0], "riemann_stage": stage_ms[1],
+                                  "k_update": stage_ms[2]},
                      "step_bytes_per_zone": bz,
                      "step_frac": bz * zones / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "cpu_baseline": cpu,

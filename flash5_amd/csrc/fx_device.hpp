@@ -39,7 +39,7 @@ struct DevParams {
   double gamma;
   int limiter;
   double beta;
-  int charLim, flat, steep, ctu3d, aux, riemann, precision;
+  int charLim, flat, steep, ctu3d, aux, riemann, precision, learned;
   double fP, smlrho, smallE, eintSwitch, tiny;
   double gammam1, gam1inv, ggprodinv, abar;
   int exactMaxIter;
@@ -494,6 +494,43 @@ __device__ __forceinline__ int exact_newton(double g, double rl, double pl, doub
   return -maxit;
 }
 
+// Newton from the better of two starts: the pressure function is evaluated
+// at both, and the iteration begins at the one with the smaller residual
+// |f(p)| (its f and f' serve the first step).  Returns iterations, negative
+// when maxit was reached; pick = 0 when pa was taken.
+__device__ __forceinline__ int exact_newton_pick(double g, double rl, double pl, double cl,
+                                                 double rr, double pr, double cr, double du,
+                                                 double pa, double pb, double tol, int maxit,
+                                                 double& pstar, int& pick) {
+  double fl, fld, fr, frd, gl, gld, gr, grd;
+  exact_fk(pa, rl, pl, cl, g, fl, fld);
+  exact_fk(pa, rr, pr, cr, g, fr, frd);
+  exact_fk(pb, rl, pl, cl, g, gl, gld);
+  exact_fk(pb, rr, pr, cr, g, gr, grd);
+  const double fa = fl + fr + du, fb = gl + gr + du;
+  pick = fabs(fa) <= fabs(fb) ? 0 : 1;
+  double p = pick ? pb : pa;
+  double f = pick ? fb : fa, fd = pick ? gld + grd : fld + frd;
+  for (int it = 1; it <= maxit; ++it) {
+    if (it > 1) {
+      exact_fk(p, rl, pl, cl, g, fl, fld);
+      exact_fk(p, rr, pr, cr, g, fr, frd);
+      f = fl + fr + du;
+      fd = fld + frd;
+    }
+    double pn = p - f / fd;
+    if (!(pn > 0.)) pn = 1.e-30;
+    const double change = 2. * fabs((pn - p) / (pn + p));
+    p = pn;
+    if (change <= tol) {
+      pstar = p;
+      return it;
+    }
+  }
+  pstar = p;
+  return -maxit;
+}
+
 struct ExactStats {
   unsigned long long iters, fallbacks;
 };
@@ -561,13 +598,18 @@ __device__ __forceinline__ int exact_flux(const DevParams& P, const double* VL, 
   double pm;
   int it;
   if (guess > 0.) {
-    it = exact_newton(g, rl, pl, cl, rr, pr, cr, du, guess, P.exactTol, P.exactMaxIter, pm);
-    if (it < 0) {
+    // seeded: start from the better of the seed and the two-shock guess;
+    // a seeded start that does not converge restarts from the two-shock guess
+    const double ts = two_shock_guess(g, rl, ul, pl, cl, rr, ur, pr, cr);
+    int pick;
+    it = exact_newton_pick(g, rl, pl, cl, rr, pr, cr, du, ts, guess, P.exactTol, P.exactMaxIter,
+                           pm, pick);
+    if (it < 0 && pick == 1) {
       es.iters += (unsigned long long)(-it);
       es.fallbacks += 1;
-      it = exact_newton(g, rl, pl, cl, rr, pr, cr, du,
-                        two_shock_guess(g, rl, ul, pl, cl, rr, ur, pr, cr), P.exactTol,
-                        P.exactMaxIter, pm);
+      it = exact_newton(g, rl, pl, cl, rr, pr, cr, du, ts, P.exactTol, P.exactMaxIter, pm);
+    } else if (it < 0) {
+      es.fallbacks += 1;
     }
   } else {
     it = exact_newton(g, rl, pl, cl, rr, pr, cr, du,

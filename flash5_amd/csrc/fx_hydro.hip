@@ -656,6 +656,10 @@ struct Lib {
   bool seedReady = false;
   float* seedY = nullptr;              // network outputs per interface / face
   size_t seedYCap = 0;
+  double* fstates = nullptr;           // face states for the separate Riemann stage
+  size_t fstatesCap = 0;
+  long lastFaces = 0;                  // faces of the last exact-solver step
+  int lastTiles = 0;
   unsigned char* dinplace = nullptr;
   size_t dinplaceCap = 0;
   DevStatus* status = nullptr;
@@ -728,6 +732,7 @@ DevParams make_dev_params(const fx_params& p) {
   d.aux = p.useAuxEint;
   d.riemann = p.riemann;
   d.precision = p.precision;
+  d.learned = p.learnedSeed;
   d.fP = p.fP;
   d.smlrho = p.smlrho;
   d.smallE = p.smallE;
@@ -875,6 +880,8 @@ void launch_march(const DevParams& P, const MarchLaunch& M, int TX, dim3 grid) {
   else launch_march_r<RS, double>(P, M, TX, grid);
 }
 
+dim3 wave_grid(long n);
+
 int record(int n) {
   if (!g.timing) return 0;
   if (!g.ev[n]) {
@@ -909,16 +916,47 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
     dIn = g.dinplace;
   }
   int TX = 0, TY = 0, NT = 0;
-  if (!g.legacy && march_shape(P, &TX, &TY, &NT)) {
-    MarchLaunch M{g.dtiles, nTiles, P.nxb / TX, (P.ndim > 1 ? P.nyb : 1) / TY, g.status, dt};
+  const bool march = !g.legacy && march_shape(P, &TX, &TY, &NT);
+  if (P.learned && (!march || !g.seedReady)) {  // the seed needs the march path and a network
+    if (st) st->code = 3;
+    return 3;
+  }
+  if (march) {
+    MarchLaunch M{g.dtiles, nTiles, P.nxb / TX, (P.ndim > 1 ? P.nyb : 1) / TY, g.status, dt,
+                  nullptr};
     const dim3 grid((unsigned)(nTiles * M.tilesX * M.tilesY));
     Launch L{g.dtiles, nTiles, nullptr, g.status, dt};
+    const long perTile = P.nface[0] + P.nface[1] + P.nface[2];
+    const long faces = perTile * nTiles;
+    if (P.riemann == FX_EXACT) {
+      if (grow(&g.fstates, &g.fstatesCap, (size_t)kFaceComps * faces, st)) return 4;
+      if (P.learned && grow(&g.seedY, &g.seedYCap, (size_t)faces, st)) return 4;
+    }
     hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
     if (record(0)) return 4;
-    if (P.riemann == FX_EXACT) launch_march<FX_EXACT>(P, M, TX, grid);
-    else launch_march<FX_HLLC>(P, M, TX, grid);
-    if (record(1)) return 4;
-    if (record(2)) return 4;
+    if (P.riemann == FX_EXACT) {
+      // exact solver: the march stores face states, a separate stage solves
+      M.fstates = g.fstates;
+      launch_march<RS_STATES>(P, M, TX, grid);
+      if (record(1)) return 4;
+      FaceLaunch F{g.dtiles, nTiles, g.fstates, P.learned ? g.seedY : nullptr, g.status};
+      g.lastFaces = faces;
+      g.lastTiles = nTiles;
+      if (P.learned) {
+        hipLaunchKernelGGL(k_face_seed, wave_grid(faces), dim3(kLearnThreads), 0, g.stream, P, F,
+                           g.seedPacked, g.seedY);
+        hipLaunchKernelGGL(k_face_solve<true>, dim3(grid_for(faces)), dim3(kThreads), 0, g.stream,
+                           P, F);
+      } else {
+        hipLaunchKernelGGL(k_face_solve<false>, dim3(grid_for(faces)), dim3(kThreads), 0,
+                           g.stream, P, F);
+      }
+      if (record(2)) return 4;
+    } else {
+      launch_march<FX_HLLC>(P, M, TX, grid);
+      if (record(1)) return 4;
+      if (record(2)) return 4;
+    }
     hipLaunchKernelGGL((k_update<false, false>), dim3(grid_for(P.icells * nTiles)), dim3(kThreads), 0,
                        g.stream, P, L, dIn);
     if (record(3)) return 4;
@@ -1066,7 +1104,7 @@ int fx_hydro_init(const fx_params* p) {
 
 void fx_hydro_finalize(void) {
   void* bufs[] = {g.states, g.flux, g.pavg, g.dtiles, g.dinplace, g.status, g.partial,
-                  g.dregTiles, g.dfaces, g.mlpPacked, g.seedPacked, g.seedY};
+                  g.dregTiles, g.dfaces, g.mlpPacked, g.seedPacked, g.seedY, g.fstates};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : g.ev)
@@ -1229,6 +1267,16 @@ int fx_riemann_batch(const double* states, long n, double gamma, const float* pa
     for (int k = 0; k < 3; ++k)
       if (hipEventElapsedTime(&g.stageMs[k], g.ev[k], g.ev[k + 1]) != hipSuccess) g.stageMs[k] = -1.f;
   return rc;
+}
+
+int fx_hydro_face_states(double* S, long cap, long* n) {
+  if (!n) return 3;
+  *n = g.lastFaces;
+  if (!g.init || g.lastFaces <= 0 || !S || cap < g.lastFaces) return 3;
+  FaceLaunch F{g.dtiles, g.lastTiles, g.fstates, nullptr, g.status};
+  hipLaunchKernelGGL(k_collect_faces, dim3(grid_for(g.lastFaces)), dim3(kThreads), 0, g.stream,
+                     g.dp, F, g.lastFaces, S);
+  return sync_or_async(nullptr);
 }
 
 int fx_hydro_set_seed_params(const float* params) {

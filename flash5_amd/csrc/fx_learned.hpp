@@ -105,6 +105,25 @@ __global__ void __launch_bounds__(kThreads) k_mlp_adam(float* __restrict__ prm,
   prm[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + eps);
 }
 
+// The network predicts y = ln(p* / P0), P0 = sqrt(pL pR).  Weak interfaces
+// (every |feature| below kSeedGate) keep the two-shock guess, which is
+// already accurate to high order there; for the others Newton starts from
+// whichever of P0 * exp(y) and the two-shock guess has the smaller residual
+// of the pressure function.
+constexpr double kSeedGate = 0.05;
+
+__device__ __forceinline__ bool seed_gate(double g, double rl, double ul, double pl, double rr,
+                                          double ur, double pr) {
+  const double P0 = sqrt(pl * pr), R0 = sqrt(rl * rr), C0 = sqrt(g * P0 / R0);
+  const double m = fmax(fmax(fabs(0.5 * log(rl / rr)), fabs(0.5 * log(pl / pr))),
+                        fabs(0.5 * (ur - ul) / C0));
+  return m >= kSeedGate;
+}
+
+__device__ __forceinline__ double seed_from_y(float y, double pl, double pr) {
+  return sqrt(pl * pr) * exp((double)fminf(fmaxf(y, -60.f), 60.f));
+}
+
 // Network seed y = ln(p*/P0) for every interface of a batch (inference only;
 // the Newton kernel below reads it, so each keeps its own register budget).
 __global__ void __launch_bounds__(kLearnThreads) k_mlp_seed_states(
@@ -117,6 +136,9 @@ __global__ void __launch_bounds__(kLearnThreads) k_mlp_seed_states(
     const long i = base + lane;
     const bool act = i < n;
     const IfaceState s = load_iface(S, n, i, act);
+    // wavefronts whose interfaces are all weak skip the network
+    const bool need = act && seed_gate(gamma, s.rl, s.ul, s.pl, s.rr, s.ur, s.pr);
+    if (__ballot(need) == 0) continue;
     float f[6];
     double P0;
     mlp_features(gamma, s.rl, s.ul, s.pl, s.rr, s.ur, s.pr, f, P0);
@@ -131,11 +153,6 @@ __global__ void __launch_bounds__(kLearnThreads) k_mlp_seed_states(
     if (act) y[i] = yo;
     wave_lds_sync();
   }
-}
-
-// start of Newton from a network output y = ln(p*/P0)
-__device__ __forceinline__ double seed_from_y(float y, double pl, double pr) {
-  return sqrt(pl * pr) * exp((double)fminf(fmaxf(y, -60.f), 60.f));
 }
 
 // Exact Riemann problems in a batch: p* by Newton (FP64 iterate and
@@ -162,15 +179,20 @@ __global__ void __launch_bounds__(kThreads) k_riemann_batch(
     }
     const double du = s.ur - s.ul;
     const double ts = two_shock_guess(g, s.rl, s.ul, s.pl, cl, s.rr, s.ur, s.pr, cr);
+    const bool use = SEEDED && seed_gate(g, s.rl, s.ul, s.pl, s.rr, s.ur, s.pr);
     double pm;
-    int it = exact_newton(g, s.rl, s.pl, cl, s.rr, s.pr, cr, du,
-                          SEEDED ? seed_from_y(yseed[i], s.pl, s.pr) : ts, tol, maxit, pm);
+    int pick = 0, it;
+    if (use)
+      it = exact_newton_pick(g, s.rl, s.pl, cl, s.rr, s.pr, cr, du, ts,
+                             seed_from_y(yseed[i], s.pl, s.pr), tol, maxit, pm, pick);
+    else
+      it = exact_newton(g, s.rl, s.pl, cl, s.rr, s.pr, cr, du, ts, tol, maxit, pm);
     int total = it < 0 ? -it : it;
-    if (SEEDED && it < 0) {
+    if (use && pick == 1 && it < 0) {
       es.fallbacks += 1;
       it = exact_newton(g, s.rl, s.pl, cl, s.rr, s.pr, cr, du, ts, tol, maxit, pm);
       total += it < 0 ? -it : it;
-    } else if (!SEEDED && it < 0) {
+    } else if (it < 0) {
       es.fallbacks += 1;
     }
     es.iters += (unsigned long long)total;
@@ -178,4 +200,164 @@ __global__ void __launch_bounds__(kThreads) k_riemann_batch(
     iters[i] = it < 0 ? -total : total;
   }
   flush_exact(st, es);
+}
+
+// ---- separate Riemann stage of the hydro step (exact solver) ----
+// The march kernel stored each face's left/right states; these kernels solve
+// one face per thread at full occupancy, optionally from the network seed.
+struct FaceLaunch {
+  const fx_tile* tiles;
+  int nTiles;
+  const double* fstates;   // [tile][dir][comp][face], from k_march<RS_STATES>
+  const float* y;          // network seed per face (flattened like gid), or null
+  DevStatus* status;
+};
+
+struct FaceRef {
+  int t, d;
+  long r;          // face index within the direction's flux array
+  long cR, cL;     // right / left cells (guard-inclusive index)
+};
+
+__device__ __forceinline__ FaceRef face_ref(const DevParams& P, long gid, long perTile) {
+  FaceRef f;
+  f.t = (int)(gid / perTile);
+  long r = gid - (long)f.t * perTile;
+  int d = 0;
+  while (d < P.ndim - 1 && r >= P.nface[d]) {
+    r -= P.nface[d];
+    ++d;
+  }
+  f.d = d;
+  f.r = r;
+  const int f0 = P.fext[d][0], f1 = P.fext[d][1];
+  const int a0 = (int)(r % f0), a1 = (int)((r / f0) % f1), a2 = (int)(r / ((long)f0 * f1));
+  f.cR = cidx(P, a0 + P.lo[0], a1 + P.lo[1], a2 + P.lo[2]);
+  f.cL = f.cR - axis_stride(P, d);
+  return f;
+}
+
+__device__ __forceinline__ void load_face_states(const DevParams& P, const FaceLaunch& F,
+                                                 const FaceRef& f, double VL[8], double VR[8]) {
+  const double* base = F.fstates + f.t * fstate_tile_stride(P) + fstate_dir_offset(P, f.d);
+  const long nf = P.nface[f.d];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    VL[comp_slot(q)] = base[q * nf + f.r];
+    VR[comp_slot(q)] = base[(6 + q) * nf + f.r];
+  }
+  const double* U = F.tiles[f.t].U;
+  VL[P_GAMC] = U[P.voff[FX_GAMC] + f.cL];
+  VL[P_GAME] = U[P.voff[FX_GAME] + f.cL];
+  VR[P_GAMC] = U[P.voff[FX_GAMC] + f.cR];
+  VR[P_GAME] = U[P.voff[FX_GAME] + f.cR];
+}
+
+template <bool SEEDED>
+__global__ void __launch_bounds__(kThreads) k_face_solve(DevParams P, FaceLaunch F) {
+  const long perTile = P.nface[0] + P.nface[1] + P.nface[2];
+  const long total = perTile * F.nTiles;
+  ExactStats es{0ULL, 0ULL};
+  for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+       gid += (long)gridDim.x * blockDim.x) {
+    const FaceRef f = face_ref(P, gid, perTile);
+    double VL[8], VR[8], Fo[8];
+    load_face_states(P, F, f, VL, VR);
+    double guess = -1.0;
+    if (SEEDED) {
+      const double g = VL[P_GAMC];
+      const double rl = VL[P_DENS], pl = VL[P_PRES], rr = VR[P_DENS], pr = VR[P_PRES];
+      const double ul = VL[P_VELX + f.d], ur = VR[P_VELX + f.d];
+      if (rl > 0. && rr > 0. && pl > 0. && pr > 0. && seed_gate(g, rl, ul, pl, rr, ur, pr))
+        guess = seed_from_y(F.y[gid], pl, pr);
+    }
+    int ierr;
+    if (f.d == 0) ierr = exact_flux<0>(P, VL, VR, guess, Fo, es);
+    else if (f.d == 1) ierr = exact_flux<1>(P, VL, VR, guess, Fo, es);
+    else ierr = exact_flux<2>(P, VL, VR, guess, Fo, es);
+    if (ierr) {
+      report_error(F.status, 2, f.t, (int)(f.cR % P.nx), (int)((f.cR / P.nx) % P.ny),
+                   (int)(f.cR / ((long)P.nx * P.ny)), f.d);
+      continue;
+    }
+    Fo[6] = 0.;
+    Fo[7] = 0.;
+    if (P.aux) {
+      if (Fo[0] > 0.) {
+        Fo[7] = Fo[0] / VL[P_DENS];
+        Fo[6] = Fo[0] * VL[P_EINT];
+      } else {
+        Fo[7] = Fo[0] / VR[P_DENS];
+        Fo[6] = Fo[0] * VR[P_EINT];
+      }
+    }
+    double* out = F.tiles[f.t].flux[f.d];
+    const long nf = P.nface[f.d];
+#pragma unroll
+    for (int q = 0; q < FX_NFLUX; ++q) out[q * nf + f.r] = Fo[q];
+  }
+  flush_exact(F.status, es);
+}
+
+// network seed ln(p*/P0) of every face, from the stored face states
+__global__ void __launch_bounds__(kLearnThreads) k_face_seed(DevParams P, FaceLaunch F,
+                                                             const MlpPacked* __restrict__ W,
+                                                             float* __restrict__ y) {
+  __shared__ float xs[kLearnWaves][64 * 8];
+  const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+  const long perTile = P.nface[0] + P.nface[1] + P.nface[2];
+  const long total = perTile * F.nTiles;
+  const long nw = (long)gridDim.x * kLearnWaves;
+  for (long base = ((long)blockIdx.x * kLearnWaves + wv) * 64; base < total; base += nw * 64) {
+    const long gid = base + lane;
+    const bool act = gid < total;
+    double rl = 1.0, ul = 0.0, pl = 1.0, rr = 1.0, ur = 0.0, pr = 1.0;
+    if (act) {
+      const FaceRef f = face_ref(P, gid, perTile);
+      const double* b = F.fstates + f.t * fstate_tile_stride(P) + fstate_dir_offset(P, f.d);
+      const long nf = P.nface[f.d];
+      rl = b[0 * nf + f.r];
+      ul = b[(1 + f.d) * nf + f.r];
+      pl = b[4 * nf + f.r];
+      rr = b[6 * nf + f.r];
+      ur = b[(7 + f.d) * nf + f.r];
+      pr = b[10 * nf + f.r];
+    }
+    // wavefronts whose faces are all weak (or invalid) skip the network
+    const bool need = act && rl > 0. && rr > 0. && pl > 0. && pr > 0. &&
+                      seed_gate(P.gamma, rl, ul, pl, rr, ur, pr);
+    if (__ballot(need) == 0) continue;
+    float x[6];
+    double P0;
+    mlp_features(P.gamma, rl, ul, pl, rr, ur, pr, x, P0);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) xs[wv][lane * 8 + k] = x[k];
+    xs[wv][lane * 8 + 6] = 0.f;
+    xs[wv][lane * 8 + 7] = 0.f;
+    wave_lds_sync();
+    MlpActs A;
+    float yo;
+    mlp_forward_wave(*W, xs[wv], lane, A, yo);
+    if (act) y[gid] = yo;
+    wave_lds_sync();
+  }
+}
+
+// face states of the last exact-solver step as Riemann problems, SoA [6][n]
+// (rhoL, uL, pL, rhoR, uR, pR with u the face-normal velocity)
+__global__ void __launch_bounds__(kThreads) k_collect_faces(DevParams P, FaceLaunch F, long n,
+                                                            double* __restrict__ S) {
+  const long perTile = P.nface[0] + P.nface[1] + P.nface[2];
+  for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < n;
+       gid += (long)gridDim.x * blockDim.x) {
+    const FaceRef f = face_ref(P, gid, perTile);
+    const double* b = F.fstates + f.t * fstate_tile_stride(P) + fstate_dir_offset(P, f.d);
+    const long nf = P.nface[f.d];
+    S[gid] = b[0 * nf + f.r];
+    S[n + gid] = b[(1 + f.d) * nf + f.r];
+    S[2 * n + gid] = b[4 * nf + f.r];
+    S[3 * n + gid] = b[6 * nf + f.r];
+    S[4 * n + gid] = b[(7 + f.d) * nf + f.r];
+    S[5 * n + gid] = b[10 * nf + f.r];
+  }
 }

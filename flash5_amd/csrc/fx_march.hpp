@@ -44,7 +44,34 @@ struct MarchLaunch {
   int tilesX, tilesY;  // column tiles per block
   DevStatus* status;
   double dt;
+  double* fstates;     // RS_STATES: per-face states for the separate Riemann stage
 };
+
+// Riemann-stage variant of the march: stage C stores each face's left and
+// right states (dens, vx, vy, vz, pres, eint) instead of solving there.
+constexpr int RS_STATES = 99;
+constexpr int kFaceComps = 12;
+
+// face-state scratch of one tile: [dir][comp 0..11][face]
+__device__ __forceinline__ long fstate_tile_stride(const DevParams& P) {
+  return (long)kFaceComps * (P.nface[0] + P.nface[1] + P.nface[2]);
+}
+__device__ __forceinline__ long fstate_dir_offset(const DevParams& P, int d) {
+  long o = 0;
+  for (int e = 0; e < d; ++e) o += (long)kFaceComps * P.nface[e];
+  return o;
+}
+
+template <int D, typename R>
+__device__ __forceinline__ void write_face_states(const DevParams& P, double* base, long r,
+                                                  const R VL[8], const R VR[8]) {
+  const long nf = P.nface[D];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    base[q * nf + r] = double(VL[comp_slot(q)]);
+    base[(6 + q) * nf + r] = double(VR[comp_slot(q)]);
+  }
+}
 
 template <int SD, typename R>
 __device__ __forceinline__ bool sigma_cell(const DevParams& P, const double* __restrict__ U, long c,
@@ -448,8 +475,14 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
           VL[P_GAME] = R(U[P.voff[FX_GAME] + c - sx]);
           VR[P_GAMC] = R(U[P.voff[FX_GAMC] + c]);
           VR[P_GAME] = R(U[P.voff[FX_GAME] + c]);
-          if (face_flux<0, RS>(P, VL, VR, F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 0); continue; }
-          write_flux<0>(P, T.flux[0], face_index(P, 0, ox + a, oy + b, k), F);
+          if constexpr (RS == RS_STATES) {
+            write_face_states<0>(P, L.fstates + t * fstate_tile_stride(P) + fstate_dir_offset(P, 0),
+                                 face_index(P, 0, ox + a, oy + b, k), VL, VR);
+            continue;
+          } else {
+            if (face_flux<0, RS>(P, VL, VR, F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 0); continue; }
+            write_flux<0>(P, T.flux[0], face_index(P, 0, ox + a, oy + b, k), F);
+          }
         } else if (w < nFX + nFY) {
           const int w2 = w - nFX;
           const int a = w2 % TX, b = w2 / TX;
@@ -460,8 +493,14 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
           VL[P_GAME] = R(U[P.voff[FX_GAME] + c - sy]);
           VR[P_GAMC] = R(U[P.voff[FX_GAMC] + c]);
           VR[P_GAME] = R(U[P.voff[FX_GAME] + c]);
-          if (face_flux<1, RS>(P, VL, VR, F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 1); continue; }
-          write_flux<1>(P, T.flux[1], face_index(P, 1, ox + a, oy + b, k), F);
+          if constexpr (RS == RS_STATES) {
+            write_face_states<1>(P, L.fstates + t * fstate_tile_stride(P) + fstate_dir_offset(P, 1),
+                                 face_index(P, 1, ox + a, oy + b, k), VL, VR);
+            continue;
+          } else {
+            if (face_flux<1, RS>(P, VL, VR, F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 1); continue; }
+            write_flux<1>(P, T.flux[1], face_index(P, 1, ox + a, oy + b, k), F);
+          }
         } else {
           // column item: z-face flux at face k, pavg of the cell, carry Wp
           const int w2 = w - nFX - nFY;
@@ -474,8 +513,14 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
             VL[P_GAME] = R(U[P.voff[FX_GAME] + c - sz]);
             VR[P_GAMC] = R(U[P.voff[FX_GAMC] + c]);
             VR[P_GAME] = R(U[P.voff[FX_GAME] + c]);
-            if (face_flux<2, RS>(P, VL, VR, F, es)) report_error(st, 2, t, ox + a, oy + b, k, 2);
-            else write_flux<2>(P, T.flux[2], face_index(P, 2, ox + a, oy + b, k), F);
+            if constexpr (RS == RS_STATES) {
+              write_face_states<2>(P,
+                                   L.fstates + t * fstate_tile_stride(P) + fstate_dir_offset(P, 2),
+                                   face_index(P, 2, ox + a, oy + b, k), VL, VR);
+            } else {
+              if (face_flux<2, RS>(P, VL, VR, F, es)) report_error(st, 2, t, ox + a, oy + b, k, 2);
+              else write_flux<2>(P, T.flux[2], face_index(P, 2, ox + a, oy + b, k), F);
+            }
           }
           if (interior && P.aux) {
             // face-averaged pressure (hy_getFaceFlux.F90:507-561 and the

@@ -6,7 +6,8 @@
 // Features of an interface (Galilean and scale invariant): with
 // P0 = sqrt(pL pR), R0 = sqrt(rhoL rhoR), C0 = sqrt(gamma P0 / R0),
 //   x = (ln rhoL/R0, ln pL/P0, (uL - ubar)/C0, ln rhoR/R0, ln pR/P0, (uR - ubar)/C0)
-// and the network predicts y = ln(p*/P0); the seed is P0 exp(y).
+// and the network predicts y = ln(p*/P0); the seed is P0 exp(y) (used where
+// the interface is not weak, see fx_learned.hpp).
 //
 // Layout.  Layers compute H^T = W X^T (units as MFMA rows, interfaces as
 // columns).  A 16x16 MFMA result tile sits in lane l as column l%16 and rows
@@ -44,6 +45,12 @@ __device__ __forceinline__ bf16x8 to_bf16x8(const float v[8]) {
 #pragma unroll
   for (int j = 0; j < 8; ++j) h[j] = f2bf(v[j]);
   return __builtin_bit_cast(bf16x8, h);
+}
+
+// tanh from the hardware exponential: 1 - 2 / (exp(2x) + 1), saturating to
+// +-1 at both ends; a few ulp in FP32, ample for a Newton seed
+__device__ __forceinline__ float act_tanh(float x) {
+  return 1.f - __fdividef(2.f, __expf(2.f * x) + 1.f);
 }
 
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
@@ -133,7 +140,7 @@ __device__ __forceinline__ void mlp_forward_wave(const MlpPacked& W, const float
     for (int nt = 0; nt < 4; ++nt) {
       f32x4 d = mfma(a, bx[nt], f32x4{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
-      for (int i = 0; i < 4; ++i) d[i] = tanhf(d[i] + W.b1[16 * mt + 4 * g + i]);
+      for (int i = 0; i < 4; ++i) d[i] = act_tanh(d[i] + W.b1[16 * mt + 4 * g + i]);
       A.h1[mt][nt] = d;
     }
   }
@@ -156,7 +163,7 @@ __device__ __forceinline__ void mlp_forward_wave(const MlpPacked& W, const float
       f32x4 d = mfma(W.w2[(mt * 2 + 0) * 64 + lane], b[0], f32x4{0.f, 0.f, 0.f, 0.f});
       d = mfma(W.w2[(mt * 2 + 1) * 64 + lane], b[1], d);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) d[i] = tanhf(d[i] + W.b2[16 * mt + 4 * g + i]);
+      for (int i = 0; i < 4; ++i) d[i] = act_tanh(d[i] + W.b2[16 * mt + 4 * g + i]);
       A.h2[mt][nt] = d;
     }
   }

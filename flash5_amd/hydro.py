@@ -100,6 +100,7 @@ EXPORTS = {
     "fx_riemann_batch": ([_DP, C.c_long, C.c_double, C.POINTER(C.c_float), C.c_int, C.c_double,
                           _DP, C.POINTER(C.c_int), C.POINTER(FxStatus)], C.c_int),
     "fx_hydro_set_seed_params": ([C.POINTER(C.c_float)], C.c_int),
+    "fx_hydro_face_states": ([_DP, C.c_long, C.POINTER(C.c_long)], C.c_int),
     "fx_hydro_set_stream": ([C.c_void_p], None),
     "fx_hydro_set_async": ([C.c_int], None),
     "fx_hydro_set_timing": ([C.c_int], None),
@@ -340,6 +341,30 @@ class Hydro:
         if rc:
             raise FxError("fx_fluxreg_conserve", st, rc)
         return st
+
+    def set_seed_params(self, params) -> None:
+        """Seed network for learnedSeed=True (float32 device tensor, FX_MLP_NPARAM)."""
+        ptr = None
+        if params is not None:
+            import torch
+            assert params.is_cuda and params.dtype == torch.float32 and params.is_contiguous()
+            ptr = C.cast(C.c_void_p(params.data_ptr()), C.POINTER(C.c_float))
+        rc = self.lib.fx_hydro_set_seed_params(ptr)
+        if rc:
+            raise FxError("fx_hydro_set_seed_params", rc=rc)
+
+    def face_states(self):
+        """Riemann problems of the last exact-solver step, (6, nfaces) float64."""
+        import torch
+        n = C.c_long(0)
+        self.lib.fx_hydro_face_states(None, 0, C.byref(n))
+        if n.value <= 0:
+            raise FxError("fx_hydro_face_states", rc=3)
+        S = torch.empty((6, n.value), dtype=torch.float64, device="cuda")
+        rc = self.lib.fx_hydro_face_states(_ptr(S), n.value, C.byref(n))
+        if rc:
+            raise FxError("fx_hydro_face_states", rc=rc)
+        return S
 
     def advance_batch(self, U, dt: float, deltas: Sequence[float]) -> FxStatus:
         """In-place step of a (nblocks, nunk, nz, ny, nx) batch of independent blocks."""

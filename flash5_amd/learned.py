@@ -147,7 +147,65 @@ def solve(S, gamma: float, params=None, maxit: int = 50, tol: float = 1e-12):
     return ps, it, st
 
 
-def targets(S, pstar):
+def targets(S, pstar, gamma: float = 1.4):
     """Training target ln(p*/P0), P0 = sqrt(pL pR)."""
     import torch
+    del gamma
     return (torch.log(pstar) - 0.5 * (torch.log(S[2]) + torch.log(S[5]))).float().contiguous()
+
+
+def study(n_test_log2: int = 24, n_train_log2: int = 22, steps: int = 3000, gamma: float = 1.4,
+          seed: int = 0x5EED0004):
+    """Learned-seed study: train on 2^n_train solver-emitted states, then solve
+    2^n_test held-out interfaces from the two-shock start and from the seed.
+    Returns a dict of iteration counts, fallbacks, p* agreement and rates."""
+    import time
+    import torch
+    t0 = time.perf_counter()
+    Str = sample_interfaces(1 << n_train_log2, seed=seed, gamma=gamma)
+    ps, _, _ = solve(Str, gamma)
+    net = SeedNet(seed=1)
+    x = net.features(Str, gamma)
+    t = targets(Str, ps, gamma)
+    hist = net.train(x, t, steps=steps, batch=8192, lr=3e-3, log_every=max(1, steps // 5))
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    del Str, ps, x, t
+    Ste = sample_interfaces(1 << n_test_log2, seed=seed + 1, gamma=gamma)
+    solve(Ste[:, :4096].contiguous(), gamma)                      # warm-up
+    solve(Ste[:, :4096].contiguous(), gamma, params=net.params)
+    out = {}
+    res = {}
+    for name, prm in (("two_shock", None), ("learned", net.params)):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        p, it, st = solve(Ste, gamma, params=prm)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - a
+        res[name] = p
+        itf = it.abs().float()
+        out[name] = {"iters_mean": itf.mean().item(), "iters_max": int(itf.max().item()),
+                     "newton_fallbacks": int(st.newton_fallbacks),
+                     "fallback_rate": int(st.newton_fallbacks) / Ste.shape[1],
+                     "not_converged": int((it < 0).sum().item()),
+                     "interfaces_per_s": Ste.shape[1] / dt, "ms": dt * 1e3}
+    rel = ((res["learned"] - res["two_shock"]).abs() / res["two_shock"])
+    return {"interfaces": Ste.shape[1], "train_states": 1 << n_train_log2, "train_steps": steps,
+            "train_s": train_s, "loss": [h[1] for h in hist], "gamma": gamma,
+            "pstar_rel_diff_max": rel.max().item(), "pstar_rel_diff_mean": rel.mean().item(),
+            **out}
+
+
+def main() -> None:
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="learned Newton seed study (2^24 interfaces)")
+    ap.add_argument("--test-log2", type=int, default=24)
+    ap.add_argument("--train-log2", type=int, default=22)
+    ap.add_argument("--steps", type=int, default=3000)
+    a = ap.parse_args()
+    print(json.dumps(study(a.test_log2, a.train_log2, a.steps)))
+
+
+if __name__ == "__main__":
+    main()

@@ -171,6 +171,8 @@ int fx_fluxreg_conserve(const fx_tile* coarse, int nCoarse, const fx_fluxreg_fac
  * Features of an interface with P0 = sqrt(pL pR), R0 = sqrt(rhoL rhoR),
  * C0 = sqrt(gamma P0/R0), ubar = (uL+uR)/2: (ln rhoL/R0, ln pL/P0,
  * (uL-ubar)/C0, ln rhoR/R0, ln pR/P0, (uR-ubar)/C0); target ln(pstar/P0).
+ * The seed P0 exp(y) is used where some feature has magnitude >= 0.05;
+ * weaker interfaces start from the two-shock guess. 
  * states: device SoA [6][n] = rhoL, uL, pL, rhoR, uR, pR. */
 enum { FX_MLP_NPARAM = 4673 };
 int fx_mlp_features(const double* states, long n, double gamma, float* x /* [n][6] */);
@@ -188,8 +190,13 @@ int fx_mlp_adam(float* params, const float* grad, float* m, float* v, int step, 
 int fx_riemann_batch(const double* states, long n, double gamma, const float* params,
                      int maxit, double tol, double* pstar, int* iters, fx_status* st);
 /* Network used when fx_params.learnedSeed = 1 and riemann = FX_EXACT
- * (NULL clears it). */
+ * (NULL clears it).  The seeded step needs the fused kernel path (2-D, 3-D). */
 int fx_hydro_set_seed_params(const float* params);
+/* The face states of the last exact-solver step as Riemann problems in the
+ * fx_riemann_batch layout (u = face-normal velocity), for training the seed
+ * on states the solver itself produced.  *n receives the face count; returns
+ * nonzero when cap < *n or no such step ran. */
+int fx_hydro_face_states(double* states, long cap, long* n);
 
 /* Work on the given HIP stream (hipStream_t as void*); NULL = default. */
 void fx_hydro_set_stream(void* stream);

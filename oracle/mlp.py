@@ -2,7 +2,8 @@
 (6 -> 64 -> 64 -> 1, tanh) and its gradients, with optional emulation of
 the BF16 rounding the GPU applies to MFMA operands (weights of layers 1-2,
 the input features, layer-1 activations feeding layer 2, and layer-2
-pre-activation gradients feeding the backward products).  Parameter order:
+pre-activation gradients feeding the backward products).  tanh is exact here;
+the GPU evaluates it from the hardware exponential (a few FP32 ulp).  Parameter order:
 W1[64][6], b1[64], W2[64][64], b2[64], w3[64], b3."""
 from __future__ import annotations
 
