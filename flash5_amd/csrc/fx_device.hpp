@@ -40,6 +40,7 @@ struct DevParams {
   int limiter;
   double beta;
   int charLim, flat, steep, ctu3d, aux, riemann, precision, learned;
+  int velSlot[3];            // var slots of velx, vely, velz
   double fP, smlrho, smallE, eintSwitch, tiny;
   double gammam1, gam1inv, ggprodinv, abar;
   int exactMaxIter;

@@ -49,7 +49,11 @@ struct Launch {
   double* states;        // scratch: [tile][band][comp][cell]
   DevStatus* status;
   double dt;
+  const double* dtp;     // device-resident dt (graph-captured steps), or null
 };
+
+// the step's dt: from device memory when the host does not know it
+__device__ __forceinline__ double step_dt(double dt, const double* dtp) { return dtp ? *dtp : dt; }
 
 __device__ void report_error(DevStatus* s, int code, int tile, int i, int j, int k, int dir) {
   if (atomicCAS(&s->code, 0, code) == 0) {
@@ -262,9 +266,10 @@ __global__ void __launch_bounds__(kThreads) k_states(DevParams P, Launch L) {
     const long c = cidx(P, ci[0], ci[1], ci[2]);
     double Wp[8], Wn[8];
     bool ok;
-    if (d == 0) ok = state_cell<0, NDIM>(P, T.U, T.del, L.dt, c, Wp, Wn, L.status);
-    else if (d == 1) ok = state_cell<1, NDIM>(P, T.U, T.del, L.dt, c, Wp, Wn, L.status);
-    else ok = state_cell<2, NDIM>(P, T.U, T.del, L.dt, c, Wp, Wn, L.status);
+    const double dt = step_dt(L.dt, L.dtp);
+    if (d == 0) ok = state_cell<0, NDIM>(P, T.U, T.del, dt, c, Wp, Wn, L.status);
+    else if (d == 1) ok = state_cell<1, NDIM>(P, T.U, T.del, dt, c, Wp, Wn, L.status);
+    else ok = state_cell<2, NDIM>(P, T.U, T.del, dt, c, Wp, Wn, L.status);
     if (!ok) {
       report_error(L.status, 1, t, ci[0], ci[1], ci[2], d);
       continue;
@@ -452,7 +457,7 @@ __global__ void __launch_bounds__(kThreads) k_update(DevParams P, Launch L,
     double* U = T.U;
     const long c = cidx(P, i, j, k);
     const double dx = T.del[0], dy = P.ndim > 1 ? T.del[1] : 1., dz = P.ndim > 2 ? T.del[2] : 1.;
-    const double dt = L.dt;
+    const double dt = step_dt(L.dt, L.dtp);
     double FL[8], FR[8], GL[8], GR[8], HL[8], HR[8];
     {
       const long nf = P.nface[0];
@@ -572,6 +577,25 @@ __global__ void __launch_bounds__(kThreads) k_dt(DevParams P, const double* __re
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// dt = cfl / max of the partial maxima, kept on the device; t += dt
+__global__ void __launch_bounds__(kThreads) k_dt_final(const double* __restrict__ partial, int n,
+                                                       double cfl, double* dtDev, double* tDev) {
+  __shared__ double red[kThreads];
+  double m = 0.;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, partial[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double dt = red[0] != 0. ? cfl / red[0] : 1.e300;
+    dtDev[0] = dt;
+    tDev[0] = tDev[0] + dt;
+  }
+}
+
 __global__ void k_reset_status(DevStatus* s) {
   s->code = 0;
   s->tile = s->i = s->j = s->k = s->dir = -1;
@@ -633,6 +657,73 @@ __global__ void __launch_bounds__(kThreads) k_fluxreg_conserve(DevParams P, RegL
   }
 }
 
+// ---- guard-cell fill on one level (Grid_fillGuardCells for a uniform block
+// grid, Grid/GridMain/.../Grid_fillGuardCells.F90) ----
+// Pass `axis` fills the guard slabs along that axis: rows interior in the
+// later axes and guard-inclusive in the earlier ones, so three passes also
+// fill edges and corners from the neighbours' already filled guards.
+// nbr[b*6 + 2*axis + side]: neighbour block, or FX_BC_OUTFLOW (zero
+// gradient) or FX_BC_REFLECT (mirror, normal velocity negated).
+__global__ void __launch_bounds__(kThreads) k_fill_guards(DevParams P, double* __restrict__ U,
+                                                          long nBlocks, long stride,
+                                                          const int* __restrict__ nbr, int axis) {
+  const int ng = P.ng;
+  const int ext[3] = {P.nx, P.ny, P.nz};
+  // extent of the slab along the two other axes
+  int r0 = 0, n0 = 1, r1 = 0, n1 = 1;   // ranges of the other axes (start, count)
+  const int oa = axis == 0 ? 1 : 0, ob = axis == 2 ? 1 : 2;
+  if (oa < P.ndim) {
+    r0 = oa < axis ? 0 : P.lo[oa];
+    n0 = oa < axis ? ext[oa] : (P.hi[oa] - P.lo[oa] + 1);
+  }
+  if (ob < P.ndim) {
+    r1 = ob < axis ? 0 : P.lo[ob];
+    n1 = ob < axis ? ext[ob] : (P.hi[ob] - P.lo[ob] + 1);
+  }
+  const long perBlock = 2L * ng * n0 * n1;
+  const long total = perBlock * nBlocks;
+  const long sAx = axis == 0 ? 1L : (axis == 1 ? (long)P.nx : (long)P.nx * P.ny);
+  const long sA = oa == 0 ? 1L : (oa == 1 ? (long)P.nx : (long)P.nx * P.ny);
+  const long sB = ob == 0 ? 1L : (ob == 1 ? (long)P.nx : (long)P.nx * P.ny);
+  const int lo = P.lo[axis], hi = P.hi[axis];
+  for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+       gid += (long)gridDim.x * blockDim.x) {
+    const long b = gid / perBlock;
+    long r = gid - b * perBlock;
+    const int ia = (int)(r % n0);
+    r /= n0;
+    const int ib = (int)(r % n1);
+    r /= n1;
+    const int gg = (int)(r % ng) + 1;          // 1..ng away from the interior
+    const int side = (int)(r / ng);
+    const long base = (long)(r0 + ia) * sA + (long)(r1 + ib) * sB;
+    const long dst = base + (long)(side ? hi + gg : lo - gg) * sAx;
+    const int nb = nbr[b * 6 + 2 * axis + side];
+    const double* src;
+    long sidx;
+    double vsign = 1.0;
+    if (nb >= 0) {
+      src = U + (long)nb * stride;
+      sidx = base + (long)(side ? lo + gg - 1 : hi - gg + 1) * sAx;
+    } else {
+      src = U + b * stride;
+      if (nb == FX_BC_REFLECT) {
+        sidx = base + (long)(side ? hi - gg + 1 : lo + gg - 1) * sAx;
+        vsign = -1.0;
+      } else {
+        sidx = base + (long)(side ? hi : lo) * sAx;
+      }
+    }
+    double* dstU = U + b * stride;
+    for (int v = 0; v < P.nunk; ++v) {
+      const long off = (long)v * P.ncell;
+      double val = src[off + sidx];
+      if (v == P.velSlot[axis]) val *= vsign;
+      dstU[off + dst] = val;
+    }
+  }
+}
+
 struct Lib {
   bool init = false;
   fx_params prm{};
@@ -658,6 +749,12 @@ struct Lib {
   size_t seedYCap = 0;
   double* fstates = nullptr;           // face states for the separate Riemann stage
   size_t fstatesCap = 0;
+  const double* dtDev = nullptr;       // set while capturing steps whose dt lives on the device
+  double* dtBuf = nullptr;             // [dt, t] of graph-captured runs
+  hipStream_t capStream = nullptr;     // library stream for graph capture (default priority)
+  std::vector<int> hnbr;               // neighbour table last uploaded (host copy)
+  int* dnbr = nullptr;
+  size_t dnbrCap = 0;
   long lastFaces = 0;                  // faces of the last exact-solver step
   int lastTiles = 0;
   unsigned char* dinplace = nullptr;
@@ -733,6 +830,7 @@ DevParams make_dev_params(const fx_params& p) {
   d.riemann = p.riemann;
   d.precision = p.precision;
   d.learned = p.learnedSeed;
+  for (int a = 0; a < 3; ++a) d.velSlot[a] = p.var[FX_VELX + a];
   d.fP = p.fP;
   d.smlrho = p.smlrho;
   d.smallE = p.smallE;
@@ -890,6 +988,16 @@ int record(int n) {
   return hipEventRecord(g.ev[n], g.stream) == hipSuccess ? 0 : 4;
 }
 
+// scratch of the separate Riemann stage, grown before (never during) a step
+int ensure_step_scratch(int nTiles, fx_status* st) {
+  const DevParams& P = g.dp;
+  if (P.riemann != FX_EXACT) return 0;
+  const long faces = (P.nface[0] + P.nface[1] + P.nface[2]) * (long)nTiles;
+  if (grow(&g.fstates, &g.fstatesCap, (size_t)kFaceComps * faces, st)) return 4;
+  if (P.learned && grow(&g.seedY, &g.seedYCap, (size_t)faces, st)) return 4;
+  return 0;
+}
+
 int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char* inPlace,
                 fx_status* st, bool tilesUploaded = false) {
   if (!g.init) {
@@ -923,15 +1031,12 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
   }
   if (march) {
     MarchLaunch M{g.dtiles, nTiles, P.nxb / TX, (P.ndim > 1 ? P.nyb : 1) / TY, g.status, dt,
-                  nullptr};
+                  nullptr, g.dtDev};
     const dim3 grid((unsigned)(nTiles * M.tilesX * M.tilesY));
-    Launch L{g.dtiles, nTiles, nullptr, g.status, dt};
+    Launch L{g.dtiles, nTiles, nullptr, g.status, dt, g.dtDev};
     const long perTile = P.nface[0] + P.nface[1] + P.nface[2];
     const long faces = perTile * nTiles;
-    if (P.riemann == FX_EXACT) {
-      if (grow(&g.fstates, &g.fstatesCap, (size_t)kFaceComps * faces, st)) return 4;
-      if (P.learned && grow(&g.seedY, &g.seedYCap, (size_t)faces, st)) return 4;
-    }
+    if (ensure_step_scratch(nTiles, st)) return 4;
     hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
     if (record(0)) return 4;
     if (P.riemann == FX_EXACT) {
@@ -962,7 +1067,7 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
     if (record(3)) return 4;
   } else {
   if (grow(&g.states, &g.statesCap, (size_t)P.scellsPerTile * kStateComps * nTiles, st)) return 4;
-  Launch L{g.dtiles, nTiles, g.states, g.status, dt};
+  Launch L{g.dtiles, nTiles, g.states, g.status, dt, g.dtDev};
   hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
   if (record(0)) return 4;
   {
@@ -1081,6 +1186,65 @@ int sync_or_async(fx_status* st) {
   return 0;
 }
 
+// tile list of a strided batch on the device (kept while the batch is unchanged)
+int prepare_batch(double* U, long nBlocks, long blockStride, const double del[3], fx_status* st) {
+  if (!g.init || !U || nBlocks < 0 || blockStride < g.dp.ncell * g.dp.nunk) return 3;
+  const bool same = g.batchValid && g.batchU == U && g.batchN == nBlocks &&
+                    g.batchStride == blockStride && g.batchDel[0] == del[0] &&
+                    g.batchDel[1] == del[1] && g.batchDel[2] == del[2];
+  if (!same) {
+    std::vector<fx_tile> tl((size_t)nBlocks);
+    for (long b = 0; b < nBlocks; ++b) {
+      fx_tile& t = tl[(size_t)b];
+      memset(&t, 0, sizeof t);
+      t.U = U + b * blockStride;
+      for (int d = 0; d < 3; ++d) t.del[d] = del[d];
+      t.level = 1;
+    }
+    if (upload_tiles(tl.data(), (int)nBlocks, true, st)) return 4;
+    g.batchU = U;
+    g.batchN = nBlocks;
+    g.batchStride = blockStride;
+    for (int d = 0; d < 3; ++d) g.batchDel[d] = del[d];
+    g.batchValid = true;
+  }
+  return 0;
+}
+
+// checked neighbour table, kept on the device until it changes
+int upload_neighbors(double* U, long nBlocks, long blockStride, const int* nbr, fx_status* st) {
+  const DevParams& P = g.dp;
+  bool ok = g.init && U && nbr && nBlocks > 0 && blockStride >= P.ncell * P.nunk;
+  for (int a = 0; ok && a < P.ndim; ++a)
+    if (P.hi[a] - P.lo[a] + 1 < P.ng) ok = false;   // neighbours supply ng interior layers
+  for (long e = 0; ok && e < nBlocks * 6; ++e) {
+    const int axis = (int)((e % 6) / 2);
+    if (axis >= P.ndim) continue;
+    if (!(nbr[e] >= 0 ? nbr[e] < nBlocks : (nbr[e] == FX_BC_OUTFLOW || nbr[e] == FX_BC_REFLECT)))
+      ok = false;
+  }
+  if (!ok) {
+    if (st) st->code = 3;
+    return 3;
+  }
+  const size_t n6 = (size_t)nBlocks * 6;
+  if (g.hnbr.size() != n6 || memcmp(g.hnbr.data(), nbr, n6 * sizeof(int)) != 0) {
+    if (grow(&g.dnbr, &g.dnbrCap, n6, st)) return 4;
+    g.hnbr.assign(nbr, nbr + n6);
+    HIPCALL(hipMemcpy(g.dnbr, g.hnbr.data(), n6 * sizeof(int), hipMemcpyHostToDevice), st);
+  }
+  return 0;
+}
+
+void launch_fill(double* U, long nBlocks, long blockStride) {
+  const DevParams& P = g.dp;
+  for (int axis = 0; axis < P.ndim; ++axis) {
+    const long slab = 2L * P.ng * (P.nx + P.ny + P.nz) * (P.nx + P.ny + P.nz);
+    hipLaunchKernelGGL(k_fill_guards, dim3(grid_for(slab * nBlocks)), dim3(kThreads), 0, g.stream,
+                       P, U, nBlocks, blockStride, g.dnbr, axis);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1104,11 +1268,13 @@ int fx_hydro_init(const fx_params* p) {
 
 void fx_hydro_finalize(void) {
   void* bufs[] = {g.states, g.flux, g.pavg, g.dtiles, g.dinplace, g.status, g.partial,
-                  g.dregTiles, g.dfaces, g.mlpPacked, g.seedPacked, g.seedY, g.fstates};
+                  g.dregTiles, g.dfaces, g.mlpPacked, g.seedPacked, g.seedY, g.fstates,
+                  g.dnbr, g.dtBuf};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : g.ev)
     if (e) (void)hipEventDestroy(e);
+  if (g.capStream) (void)hipStreamDestroy(g.capStream);
   g = Lib{};
 }
 
@@ -1131,7 +1297,7 @@ int fx_hydro_update_solution(const fx_tile* tiles, int nTiles, double dt, double
   }
   const DevParams& P = g.dp;
   if (upload_tiles(tiles, nTiles, false, st)) return 4;
-  Launch L{g.dtiles, nTiles, nullptr, g.status, dt};
+  Launch L{g.dtiles, nTiles, nullptr, g.status, dt, g.dtDev};
   hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
   hipLaunchKernelGGL((k_update<false, false>), dim3(grid_for(P.icells * nTiles)), dim3(kThreads), 0,
                      g.stream, P, L, (const unsigned char*)nullptr);
@@ -1141,28 +1307,11 @@ int fx_hydro_update_solution(const fx_tile* tiles, int nTiles, double dt, double
 
 int fx_hydro_advance_batch(double* U, long nBlocks, long blockStride, const double del[3], double dt,
                            fx_status* st) {
-  if (!g.init || !U || nBlocks < 0 || blockStride < g.dp.ncell * g.dp.nunk) return 3;
-  const bool same = g.batchValid && g.batchU == U && g.batchN == nBlocks &&
-                    g.batchStride == blockStride && g.batchDel[0] == del[0] &&
-                    g.batchDel[1] == del[1] && g.batchDel[2] == del[2];
-  if (!same) {
-    std::vector<fx_tile> tl((size_t)nBlocks);
-    for (long b = 0; b < nBlocks; ++b) {
-      fx_tile& t = tl[(size_t)b];
-      memset(&t, 0, sizeof t);
-      t.U = U + b * blockStride;
-      for (int d = 0; d < 3; ++d) t.del[d] = del[d];
-      t.level = 1;
-    }
-    if (upload_tiles(tl.data(), (int)nBlocks, true, st)) return 4;
-    g.batchU = U;
-    g.batchN = nBlocks;
-    g.batchStride = blockStride;
-    for (int d = 0; d < 3; ++d) g.batchDel[d] = del[d];
-    g.batchValid = true;
-  }
+  const int rc = prepare_batch(U, nBlocks, blockStride, del, st);
+  if (rc) return rc;
   return run_compute(nullptr, (int)nBlocks, dt, nullptr, st, true);
 }
+
 
 int fx_hydro_compute_dt(const double* U, long nBlocks, long blockStride, const double del[3],
                         double cfl, double* dtOut) {
@@ -1267,6 +1416,87 @@ int fx_riemann_batch(const double* states, long n, double gamma, const float* pa
     for (int k = 0; k < 3; ++k)
       if (hipEventElapsedTime(&g.stageMs[k], g.ev[k], g.ev[k + 1]) != hipSuccess) g.stageMs[k] = -1.f;
   return rc;
+}
+
+int fx_grid_fill_guards(double* U, long nBlocks, long blockStride, const int* nbr,
+                        fx_status* st) {
+  if (st) memset(st, 0, sizeof *st);
+  int rc = upload_neighbors(U, nBlocks, blockStride, nbr, st);
+  if (rc) return rc;
+  launch_fill(U, nBlocks, blockStride);
+  HIPCALL(hipGetLastError(), st);
+  if (g.async) return 0;
+  HIPCALL(hipStreamSynchronize(g.stream), st);
+  return 0;
+}
+
+int fx_grid_run_graph(double* U, long nBlocks, long blockStride, const int* nbr,
+                      const double del[3], double cfl, int stepsPerGraph, int nReplays,
+                      double* tOut, fx_status* st) {
+  if (st) memset(st, 0, sizeof *st);
+  if (stepsPerGraph < 1 || nReplays < 1 || !(cfl > 0.) || !del) {
+    if (st) st->code = 3;
+    return 3;
+  }
+  if (g.dp.learned) {            // the seeded step is not offered under capture
+    if (st) st->code = 3;
+    return 3;
+  }
+  int rc = upload_neighbors(U, nBlocks, blockStride, nbr, st);
+  if (!rc) rc = prepare_batch(U, nBlocks, blockStride, del, st);
+  if (!rc) rc = ensure_step_scratch((int)nBlocks, st);
+  if (rc) return rc;
+  const DevParams& P = g.dp;
+  const int dtGrid = grid_for(P.icells * nBlocks);
+  if (g.partialCap < dtGrid) {
+    if (g.partial) HIPCALL(hipFree(g.partial), st);
+    g.partial = nullptr;
+    HIPCALL(hipMalloc((void**)&g.partial, sizeof(double) * dtGrid), st);
+    g.partialCap = dtGrid;
+  }
+  if (!g.dtBuf) HIPCALL(hipMalloc((void**)&g.dtBuf, 2 * sizeof(double)), st);
+  if (!g.capStream) HIPCALL(hipStreamCreateWithFlags(&g.capStream, hipStreamNonBlocking), st);
+  const hipStream_t saved = g.stream;
+  const int savedAsync = g.async, savedTiming = g.timing;
+  // the clock starts at zero; dt and t stay on the device for the whole run
+  if (saved) HIPCALL(hipStreamSynchronize(saved), st);
+  g.stream = g.capStream;
+  double zero2[2] = {0.0, 0.0};
+  HIPCALL(hipMemcpy(g.dtBuf, zero2, sizeof zero2, hipMemcpyHostToDevice), st);
+  g.async = 1;
+  g.timing = 0;
+  g.dtDev = g.dtBuf;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int err = 0;
+  if (hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) err = 4;
+  for (int s = 0; !err && s < stepsPerGraph; ++s) {
+    launch_fill(U, nBlocks, blockStride);
+    hipLaunchKernelGGL(k_dt, dim3(dtGrid), dim3(kThreads), 0, g.stream, P, U, nBlocks, blockStride,
+                       1. / del[0], 1. / del[1], 1. / del[2], g.partial);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kThreads), 0, g.stream, g.partial, dtGrid, cfl,
+                       g.dtBuf, g.dtBuf + 1);
+    if (run_compute(nullptr, (int)nBlocks, 0.0, nullptr, nullptr, true)) err = 4;
+  }
+  if (hipStreamEndCapture(g.stream, &graph) != hipSuccess) err = 4;
+  if (!err && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) err = 4;
+  for (int r = 0; !err && r < nReplays; ++r)
+    if (hipGraphLaunch(exec, g.stream) != hipSuccess) err = 4;
+  if (!err && hipStreamSynchronize(g.stream) != hipSuccess) err = 4;
+  double td[2] = {0.0, 0.0};
+  if (!err && hipMemcpy(td, g.dtBuf, sizeof td, hipMemcpyDeviceToHost) != hipSuccess) err = 4;
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  g.dtDev = nullptr;
+  g.async = savedAsync;
+  g.timing = savedTiming;
+  g.stream = saved;
+  if (err) {
+    if (st) st->code = 4;
+    return 4;
+  }
+  if (tOut) *tOut = td[1];
+  return finish(st);
 }
 
 int fx_hydro_face_states(double* S, long cap, long* n) {

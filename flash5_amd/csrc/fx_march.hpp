@@ -45,6 +45,7 @@ struct MarchLaunch {
   DevStatus* status;
   double dt;
   double* fstates;     // RS_STATES: per-face states for the separate Riemann stage
+  const double* dtp;   // device-resident dt (graph-captured steps), or null
 };
 
 // Riemann-stage variant of the march: stage C stores each face's left and
@@ -213,7 +214,7 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
   const int ey = (oy + TY > P.hi[1]) ? 1 : 0;
   const fx_tile& T = L.tiles[t];
   const double* __restrict__ U = T.U;
-  const double dt = L.dt;
+  const double dt = step_dt(L.dt, L.dtp);
   const double dx = T.del[0], dy = T.del[1], dz = T.del[2];
   const long sx = 1, sy = P.nx, sz = (long)P.nx * P.ny;
   const int klo = P.lo[2], khi = P.hi[2];

@@ -1,0 +1,287 @@
+"""Multi-step drivers on uniform block grids: the Sod shock tube along the
+diagonal (FLASH's Simulation/SimulationMain/Sod) and a Kelvin-Helmholtz
+shear layer.
+
+Each step does what FLASH's driver and Hydro do on one level:
+guard-cell fill (Grid_fillGuardCells, Hydro.F90:121), the unsplit update of
+every block in place, and the CFL time step from the updated interior cells
+(Hydro_computeDt.F90:186-282, min over blocks).  Everything runs on the GPU
+through libfx_hydro.so; the host only loops.  FLASH's driver also limits how
+fast dt may grow between steps (Driver_computeDt); that belongs to the
+driver, not the hydro path, and is left out: every step uses the CFL dt, and
+the last one is shortened to land on tmax.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from .hydro import FxError, FxStatus, Hydro, HydroParams, load_library
+
+BC_OUTFLOW, BC_REFLECT = -1, -2
+DENS, VELX, VELY, VELZ, PRES, ENER, GAMC, GAME, TEMP, EINT = range(10)
+
+
+def uniform_neighbors(nblk, periodic, ndim: int) -> np.ndarray:
+    """Neighbour table [nBlocks][6] (-x, +x, -y, +y, -z, +z) of a uniform
+    block grid, blocks numbered b = bi + nbx*(bj + nby*bk)."""
+    nblk = tuple(nblk) + (1,) * (3 - len(nblk))
+    periodic = tuple(periodic) + (False,) * (3 - len(periodic))
+    nb = np.full((nblk[0] * nblk[1] * nblk[2], 6), BC_OUTFLOW, dtype=np.int32)
+    for bk in range(nblk[2]):
+        for bj in range(nblk[1]):
+            for bi in range(nblk[0]):
+                b = bi + nblk[0] * (bj + nblk[1] * bk)
+                idx = [bi, bj, bk]
+                for a in range(ndim):
+                    for side in (0, 1):
+                        n = list(idx)
+                        n[a] += 1 if side else -1
+                        if n[a] < 0 or n[a] >= nblk[a]:
+                            if not periodic[a]:
+                                continue
+                            n[a] %= nblk[a]
+                        nb[b, 2 * a + side] = n[0] + nblk[0] * (n[1] + nblk[1] * n[2])
+    return nb
+
+
+def eos_fill(U, gamma: float, gas_constant: float = 8.3144598e7, abar: float = 1.0):
+    """Ideal-gas EOS (MODE_DENS_PRES) on numpy blocks: eint, ener, temp,
+    gamc, game from dens, velocities and pres (eos_idealGamma.F90)."""
+    ekin = 0.5 * (U[:, VELX] ** 2 + U[:, VELY] ** 2 + U[:, VELZ] ** 2)
+    U[:, EINT] = U[:, PRES] * (1.0 / (gamma - 1.0)) / U[:, DENS]
+    U[:, ENER] = U[:, EINT] + ekin
+    U[:, TEMP] = U[:, EINT] / ((1.0 / (gamma - 1.0)) * gas_constant) * abar
+    U[:, GAMC] = gamma
+    U[:, GAME] = gamma
+    return U
+
+
+def sod_blocks(nblk: int = 16, nb: int = 8, ng: int = 4, gamma: float = 1.4,
+               angle_deg: float = 45.0, posn: float = 1.0):
+    """FLASH's Sod setup on [0,1]^2 (Sod/Simulation_initBlock.F90:140-223,
+    Simulation_init.F90:106-121): left state (1, 0, 1), right (0.125, 0, 0.1),
+    the discontinuity x = posn - y*yCos/xCos, cells it cuts take the average.
+    Returns numpy blocks [nBlocks][10][1][ny][nx] (guards left for the fill)."""
+    xcos = math.cos(angle_deg * 0.0174532925)
+    ycos = math.sqrt(1.0 - xcos * xcos)
+    nx = nb + 2 * ng
+    U = np.zeros((nblk * nblk, 10, 1, nx, nx))
+    h = 1.0 / (nblk * nb)
+    left = (1.0, 1.0)           # rho, p
+    right = (0.125, 0.1)
+    for bj in range(nblk):
+        for bi in range(nblk):
+            b = bi + nblk * bj
+            for j in range(nx):
+                y = (bj * nb + j - ng + 0.5) * h
+                lposn = posn - y * ycos / xcos
+                for i in range(nx):
+                    xl = (bi * nb + i - ng) * h
+                    xr = xl + h
+                    if xr <= lposn:
+                        rho, p = left
+                    elif xl < lposn < xr:
+                        rho, p = 0.5 * (left[0] + right[0]), 0.5 * (left[1] + right[1])
+                    else:
+                        rho, p = right
+                    U[b, DENS, 0, j, i] = rho
+                    U[b, PRES, 0, j, i] = p
+    return eos_fill(U, gamma)
+
+
+def kh_blocks(ncells: int = 2048, nb: int = 16, ng: int = 4, gamma: float = 1.4,
+              seed: int = 0x5EED0003, width: float = 0.01):
+    """Kelvin-Helmholtz shear layer on the periodic unit square: rho = 2 for
+    |y - 0.5| < 0.25 else 1, u = +0.5 inside and -0.5 outside joined by tanh
+    profiles of the given width, v = 0.01 sin(4 pi x) plus 1e-4 seeded white
+    noise, p = 2.5."""
+    nblk = ncells // nb
+    nx = nb + 2 * ng
+    rng = np.random.default_rng(seed)
+    h = 1.0 / ncells
+    U = np.zeros((nblk * nblk, 10, 1, nx, nx))
+    xs = (np.arange(nx) - ng + 0.5) * h
+    for bj in range(nblk):
+        y = bj * nb * h + xs
+        inside = np.abs(y - 0.5) < 0.25
+        u = 0.5 * (np.tanh((y - 0.25) / width) - np.tanh((y - 0.75) / width)) - 0.5
+        for bi in range(nblk):
+            b = bi + nblk * bj
+            x = bi * nb * h + xs
+            U[b, DENS, 0] = np.where(inside, 2.0, 1.0)[:, None]
+            U[b, VELX, 0] = u[:, None]
+            U[b, VELY, 0] = 0.01 * np.sin(4 * np.pi * x)[None, :] + 1e-4 * rng.standard_normal((nx, nx))
+            U[b, PRES, 0] = 2.5
+    return eos_fill(U, gamma)
+
+
+@dataclass
+class RunStats:
+    steps: int
+    time: float
+    wall_s: float
+    zone_updates: int
+    fallbacks: int
+
+    @property
+    def zone_updates_per_s(self) -> float:
+        return self.zone_updates / self.wall_s if self.wall_s > 0 else 0.0
+
+
+class UniformGrid:
+    """A single-level grid of equal blocks advanced by the hot path."""
+
+    def __init__(self, params: HydroParams, U, nbr: np.ndarray, deltas, cfl: float = 0.8):
+        import torch
+        self.params = params
+        self.hydro = Hydro(params)
+        self.lib = load_library()
+        self.U = U if isinstance(U, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(U))
+        self.U = self.U.to("cuda").contiguous()
+        self.nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+        self.deltas = tuple(float(d) for d in deltas)
+        self.cfl = cfl
+        self.t = 0.0
+        self.nsteps = 0
+
+    def fill_guards(self) -> None:
+        import ctypes as C
+        st = FxStatus()
+        nb = self.U.shape[0]
+        rc = self.lib.fx_grid_fill_guards(
+            C.cast(C.c_void_p(self.U.data_ptr()), C.POINTER(C.c_double)), nb, self.U[0].numel(),
+            self.nbr.ctypes.data_as(C.POINTER(C.c_int)), C.byref(st))
+        if rc:
+            raise FxError("fx_grid_fill_guards", st, rc)
+
+    def step(self, dt_max: float = math.inf) -> float:
+        """Guard fill, CFL dt (capped by dt_max), unsplit update in place."""
+        self.fill_guards()
+        dt = min(self.hydro.compute_dt(self.U, self.deltas, self.cfl), dt_max)
+        self.hydro.advance_batch(self.U, dt, self.deltas)
+        self.t += dt
+        self.nsteps += 1
+        return dt
+
+    def run_graph(self, steps_per_graph: int, replays: int) -> float:
+        """steps_per_graph whole steps (guard fill, device CFL dt, update)
+        captured once in a HIP graph and replayed; returns the simulated time
+        the replays covered."""
+        import ctypes as C
+        st = FxStatus()
+        t = C.c_double(0.0)
+        self.hydro._reinit()
+        rc = self.lib.fx_grid_run_graph(
+            C.cast(C.c_void_p(self.U.data_ptr()), C.POINTER(C.c_double)), self.U.shape[0],
+            self.U[0].numel(), self.nbr.ctypes.data_as(C.POINTER(C.c_int)),
+            (C.c_double * 3)(*self.deltas), self.cfl, steps_per_graph, replays, C.byref(t),
+            C.byref(st))
+        if rc:
+            raise FxError("fx_grid_run_graph", st, rc)
+        self.t += t.value
+        self.nsteps += steps_per_graph * replays
+        return t.value
+
+    def run(self, tmax: float = math.inf, nsteps: int | None = None) -> RunStats:
+        import torch
+        p = self.params
+        zones = self.U.shape[0] * p.nxb * (p.nyb if p.ndim > 1 else 1) * (p.nzb if p.ndim > 2 else 1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n = 0
+        fb = 0
+        while (nsteps is None or n < nsteps) and self.t < tmax * (1 - 1e-14):
+            self.step(tmax - self.t)
+            fb += int(self.hydro.last_status.fallbacks)
+            n += 1
+        torch.cuda.synchronize()
+        return RunStats(n, self.t, time.perf_counter() - t0, n * zones, fb)
+
+
+def sod(nblk: int = 16, nb: int = 8, tmax: float = 0.2, **kw):
+    """Config-1 Sod problem: 128x128 cells as 16x16 blocks of 8x8, PPM with
+    MC limiting (Sod/flash.par) and HLLC, flattening on, eintSwitch 1e-4,
+    outflow on every side, CFL 0.8, to t = 0.2."""
+    hp = HydroParams(ndim=2, nxb=nb, nyb=nb, nzb=1, gamma=1.4, slopeLimiter="mc",
+                     use_flattening=True, eintSwitch=1e-4, **kw)
+    U = sod_blocks(nblk, nb)
+    grid = UniformGrid(hp, U, uniform_neighbors((nblk, nblk), (False, False), 2),
+                       (1.0 / (nblk * nb),) * 2 + (1.0,))
+    return grid, grid.run(tmax=tmax)
+
+
+def kh(ncells: int = 2048, nb: int = 16, nsteps: int = 200, **kw):
+    """Config-3 Kelvin-Helmholtz run: 2048^2 cells as 16x16 blocks, periodic."""
+    hp = HydroParams(ndim=2, nxb=nb, nyb=nb, nzb=1, gamma=1.4, **kw)
+    nblk = ncells // nb
+    grid = UniformGrid(hp, kh_blocks(ncells, nb), uniform_neighbors((nblk, nblk), (True, True), 2),
+                       (1.0 / ncells,) * 2 + (1.0,))
+    return grid, grid.run(nsteps=nsteps)
+
+
+def kh_precision_study(ncells: int = 2048, nb: int = 16, nsteps: int = 200):
+    """Config-3 precision study: the same Kelvin-Helmholtz run in FP64 and in
+    the mixed mode (FP32 reconstruction and fluxes, FP64 update), stepped with
+    the FP64 run's dt sequence so both integrate the same times, then the
+    per-variable L1 / Linf (sfocu norms) of mixed against FP64 over all
+    interior cells.  The FP64 GPU path agrees with the FP64 oracle bit for
+    bit, so it stands in for the oracle at this size."""
+    import torch
+    from . import sfocu
+    from .hydro import PREC_FP64, PREC_MIXED
+    nblk = ncells // nb
+    U0 = kh_blocks(ncells, nb)
+    nbr = uniform_neighbors((nblk, nblk), (True, True), 2)
+    deltas = (1.0 / ncells,) * 2 + (1.0,)
+    runs = {}
+    dts = []
+    for name, prec in (("fp64", PREC_FP64), ("mixed", PREC_MIXED)):
+        hp = HydroParams(ndim=2, nxb=nb, nyb=nb, nzb=1, gamma=1.4, precision=prec)
+        grid = UniformGrid(hp, U0, nbr, deltas)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in range(nsteps):
+            if name == "fp64":
+                dts.append(grid.step())
+            else:
+                grid.fill_guards()
+                grid.hydro.advance_batch(grid.U, dts[s], deltas)
+                grid.t += dts[s]
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        runs[name] = (grid.U.cpu().numpy(), wall, grid.t)
+        del grid
+    names = {"dens": DENS, "velx": VELX, "vely": VELY, "pres": PRES, "ener": ENER, "eint": EINT,
+             "temp": TEMP}
+    Ua, Ub = runs["mixed"][0], runs["fp64"][0]
+    norms = sfocu.compare(Ua, Ub, 1, names, 2, 4)
+    zones = ncells * ncells * nsteps
+    return {"cells": ncells * ncells, "steps": nsteps, "t_end": runs["fp64"][2],
+            "l1": {k: v[0] for k, v in norms.items()}, "linf": {k: v[1] for k, v in norms.items()},
+            "fp64_zone_updates_per_s": zones / runs["fp64"][1],
+            "mixed_zone_updates_per_s": zones / runs["mixed"][1]}
+
+
+def main() -> None:
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="multi-step runs: Sod (config 1), KH precision study")
+    ap.add_argument("case", choices=["sod", "kh"])
+    ap.add_argument("--ncells", type=int, default=2048)
+    ap.add_argument("--steps", type=int, default=200)
+    a = ap.parse_args()
+    if a.case == "sod":
+        grid, st = sod()
+        print(json.dumps({"case": "sod 128^2 16x16 blocks of 8x8", "steps": st.steps,
+                          "t": grid.t, "zone_updates_per_s": st.zone_updates_per_s,
+                          "fallbacks": st.fallbacks}))
+    else:
+        print(json.dumps(kh_precision_study(a.ncells, 16, a.steps)))
+
+
+if __name__ == "__main__":
+    main()

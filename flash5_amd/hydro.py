@@ -101,6 +101,10 @@ EXPORTS = {
                           _DP, C.POINTER(C.c_int), C.POINTER(FxStatus)], C.c_int),
     "fx_hydro_set_seed_params": ([C.POINTER(C.c_float)], C.c_int),
     "fx_hydro_face_states": ([_DP, C.c_long, C.POINTER(C.c_long)], C.c_int),
+    "fx_grid_fill_guards": ([_DP, C.c_long, C.c_long, C.POINTER(C.c_int), C.POINTER(FxStatus)],
+                            C.c_int),
+    "fx_grid_run_graph": ([_DP, C.c_long, C.c_long, C.POINTER(C.c_int), C.c_double * 3, C.c_double,
+                           C.c_int, C.c_int, _DP, C.POINTER(FxStatus)], C.c_int),
     "fx_hydro_set_stream": ([C.c_void_p], None),
     "fx_hydro_set_async": ([C.c_int], None),
     "fx_hydro_set_timing": ([C.c_int], None),

@@ -198,6 +198,29 @@ int fx_hydro_set_seed_params(const float* params);
  * nonzero when cap < *n or no such step ran. */
 int fx_hydro_face_states(double* states, long cap, long* n);
 
+/* ---- guard cells on one level of equal blocks ----
+ * Grid_fillGuardCells(CENTER, ALLDIR) for a uniform grid of independent
+ * block storage (Grid/GridMain/UG/Grid_fillGuardCells.F90): every variable
+ * of every guard cell is copied from the neighbour block's interior, or set
+ * by the boundary condition.  nbr: host array [nBlocks][6] in the order
+ * -x, +x, -y, +y, -z, +z holding a block index or one of the codes below;
+ * it is checked, then kept on the device until it changes.  Blocks need at
+ * least NGUARD interior cells per axis. */
+enum { FX_BC_OUTFLOW = -1,   /* zero gradient */
+       FX_BC_REFLECT = -2 }; /* mirror, normal velocity negated */
+int fx_grid_fill_guards(double* U, long nBlocks, long blockStride, const int* nbr,
+                        fx_status* st);
+
+/* Whole steps of a uniform grid captured in a HIP graph: each captured step
+ * fills the guard cells (as fx_grid_fill_guards), computes the CFL dt on
+ * the device (as fx_hydro_compute_dt, kept in device memory), and advances
+ * every block (as fx_hydro_advance_batch with that dt).  stepsPerGraph steps
+ * are captured once and the graph is replayed nReplays times; *tOut gets the
+ * elapsed simulation time (the clock starts at 0).  Not with learnedSeed. */
+int fx_grid_run_graph(double* U, long nBlocks, long blockStride, const int* nbr,
+                      const double del[3], double cfl, int stepsPerGraph, int nReplays,
+                      double* tOut, fx_status* st);
+
 /* Work on the given HIP stream (hipStream_t as void*); NULL = default. */
 void fx_hydro_set_stream(void* stream);
 /* 1: entry points return without synchronising (needed for graph capture). */

@@ -88,6 +88,8 @@ def lib():
                                        C.POINTER(Stats)]
         L.ox_exact_pstar.argtypes = [C.c_double] * 8 + [C.c_double, C.c_int, _DP, _DP]
         L.ox_exact_pstar.restype = C.c_int
+        L.ox_exact_sample.argtypes = [C.c_double] * 8 + [_DP]
+        L.ox_exact_sample.restype = C.c_int
         L.ox_fluxreg_put.argtypes = [C.POINTER(Params), C.POINTER(_DP), C.c_void_p, C.c_int, _DP]
         L.ox_fluxreg_conserve.argtypes = [C.POINTER(Params), C.POINTER(_DP), C.c_void_p, C.c_int,
                                           _DP]
@@ -269,3 +271,10 @@ def fluxreg_conserve(p: Params, coarse_fluxes, faces, reg) -> None:
     reg = np.ascontiguousarray(reg, dtype=np.float64)
     lib().ox_fluxreg_conserve(C.byref(p), _flux_ptrs(coarse_fluxes),
                               C.cast(_faces_c(faces), C.c_void_p), len(faces), _p(reg))
+
+
+def exact_sample(gamma, rl, ul, pl, rr, ur, pr, S):
+    """Exact (rho, u, p) at x/t = S."""
+    out = np.zeros(3)
+    lib().ox_exact_sample(gamma, rl, ul, pl, rr, ur, pr, S, _p(out))
+    return out

@@ -549,12 +549,12 @@ int ox_exact_pstar(double g, double rl, double ul, double pl, double rr,
   return ok ? it : -maxit;
 }
 
-/* sample the exact solution at S = x/t = 0 (Toro sec. 4.5) */
-static void exact_sample(double g, double rl, double ul, double pl, double rr,
-                         double ur, double pr, double pm, double um,
-                         double* r, double* u, double* pp, int* leftside) {
+/* sample the exact solution at S = x/t (Toro sec. 4.5); the Godunov flux
+ * uses S = 0 */
+static void exact_sample_at(double g, double rl, double ul, double pl, double rr,
+                            double ur, double pr, double pm, double um, double S,
+                            double* r, double* u, double* pp, int* leftside) {
   double cl = sqrt(g * pl / rl), cr = sqrt(g * pr / rr);
-  const double S = 0.;
   if (S <= um) {                                     /* left of contact */
     *leftside = 1;
     if (pm <= pl) {                                  /* left rarefaction */
@@ -594,6 +594,21 @@ static void exact_sample(double g, double rl, double ul, double pl, double rr,
   *u = 2. / (g + 1.) * (-cr + (g - 1.) / 2. * ur + S);
   *r = rr * pow(c / cr, 2. / (g - 1.));
   *pp = pr * pow(c / cr, 2. * g / (g - 1.));
+}
+
+static void exact_sample(double g, double rl, double ul, double pl, double rr,
+                         double ur, double pr, double pm, double um,
+                         double* r, double* u, double* pp, int* leftside) {
+  exact_sample_at(g, rl, ul, pl, rr, ur, pr, pm, um, 0., r, u, pp, leftside);
+}
+
+int ox_exact_sample(double g, double rl, double ul, double pl, double rr, double ur,
+                    double pr, double S, double out[3]) {
+  double pm, um;
+  int it = ox_exact_pstar(g, rl, ul, pl, rr, ur, pr, -1., 1e-14, 100, &pm, &um);
+  int left;
+  exact_sample_at(g, rl, ul, pl, rr, ur, pr, pm, um, S, &out[0], &out[1], &out[2], &left);
+  return it;
 }
 
 int ox_exact_riemann(const ox_params* p, int dir, const double VL[9],

@@ -147,6 +147,11 @@ int ox_exact_pstar(double gamma, double rl, double ul, double pl, double rr,
                    double ur, double pr, double guess, double tol, int maxit,
                    double* pstar, double* ustar);
 
+/* Exact solution (rho, u, p) at x/t = S of a Riemann problem (Toro sec.
+ * 4.5); returns the Newton iterations spent on p*. */
+int ox_exact_sample(double gamma, double rl, double ul, double pl, double rr,
+                    double ur, double pr, double S, double out[3]);
+
 /* ---- flux register (Cartesian, refinement ratio 2) ----
  * Grid_putFluxData -> Grid_addFineToFluxRegister(zeroFullRegister)
  * (Grid/GridMain/AMR/Amrex/Grid_addFineToFluxRegister.F90:128-171) and
