@@ -473,45 +473,33 @@ __device__ __forceinline__ double two_shock_guess(double g, double rl, double ul
   return fmax(1.e-30, pts);
 }
 
-// Newton from p0; returns iterations (negative when maxit was reached)
-__device__ __forceinline__ int exact_newton(double g, double rl, double pl, double cl, double rr,
-                                            double pr, double cr, double du, double p0, double tol,
-                                            int maxit, double& pstar) {
-  double p = p0;
-  for (int it = 1; it <= maxit; ++it) {
-    double fl, fld, fr, frd;
-    exact_fk(p, rl, pl, cl, g, fl, fld);
-    exact_fk(p, rr, pr, cr, g, fr, frd);
-    double pn = p - (fl + fr + du) / (fld + frd);
-    if (!(pn > 0.)) pn = 1.e-30;
-    const double change = 2. * fabs((pn - p) / (pn + p));
-    p = pn;
-    if (change <= tol) {
-      pstar = p;
-      return it;
-    }
-  }
-  pstar = p;
-  return -maxit;
-}
-
 // Newton from the better of two starts: the pressure function is evaluated
 // at both, and the iteration begins at the one with the smaller residual
-// |f(p)| (its f and f' serve the first step).  Returns iterations, negative
-// when maxit was reached; pick = 0 when pa was taken.
+// |f(p)| (its f and f' serve the first step).  pb <= 0 means no second
+// start: plain Newton from pa, in the CPU oracle's operation order.
+// Returns iterations, negative when maxit was reached; pick = 0 when pa was
+// taken.
 __device__ __forceinline__ int exact_newton_pick(double g, double rl, double pl, double cl,
                                                  double rr, double pr, double cr, double du,
                                                  double pa, double pb, double tol, int maxit,
                                                  double& pstar, int& pick) {
-  double fl, fld, fr, frd, gl, gld, gr, grd;
+  double fl, fld, fr, frd;
   exact_fk(pa, rl, pl, cl, g, fl, fld);
   exact_fk(pa, rr, pr, cr, g, fr, frd);
-  exact_fk(pb, rl, pl, cl, g, gl, gld);
-  exact_fk(pb, rr, pr, cr, g, gr, grd);
-  const double fa = fl + fr + du, fb = gl + gr + du;
-  pick = fabs(fa) <= fabs(fb) ? 0 : 1;
-  double p = pick ? pb : pa;
-  double f = pick ? fb : fa, fd = pick ? gld + grd : fld + frd;
+  double p = pa, f = fl + fr + du, fd = fld + frd;
+  pick = 0;
+  if (pb > 0.) {
+    double gl, gld, gr, grd;
+    exact_fk(pb, rl, pl, cl, g, gl, gld);
+    exact_fk(pb, rr, pr, cr, g, gr, grd);
+    const double fb = gl + gr + du;
+    if (!(fabs(f) <= fabs(fb))) {
+      pick = 1;
+      p = pb;
+      f = fb;
+      fd = gld + grd;
+    }
+  }
   for (int it = 1; it <= maxit; ++it) {
     if (it > 1) {
       exact_fk(p, rl, pl, cl, g, fl, fld);
@@ -583,56 +571,105 @@ __device__ __forceinline__ void exact_sample(double g, double rl, double ul, dou
   pp = pr * pow(c / cr, 2. * g / (g - 1.));
 }
 
-// Godunov flux from the exact solution; returns 0 ok, 1 non-positive state,
-// 2 vacuum generated.  guess > 0 seeds Newton; when that start does not
-// converge, Newton restarts from the two-shock guess (counted as fallback).
-template <int DIR>
-__device__ __forceinline__ int exact_flux(const DevParams& P, const double* VL, const double* VR,
-                                          double guess, double F[6], ExactStats& es) {
-  const double g = VL[P_GAMC];
-  if (!(VL[P_DENS] > 0. && VR[P_DENS] > 0. && VL[P_PRES] > 0. && VR[P_PRES] > 0.)) return 1;
-  const double rl = VL[P_DENS], ul = VL[P_VELX + DIR], pl = VL[P_PRES];
-  const double rr = VR[P_DENS], ur = VR[P_VELX + DIR], pr = VR[P_PRES];
+// Exact solution at x/t = 0 of the face problem (rl, ul, pl | rr, ur, pr)
+// with the face-normal velocities ul, ur; returns 0 ok, 1 non-positive state,
+// 2 vacuum generated.  guess > 0 seeds Newton: it starts from the better of
+// the seed and the two-shock guess, and a seeded start that does not
+// converge restarts from the two-shock guess (counted as fallback).  One
+// Newton loop serves both passes, which keeps a seeded kernel's register
+// budget that of the unseeded one.  SEEDED = false ignores guess.
+template <bool SEEDED>
+__device__ __forceinline__ int exact_riemann(const DevParams& P, double g, double rl, double ul,
+                                             double pl, double rr, double ur, double pr,
+                                             double guess, ExactStats& es, double& r, double& u,
+                                             double& pp, bool& left) {
+  if (!(rl > 0. && rr > 0. && pl > 0. && pr > 0.)) return 1;
   const double cl = sqrt(g * pl / rl), cr = sqrt(g * pr / rr);
   if (2. / (g - 1.) * (cl + cr) <= ur - ul) return 2;
   const double du = ur - ul;
+  const double ts = two_shock_guess(g, rl, ul, pl, cl, rr, ur, pr, cr);
   double pm;
-  int it;
-  if (guess > 0.) {
-    // seeded: start from the better of the seed and the two-shock guess;
-    // a seeded start that does not converge restarts from the two-shock guess
-    const double ts = two_shock_guess(g, rl, ul, pl, cl, rr, ur, pr, cr);
-    int pick;
-    it = exact_newton_pick(g, rl, pl, cl, rr, pr, cr, du, ts, guess, P.exactTol, P.exactMaxIter,
-                           pm, pick);
-    if (it < 0 && pick == 1) {
-      es.iters += (unsigned long long)(-it);
-      es.fallbacks += 1;
-      it = exact_newton(g, rl, pl, cl, rr, pr, cr, du, ts, P.exactTol, P.exactMaxIter, pm);
-    } else if (it < 0) {
-      es.fallbacks += 1;
-    }
-  } else {
-    it = exact_newton(g, rl, pl, cl, rr, pr, cr, du,
-                      two_shock_guess(g, rl, ul, pl, cl, rr, ur, pr, cr), P.exactTol,
-                      P.exactMaxIter, pm);
+  int it, pick;
+  if constexpr (!SEEDED) {
+    it = exact_newton_pick(g, rl, pl, cl, rr, pr, cr, du, ts, -1.0, P.exactTol, P.exactMaxIter, pm,
+                           pick);
     if (it < 0) es.fallbacks += 1;
+  } else {
+    double pb = guess;
+    for (;;) {
+      it = exact_newton_pick(g, rl, pl, cl, rr, pr, cr, du, ts, pb, P.exactTol, P.exactMaxIter,
+                             pm, pick);
+      if (it < 0 && (pb > 0. || guess <= 0.)) es.fallbacks += 1;   // a restart counts once
+      if (it >= 0 || pick == 0) break;
+      es.iters += (unsigned long long)(-it);
+      pb = -1.0;
+    }
   }
   es.iters += (unsigned long long)(it < 0 ? -it : it);
   double fl, fld, fr, frd;
   exact_fk(pm, rl, pl, cl, g, fl, fld);
   exact_fk(pm, rr, pr, cr, g, fr, frd);
   const double um = 0.5 * (ul + ur) + 0.5 * (fr - fl);
+  exact_sample(g, rl, ul, pl, cl, rr, ur, pr, cr, pm, um, r, u, pp, left);
+  return 0;
+}
+
+// prim2flx with the direction chosen at run time (same operations and
+// order as prim2flx<DIR>)
+__device__ __forceinline__ void prim2flx_dir(int d, double r, double vx, double vy, double vz,
+                                             double p, double game, double fP, double F[6]) {
+  const double u2 = vx * vx + vy * vy + vz * vz;
+  const double E = 0.5 * r * u2 + p / (game - 1.);
+  const double un = d == 0 ? vx : (d == 1 ? vy : vz);
+  F[5] = p;
+  F[0] = r * un;
+  F[1] = F[0] * vx;
+  F[2] = F[0] * vy;
+  F[3] = F[0] * vz;
+  const double fp = fP * p;
+  if (d == 0) F[1] = F[1] + fp;
+  else if (d == 1) F[2] = F[2] + fp;
+  else F[3] = F[3] + fp;
+  F[4] = (E + p) * un;
+}
+
+// Godunov flux F[0..7] across a face normal to direction d from the exact
+// solution, with the aux internal-energy fluxes (hy_getFaceFlux.F90:428-442).
+// The face states sit in global memory as components 0..5 = dens, velx,
+// vely, velz, pres, eint at Lb[q * s] (left) and Rb[q * s] (right); reading
+// them with the run-time offset d keeps them out of private arrays, and one
+// copy of the solver serves all directions.  gamc is the left cell's, game
+// the cells' on either side.  Returns exact_riemann's codes.
+template <bool SEEDED>
+__device__ __forceinline__ int exact_face_flux(const DevParams& P, int d, const double* Lb,
+                                               const double* Rb, long s, double gamc,
+                                               double gameL, double gameR, double guess,
+                                               double F[8], ExactStats& es) {
+  const double rl = Lb[0], pl = Lb[4 * s], rr = Rb[0], pr = Rb[4 * s];
+  const double ul = Lb[(1 + d) * s], ur = Rb[(1 + d) * s];
   double r, u, pp;
   bool left;
-  exact_sample(g, rl, ul, pl, cl, rr, ur, pr, cr, pm, um, r, u, pp, left);
-  double W[8];
-#pragma unroll
-  for (int v = 0; v < 8; ++v) W[v] = left ? VL[v] : VR[v];
-  W[P_DENS] = r;
-  W[P_VELX + DIR] = u;
-  W[P_PRES] = pp;
-  prim2flx<DIR>(W, P.fP, F);
+  const int e = exact_riemann<SEEDED>(P, gamc, rl, ul, pl, rr, ur, pr, guess, es, r, u, pp, left);
+  if (e) return e;
+  // the sampled state takes its tangential velocities and game from the side
+  // of the contact the face lies on
+  const double* Sb = left ? Lb : Rb;
+  double vx = Sb[s], vy = Sb[2 * s], vz = Sb[3 * s];
+  if (d == 0) vx = u;
+  else if (d == 1) vy = u;
+  else vz = u;
+  prim2flx_dir(d, r, vx, vy, vz, pp, left ? gameL : gameR, P.fP, F);
+  F[6] = 0.;
+  F[7] = 0.;
+  if (P.aux) {
+    if (F[0] > 0.) {
+      F[7] = F[0] / rl;
+      F[6] = F[0] * Lb[5 * s];
+    } else {
+      F[7] = F[0] / rr;
+      F[6] = F[0] * Rb[5 * s];
+    }
+  }
   return 0;
 }
 

@@ -301,36 +301,11 @@ __device__ __forceinline__ long band_index(const DevParams& P, int d, int i, int
   return ((long)cc * P.bext[d][1] + b) * P.bext[d][0] + a;
 }
 
-// Face flux densities F[0..7] (hy_getFaceFlux.F90:324-442): the Riemann
-// solver RS on the face states VL, VR, then the aux internal-energy fluxes.
-// HLLC runs in the reconstruction type R; the exact solver always runs its
-// Newton iteration and convergence test in FP64.
-template <int D, int RS, typename R>
-__device__ __forceinline__ int face_flux(const DevParams& P, const R* VL, const R* VR, double F[8],
-                                         ExactStats& es) {
-  R f0;
-  if constexpr (RS == FX_EXACT) {
-    double L[8], Rt[8];
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-      L[v] = double(VL[v]);
-      Rt[v] = double(VR[v]);
-    }
-    const int ierr = exact_flux<D>(P, L, Rt, -1.0, F, es);
-    if (ierr) return ierr;
-    f0 = R(F[0]);
-  } else if constexpr (sizeof(R) == sizeof(double)) {
-    const int ierr = hllc<D>(P, VL, VR, F);
-    if (ierr) return ierr;
-    f0 = F[0];
-  } else {
-    R Fr[6];
-    const int ierr = hllc<D>(P, VL, VR, Fr);
-    if (ierr) return ierr;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) F[q] = double(Fr[q]);
-    f0 = Fr[0];
-  }
+// aux internal-energy fluxes F[6], F[7] from the mass flux f0 (upwind face
+// state)
+template <typename R>
+__device__ __forceinline__ void aux_fluxes(const DevParams& P, R f0, const R* VL, const R* VR,
+                                           double F[8]) {
   F[6] = 0.;
   F[7] = 0.;
   if (P.aux) {
@@ -342,11 +317,35 @@ __device__ __forceinline__ int face_flux(const DevParams& P, const R* VL, const 
       F[6] = double(f0 * VR[P_EINT]);
     }
   }
+}
+
+// Face flux densities F[0..7] (hy_getFaceFlux.F90:324-442): HLLC on the
+// face states VL, VR in the reconstruction type R, then the aux
+// internal-energy fluxes.  The exact solver takes exact_face_flux.
+template <int D, int RS, typename R>
+__device__ __forceinline__ int face_flux(const DevParams& P, const R* VL, const R* VR, double F[8],
+                                         ExactStats& es) {
+  static_assert(RS == FX_HLLC, "face_flux serves HLLC");
+  R f0;
+  if constexpr (sizeof(R) == sizeof(double)) {
+    const int ierr = hllc<D>(P, VL, VR, F);
+    if (ierr) return ierr;
+    f0 = F[0];
+  } else {
+    R Fr[6];
+    const int ierr = hllc<D>(P, VL, VR, Fr);
+    if (ierr) return ierr;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) F[q] = double(Fr[q]);
+    f0 = Fr[0];
+  }
+  aux_fluxes(P, f0, VL, VR, F);
   return 0;
 }
 
 template <int RS>
-__global__ void __launch_bounds__(kThreads) k_fluxes(DevParams P, Launch L) {
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3)))
+k_fluxes(DevParams P, Launch L) {
   const long perTile = P.nface[0] + P.nface[1] + P.nface[2];
   const long total = perTile * L.nTiles;
   ExactStats es{0ULL, 0ULL};
@@ -374,31 +373,37 @@ __global__ void __launch_bounds__(kThreads) k_fluxes(DevParams P, Launch L) {
     const long bc = P.bcells[d];
     const long bl = band_index(P, d, im, jm, km);
     const long br = band_index(P, d, i, j, k);
-    double VL[8], VR[8];
-    VL[P_DENS] = S[0 * bc + bl];
-    VL[P_VELX] = S[1 * bc + bl];
-    VL[P_VELY] = S[2 * bc + bl];
-    VL[P_VELZ] = S[3 * bc + bl];
-    VL[P_PRES] = S[4 * bc + bl];
-    VL[P_EINT] = S[5 * bc + bl];
-    VR[P_DENS] = S[6 * bc + br];
-    VR[P_VELX] = S[7 * bc + br];
-    VR[P_VELY] = S[8 * bc + br];
-    VR[P_VELZ] = S[9 * bc + br];
-    VR[P_PRES] = S[10 * bc + br];
-    VR[P_EINT] = S[11 * bc + br];
-    // face-state gammas equal the cell's (PPM.F90:733-736 under
-    // FLASH_EOS_GAMMA, and the first-order fallback)
     const long cl = cidx(P, im, jm, km), cr = cidx(P, i, j, k);
-    VL[P_GAMC] = U[P.voff[FX_GAMC] + cl];
-    VL[P_GAME] = U[P.voff[FX_GAME] + cl];
-    VR[P_GAMC] = U[P.voff[FX_GAMC] + cr];
-    VR[P_GAME] = U[P.voff[FX_GAME] + cr];
     double F[8];
     int ierr;
-    if (d == 0) ierr = face_flux<0, RS>(P, VL, VR, F, es);
-    else if (d == 1) ierr = face_flux<1, RS>(P, VL, VR, F, es);
-    else ierr = face_flux<2, RS>(P, VL, VR, F, es);
+    if constexpr (RS == FX_EXACT) {
+      ierr = exact_face_flux<false>(P, d, S + bl, S + 6 * bc + br, bc, U[P.voff[FX_GAMC] + cl],
+                                    U[P.voff[FX_GAME] + cl], U[P.voff[FX_GAME] + cr], -1.0, F,
+                                    es);
+    } else {
+      double VL[8], VR[8];
+      VL[P_DENS] = S[0 * bc + bl];
+      VL[P_VELX] = S[1 * bc + bl];
+      VL[P_VELY] = S[2 * bc + bl];
+      VL[P_VELZ] = S[3 * bc + bl];
+      VL[P_PRES] = S[4 * bc + bl];
+      VL[P_EINT] = S[5 * bc + bl];
+      VR[P_DENS] = S[6 * bc + br];
+      VR[P_VELX] = S[7 * bc + br];
+      VR[P_VELY] = S[8 * bc + br];
+      VR[P_VELZ] = S[9 * bc + br];
+      VR[P_PRES] = S[10 * bc + br];
+      VR[P_EINT] = S[11 * bc + br];
+      // face-state gammas equal the cell's (PPM.F90:733-736 under
+      // FLASH_EOS_GAMMA, and the first-order fallback)
+      VL[P_GAMC] = U[P.voff[FX_GAMC] + cl];
+      VL[P_GAME] = U[P.voff[FX_GAME] + cl];
+      VR[P_GAMC] = U[P.voff[FX_GAMC] + cr];
+      VR[P_GAME] = U[P.voff[FX_GAME] + cr];
+      if (d == 0) ierr = face_flux<0, RS>(P, VL, VR, F, es);
+      else if (d == 1) ierr = face_flux<1, RS>(P, VL, VR, F, es);
+      else ierr = face_flux<2, RS>(P, VL, VR, F, es);
+    }
     if (ierr) {
       report_error(L.status, 2, t, i, j, k, d);
       continue;
@@ -745,6 +750,7 @@ struct Lib {
   MlpPacked* mlpPacked = nullptr;      // BF16 fragments for fx_mlp_* / fx_riemann_batch
   MlpPacked* seedPacked = nullptr;     // network seeding the exact solver in hydro steps
   bool seedReady = false;
+  double seedGate = 0.2;               // fx_mlp_set_gate
   float* seedY = nullptr;              // network outputs per interface / face
   size_t seedYCap = 0;
   double* fstates = nullptr;           // face states for the separate Riemann stage
@@ -1044,7 +1050,8 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
       M.fstates = g.fstates;
       launch_march<RS_STATES>(P, M, TX, grid);
       if (record(1)) return 4;
-      FaceLaunch F{g.dtiles, nTiles, g.fstates, P.learned ? g.seedY : nullptr, g.status};
+      FaceLaunch F{g.dtiles, nTiles, g.fstates, P.learned ? g.seedY : nullptr, g.status,
+                   g.seedGate};
       g.lastFaces = faces;
       g.lastTiles = nTiles;
       if (P.learned) {
@@ -1398,15 +1405,16 @@ int fx_riemann_batch(const double* states, long n, double gamma, const float* pa
     if (grow(&g.seedY, &g.seedYCap, (size_t)n, st)) return 4;
     if (record(0)) return 4;
     hipLaunchKernelGGL(k_mlp_seed_states, wave_grid(n), dim3(kLearnThreads), 0, g.stream, states,
-                       n, gamma, g.mlpPacked, g.seedY);
+                       n, gamma, g.mlpPacked, g.seedY, g.seedGate);
     if (record(1)) return 4;
     hipLaunchKernelGGL(k_riemann_batch<true>, dim3(grid_for(n)), dim3(kThreads), 0, g.stream,
-                       states, n, gamma, g.seedY, tol, maxit, pstar, iters, g.status);
+                       states, n, gamma, g.seedY, tol, maxit, pstar, iters, g.status, g.seedGate);
     if (record(2)) return 4;
   } else {
     if (record(0) || record(1)) return 4;
     hipLaunchKernelGGL(k_riemann_batch<false>, dim3(grid_for(n)), dim3(kThreads), 0, g.stream,
-                       states, n, gamma, (const float*)nullptr, tol, maxit, pstar, iters, g.status);
+                       states, n, gamma, (const float*)nullptr, tol, maxit, pstar, iters, g.status,
+                       g.seedGate);
     if (record(2)) return 4;
   }
   if (record(3)) return 4;
@@ -1503,10 +1511,16 @@ int fx_hydro_face_states(double* S, long cap, long* n) {
   if (!n) return 3;
   *n = g.lastFaces;
   if (!g.init || g.lastFaces <= 0 || !S || cap < g.lastFaces) return 3;
-  FaceLaunch F{g.dtiles, g.lastTiles, g.fstates, nullptr, g.status};
+  FaceLaunch F{g.dtiles, g.lastTiles, g.fstates, nullptr, g.status, g.seedGate};
   hipLaunchKernelGGL(k_collect_faces, dim3(grid_for(g.lastFaces)), dim3(kThreads), 0, g.stream,
                      g.dp, F, g.lastFaces, S);
   return sync_or_async(nullptr);
+}
+
+int fx_mlp_set_gate(double gate) {
+  if (!(gate >= 0.0)) return 3;
+  g.seedGate = gate;
+  return 0;
 }
 
 int fx_hydro_set_seed_params(const float* params) {

@@ -106,19 +106,20 @@ __global__ void __launch_bounds__(kThreads) k_mlp_adam(float* __restrict__ prm,
 }
 
 // The network predicts y = ln(p* / P0), P0 = sqrt(pL pR).  Weak interfaces
-// (every |feature| below kSeedGate) keep the two-shock guess, which is
+// (every |feature| below the gate, fx_mlp_set_gate) keep the two-shock guess, which is
 // already accurate to high order there; for the others Newton starts from
 // whichever of P0 * exp(y) and the two-shock guess has the smaller residual
 // of the pressure function.
-constexpr double kSeedGate = 0.05;
-
-__device__ __forceinline__ bool seed_gate(double g, double rl, double ul, double pl, double rr,
-                                          double ur, double pr) {
+// largest feature magnitude of an interface
+__device__ __forceinline__ double seed_strength(double g, double rl, double ul, double pl,
+                                                double rr, double ur, double pr) {
   const double P0 = sqrt(pl * pr), R0 = sqrt(rl * rr), C0 = sqrt(g * P0 / R0);
-  const double m = fmax(fmax(fabs(0.5 * log(rl / rr)), fabs(0.5 * log(pl / pr))),
-                        fabs(0.5 * (ur - ul) / C0));
-  return m >= kSeedGate;
+  return fmax(fmax(fabs(0.5 * log(rl / rr)), fabs(0.5 * log(pl / pr))),
+              fabs(0.5 * (ur - ul) / C0));
 }
+
+// seed value of a face whose Newton iteration keeps the two-shock start
+constexpr float kNoSeed = 3.0e38f;
 
 __device__ __forceinline__ double seed_from_y(float y, double pl, double pr) {
   return sqrt(pl * pr) * exp((double)fminf(fmaxf(y, -60.f), 60.f));
@@ -128,7 +129,7 @@ __device__ __forceinline__ double seed_from_y(float y, double pl, double pr) {
 // the Newton kernel below reads it, so each keeps its own register budget).
 __global__ void __launch_bounds__(kLearnThreads) k_mlp_seed_states(
     const double* __restrict__ S, long n, double gamma, const MlpPacked* __restrict__ W,
-    float* __restrict__ y) {
+    float* __restrict__ y, double gate) {
   __shared__ float xs[kLearnWaves][64 * 8];
   const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
   const long nw = (long)gridDim.x * kLearnWaves;
@@ -137,7 +138,7 @@ __global__ void __launch_bounds__(kLearnThreads) k_mlp_seed_states(
     const bool act = i < n;
     const IfaceState s = load_iface(S, n, i, act);
     // wavefronts whose interfaces are all weak skip the network
-    const bool need = act && seed_gate(gamma, s.rl, s.ul, s.pl, s.rr, s.ur, s.pr);
+    const bool need = act && seed_strength(gamma, s.rl, s.ul, s.pl, s.rr, s.ur, s.pr) >= gate;
     if (__ballot(need) == 0) continue;
     float f[6];
     double P0;
@@ -163,7 +164,8 @@ __global__ void __launch_bounds__(kLearnThreads) k_mlp_seed_states(
 template <bool SEEDED>
 __global__ void __launch_bounds__(kThreads) k_riemann_batch(
     const double* __restrict__ S, long n, double gamma, const float* __restrict__ yseed,
-    double tol, int maxit, double* __restrict__ pstar, int* __restrict__ iters, DevStatus* st) {
+    double tol, int maxit, double* __restrict__ pstar, int* __restrict__ iters, DevStatus* st,
+    double gate) {
   ExactStats es{0ULL, 0ULL};
   const double g = gamma;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -179,21 +181,17 @@ __global__ void __launch_bounds__(kThreads) k_riemann_batch(
     }
     const double du = s.ur - s.ul;
     const double ts = two_shock_guess(g, s.rl, s.ul, s.pl, cl, s.rr, s.ur, s.pr, cr);
-    const bool use = SEEDED && seed_gate(g, s.rl, s.ul, s.pl, s.rr, s.ur, s.pr);
-    double pm;
-    int pick = 0, it;
-    if (use)
-      it = exact_newton_pick(g, s.rl, s.pl, cl, s.rr, s.pr, cr, du, ts,
-                             seed_from_y(yseed[i], s.pl, s.pr), tol, maxit, pm, pick);
-    else
-      it = exact_newton(g, s.rl, s.pl, cl, s.rr, s.pr, cr, du, ts, tol, maxit, pm);
-    int total = it < 0 ? -it : it;
-    if (use && pick == 1 && it < 0) {
-      es.fallbacks += 1;
-      it = exact_newton(g, s.rl, s.pl, cl, s.rr, s.pr, cr, du, ts, tol, maxit, pm);
+    const bool use = SEEDED && seed_strength(g, s.rl, s.ul, s.pl, s.rr, s.ur, s.pr) >= gate;
+    double pm, pb = use ? seed_from_y(yseed[i], s.pl, s.pr) : -1.0;
+    int it, total = 0;
+    // one Newton loop for the seeded start and its two-shock restart
+    for (;;) {
+      int pick;
+      it = exact_newton_pick(g, s.rl, s.pl, cl, s.rr, s.pr, cr, du, ts, pb, tol, maxit, pm, pick);
       total += it < 0 ? -it : it;
-    } else if (it < 0) {
-      es.fallbacks += 1;
+      if (it < 0 && (pb > 0. || !use)) es.fallbacks += 1;   // a restart counts once
+      if (it >= 0 || pick == 0) break;
+      pb = -1.0;
     }
     es.iters += (unsigned long long)total;
     pstar[i] = pm;
@@ -211,6 +209,7 @@ struct FaceLaunch {
   const double* fstates;   // [tile][dir][comp][face], from k_march<RS_STATES>
   const float* y;          // network seed per face (flattened like gid), or null
   DevStatus* status;
+  double gate;             // faces weaker than this keep the two-shock start
 };
 
 struct FaceRef {
@@ -237,62 +236,40 @@ __device__ __forceinline__ FaceRef face_ref(const DevParams& P, long gid, long p
   return f;
 }
 
-__device__ __forceinline__ void load_face_states(const DevParams& P, const FaceLaunch& F,
-                                                 const FaceRef& f, double VL[8], double VR[8]) {
-  const double* base = F.fstates + f.t * fstate_tile_stride(P) + fstate_dir_offset(P, f.d);
-  const long nf = P.nface[f.d];
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    VL[comp_slot(q)] = base[q * nf + f.r];
-    VR[comp_slot(q)] = base[(6 + q) * nf + f.r];
-  }
-  const double* U = F.tiles[f.t].U;
-  VL[P_GAMC] = U[P.voff[FX_GAMC] + f.cL];
-  VL[P_GAME] = U[P.voff[FX_GAME] + f.cL];
-  VR[P_GAMC] = U[P.voff[FX_GAMC] + f.cR];
-  VR[P_GAME] = U[P.voff[FX_GAME] + f.cR];
-}
-
+// One code path serves the three directions: the face-normal velocity and
+// the flux component that takes the pressure are picked at run time.
 template <bool SEEDED>
-__global__ void __launch_bounds__(kThreads) k_face_solve(DevParams P, FaceLaunch F) {
+// (3 waves per SIMD fit without spilling the solver's registers)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3)))
+k_face_solve(DevParams P, FaceLaunch F) {
   const long perTile = P.nface[0] + P.nface[1] + P.nface[2];
   const long total = perTile * F.nTiles;
   ExactStats es{0ULL, 0ULL};
   for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
        gid += (long)gridDim.x * blockDim.x) {
     const FaceRef f = face_ref(P, gid, perTile);
-    double VL[8], VR[8], Fo[8];
-    load_face_states(P, F, f, VL, VR);
+    const int d = f.d;
+    const double* b = F.fstates + f.t * fstate_tile_stride(P) + fstate_dir_offset(P, d);
+    const long nf = P.nface[d];
+    // face states: comps 0..5 dens, velx, vely, velz, pres, eint; left, then right
+    const double* Lb = b + f.r;
+    const double* Rb = b + 6 * nf + f.r;
     double guess = -1.0;
     if (SEEDED) {
-      const double g = VL[P_GAMC];
-      const double rl = VL[P_DENS], pl = VL[P_PRES], rr = VR[P_DENS], pr = VR[P_PRES];
-      const double ul = VL[P_VELX + f.d], ur = VR[P_VELX + f.d];
-      if (rl > 0. && rr > 0. && pl > 0. && pr > 0. && seed_gate(g, rl, ul, pl, rr, ur, pr))
-        guess = seed_from_y(F.y[gid], pl, pr);
+      const float y = F.y[gid];
+      if (y != kNoSeed) guess = seed_from_y(y, Lb[4 * nf], Rb[4 * nf]);
     }
-    int ierr;
-    if (f.d == 0) ierr = exact_flux<0>(P, VL, VR, guess, Fo, es);
-    else if (f.d == 1) ierr = exact_flux<1>(P, VL, VR, guess, Fo, es);
-    else ierr = exact_flux<2>(P, VL, VR, guess, Fo, es);
+    const double* U = F.tiles[f.t].U;
+    double Fo[8];
+    const int ierr = exact_face_flux<SEEDED>(P, d, Lb, Rb, nf, U[P.voff[FX_GAMC] + f.cL],
+                                             U[P.voff[FX_GAME] + f.cL], U[P.voff[FX_GAME] + f.cR],
+                                             guess, Fo, es);
     if (ierr) {
       report_error(F.status, 2, f.t, (int)(f.cR % P.nx), (int)((f.cR / P.nx) % P.ny),
-                   (int)(f.cR / ((long)P.nx * P.ny)), f.d);
+                   (int)(f.cR / ((long)P.nx * P.ny)), d);
       continue;
     }
-    Fo[6] = 0.;
-    Fo[7] = 0.;
-    if (P.aux) {
-      if (Fo[0] > 0.) {
-        Fo[7] = Fo[0] / VL[P_DENS];
-        Fo[6] = Fo[0] * VL[P_EINT];
-      } else {
-        Fo[7] = Fo[0] / VR[P_DENS];
-        Fo[6] = Fo[0] * VR[P_EINT];
-      }
-    }
-    double* out = F.tiles[f.t].flux[f.d];
-    const long nf = P.nface[f.d];
+    double* out = F.tiles[f.t].flux[d];
 #pragma unroll
     for (int q = 0; q < FX_NFLUX; ++q) out[q * nf + f.r] = Fo[q];
   }
@@ -323,10 +300,14 @@ __global__ void __launch_bounds__(kLearnThreads) k_face_seed(DevParams P, FaceLa
       ur = b[(7 + f.d) * nf + f.r];
       pr = b[10 * nf + f.r];
     }
-    // wavefronts whose faces are all weak (or invalid) skip the network
+    // weak (or invalid) faces get no seed; wavefronts of such faces only
+    // skip the network
     const bool need = act && rl > 0. && rr > 0. && pl > 0. && pr > 0. &&
-                      seed_gate(P.gamma, rl, ul, pl, rr, ur, pr);
-    if (__ballot(need) == 0) continue;
+                      seed_strength(P.gamma, rl, ul, pl, rr, ur, pr) >= F.gate;
+    if (__ballot(need) == 0) {
+      if (act) y[gid] = kNoSeed;
+      continue;
+    }
     float x[6];
     double P0;
     mlp_features(P.gamma, rl, ul, pl, rr, ur, pr, x, P0);
@@ -338,7 +319,7 @@ __global__ void __launch_bounds__(kLearnThreads) k_face_seed(DevParams P, FaceLa
     MlpActs A;
     float yo;
     mlp_forward_wave(*W, xs[wv], lane, A, yo);
-    if (act) y[gid] = yo;
+    if (act) y[gid] = need ? yo : kNoSeed;
     wave_lds_sync();
   }
 }

@@ -275,10 +275,27 @@ def main() -> None:
     ap.add_argument("--steps", type=int, default=200)
     a = ap.parse_args()
     if a.case == "sod":
+        import torch
         grid, st = sod()
+        # the same grid advanced 60 steps eagerly and 60 steps as replays of
+        # 20 graph-captured steps, from the initial state
+        rates = {}
+        for mode in ("eager", "graph"):
+            g2, _ = sod(tmax=0.0)
+            zones = g2.U.shape[0] * 64
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if mode == "eager":
+                for _ in range(60):
+                    g2.step()
+            else:
+                g2.run_graph(steps_per_graph=20, replays=3)
+            torch.cuda.synchronize()
+            rates[mode] = 60 * zones / (time.perf_counter() - t0)
         print(json.dumps({"case": "sod 128^2 16x16 blocks of 8x8", "steps": st.steps,
                           "t": grid.t, "zone_updates_per_s": st.zone_updates_per_s,
-                          "fallbacks": st.fallbacks}))
+                          "fallbacks": st.fallbacks,
+                          "first_60_steps_zone_updates_per_s": rates}))
     else:
         print(json.dumps(kh_precision_study(a.ncells, 16, a.steps)))
 

@@ -100,6 +100,7 @@ EXPORTS = {
     "fx_riemann_batch": ([_DP, C.c_long, C.c_double, C.POINTER(C.c_float), C.c_int, C.c_double,
                           _DP, C.POINTER(C.c_int), C.POINTER(FxStatus)], C.c_int),
     "fx_hydro_set_seed_params": ([C.POINTER(C.c_float)], C.c_int),
+    "fx_mlp_set_gate": ([C.c_double], C.c_int),
     "fx_hydro_face_states": ([_DP, C.c_long, C.POINTER(C.c_long)], C.c_int),
     "fx_grid_fill_guards": ([_DP, C.c_long, C.c_long, C.POINTER(C.c_int), C.POINTER(FxStatus)],
                             C.c_int),

@@ -132,6 +132,13 @@ class SeedNet:
         return hist
 
 
+def set_gate(gate: float) -> None:
+    """Interfaces whose largest |feature| is below gate keep the two-shock start."""
+    rc = load_library().fx_mlp_set_gate(gate)
+    if rc:
+        raise FxError("fx_mlp_set_gate", rc=rc)
+
+
 def solve(S, gamma: float, params=None, maxit: int = 50, tol: float = 1e-12):
     """Batched exact Riemann problems: (pstar float64, iters int32, status)."""
     import torch
@@ -155,7 +162,7 @@ def targets(S, pstar, gamma: float = 1.4):
 
 
 def study(n_test_log2: int = 24, n_train_log2: int = 22, steps: int = 3000, gamma: float = 1.4,
-          seed: int = 0x5EED0004):
+          seed: int = 0x5EED0004, keep_params: bool = False):
     """Learned-seed study: train on 2^n_train solver-emitted states, then solve
     2^n_test held-out interfaces from the two-shock start and from the seed.
     Returns a dict of iteration counts, fallbacks, p* agreement and rates."""
@@ -190,10 +197,60 @@ def study(n_test_log2: int = 24, n_train_log2: int = 22, steps: int = 3000, gamm
                      "not_converged": int((it < 0).sum().item()),
                      "interfaces_per_s": Ste.shape[1] / dt, "ms": dt * 1e3}
     rel = ((res["learned"] - res["two_shock"]).abs() / res["two_shock"])
-    return {"interfaces": Ste.shape[1], "train_states": 1 << n_train_log2, "train_steps": steps,
-            "train_s": train_s, "loss": [h[1] for h in hist], "gamma": gamma,
-            "pstar_rel_diff_max": rel.max().item(), "pstar_rel_diff_mean": rel.mean().item(),
-            **out}
+    r = {"interfaces": Ste.shape[1], "train_states": 1 << n_train_log2, "train_steps": steps,
+         "train_s": train_s, "loss": [h[1] for h in hist], "gamma": gamma,
+         "pstar_rel_diff_max": rel.max().item(), "pstar_rel_diff_mean": rel.mean().item(), **out}
+    if keep_params:
+        r["params"] = net.params
+    return r
+
+
+def strength(S, gamma: float):
+    """Largest |feature| of each interface (the gate's measure)."""
+    import torch
+    rl, ul, pl, rr, ur, pr = S
+    P0, R0 = torch.sqrt(pl * pr), torch.sqrt(rl * rr)
+    C0 = torch.sqrt(gamma * P0 / R0)
+    return torch.maximum(torch.maximum((0.5 * torch.log(rl / rr)).abs(),
+                                       (0.5 * torch.log(pl / pr)).abs()),
+                         (0.5 * (ur - ul) / C0).abs())
+
+
+def strength_bins(params, n_log2: int = 22, gamma: float = 1.4, seed: int = 0x5EED0005,
+                  edges=(0, 0.02, 0.05, 0.1, 0.2, 0.5, 1, 2, 100)):
+    """Mean Newton iterations per strength bin from the two-shock start and
+    from the seed with the gate off (held-out interfaces)."""
+    S = sample_interfaces(1 << n_log2, seed=seed, gamma=gamma)
+    m = strength(S, gamma)
+    _, i0, _ = solve(S, gamma)
+    set_gate(0.0)
+    try:
+        _, i1, _ = solve(S, gamma, params=params)
+    finally:
+        set_gate(0.2)
+    rows = []
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        sel = (m >= lo) & (m < hi)
+        if int(sel.sum()):
+            rows.append({"bin": [lo, hi], "n": int(sel.sum()),
+                         "two_shock": i0[sel].abs().float().mean().item(),
+                         "seeded": i1[sel].abs().float().mean().item()})
+    return rows
+
+
+def batch_face_strengths(blocks: int = 256, n: int = 16, gamma: float = 5.0 / 3.0):
+    """Strength quantiles of the face states of one exact-solver step of the
+    3-D batch workload."""
+    from . import workloads as W
+    from .hydro import Hydro, HydroParams
+    U = W.fourier_blocks(blocks, n=n, gamma=gamma, device="cuda")
+    dt = W.cfl_dt(U, 3, 4, (1.0 / n,) * 3, 0.8)
+    hy = Hydro(HydroParams(ndim=3, nxb=n, nyb=n, nzb=n, gamma=gamma, RiemannSolver="exact"))
+    hy.advance_batch(U, dt, (1.0 / n,) * 3)
+    m = strength(hy.face_states(), gamma)
+    return {"faces": int(m.numel()),
+            "quantiles": {str(q): m.quantile(q).item() for q in (0.5, 0.9, 0.99, 0.999)},
+            "frac_at_or_above": {str(t): (m >= t).float().mean().item() for t in (0.05, 0.1, 0.2, 0.5)}}
 
 
 def main() -> None:
@@ -203,8 +260,15 @@ def main() -> None:
     ap.add_argument("--test-log2", type=int, default=24)
     ap.add_argument("--train-log2", type=int, default=22)
     ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--bins", action="store_true",
+                    help="also: iterations per strength bin, and batch-workload face strengths")
     a = ap.parse_args()
-    print(json.dumps(study(a.test_log2, a.train_log2, a.steps)))
+    res = study(a.test_log2, a.train_log2, a.steps, keep_params=a.bins)
+    if a.bins:
+        params = res.pop("params")
+        res["strength_bins"] = strength_bins(params)
+        res["batch_face_strengths"] = batch_face_strengths()
+    print(json.dumps(res))
 
 
 if __name__ == "__main__":

@@ -171,8 +171,10 @@ int fx_fluxreg_conserve(const fx_tile* coarse, int nCoarse, const fx_fluxreg_fac
  * Features of an interface with P0 = sqrt(pL pR), R0 = sqrt(rhoL rhoR),
  * C0 = sqrt(gamma P0/R0), ubar = (uL+uR)/2: (ln rhoL/R0, ln pL/P0,
  * (uL-ubar)/C0, ln rhoR/R0, ln pR/P0, (uR-ubar)/C0); target ln(pstar/P0).
- * The seed P0 exp(y) is used where some feature has magnitude >= 0.05;
- * weaker interfaces start from the two-shock guess. 
+ * The seed P0 exp(y) is considered where some feature has magnitude at or
+ * above the gate (fx_mlp_set_gate, default 0.2); Newton then starts from
+ * whichever of the seed and the two-shock guess has the smaller residual of
+ * the pressure function.  Weaker interfaces start from the two-shock guess. 
  * states: device SoA [6][n] = rhoL, uL, pL, rhoR, uR, pR. */
 enum { FX_MLP_NPARAM = 4673 };
 int fx_mlp_features(const double* states, long n, double gamma, float* x /* [n][6] */);
@@ -189,6 +191,7 @@ int fx_mlp_adam(float* params, const float* grad, float* m, float* v, int step, 
  * negative when even the restart did not converge. */
 int fx_riemann_batch(const double* states, long n, double gamma, const float* params,
                      int maxit, double tol, double* pstar, int* iters, fx_status* st);
+int fx_mlp_set_gate(double gate);
 /* Network used when fx_params.learnedSeed = 1 and riemann = FX_EXACT
  * (NULL clears it).  The seeded step needs the fused kernel path (2-D, 3-D). */
 int fx_hydro_set_seed_params(const float* params);
