@@ -18,6 +18,7 @@
 // the reference stores face states for (hy_getRiemannState.F90:553-591).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -51,6 +52,26 @@ struct Launch {
   double dt;
   const double* dtp;     // device-resident dt (graph-captured steps), or null
 };
+
+// Tiles of several block classes in one launch: each tile's class selects its
+// parameters, and per-tile prefix offsets map a flat cell (or face) index to
+// its tile.
+struct ClassMap {
+  const DevParams* cls;        // parameters per class
+  const unsigned char* tcls;   // class of each tile
+  const long* start;           // items before tile t; start[nTiles] = all items
+};
+
+// t with start[t] <= gid < start[t + 1] (start non-decreasing, start[0] = 0)
+__device__ __forceinline__ int find_tile(const long* __restrict__ start, int n, long gid) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= gid) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 
 // the step's dt: from device memory when the host does not know it
 __device__ __forceinline__ double step_dt(double dt, const double* dtp) { return dtp ? *dtp : dt; }
@@ -425,21 +446,19 @@ __device__ __forceinline__ long face_index(const DevParams& P, int d, int i, int
 
 // FROM_STATES: face-averaged pressure from this call's states; otherwise
 // read from the tile's pavg array (hy_updateSolution path).
+// Interior cell r of tile t: conservative update, energy fix and EOS
+// (hy_unsplitUpdate.F90:440-915, hy_energyFix.F90:188-251, Eos_wrapped).
 template <bool FROM_STATES, bool WRITE_PAVG_ONLY>
-__global__ void __launch_bounds__(kThreads) k_update(DevParams P, Launch L,
-                                                     const unsigned char* __restrict__ inPlace) {
-  const long perTile = P.icells;
-  const long total = perTile * L.nTiles;
+__device__ __forceinline__ void update_cell(const DevParams& P, const Launch& L, int t, long r,
+                                            const unsigned char* __restrict__ inPlace) {
   const int nxi = P.nxb, nyi = P.ndim > 1 ? P.nyb : 1;
-  for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
-       gid += (long)gridDim.x * blockDim.x) {
-    const int t = (int)(gid / perTile);
-    const long r = gid - (long)t * perTile;
+  {
     const fx_tile& T = L.tiles[t];
     const bool doUpdate = WRITE_PAVG_ONLY ? false : (inPlace ? inPlace[t] != 0 : true);
-    const int i = (int)(r % nxi) + P.lo[0];
-    const int j = (int)((r / nxi) % nyi) + P.lo[1];
-    const int k = (int)(r / ((long)nxi * nyi)) + P.lo[2];
+    const int ri = (int)r;  // a tile holds < 2^31 cells
+    const int i = ri % nxi + P.lo[0];
+    const int j = (ri / nxi) % nyi + P.lo[1];
+    const int k = ri / (nxi * nyi) + P.lo[2];
     double pres = 0.;
     if (P.aux) {
       if (FROM_STATES) {
@@ -458,7 +477,7 @@ __global__ void __launch_bounds__(kThreads) k_update(DevParams P, Launch L,
         pres = T.pavg[r];
       }
     }
-    if (!doUpdate) continue;
+    if (!doUpdate) return;
     double* U = T.U;
     const long c = cidx(P, i, j, k);
     const double dx = T.del[0], dy = P.ndim > 1 ? T.del[1] : 1., dz = P.ndim > 2 ? T.del[2] : 1.;
@@ -548,6 +567,29 @@ __global__ void __launch_bounds__(kThreads) k_update(DevParams P, Launch L,
   }
 }
 
+template <bool FROM_STATES, bool WRITE_PAVG_ONLY>
+__global__ void __launch_bounds__(kThreads) k_update(DevParams P, Launch L,
+                                                     const unsigned char* __restrict__ inPlace) {
+  const long perTile = P.icells;
+  const long total = perTile * L.nTiles;
+  for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+       gid += (long)gridDim.x * blockDim.x) {
+    const int t = (int)(gid / perTile);
+    update_cell<FROM_STATES, WRITE_PAVG_ONLY>(P, L, t, gid - (long)t * perTile, inPlace);
+  }
+}
+
+// the update over tiles of several block classes in one launch
+__global__ void __launch_bounds__(kThreads) k_update_multi(Launch L, ClassMap M,
+                                                           const unsigned char* __restrict__ inPlace) {
+  const long total = M.start[L.nTiles];
+  for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+       gid += (long)gridDim.x * blockDim.x) {
+    const int t = find_tile(M.start, L.nTiles, gid);
+    update_cell<false, false>(M.cls[M.tcls[t]], L, t, gid - M.start[t], inPlace);
+  }
+}
+
 // per-workgroup max of (|v_d| + c)/del_d over interior cells
 __global__ void __launch_bounds__(kThreads) k_dt(DevParams P, const double* __restrict__ Ubase,
                                                  long nBlocks, long stride, double idx, double idy,
@@ -618,6 +660,8 @@ struct RegLaunch {
   const fx_fluxreg_face* faces;
   int nFaces;
   double* reg;
+  const DevParams* cls;        // block classes, or null: every tile has P's shape
+  const unsigned char* tcls;   // class of each tile of the list
 };
 
 __device__ __forceinline__ long reg_face(const DevParams& P, int d, int a, int b, int c) {
@@ -634,7 +678,8 @@ __global__ void __launch_bounds__(kThreads) k_fluxreg_put(DevParams P, RegLaunch
     const int e = (int)(gid - (long)q * R.nFaces);
     const fx_fluxreg_face f = R.faces[e];
     const int d = f.dir;
-    const double* F = R.tiles[f.fine_tile].flux[d] + (long)q * P.nface[d];
+    const DevParams& Pt = R.cls ? R.cls[R.tcls[f.fine_tile]] : P;
+    const double* F = R.tiles[f.fine_tile].flux[d] + (long)q * Pt.nface[d];
     const int inner = d == 0 ? 1 : 0, outer = d == 2 ? 1 : 2;
     const int nIn = P.ndim > 1 ? 2 : 1, nOut = P.ndim > 2 ? 2 : 1;
     double acc = 0.0;
@@ -643,7 +688,7 @@ __global__ void __launch_bounds__(kThreads) k_fluxreg_put(DevParams P, RegLaunch
         int idx[3] = {f.fi, f.fj, f.fk};
         idx[inner] += in;
         idx[outer] += o;
-        acc = acc + coef * F[reg_face(P, d, idx[0], idx[1], idx[2])];
+        acc = acc + coef * F[reg_face(Pt, d, idx[0], idx[1], idx[2])];
       }
     R.reg[gid] = acc;
   }
@@ -657,8 +702,9 @@ __global__ void __launch_bounds__(kThreads) k_fluxreg_conserve(DevParams P, RegL
     const int e = (int)(gid - (long)q * R.nFaces);
     const fx_fluxreg_face f = R.faces[e];
     const int d = f.dir;
-    double* F = R.tiles[f.coarse_tile].flux[d] + (long)q * P.nface[d];
-    F[reg_face(P, d, f.ci, f.cj, f.ck)] = R.reg[gid] * 1.0;
+    const DevParams& Pt = R.cls ? R.cls[R.tcls[f.coarse_tile]] : P;
+    double* F = R.tiles[f.coarse_tile].flux[d] + (long)q * Pt.nface[d];
+    F[reg_face(Pt, d, f.ci, f.cj, f.ck)] = R.reg[gid] * 1.0;
   }
 }
 
@@ -747,6 +793,9 @@ struct Lib {
   size_t dregTilesCap = 0;
   fx_fluxreg_face* dfaces = nullptr;   // face list of a register call
   size_t dfacesCap = 0;
+  unsigned char* dregCls = nullptr;    // tile classes of a register call
+  size_t dregClsCap = 0;
+  std::vector<unsigned char> hregCls;
   MlpPacked* mlpPacked = nullptr;      // BF16 fragments for fx_mlp_* / fx_riemann_batch
   MlpPacked* seedPacked = nullptr;     // network seeding the exact solver in hydro steps
   bool seedReady = false;
@@ -770,6 +819,27 @@ struct Lib {
   int partialCap = 0;
   std::vector<fx_tile> htiles;
   int numCU = 256;
+  // block classes (fx_hydro_set_block_classes); empty: the fx_hydro_init shape
+  std::vector<DevParams> hcls;
+  DevParams* dcls = nullptr;
+  size_t dclsCap = 0;
+  // the uploaded tile list: one shape (cur) or several classes (multi)
+  DevParams cur{};
+  bool multi = false;
+  bool lastMulti = false;              // multi state of the last exact-solver step
+  std::vector<unsigned char> htcls;
+  std::vector<long> hcellStart, hfaceStart;
+  unsigned char* dtcls = nullptr;
+  size_t dtclsCap = 0;
+  long* dcellStart = nullptr;
+  size_t dcellStartCap = 0;
+  long* dfaceStart = nullptr;
+  size_t dfaceStartCap = 0;
+  MarchItem* ditems = nullptr;         // column tiles of a multi-class march
+  size_t ditemsCap = 0;
+  int nItems = 0;
+  int itemsTX = 0;                     // column-tile width the items were built for
+  int persist = 0;                     // workgroups per CU of the multi-class march (0: occupancy)
   // batch tile-list cache (fx_hydro_advance_batch)
   double* batchU = nullptr;
   long batchN = -1, batchStride = -1;
@@ -930,22 +1000,56 @@ int finish(fx_status* st) {
   return h.code;
 }
 
+// parameters of tile t of the uploaded list
+const DevParams& tile_params(int t) { return g.multi ? g.hcls[g.htcls[t]] : g.cur; }
+
+// Classifies the tiles (useClasses: by fx_tile.block_class once classes are
+// set; otherwise every tile has the fx_hydro_init shape) and builds the
+// per-tile cell and face prefix offsets.  Returns 3 for a class out of range.
+int classify_tiles(const fx_tile* tiles, int nTiles, bool useClasses) {
+  g.multi = false;
+  g.cur = g.dp;
+  g.htcls.assign((size_t)nTiles, 0);
+  if (useClasses && !g.hcls.empty()) {
+    for (int t = 0; t < nTiles; ++t) {
+      const int c = tiles[t].block_class;
+      if (c < 0 || c >= (int)g.hcls.size()) return 3;
+      g.htcls[t] = (unsigned char)c;
+      if (c != g.htcls[0]) g.multi = true;
+    }
+    if (nTiles > 0) g.cur = g.hcls[g.htcls[0]];
+  }
+  g.hcellStart.assign((size_t)nTiles + 1, 0);
+  g.hfaceStart.assign((size_t)nTiles + 1, 0);
+  for (int t = 0; t < nTiles; ++t) {
+    const DevParams& P = tile_params(t);
+    g.hcellStart[t + 1] = g.hcellStart[t] + P.icells;
+    g.hfaceStart[t + 1] = g.hfaceStart[t] + P.nface[0] + P.nface[1] + P.nface[2];
+  }
+  return 0;
+}
+
 // Uploads the tile list, substituting library-owned flux arrays where a
 // tile has none.
-int upload_tiles(const fx_tile* tiles, int nTiles, bool needFlux, fx_status* st) {
+int upload_tiles(const fx_tile* tiles, int nTiles, bool needFlux, fx_status* st,
+                 bool useClasses = true) {
   g.batchValid = false;
+  if (classify_tiles(tiles, nTiles, useClasses)) {
+    if (st) st->code = 3;
+    return 3;
+  }
   g.htiles.assign(tiles, tiles + nTiles);
-  const DevParams& P = g.dp;
-  const size_t perTileFlux = (size_t)(P.nface[0] + P.nface[1] + P.nface[2]) * FX_NFLUX;
+  const int ndim = g.dp.ndim;
   size_t missing = 0;
   for (int t = 0; t < nTiles; ++t)
-    for (int d = 0; d < P.ndim; ++d)
+    for (int d = 0; d < ndim; ++d)
       if (!g.htiles[t].flux[d]) ++missing;
   if (needFlux && missing) {
-    if (grow(&g.flux, &g.fluxCap, perTileFlux * (size_t)nTiles, st)) return 4;
+    if (grow(&g.flux, &g.fluxCap, (size_t)g.hfaceStart[nTiles] * FX_NFLUX, st)) return 4;
     for (int t = 0; t < nTiles; ++t) {
-      double* base = g.flux + perTileFlux * t;
-      for (int d = 0; d < P.ndim; ++d) {
+      const DevParams& P = tile_params(t);
+      double* base = g.flux + (size_t)g.hfaceStart[t] * FX_NFLUX;
+      for (int d = 0; d < ndim; ++d) {
         if (!g.htiles[t].flux[d]) g.htiles[t].flux[d] = base;
         base += (size_t)P.nface[d] * FX_NFLUX;
       }
@@ -956,15 +1060,64 @@ int upload_tiles(const fx_tile* tiles, int nTiles, bool needFlux, fx_status* st)
     for (int t = 0; t < nTiles; ++t)
       if (!g.htiles[t].pavg) ++missingP;
     if (missingP) {
-      if (grow(&g.pavg, &g.pavgCap, (size_t)P.icells * nTiles, st)) return 4;
+      if (grow(&g.pavg, &g.pavgCap, (size_t)g.hcellStart[nTiles], st)) return 4;
       for (int t = 0; t < nTiles; ++t)
-        if (!g.htiles[t].pavg) g.htiles[t].pavg = g.pavg + (size_t)P.icells * t;
+        if (!g.htiles[t].pavg) g.htiles[t].pavg = g.pavg + (size_t)g.hcellStart[t];
     }
   }
   if (grow(&g.dtiles, &g.dtilesCap, (size_t)nTiles, st)) return 4;
   HIPCALL(hipMemcpyAsync(g.dtiles, g.htiles.data(), sizeof(fx_tile) * nTiles, hipMemcpyHostToDevice,
                          g.stream),
           st);
+  if (g.multi) {
+    if (grow(&g.dtcls, &g.dtclsCap, (size_t)nTiles, st)) return 4;
+    if (grow(&g.dcellStart, &g.dcellStartCap, (size_t)nTiles + 1, st)) return 4;
+    if (grow(&g.dfaceStart, &g.dfaceStartCap, (size_t)nTiles + 1, st)) return 4;
+    HIPCALL(hipMemcpyAsync(g.dtcls, g.htcls.data(), (size_t)nTiles, hipMemcpyHostToDevice, g.stream),
+            st);
+    HIPCALL(hipMemcpyAsync(g.dcellStart, g.hcellStart.data(), sizeof(long) * (nTiles + 1),
+                           hipMemcpyHostToDevice, g.stream),
+            st);
+    HIPCALL(hipMemcpyAsync(g.dfaceStart, g.hfaceStart.data(), sizeof(long) * (nTiles + 1),
+                           hipMemcpyHostToDevice, g.stream),
+            st);
+    g.nItems = 0;  // march items are built for the tile shape of the call
+  }
+  return 0;
+}
+
+// Column tile shared by every class of a multi-class march.
+bool multi_march_shape(int* TX) {
+  bool all16 = true;
+  for (const DevParams& P : g.hcls) {
+    if (P.ndim < 2 || P.nxb % 8 || P.nyb % 8) return false;
+    if (P.nxb % 16) all16 = false;
+  }
+  *TX = all16 ? 16 : 8;
+  return true;
+}
+
+// Column-tile work items of the uploaded multi-class list, longest columns
+// first (stable in tile order within a class).
+int build_march_items(int nTiles, int TX, fx_status* st) {
+  if (g.nItems > 0 && g.itemsTX == TX) return 0;
+  const int TY = 8;
+  std::vector<MarchItem> items;
+  for (int t = 0; t < nTiles; ++t) {
+    const DevParams& P = tile_params(t);
+    for (int ty = 0; ty < P.nyb / TY; ++ty)
+      for (int tx = 0; tx < P.nxb / TX; ++tx)
+        items.push_back(MarchItem{t, (int)g.htcls[t], P.lo[0] + tx * TX, P.lo[1] + ty * TY});
+  }
+  std::stable_sort(items.begin(), items.end(), [](const MarchItem& a, const MarchItem& b) {
+    return g.hcls[a.cls].nzb > g.hcls[b.cls].nzb;
+  });
+  if (grow(&g.ditems, &g.ditemsCap, items.size(), st)) return 4;
+  HIPCALL(hipMemcpyAsync(g.ditems, items.data(), sizeof(MarchItem) * items.size(),
+                         hipMemcpyHostToDevice, g.stream),
+          st);
+  g.nItems = (int)items.size();
+  g.itemsTX = TX;
   return 0;
 }
 
@@ -984,6 +1137,40 @@ void launch_march(const DevParams& P, const MarchLaunch& M, int TX, dim3 grid) {
   else launch_march_r<RS, double>(P, M, TX, grid);
 }
 
+// Persistent multi-class march: as many workgroups as stay resident (or
+// g.persist per CU), each taking every G-th column tile.
+template <int NDIM, int TX, int TY, int NT, int RS, typename R>
+void launch_march_multi_k(const MarchLaunch& M) {
+  static int resident = -1;
+  if (resident < 0) {
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_march_multi<NDIM, TX, TY, NT, RS, R>,
+                                                     NT, 0) != hipSuccess ||
+        per < 1)
+      per = 1;
+    resident = per;
+  }
+  long G = (long)g.numCU * (g.persist > 0 ? g.persist : resident);
+  if (G > g.nItems) G = g.nItems;
+  if (G < 1) G = 1;
+  hipLaunchKernelGGL((k_march_multi<NDIM, TX, TY, NT, RS, R>), dim3((unsigned)G), dim3(NT), 0,
+                     g.stream, g.dcls, M, g.ditems, g.nItems, g.dfaceStart);
+}
+
+template <int RS, typename R>
+void launch_march_multi_r(int ndim, int TX, const MarchLaunch& M) {
+  if (ndim == 3 && TX == 16) launch_march_multi_k<3, 16, 8, 512, RS, R>(M);
+  else if (ndim == 3) launch_march_multi_k<3, 8, 8, 256, RS, R>(M);
+  else if (TX == 16) launch_march_multi_k<2, 16, 8, 256, RS, R>(M);
+  else launch_march_multi_k<2, 8, 8, 256, RS, R>(M);
+}
+
+template <int RS>
+void launch_march_multi(int ndim, int precision, int TX, const MarchLaunch& M) {
+  if (precision == FX_PREC_MIXED) launch_march_multi_r<RS, float>(ndim, TX, M);
+  else launch_march_multi_r<RS, double>(ndim, TX, M);
+}
+
 dim3 wave_grid(long n);
 
 int record(int n) {
@@ -998,7 +1185,8 @@ int record(int n) {
 int ensure_step_scratch(int nTiles, fx_status* st) {
   const DevParams& P = g.dp;
   if (P.riemann != FX_EXACT) return 0;
-  const long faces = (P.nface[0] + P.nface[1] + P.nface[2]) * (long)nTiles;
+  const long faces = (long)nTiles <= (long)g.hfaceStart.size() - 1 ? g.hfaceStart[nTiles]
+                                                                     : 0;
   if (grow(&g.fstates, &g.fstatesCap, (size_t)kFaceComps * faces, st)) return 4;
   if (P.learned && grow(&g.seedY, &g.seedYCap, (size_t)faces, st)) return 4;
   return 0;
@@ -1012,8 +1200,11 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
   }
   if (st) memset(st, 0, sizeof *st);
   if (nTiles <= 0) return 0;
-  const DevParams& P = g.dp;
-  if (!tilesUploaded && upload_tiles(tiles, nTiles, true, st)) return 4;
+  if (!tilesUploaded) {
+    const int rc = upload_tiles(tiles, nTiles, true, st);
+    if (rc) return rc;
+  }
+  const DevParams& P = g.cur;
   const unsigned char* dIn = nullptr;
   bool anyFluxOnly = false;
   if (inPlace) {
@@ -1030,12 +1221,52 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
     dIn = g.dinplace;
   }
   int TX = 0, TY = 0, NT = 0;
-  const bool march = !g.legacy && march_shape(P, &TX, &TY, &NT);
+  const bool march = g.multi ? multi_march_shape(&TX) : (!g.legacy && march_shape(P, &TX, &TY, &NT));
   if (P.learned && (!march || !g.seedReady)) {  // the seed needs the march path and a network
     if (st) st->code = 3;
     return 3;
   }
-  if (march) {
+  if (g.multi && !march) {  // classes admit only the march path
+    if (st) st->code = 3;
+    return 3;
+  }
+  if (g.multi) {
+    if (build_march_items(nTiles, TX, st)) return 4;
+    MarchLaunch M{g.dtiles, nTiles, 0, 0, g.status, dt, nullptr, g.dtDev};
+    Launch L{g.dtiles, nTiles, nullptr, g.status, dt, g.dtDev};
+    const long faces = g.hfaceStart[nTiles];
+    if (ensure_step_scratch(nTiles, st)) return 4;
+    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
+    if (record(0)) return 4;
+    if (P.riemann == FX_EXACT) {
+      M.fstates = g.fstates;
+      launch_march_multi<RS_STATES>(P.ndim, P.precision, TX, M);
+      if (record(1)) return 4;
+      FaceLaunch F{g.dtiles, nTiles, g.fstates, P.learned ? g.seedY : nullptr, g.status,
+                   g.seedGate, g.dcls, g.dtcls, g.dfaceStart};
+      g.lastFaces = faces;
+      g.lastTiles = nTiles;
+      g.lastMulti = true;
+      if (P.learned) {
+        hipLaunchKernelGGL(k_face_seed<true>, wave_grid(faces), dim3(kLearnThreads), 0, g.stream, P,
+                           F, g.seedPacked, g.seedY);
+        hipLaunchKernelGGL((k_face_solve<true, true>), dim3(grid_for(faces)), dim3(kThreads), 0,
+                           g.stream, P, F);
+      } else {
+        hipLaunchKernelGGL((k_face_solve<false, true>), dim3(grid_for(faces)), dim3(kThreads), 0,
+                           g.stream, P, F);
+      }
+      if (record(2)) return 4;
+    } else {
+      launch_march_multi<FX_HLLC>(P.ndim, P.precision, TX, M);
+      if (record(1)) return 4;
+      if (record(2)) return 4;
+    }
+    const ClassMap C{g.dcls, g.dtcls, g.dcellStart};
+    hipLaunchKernelGGL(k_update_multi, dim3(grid_for(g.hcellStart[nTiles])), dim3(kThreads), 0,
+                       g.stream, L, C, dIn);
+    if (record(3)) return 4;
+  } else if (march) {
     MarchLaunch M{g.dtiles, nTiles, P.nxb / TX, (P.ndim > 1 ? P.nyb : 1) / TY, g.status, dt,
                   nullptr, g.dtDev};
     const dim3 grid((unsigned)(nTiles * M.tilesX * M.tilesY));
@@ -1051,16 +1282,17 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
       launch_march<RS_STATES>(P, M, TX, grid);
       if (record(1)) return 4;
       FaceLaunch F{g.dtiles, nTiles, g.fstates, P.learned ? g.seedY : nullptr, g.status,
-                   g.seedGate};
+                   g.seedGate, nullptr, nullptr, nullptr};
       g.lastFaces = faces;
       g.lastTiles = nTiles;
+      g.lastMulti = false;
       if (P.learned) {
-        hipLaunchKernelGGL(k_face_seed, wave_grid(faces), dim3(kLearnThreads), 0, g.stream, P, F,
-                           g.seedPacked, g.seedY);
-        hipLaunchKernelGGL(k_face_solve<true>, dim3(grid_for(faces)), dim3(kThreads), 0, g.stream,
-                           P, F);
+        hipLaunchKernelGGL(k_face_seed<false>, wave_grid(faces), dim3(kLearnThreads), 0, g.stream,
+                           P, F, g.seedPacked, g.seedY);
+        hipLaunchKernelGGL((k_face_solve<true, false>), dim3(grid_for(faces)), dim3(kThreads), 0,
+                           g.stream, P, F);
       } else {
-        hipLaunchKernelGGL(k_face_solve<false>, dim3(grid_for(faces)), dim3(kThreads), 0,
+        hipLaunchKernelGGL((k_face_solve<false, false>), dim3(grid_for(faces)), dim3(kThreads), 0,
                            g.stream, P, F);
       }
       if (record(2)) return 4;
@@ -1108,11 +1340,15 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
 // anything is launched; the kernels index without further checks.
 int check_reg_faces(const fx_tile* tiles, int nTiles, const fx_fluxreg_face* faces, int nFaces,
                     bool fine) {
-  const DevParams& P = g.dp;
+  for (int t = 0; t < nTiles; ++t)
+    if (!g.hcls.empty() && (tiles[t].block_class < 0 || tiles[t].block_class >= (int)g.hcls.size()))
+      return 3;
   for (int e = 0; e < nFaces; ++e) {
     const fx_fluxreg_face& f = faces[e];
     const int t = fine ? f.fine_tile : f.coarse_tile;
-    if (t < 0 || t >= nTiles || f.dir < 0 || f.dir >= P.ndim) return 3;
+    if (t < 0 || t >= nTiles) return 3;
+    const DevParams& P = g.hcls.empty() ? g.dp : g.hcls[tiles[t].block_class];
+    if (f.dir < 0 || f.dir >= P.ndim) return 3;
     if (!tiles[t].flux[f.dir]) return 3;
     const int* ext = P.fext[f.dir];
     int idx[3] = {fine ? f.fi : f.ci, fine ? f.fj : f.cj, fine ? f.fk : f.ck};
@@ -1146,7 +1382,17 @@ int run_fluxreg(const fx_tile* tiles, int nTiles, const fx_fluxreg_face* faces, 
                          g.stream),
           st);
   const DevParams& P = g.dp;
-  RegLaunch R{g.dregTiles, g.dfaces, nFaces, reg};
+  RegLaunch R{g.dregTiles, g.dfaces, nFaces, reg, nullptr, nullptr};
+  if (!g.hcls.empty()) {
+    g.hregCls.resize((size_t)nTiles);
+    for (int t = 0; t < nTiles; ++t) g.hregCls[t] = (unsigned char)tiles[t].block_class;
+    if (grow(&g.dregCls, &g.dregClsCap, (size_t)nTiles, st)) return 4;
+    HIPCALL(hipMemcpyAsync(g.dregCls, g.hregCls.data(), (size_t)nTiles, hipMemcpyHostToDevice,
+                           g.stream),
+            st);
+    R.cls = g.dcls;
+    R.tcls = g.dregCls;
+  }
   const dim3 grid(grid_for((long)nFaces * FX_NFLUX));
   if (put) {
     const double coef = P.ndim == 3 ? 0.25 : (P.ndim == 2 ? 0.5 : 1.0);
@@ -1208,7 +1454,7 @@ int prepare_batch(double* U, long nBlocks, long blockStride, const double del[3]
       for (int d = 0; d < 3; ++d) t.del[d] = del[d];
       t.level = 1;
     }
-    if (upload_tiles(tl.data(), (int)nBlocks, true, st)) return 4;
+    if (upload_tiles(tl.data(), (int)nBlocks, true, st, false)) return 4;
     g.batchU = U;
     g.batchN = nBlocks;
     g.batchStride = blockStride;
@@ -1259,7 +1505,10 @@ extern "C" {
 int fx_hydro_init(const fx_params* p) {
   int rc = validate(p);
   if (rc) return rc;
-  if (!g.init || memcmp(&g.prm, p, sizeof *p) != 0) g.batchValid = false;
+  if (!g.init || memcmp(&g.prm, p, sizeof *p) != 0) {
+    g.batchValid = false;
+    g.hcls.clear();  // classes carry the knobs of the init they were set under
+  }
   g.prm = *p;
   g.dp = make_dev_params(*p);
   if (!g.status) {
@@ -1276,7 +1525,8 @@ int fx_hydro_init(const fx_params* p) {
 void fx_hydro_finalize(void) {
   void* bufs[] = {g.states, g.flux, g.pavg, g.dtiles, g.dinplace, g.status, g.partial,
                   g.dregTiles, g.dfaces, g.mlpPacked, g.seedPacked, g.seedY, g.fstates,
-                  g.dnbr, g.dtBuf};
+                  g.dnbr, g.dtBuf, g.dcls, g.dtcls, g.dcellStart, g.dfaceStart,
+                  g.ditems, g.dregCls};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : g.ev)
@@ -1302,12 +1552,21 @@ int fx_hydro_update_solution(const fx_tile* tiles, int nTiles, double dt, double
       if (!tiles[t].flux[d]) return 3;
     if (g.dp.aux && !tiles[t].pavg) return 3;
   }
-  const DevParams& P = g.dp;
-  if (upload_tiles(tiles, nTiles, false, st)) return 4;
+  {
+    const int rc = upload_tiles(tiles, nTiles, false, st);
+    if (rc) return rc;
+  }
+  const DevParams& P = g.cur;
   Launch L{g.dtiles, nTiles, nullptr, g.status, dt, g.dtDev};
   hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
-  hipLaunchKernelGGL((k_update<false, false>), dim3(grid_for(P.icells * nTiles)), dim3(kThreads), 0,
-                     g.stream, P, L, (const unsigned char*)nullptr);
+  if (g.multi) {
+    const ClassMap C{g.dcls, g.dtcls, g.dcellStart};
+    hipLaunchKernelGGL(k_update_multi, dim3(grid_for(g.hcellStart[nTiles])), dim3(kThreads), 0,
+                       g.stream, L, C, (const unsigned char*)nullptr);
+  } else {
+    hipLaunchKernelGGL((k_update<false, false>), dim3(grid_for(P.icells * nTiles)), dim3(kThreads),
+                       0, g.stream, P, L, (const unsigned char*)nullptr);
+  }
   HIPCALL(hipGetLastError(), st);
   return finish(st);
 }
@@ -1511,9 +1770,14 @@ int fx_hydro_face_states(double* S, long cap, long* n) {
   if (!n) return 3;
   *n = g.lastFaces;
   if (!g.init || g.lastFaces <= 0 || !S || cap < g.lastFaces) return 3;
-  FaceLaunch F{g.dtiles, g.lastTiles, g.fstates, nullptr, g.status, g.seedGate};
-  hipLaunchKernelGGL(k_collect_faces, dim3(grid_for(g.lastFaces)), dim3(kThreads), 0, g.stream,
-                     g.dp, F, g.lastFaces, S);
+  FaceLaunch F{g.dtiles, g.lastTiles, g.fstates, nullptr, g.status, g.seedGate,
+               g.dcls, g.dtcls, g.dfaceStart};
+  if (g.lastMulti)
+    hipLaunchKernelGGL(k_collect_faces<true>, dim3(grid_for(g.lastFaces)), dim3(kThreads), 0,
+                       g.stream, g.cur, F, g.lastFaces, S);
+  else
+    hipLaunchKernelGGL(k_collect_faces<false>, dim3(grid_for(g.lastFaces)), dim3(kThreads), 0,
+                       g.stream, g.cur, F, g.lastFaces, S);
   return sync_or_async(nullptr);
 }
 
@@ -1546,6 +1810,30 @@ int fx_hydro_stage_times(float* ms, int n) {
   return 3;
 }
 void fx_hydro_set_async(int a) { g.async = a; }
+
+int fx_hydro_set_block_classes(const int* nb, int nClasses) {
+  if (!g.init || nClasses < 0 || nClasses > FX_MAX_CLASSES || (nClasses > 0 && !nb)) return 3;
+  std::vector<DevParams> cls;
+  for (int c = 0; c < nClasses; ++c) {
+    fx_params p = g.prm;
+    p.nxb = nb[3 * c];
+    p.nyb = p.ndim > 1 ? nb[3 * c + 1] : 1;
+    p.nzb = p.ndim > 2 ? nb[3 * c + 2] : 1;
+    if (validate(&p)) return 3;
+    if (p.ndim < 2 || p.nxb % 8 || p.nyb % 8) return 3;  // the column march's tiles
+    cls.push_back(make_dev_params(p));
+  }
+  if (nClasses > 0) {
+    if (grow(&g.dcls, &g.dclsCap, (size_t)nClasses, nullptr)) return 4;
+    if (hipMemcpy(g.dcls, cls.data(), sizeof(DevParams) * nClasses, hipMemcpyHostToDevice) !=
+        hipSuccess)
+      return 4;
+  }
+  g.hcls = cls;
+  g.nItems = 0;
+  g.batchValid = false;
+  return 0;
+}
 const char* fx_hydro_version(void) { return "flash5_amd fx_hydro 0.1 (gfx950)"; }
 
 }  // extern "C"

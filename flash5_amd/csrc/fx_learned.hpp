@@ -206,10 +206,15 @@ __global__ void __launch_bounds__(kThreads) k_riemann_batch(
 struct FaceLaunch {
   const fx_tile* tiles;
   int nTiles;
-  const double* fstates;   // [tile][dir][comp][face], from k_march<RS_STATES>
+  const double* fstates;   // [tile][dir][comp][face], from the march's RS_STATES stage
   const float* y;          // network seed per face (flattened like gid), or null
   DevStatus* status;
   double gate;             // faces weaker than this keep the two-shock start
+  // tiles of several block classes (the MULTI kernels): parameters per class,
+  // class of each tile, faces before each tile (faceStart[nTiles] = all)
+  const DevParams* cls;
+  const unsigned char* tcls;
+  const long* faceStart;
 };
 
 struct FaceRef {
@@ -218,19 +223,51 @@ struct FaceRef {
   long cR, cL;     // right / left cells (guard-inclusive index)
 };
 
-__device__ __forceinline__ FaceRef face_ref(const DevParams& P, long gid, long perTile) {
+// all faces of the launch
+template <bool MULTI>
+__device__ __forceinline__ long face_total(const DevParams& P0, const FaceLaunch& F) {
+  if constexpr (MULTI) return F.faceStart[F.nTiles];
+  else return (P0.nface[0] + P0.nface[1] + P0.nface[2]) * F.nTiles;
+}
+
+// Tile t holding flat face gid, gid's index r within it, and the tile's
+// parameters: the launch's own, or its class's (MULTI).
+template <bool MULTI>
+__device__ __forceinline__ const DevParams& face_tile(const DevParams& P0, const FaceLaunch& F,
+                                                      long gid, int& t, long& r) {
+  if constexpr (MULTI) {
+    t = find_tile(F.faceStart, F.nTiles, gid);
+    r = gid - F.faceStart[t];
+    return F.cls[F.tcls[t]];
+  } else {
+    const long perTile = P0.nface[0] + P0.nface[1] + P0.nface[2];
+    t = (int)(gid / perTile);
+    r = gid - (long)t * perTile;
+    return P0;
+  }
+}
+
+// offset of tile t's face states in F.fstates
+template <bool MULTI>
+__device__ __forceinline__ long face_state_offset(const DevParams& P, const FaceLaunch& F, int t) {
+  if constexpr (MULTI) return kFaceComps * F.faceStart[t];
+  else return t * fstate_tile_stride(P);
+}
+
+// face rt of tile t (index arithmetic in 32 bits: a tile holds < 2^31 faces)
+__device__ __forceinline__ FaceRef face_ref(const DevParams& P, int t, long rt) {
   FaceRef f;
-  f.t = (int)(gid / perTile);
-  long r = gid - (long)f.t * perTile;
+  f.t = t;
+  int r = (int)rt;
   int d = 0;
-  while (d < P.ndim - 1 && r >= P.nface[d]) {
-    r -= P.nface[d];
+  while (d < P.ndim - 1 && r >= (int)P.nface[d]) {
+    r -= (int)P.nface[d];
     ++d;
   }
   f.d = d;
   f.r = r;
   const int f0 = P.fext[d][0], f1 = P.fext[d][1];
-  const int a0 = (int)(r % f0), a1 = (int)((r / f0) % f1), a2 = (int)(r / ((long)f0 * f1));
+  const int a0 = r % f0, a1 = (r / f0) % f1, a2 = r / (f0 * f1);
   f.cR = cidx(P, a0 + P.lo[0], a1 + P.lo[1], a2 + P.lo[2]);
   f.cL = f.cR - axis_stride(P, d);
   return f;
@@ -238,18 +275,20 @@ __device__ __forceinline__ FaceRef face_ref(const DevParams& P, long gid, long p
 
 // One code path serves the three directions: the face-normal velocity and
 // the flux component that takes the pressure are picked at run time.
-template <bool SEEDED>
 // (3 waves per SIMD fit without spilling the solver's registers)
+template <bool SEEDED, bool MULTI>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3)))
-k_face_solve(DevParams P, FaceLaunch F) {
-  const long perTile = P.nface[0] + P.nface[1] + P.nface[2];
-  const long total = perTile * F.nTiles;
+k_face_solve(DevParams P0, FaceLaunch F) {
+  const long total = face_total<MULTI>(P0, F);
   ExactStats es{0ULL, 0ULL};
   for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
        gid += (long)gridDim.x * blockDim.x) {
-    const FaceRef f = face_ref(P, gid, perTile);
+    int t;
+    long rt;
+    const DevParams& P = face_tile<MULTI>(P0, F, gid, t, rt);
+    const FaceRef f = face_ref(P, t, rt);
     const int d = f.d;
-    const double* b = F.fstates + f.t * fstate_tile_stride(P) + fstate_dir_offset(P, d);
+    const double* b = F.fstates + face_state_offset<MULTI>(P, F, t) + fstate_dir_offset(P, d);
     const long nf = P.nface[d];
     // face states: comps 0..5 dens, velx, vely, velz, pres, eint; left, then right
     const double* Lb = b + f.r;
@@ -277,21 +316,24 @@ k_face_solve(DevParams P, FaceLaunch F) {
 }
 
 // network seed ln(p*/P0) of every face, from the stored face states
-__global__ void __launch_bounds__(kLearnThreads) k_face_seed(DevParams P, FaceLaunch F,
+template <bool MULTI>
+__global__ void __launch_bounds__(kLearnThreads) k_face_seed(DevParams P0, FaceLaunch F,
                                                              const MlpPacked* __restrict__ W,
                                                              float* __restrict__ y) {
   __shared__ float xs[kLearnWaves][64 * 8];
   const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
-  const long perTile = P.nface[0] + P.nface[1] + P.nface[2];
-  const long total = perTile * F.nTiles;
+  const long total = face_total<MULTI>(P0, F);
   const long nw = (long)gridDim.x * kLearnWaves;
   for (long base = ((long)blockIdx.x * kLearnWaves + wv) * 64; base < total; base += nw * 64) {
     const long gid = base + lane;
     const bool act = gid < total;
     double rl = 1.0, ul = 0.0, pl = 1.0, rr = 1.0, ur = 0.0, pr = 1.0;
     if (act) {
-      const FaceRef f = face_ref(P, gid, perTile);
-      const double* b = F.fstates + f.t * fstate_tile_stride(P) + fstate_dir_offset(P, f.d);
+      int t;
+      long rt;
+      const DevParams& P = face_tile<MULTI>(P0, F, gid, t, rt);
+      const FaceRef f = face_ref(P, t, rt);
+      const double* b = F.fstates + face_state_offset<MULTI>(P, F, t) + fstate_dir_offset(P, f.d);
       const long nf = P.nface[f.d];
       rl = b[0 * nf + f.r];
       ul = b[(1 + f.d) * nf + f.r];
@@ -303,14 +345,14 @@ __global__ void __launch_bounds__(kLearnThreads) k_face_seed(DevParams P, FaceLa
     // weak (or invalid) faces get no seed; wavefronts of such faces only
     // skip the network
     const bool need = act && rl > 0. && rr > 0. && pl > 0. && pr > 0. &&
-                      seed_strength(P.gamma, rl, ul, pl, rr, ur, pr) >= F.gate;
+                      seed_strength(P0.gamma, rl, ul, pl, rr, ur, pr) >= F.gate;
     if (__ballot(need) == 0) {
       if (act) y[gid] = kNoSeed;
       continue;
     }
     float x[6];
-    double P0;
-    mlp_features(P.gamma, rl, ul, pl, rr, ur, pr, x, P0);
+    double pref;
+    mlp_features(P0.gamma, rl, ul, pl, rr, ur, pr, x, pref);
 #pragma unroll
     for (int k = 0; k < 6; ++k) xs[wv][lane * 8 + k] = x[k];
     xs[wv][lane * 8 + 6] = 0.f;
@@ -326,13 +368,16 @@ __global__ void __launch_bounds__(kLearnThreads) k_face_seed(DevParams P, FaceLa
 
 // face states of the last exact-solver step as Riemann problems, SoA [6][n]
 // (rhoL, uL, pL, rhoR, uR, pR with u the face-normal velocity)
-__global__ void __launch_bounds__(kThreads) k_collect_faces(DevParams P, FaceLaunch F, long n,
+template <bool MULTI>
+__global__ void __launch_bounds__(kThreads) k_collect_faces(DevParams P0, FaceLaunch F, long n,
                                                             double* __restrict__ S) {
-  const long perTile = P.nface[0] + P.nface[1] + P.nface[2];
   for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < n;
        gid += (long)gridDim.x * blockDim.x) {
-    const FaceRef f = face_ref(P, gid, perTile);
-    const double* b = F.fstates + f.t * fstate_tile_stride(P) + fstate_dir_offset(P, f.d);
+    int t;
+    long rt;
+    const DevParams& P = face_tile<MULTI>(P0, F, gid, t, rt);
+    const FaceRef f = face_ref(P, t, rt);
+    const double* b = F.fstates + face_state_offset<MULTI>(P, F, t) + fstate_dir_offset(P, f.d);
     const long nf = P.nface[f.d];
     S[gid] = b[0 * nf + f.r];
     S[n + gid] = b[(1 + f.d) * nf + f.r];

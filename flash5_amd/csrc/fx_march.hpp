@@ -183,8 +183,13 @@ __device__ __forceinline__ R flat_tilde_vals(R pm2, R pm1, R pp1, R pp2, R vm1, 
   return ft;
 }
 
+// One column tile (TX x TY columns at ox, oy, all planes) of tile t.  Every
+// thread of the workgroup calls it with the same arguments; it ends on a
+// barrier, so a workgroup may call it again for another column tile.  fsb:
+// tile t's face-state scratch (RS_STATES).
 template <int NDIM, int TX, int TY, int NT, int RS, typename R>
-__global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
+__device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch& L, int t, int ox,
+                                           int oy, double* fsb, ExactStats& es) {
   using I = TileIdx<TX, TY>;
   constexpr int NR = NDIM == 3 ? 3 : 1;        // ring depth
   constexpr int NCOL = TX * TY;
@@ -202,11 +207,6 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
   __shared__ R WZP[Z ? 6 * NCOL : 1];      // z-face Wp of plane k-1
 
   const int tid = threadIdx.x;
-  const int tilesPerBlock = L.tilesX * L.tilesY;
-  const int t = blockIdx.x / tilesPerBlock;
-  const int tr = blockIdx.x - t * tilesPerBlock;
-  const int ox = P.lo[0] + (tr % L.tilesX) * TX;
-  const int oy = P.lo[1] + (tr / L.tilesX) * TY;
   // A tile owns its high x (y) face row only at the block's high side;
   // otherwise the neighbouring tile computes that row (and the states
   // behind it) as its low row.
@@ -220,7 +220,6 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
   const int klo = P.lo[2], khi = P.hi[2];
   DevStatus* st = L.status;
   bool bad = false;  // eigen failure seen by this thread
-  ExactStats es{0ULL, 0ULL};
 
   auto ring = [](int k) { return NR == 3 ? (k % 3) : 0; };
 
@@ -477,7 +476,7 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
           VR[P_GAMC] = R(U[P.voff[FX_GAMC] + c]);
           VR[P_GAME] = R(U[P.voff[FX_GAME] + c]);
           if constexpr (RS == RS_STATES) {
-            write_face_states<0>(P, L.fstates + t * fstate_tile_stride(P) + fstate_dir_offset(P, 0),
+            write_face_states<0>(P, fsb + fstate_dir_offset(P, 0),
                                  face_index(P, 0, ox + a, oy + b, k), VL, VR);
             continue;
           } else {
@@ -495,7 +494,7 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
           VR[P_GAMC] = R(U[P.voff[FX_GAMC] + c]);
           VR[P_GAME] = R(U[P.voff[FX_GAME] + c]);
           if constexpr (RS == RS_STATES) {
-            write_face_states<1>(P, L.fstates + t * fstate_tile_stride(P) + fstate_dir_offset(P, 1),
+            write_face_states<1>(P, fsb + fstate_dir_offset(P, 1),
                                  face_index(P, 1, ox + a, oy + b, k), VL, VR);
             continue;
           } else {
@@ -516,7 +515,7 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
             VR[P_GAME] = R(U[P.voff[FX_GAME] + c]);
             if constexpr (RS == RS_STATES) {
               write_face_states<2>(P,
-                                   L.fstates + t * fstate_tile_stride(P) + fstate_dir_offset(P, 2),
+                                   fsb + fstate_dir_offset(P, 2),
                                    face_index(P, 2, ox + a, oy + b, k), VL, VR);
             } else {
               if (face_flux<2, RS>(P, VL, VR, F, es)) report_error(st, 2, t, ox + a, oy + b, k, 2);
@@ -542,5 +541,41 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
     __syncthreads();
   }
   if (bad) report_error(st, 1, t, ox, oy, -1, -1);
-  flush_exact(st, es);
+}
+
+// one workgroup per column tile of a batch of equal-shape tiles
+template <int NDIM, int TX, int TY, int NT, int RS, typename R>
+__global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
+  const int tilesPerBlock = L.tilesX * L.tilesY;
+  const int t = blockIdx.x / tilesPerBlock;
+  const int tr = blockIdx.x - t * tilesPerBlock;
+  const int ox = P.lo[0] + (tr % L.tilesX) * TX;
+  const int oy = P.lo[1] + (tr / L.tilesX) * TY;
+  ExactStats es{0ULL, 0ULL};
+  double* fsb = RS == RS_STATES ? L.fstates + t * fstate_tile_stride(P) : nullptr;
+  march_tile<NDIM, TX, TY, NT, RS, R>(P, L, t, ox, oy, fsb, es);
+  flush_exact(L.status, es);
+}
+
+// Tiles of several block classes in one persistent launch: workgroup b takes
+// the column tiles b, b + G, b + 2G, ... of the item list (G = grid size).
+// The host orders the items by falling column height, so the round-robin
+// deals the long columns first and the short ones fill in behind them.
+struct MarchItem {
+  int t, cls, ox, oy;  // tile, its class, column-tile origin (guard-inclusive)
+};
+
+template <int NDIM, int TX, int TY, int NT, int RS, typename R>
+__global__ void __launch_bounds__(NT) k_march_multi(const DevParams* __restrict__ cls,
+                                                    MarchLaunch L,
+                                                    const MarchItem* __restrict__ items,
+                                                    int nItems, const long* __restrict__ faceStart) {
+  ExactStats es{0ULL, 0ULL};
+  for (int it = blockIdx.x; it < nItems; it += gridDim.x) {
+    const MarchItem w = items[it];
+    const DevParams& P = cls[w.cls];
+    double* fsb = RS == RS_STATES ? L.fstates + kFaceComps * faceStart[w.t] : nullptr;
+    march_tile<NDIM, TX, TY, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, es);
+  }
+  flush_exact(L.status, es);
 }

@@ -58,7 +58,7 @@ _DP = C.POINTER(C.c_double)
 
 class FxTile(C.Structure):
     _fields_ = [("U", _DP), ("flux", _DP * 3), ("pavg", _DP), ("del_", C.c_double * 3),
-                ("level", C.c_int), ("pad", C.c_int)]
+                ("level", C.c_int), ("block_class", C.c_int)]
 
 
 class FxStatus(C.Structure):
@@ -110,6 +110,7 @@ EXPORTS = {
     "fx_hydro_set_async": ([C.c_int], None),
     "fx_hydro_set_timing": ([C.c_int], None),
     "fx_hydro_set_kernel_path": ([C.c_int], None),
+    "fx_hydro_set_block_classes": ([C.POINTER(C.c_int), C.c_int], C.c_int),
     "fx_hydro_stage_times": ([C.POINTER(C.c_float), C.c_int], C.c_int),
     "fx_hydro_version": ([], C.c_char_p),
 }
@@ -229,6 +230,7 @@ class Tile:
     flux: Optional[List[object]] = None   # per-direction flux tensors or None
     pavg: Optional[object] = None         # interior-cell pressure average or None
     level: int = 1
+    block_class: int = 0                  # index into Hydro.set_block_classes's shapes
 
 
 def _ptr(t) -> _DP:
@@ -257,11 +259,36 @@ class Hydro:
         if rc:
             raise FxError("fx_hydro_init", rc=rc)
         self.last_status = FxStatus()
+        self._classes = None
 
     def _reinit(self):
         rc = self.lib.fx_hydro_init(C.byref(self._cp))
         if rc:
             raise FxError("fx_hydro_init", rc=rc)
+        if self._classes is not None:
+            self._apply_classes()
+
+    def _apply_classes(self):
+        flat = [int(v) for shape in self._classes for v in shape]
+        arr = (C.c_int * max(1, len(flat)))(*flat)
+        rc = self.lib.fx_hydro_set_block_classes(arr, len(self._classes))
+        if rc:
+            raise FxError("fx_hydro_set_block_classes", rc=rc)
+
+    def set_block_classes(self, shapes) -> None:
+        """Interior shapes (nxb, nyb, nzb) of the block classes; a Tile picks
+        its shape by block_class.  Tiles of several classes go through one
+        persistent launch per stage.  An empty list returns to the params
+        shape."""
+        self._classes = [tuple(int(v) for v in sh) + (1,) * (3 - len(sh)) for sh in shapes]
+        self._apply_classes()
+        if not self._classes:
+            self._classes = None
+
+    def class_params(self, c: int) -> HydroParams:
+        """HydroParams of block class c (the params with that class's shape)."""
+        nx, ny, nz = self._classes[c]
+        return dataclasses.replace(self.params, nxb=nx, nyb=ny, nzb=nz)
 
     def set_kernel_path(self, path: int) -> None:
         """0 = fused column-march kernel (default), 1 = three-kernel path."""
@@ -289,6 +316,7 @@ class Hydro:
             for d in range(3):
                 arr[n].del_[d] = float(t.deltas[d]) if d < len(t.deltas) else 1.0
             arr[n].level = t.level
+            arr[n].block_class = t.block_class
         return arr
 
     def compute_fluxes(self, tiles: Sequence[Tile], dt: float, dtOld: float = 0.0,

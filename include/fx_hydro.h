@@ -93,7 +93,8 @@ typedef struct {
                                reference's scrch_Ptr HY_VAR1), or NULL */
   double del[3];            /* cell widths (tileDesc%deltas) */
   int level;                /* refinement level, 1-based */
-  int pad;
+  int block_class;          /* index into fx_hydro_set_block_classes's list;
+                               ignored (0) while no classes are set */
 } fx_tile;
 
 typedef struct {
@@ -236,6 +237,23 @@ int fx_hydro_stage_times(float* ms, int n);
 /* Kernel path: 0 = fused column-march kernel where the block shape allows
  * (default), 1 = three-kernel path (states, fluxes, update). */
 void fx_hydro_set_kernel_path(int path);
+
+/* Block classes: tiles of several interior shapes in one call, e.g. levels
+ * whose blocks differ in size.  No counterpart in the reference, where every
+ * block has the compile-time shape NXB x NYB x NZB; it serves callers that
+ * batch levels of different block sizes.  nb[3*c + a] is class c's interior
+ * extent along axis a (1 beyond ndim); every physics knob stays that of
+ * fx_hydro_init.  Once set, fx_hydro_compute_fluxes, fx_hydro_update_solution
+ * and the flux-register calls read each tile's shape from
+ * fx_tile.block_class; tiles of several classes go through one persistent
+ * launch per stage.  The shapes must admit the fused column-march kernel
+ * (interior x and y extents multiples of 8, 2-D or 3-D).  nClasses = 0
+ * returns to the fx_hydro_init shape.  The batch entry points
+ * (fx_hydro_advance_batch, fx_grid_*) keep the fx_hydro_init shape.
+ * fx_hydro_init with different knobs clears the classes. 
+ * Returns 0, or 3 for a bad shape or count (at most FX_MAX_CLASSES). */
+#define FX_MAX_CLASSES 8
+int fx_hydro_set_block_classes(const int* nb, int nClasses);
 /* Library version string. */
 const char* fx_hydro_version(void);
 
