@@ -1,0 +1,280 @@
+// fx_amr.hpp -- device side of multi-level block grids: guard fill with
+// coarse-to-fine prolongation, restriction, and the CFL reduction over all
+// levels.  Included by fx_hydro.hip after the hydro kernels.
+//
+// Reference semantics (paths relative to source/Grid/GridMain/AMR/Amrex):
+//   guard fill       Grid_fillGuardCells.F90:287-369: same-level copies, then
+//                    amrex_fillpatch from the next coarser level with the
+//                    interpolator amrexInterpolator = CELL_CONSERVATIVE_LINEAR
+//                    (Config:39-40), on data in conserved form
+//                    (gr_primitiveToConserve.F90:107-116) and back, then EOS
+//                    on the guard cells (Hydro.F90:121).
+//   restriction      gr_restrictAllLevels.F90:86-130: PER_MASS variables
+//                    times density on the leaves, amrex_average_down finest to
+//                    coarsest, then back to per-mass form on every block.
+// AMReX itself is not part of the reference tree; the interpolation below is
+// the published form of its cell-conservative linear interpolator: per
+// component, MC-limited central slopes of the coarse data, evaluated at the
+// fine-cell offsets +-1/4 of a coarse cell.  PER_MASS variables (velx, vely,
+// velz, ener, eint; hydro unsplit Config:44-53) are interpolated and averaged
+// as density-weighted quantities.
+#pragma once
+
+struct AmrBlock {
+  double* U;      // guard-inclusive storage of the block
+  int level;      // 0-based
+  int cls;        // block class
+  int pos[3];     // block position at its level, in blocks
+};
+
+struct AmrLevelDev {
+  int nb[3];           // interior cells per block along each axis (1 beyond ndim)
+  int ext[3];          // blocks per axis that would cover the domain at this level
+  const int* index;    // dense [(z * ext1 + y) * ext0 + x] -> block, or -1
+  double del[3];       // cell widths
+};
+
+struct AmrLaunch {
+  const AmrBlock* blocks;
+  int nBlocks;
+  const AmrLevelDev* levels;
+  const DevParams* cls;
+  const int* list;          // block subset of the launch (restriction: one level), or null
+  int nList;                // blocks in the launch
+  const long* start;        // prefix over the launch's blocks of its work items
+  int periodic[3];
+  unsigned long long perMass;  // bit v: variable slot v is PER_MASS
+  DevStatus* status;
+};
+
+// global cell g (level lev) wrapped or clamped into the domain: periodic
+// axes wrap, the others clamp (zero-gradient outflow)
+__device__ __forceinline__ void amr_domain(const AmrLaunch& A, int lev, int ndim, int g[3]) {
+  const AmrLevelDev& L = A.levels[lev];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (a >= ndim) break;
+    const int n = L.ext[a] * L.nb[a];
+    if (A.periodic[a]) g[a] = ((g[a] % n) + n) % n;
+    else g[a] = g[a] < 0 ? 0 : (g[a] >= n ? n - 1 : g[a]);
+  }
+}
+
+// block holding global cell g of level lev and the cell's guard-inclusive
+// index in it; false when the level has no block there
+__device__ __forceinline__ bool amr_find(const AmrLaunch& A, int lev, const int g[3], int& b,
+                                         long& c) {
+  const AmrLevelDev& L = A.levels[lev];
+  const int bx = g[0] / L.nb[0], by = g[1] / L.nb[1], bz = g[2] / L.nb[2];
+  const int id = L.index[((long)bz * L.ext[1] + by) * L.ext[0] + bx];
+  if (id < 0) return false;
+  const DevParams& P = A.cls[A.blocks[id].cls];
+  c = cidx(P, g[0] - bx * L.nb[0] + P.lo[0], g[1] - by * L.nb[1] + P.lo[1],
+           g[2] - bz * L.nb[2] + P.lo[2]);
+  b = id;
+  return true;
+}
+
+// conserved-form value of variable v at cell c of U
+__device__ __forceinline__ double amr_q(const AmrLaunch& A, const DevParams& P, const double* U,
+                                        long c, int v) {
+  const double x = U[(long)v * P.ncell + c];
+  return (A.perMass >> v) & 1ULL ? U[P.voff[FX_DENS] + c] * x : x;
+}
+
+// MC-limited slope from the left and right differences
+__device__ __forceinline__ double amr_mc(double dl, double dr) {
+  if (!(dl * dr > 0.)) return 0.;
+  const double dc = 0.5 * (dl + dr);
+  const double m = fmin(fabs(dc), fmin(2. * fabs(dl), 2. * fabs(dr)));
+  return copysign(m, dc);
+}
+
+// Guard cells of every block: same-level copy where the level has a block,
+// else conservative linear prolongation from the next coarser level followed
+// by the EOS (MODE_DENS_EI).  One thread per cell of the guard-inclusive box;
+// interior cells return at once.
+__global__ void __launch_bounds__(kThreads) k_amr_fill(AmrLaunch A) {
+  const long total = A.start[A.nBlocks];
+  for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+       gid += (long)gridDim.x * blockDim.x) {
+    const int b = find_tile(A.start, A.nBlocks, gid);
+    const int r = (int)(gid - A.start[b]);
+    const AmrBlock& B = A.blocks[b];
+    const DevParams& P = A.cls[B.cls];
+    const int i = r % P.nx, j = (r / P.nx) % P.ny, k = r / (P.nx * P.ny);
+    const int loc[3] = {i, j, k};
+    bool inside = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (a < P.ndim && (loc[a] < P.lo[a] || loc[a] > P.hi[a])) inside = false;
+    if (inside) continue;
+    const int lev = B.level;
+    const AmrLevelDev& Lf = A.levels[lev];
+    int g[3] = {0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (a < P.ndim) g[a] = B.pos[a] * Lf.nb[a] + loc[a] - P.lo[a];
+    amr_domain(A, lev, P.ndim, g);
+    double* Ud = B.U;
+    const long cd = (long)r;
+    int sb;
+    long sc;
+    if (amr_find(A, lev, g, sb, sc)) {
+      const AmrBlock& S = A.blocks[sb];
+      const DevParams& PS = A.cls[S.cls];
+      for (int v = 0; v < P.nunk; ++v) Ud[(long)v * P.ncell + cd] = S.U[(long)v * PS.ncell + sc];
+      continue;
+    }
+    // prolongation from level lev - 1 (the coarse cell and its neighbours)
+    if (lev == 0) {
+      report_error(A.status, 3, b, i, j, k, -1);
+      continue;
+    }
+    int gc[3] = {g[0] / 2, g[1] / 2, g[2] / 2};
+    int off[3] = {g[0] - 2 * gc[0], g[1] - 2 * gc[1], g[2] - 2 * gc[2]};
+    int cb;
+    long cc;
+    if (!amr_find(A, lev - 1, gc, cb, cc)) {  // the levels are not properly nested
+      report_error(A.status, 3, b, i, j, k, -1);
+      continue;
+    }
+    const AmrBlock& Cb = A.blocks[cb];
+    const DevParams& PC = A.cls[Cb.cls];
+    int nbm[3], nbp[3];
+    long ncm[3], ncp[3];
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (a >= P.ndim) break;
+      int gm[3] = {gc[0], gc[1], gc[2]}, gp[3] = {gc[0], gc[1], gc[2]};
+      gm[a] -= 1;
+      gp[a] += 1;
+      amr_domain(A, lev - 1, P.ndim, gm);
+      amr_domain(A, lev - 1, P.ndim, gp);
+      ok = ok && amr_find(A, lev - 1, gm, nbm[a], ncm[a]) && amr_find(A, lev - 1, gp, nbp[a], ncp[a]);
+    }
+    if (!ok) {
+      report_error(A.status, 3, b, i, j, k, -1);
+      continue;
+    }
+    const double dens = [&] {
+      const int v = (int)(P.voff[FX_DENS] / P.ncell);
+      const double q0 = amr_q(A, PC, Cb.U, cc, v);
+      double q = q0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (a >= P.ndim) break;
+        const AmrBlock& M = A.blocks[nbm[a]];
+        const AmrBlock& Q = A.blocks[nbp[a]];
+        const double s = amr_mc(q0 - amr_q(A, A.cls[M.cls], M.U, ncm[a], v),
+                                amr_q(A, A.cls[Q.cls], Q.U, ncp[a], v) - q0);
+        q = q + s * (off[a] ? 0.25 : -0.25);
+      }
+      return q;
+    }();
+    for (int v = 0; v < P.nunk; ++v) {
+      const double q0 = amr_q(A, PC, Cb.U, cc, v);
+      double q = q0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (a >= P.ndim) break;
+        const AmrBlock& M = A.blocks[nbm[a]];
+        const AmrBlock& Q = A.blocks[nbp[a]];
+        const double s = amr_mc(q0 - amr_q(A, A.cls[M.cls], M.U, ncm[a], v),
+                                amr_q(A, A.cls[Q.cls], Q.U, ncp[a], v) - q0);
+        q = q + s * (off[a] ? 0.25 : -0.25);
+      }
+      Ud[(long)v * P.ncell + cd] = (A.perMass >> v) & 1ULL ? q / dens : q;
+    }
+    const EosOut eo = eos_dens_ei(P, dens, Ud[P.voff[FX_VELX] + cd], Ud[P.voff[FX_VELY] + cd],
+                                  Ud[P.voff[FX_VELZ] + cd], Ud[P.voff[FX_EINT] + cd],
+                                  Ud[P.voff[FX_ENER] + cd]);
+    Ud[P.voff[FX_PRES] + cd] = eo.pres;
+    Ud[P.voff[FX_TEMP] + cd] = eo.temp;
+    Ud[P.voff[FX_ENER] + cd] = eo.ener;
+    Ud[P.voff[FX_EINT] + cd] = eo.eint;
+    Ud[P.voff[FX_GAMC] + cd] = P.gamma;
+    Ud[P.voff[FX_GAME] + cd] = eo.game;
+  }
+}
+
+// Restriction of one level onto its parent level: every interior cell of
+// the parent covered by the listed fine blocks becomes the average of its
+// 2^ndim children (z, then y, then x outermost to innermost), PER_MASS
+// variables density-weighted.  One thread per covered parent cell.
+__global__ void __launch_bounds__(kThreads) k_amr_restrict(AmrLaunch A) {
+  const long total = A.start[A.nList];
+  for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+       gid += (long)gridDim.x * blockDim.x) {
+    const int q = find_tile(A.start, A.nList, gid);
+    const int r = (int)(gid - A.start[q]);
+    const int fb = A.list[q];
+    const AmrBlock& F = A.blocks[fb];
+    const DevParams& P = A.cls[F.cls];
+    const int hx = P.nxb / 2, hy = P.ndim > 1 ? P.nyb / 2 : 1;
+    const int c[3] = {r % hx, (r / hx) % hy, r / (hx * hy)};
+    int G[3] = {0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (a < P.ndim) G[a] = F.pos[a] * (A.levels[F.level].nb[a] / 2) + c[a];
+    int pb;
+    long pc;
+    if (!amr_find(A, F.level - 1, G, pb, pc)) {
+      report_error(A.status, 3, fb, c[0], c[1], c[2], -1);
+      continue;
+    }
+    const AmrBlock& Pb = A.blocks[pb];
+    const DevParams& PP = A.cls[Pb.cls];
+    const int nz = P.ndim > 2 ? 2 : 1, ny = P.ndim > 1 ? 2 : 1;
+    const double frac = 1.0 / (double)(2 * ny * nz);
+    // children sum of variable v in conserved form
+    auto child_sum = [&](int v) {
+      double s = 0.;
+      for (int oz = 0; oz < nz; ++oz)
+        for (int oy = 0; oy < ny; ++oy)
+          for (int ox = 0; ox < 2; ++ox) {
+            const long fc = cidx(P, 2 * c[0] + ox + P.lo[0], 2 * c[1] + oy + P.lo[1],
+                                 2 * c[2] + oz + P.lo[2]);
+            s = s + amr_q(A, P, F.U, fc, v);
+          }
+      return s;
+    };
+    const double dens = child_sum((int)(P.voff[FX_DENS] / P.ncell)) * frac;
+    for (int v = 0; v < P.nunk; ++v) {
+      const double avg = child_sum(v) * frac;
+      Pb.U[(long)v * PP.ncell + pc] = (A.perMass >> v) & 1ULL ? avg / dens : avg;
+    }
+  }
+}
+
+// per-workgroup max of (|v_d| + c) / del_d over the interior cells of every
+// block of every level
+__global__ void __launch_bounds__(kThreads) k_amr_dt(AmrLaunch A, double* __restrict__ partial) {
+  __shared__ double red[kThreads];
+  const long total = A.start[A.nBlocks];
+  double m = 0.;
+  for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+       gid += (long)gridDim.x * blockDim.x) {
+    const int b = find_tile(A.start, A.nBlocks, gid);
+    const int r = (int)(gid - A.start[b]);
+    const AmrBlock& B = A.blocks[b];
+    const DevParams& P = A.cls[B.cls];
+    const AmrLevelDev& L = A.levels[B.level];
+    const int nxi = P.nxb, nyi = P.ndim > 1 ? P.nyb : 1;
+    const long c = cidx(P, r % nxi + P.lo[0], (r / nxi) % nyi + P.lo[1], r / (nxi * nyi) + P.lo[2]);
+    const double* U = B.U;
+    const double cs = sqrt(U[P.voff[FX_GAMC] + c] * U[P.voff[FX_PRES] + c] / U[P.voff[FX_DENS] + c]);
+    double dtl = (fabs(U[P.voff[FX_VELX] + c] - 0.) + cs) * (1. / L.del[0]);
+    if (P.ndim > 1) dtl = fmax(dtl, (fabs(U[P.voff[FX_VELY] + c] - 0.) + cs) * (1. / L.del[1]));
+    if (P.ndim > 2) dtl = fmax(dtl, (fabs(U[P.voff[FX_VELZ] + c] - 0.) + cs) * (1. / L.del[2]));
+    m = fmax(m, dtl);
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}

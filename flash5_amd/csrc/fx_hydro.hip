@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -443,6 +444,7 @@ __device__ __forceinline__ long face_index(const DevParams& P, int d, int i, int
 
 #include "fx_march.hpp"
 #include "fx_learned.hpp"
+#include "fx_amr.hpp"
 
 // FROM_STATES: face-averaged pressure from this call's states; otherwise
 // read from the tile's pavg array (hy_updateSolution path).
@@ -840,6 +842,32 @@ struct Lib {
   int nItems = 0;
   int itemsTX = 0;                     // column-tile width the items were built for
   int persist = 0;                     // workgroups per CU of the multi-class march (0: occupancy)
+  unsigned long uploadGen = 0;         // counts tile-list uploads
+  // multi-level grid (fx_amr_*)
+  struct Amr {
+    bool ready = false;
+    int nBlocks = 0, nLevels = 0;
+    int periodic[3] = {0, 0, 0};
+    unsigned long long perMass = 0;
+    std::vector<fx_tile> tiles;
+    std::vector<AmrBlock> hblocks;
+    AmrBlock* dblocks = nullptr;
+    AmrLevelDev* dlevels = nullptr;
+    std::vector<int*> dindex;                 // per level
+    long* dboxStart = nullptr;                // guard-inclusive cells before each block
+    long* dcellStart = nullptr;               // interior cells before each block
+    std::vector<int*> dlist;                  // per fine level: its blocks
+    std::vector<long*> drstart;               // per fine level: covered parent cells before each
+    std::vector<int> nlist;
+    std::vector<fx_fluxreg_face*> dfaces;     // per level pair (coarse level L, L+1)
+    std::vector<int> nfaces;
+    std::vector<double*> dreg;
+    long boxCells = 0, cells = 0;
+    double* flux = nullptr;                   // the grid's own flux and pavg arrays
+    double* pavg = nullptr;
+    unsigned long gen = 0;                    // upload generation of the grid's tile list
+    int TX = 8, TY = 8;
+  } amr;
   // batch tile-list cache (fx_hydro_advance_batch)
   double* batchU = nullptr;
   long batchN = -1, batchStride = -1;
@@ -1034,6 +1062,7 @@ int classify_tiles(const fx_tile* tiles, int nTiles, bool useClasses) {
 int upload_tiles(const fx_tile* tiles, int nTiles, bool needFlux, fx_status* st,
                  bool useClasses = true) {
   g.batchValid = false;
+  ++g.uploadGen;
   if (classify_tiles(tiles, nTiles, useClasses)) {
     if (st) st->code = 3;
     return 3;
@@ -1181,6 +1210,65 @@ int record(int n) {
   return hipEventRecord(g.ev[n], g.stream) == hipSuccess ? 0 : 4;
 }
 
+// Flux stages of a step on the uploaded tiles (march path): the column march,
+// then the exact solver's face stage (records events 1 and 2).  Launches
+// only: scratch and work items are in place.
+int launch_flux_stages(int nTiles, double dt, int TX, int TY) {
+  const DevParams& P = g.cur;
+  MarchLaunch M{g.dtiles, nTiles, g.multi ? 0 : P.nxb / TX,
+                g.multi ? 0 : (P.ndim > 1 ? P.nyb : 1) / TY, g.status, dt, nullptr, g.dtDev};
+  const dim3 grid((unsigned)(g.multi ? 1 : nTiles * M.tilesX * M.tilesY));
+  if (P.riemann == FX_EXACT) {
+    // exact solver: the march stores face states, a separate stage solves
+    M.fstates = g.fstates;
+    if (g.multi) launch_march_multi<RS_STATES>(P.ndim, P.precision, TX, M);
+    else launch_march<RS_STATES>(P, M, TX, grid);
+    if (record(1)) return 4;
+    const long faces = g.hfaceStart[nTiles];
+    FaceLaunch F{g.dtiles, nTiles, g.fstates, P.learned ? g.seedY : nullptr, g.status, g.seedGate,
+                 g.multi ? g.dcls : nullptr, g.multi ? g.dtcls : nullptr,
+                 g.multi ? g.dfaceStart : nullptr};
+    g.lastFaces = faces;
+    g.lastTiles = nTiles;
+    g.lastMulti = g.multi;
+    const dim3 gf(grid_for(faces));
+    if (g.multi && P.learned) {
+      hipLaunchKernelGGL(k_face_seed<true>, wave_grid(faces), dim3(kLearnThreads), 0, g.stream, P,
+                         F, g.seedPacked, g.seedY);
+      hipLaunchKernelGGL((k_face_solve<true, true>), gf, dim3(kThreads), 0, g.stream, P, F);
+    } else if (g.multi) {
+      hipLaunchKernelGGL((k_face_solve<false, true>), gf, dim3(kThreads), 0, g.stream, P, F);
+    } else if (P.learned) {
+      hipLaunchKernelGGL(k_face_seed<false>, wave_grid(faces), dim3(kLearnThreads), 0, g.stream,
+                         P, F, g.seedPacked, g.seedY);
+      hipLaunchKernelGGL((k_face_solve<true, false>), gf, dim3(kThreads), 0, g.stream, P, F);
+    } else {
+      hipLaunchKernelGGL((k_face_solve<false, false>), gf, dim3(kThreads), 0, g.stream, P, F);
+    }
+    if (record(2)) return 4;
+  } else {
+    if (g.multi) launch_march_multi<FX_HLLC>(P.ndim, P.precision, TX, M);
+    else launch_march<FX_HLLC>(P, M, TX, grid);
+    if (record(1)) return 4;
+    if (record(2)) return 4;
+  }
+  return 0;
+}
+
+// update stage of a step on the uploaded tiles (dIn: per-tile in-place flags
+// on the device, or null for all)
+void launch_update(int nTiles, double dt, const unsigned char* dIn) {
+  Launch L{g.dtiles, nTiles, nullptr, g.status, dt, g.dtDev};
+  if (g.multi) {
+    const ClassMap C{g.dcls, g.dtcls, g.dcellStart};
+    hipLaunchKernelGGL(k_update_multi, dim3(grid_for(g.hcellStart[nTiles])), dim3(kThreads), 0,
+                       g.stream, L, C, dIn);
+  } else {
+    hipLaunchKernelGGL((k_update<false, false>), dim3(grid_for(g.cur.icells * nTiles)),
+                       dim3(kThreads), 0, g.stream, g.cur, L, dIn);
+  }
+}
+
 // scratch of the separate Riemann stage, grown before (never during) a step
 int ensure_step_scratch(int nTiles, fx_status* st) {
   const DevParams& P = g.dp;
@@ -1220,7 +1308,7 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
     HIPCALL(hipMemcpyAsync(g.dinplace, inPlace, nTiles, hipMemcpyHostToDevice, g.stream), st);
     dIn = g.dinplace;
   }
-  int TX = 0, TY = 0, NT = 0;
+  int TX = 0, TY = 8, NT = 0;
   const bool march = g.multi ? multi_march_shape(&TX) : (!g.legacy && march_shape(P, &TX, &TY, &NT));
   if (P.learned && (!march || !g.seedReady)) {  // the seed needs the march path and a network
     if (st) st->code = 3;
@@ -1230,79 +1318,13 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
     if (st) st->code = 3;
     return 3;
   }
-  if (g.multi) {
-    if (build_march_items(nTiles, TX, st)) return 4;
-    MarchLaunch M{g.dtiles, nTiles, 0, 0, g.status, dt, nullptr, g.dtDev};
-    Launch L{g.dtiles, nTiles, nullptr, g.status, dt, g.dtDev};
-    const long faces = g.hfaceStart[nTiles];
+  if (march) {
+    if (g.multi && build_march_items(nTiles, TX, st)) return 4;
     if (ensure_step_scratch(nTiles, st)) return 4;
     hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
     if (record(0)) return 4;
-    if (P.riemann == FX_EXACT) {
-      M.fstates = g.fstates;
-      launch_march_multi<RS_STATES>(P.ndim, P.precision, TX, M);
-      if (record(1)) return 4;
-      FaceLaunch F{g.dtiles, nTiles, g.fstates, P.learned ? g.seedY : nullptr, g.status,
-                   g.seedGate, g.dcls, g.dtcls, g.dfaceStart};
-      g.lastFaces = faces;
-      g.lastTiles = nTiles;
-      g.lastMulti = true;
-      if (P.learned) {
-        hipLaunchKernelGGL(k_face_seed<true>, wave_grid(faces), dim3(kLearnThreads), 0, g.stream, P,
-                           F, g.seedPacked, g.seedY);
-        hipLaunchKernelGGL((k_face_solve<true, true>), dim3(grid_for(faces)), dim3(kThreads), 0,
-                           g.stream, P, F);
-      } else {
-        hipLaunchKernelGGL((k_face_solve<false, true>), dim3(grid_for(faces)), dim3(kThreads), 0,
-                           g.stream, P, F);
-      }
-      if (record(2)) return 4;
-    } else {
-      launch_march_multi<FX_HLLC>(P.ndim, P.precision, TX, M);
-      if (record(1)) return 4;
-      if (record(2)) return 4;
-    }
-    const ClassMap C{g.dcls, g.dtcls, g.dcellStart};
-    hipLaunchKernelGGL(k_update_multi, dim3(grid_for(g.hcellStart[nTiles])), dim3(kThreads), 0,
-                       g.stream, L, C, dIn);
-    if (record(3)) return 4;
-  } else if (march) {
-    MarchLaunch M{g.dtiles, nTiles, P.nxb / TX, (P.ndim > 1 ? P.nyb : 1) / TY, g.status, dt,
-                  nullptr, g.dtDev};
-    const dim3 grid((unsigned)(nTiles * M.tilesX * M.tilesY));
-    Launch L{g.dtiles, nTiles, nullptr, g.status, dt, g.dtDev};
-    const long perTile = P.nface[0] + P.nface[1] + P.nface[2];
-    const long faces = perTile * nTiles;
-    if (ensure_step_scratch(nTiles, st)) return 4;
-    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
-    if (record(0)) return 4;
-    if (P.riemann == FX_EXACT) {
-      // exact solver: the march stores face states, a separate stage solves
-      M.fstates = g.fstates;
-      launch_march<RS_STATES>(P, M, TX, grid);
-      if (record(1)) return 4;
-      FaceLaunch F{g.dtiles, nTiles, g.fstates, P.learned ? g.seedY : nullptr, g.status,
-                   g.seedGate, nullptr, nullptr, nullptr};
-      g.lastFaces = faces;
-      g.lastTiles = nTiles;
-      g.lastMulti = false;
-      if (P.learned) {
-        hipLaunchKernelGGL(k_face_seed<false>, wave_grid(faces), dim3(kLearnThreads), 0, g.stream,
-                           P, F, g.seedPacked, g.seedY);
-        hipLaunchKernelGGL((k_face_solve<true, false>), dim3(grid_for(faces)), dim3(kThreads), 0,
-                           g.stream, P, F);
-      } else {
-        hipLaunchKernelGGL((k_face_solve<false, false>), dim3(grid_for(faces)), dim3(kThreads), 0,
-                           g.stream, P, F);
-      }
-      if (record(2)) return 4;
-    } else {
-      launch_march<FX_HLLC>(P, M, TX, grid);
-      if (record(1)) return 4;
-      if (record(2)) return 4;
-    }
-    hipLaunchKernelGGL((k_update<false, false>), dim3(grid_for(P.icells * nTiles)), dim3(kThreads), 0,
-                       g.stream, P, L, dIn);
+    if (launch_flux_stages(nTiles, dt, TX, TY)) return 4;
+    launch_update(nTiles, dt, dIn);
     if (record(3)) return 4;
   } else {
   if (grow(&g.states, &g.statesCap, (size_t)P.scellsPerTile * kStateComps * nTiles, st)) return 4;
@@ -1500,6 +1522,128 @@ void launch_fill(double* U, long nBlocks, long blockStride) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------
+// multi-level grids (fx_amr_*)
+// ---------------------------------------------------------------------
+void amr_free() {
+  Lib::Amr& A = g.amr;
+  void* bufs[] = {A.dblocks, A.dlevels, A.dboxStart, A.dcellStart, A.flux, A.pavg};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  for (int* q : A.dindex) if (q) (void)hipFree(q);
+  for (int* q : A.dlist) if (q) (void)hipFree(q);
+  for (long* q : A.drstart) if (q) (void)hipFree(q);
+  for (fx_fluxreg_face* q : A.dfaces) if (q) (void)hipFree(q);
+  for (double* q : A.dreg) if (q) (void)hipFree(q);
+  A = Lib::Amr{};
+}
+
+template <typename T>
+int upload_vec(T** dst, const std::vector<T>& v, fx_status* st) {
+  *dst = nullptr;
+  if (v.empty()) return 0;
+  HIPCALL(hipMalloc((void**)dst, sizeof(T) * v.size()), st);
+  HIPCALL(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice), st);
+  return 0;
+}
+
+AmrLaunch amr_launch() {
+  const Lib::Amr& A = g.amr;
+  AmrLaunch L{};
+  L.blocks = A.dblocks;
+  L.nBlocks = A.nBlocks;
+  L.levels = A.dlevels;
+  L.cls = g.dcls;
+  L.list = nullptr;
+  L.nList = A.nBlocks;
+  L.start = A.dboxStart;
+  for (int a = 0; a < 3; ++a) L.periodic[a] = A.periodic[a];
+  L.perMass = A.perMass;
+  L.status = g.status;
+  return L;
+}
+
+// the grid's tile list on the device (another call may have replaced it)
+int amr_ensure_uploaded(fx_status* st) {
+  Lib::Amr& A = g.amr;
+  if (A.gen == g.uploadGen) return 0;
+  const int rc = upload_tiles(A.tiles.data(), A.nBlocks, true, st);
+  if (rc) return rc;
+  A.gen = g.uploadGen;
+  if (g.multi && build_march_items(A.nBlocks, A.TX, st)) return 4;
+  return ensure_step_scratch(A.nBlocks, st);
+}
+
+void amr_launch_fill() {
+  const AmrLaunch L = amr_launch();
+  hipLaunchKernelGGL(k_amr_fill, dim3(grid_for(g.amr.boxCells)), dim3(kThreads), 0, g.stream, L);
+}
+
+// fine-to-coarse flux correction, finest level pair first
+void amr_launch_register() {
+  const Lib::Amr& A = g.amr;
+  const DevParams& P = g.dp;
+  const double coef = P.ndim == 3 ? 0.25 : (P.ndim == 2 ? 0.5 : 1.0);
+  for (int L = A.nLevels - 2; L >= 0; --L) {
+    const int n = A.nfaces[L];
+    if (n <= 0) continue;
+    RegLaunch R{g.dtiles, A.dfaces[L], n, A.dreg[L], g.dcls, g.multi ? g.dtcls : nullptr};
+    if (!g.multi) R.cls = nullptr;
+    const dim3 grid(grid_for((long)n * FX_NFLUX));
+    hipLaunchKernelGGL(k_fluxreg_put, grid, dim3(kThreads), 0, g.stream, g.cur, R, coef);
+    hipLaunchKernelGGL(k_fluxreg_conserve, grid, dim3(kThreads), 0, g.stream, g.cur, R);
+  }
+}
+
+// restriction, finest level first
+void amr_launch_restrict() {
+  const Lib::Amr& A = g.amr;
+  for (int L = A.nLevels - 1; L >= 1; --L) {
+    if (A.nlist[L] <= 0) continue;
+    AmrLaunch R = amr_launch();
+    R.list = A.dlist[L];
+    R.nList = A.nlist[L];
+    R.start = A.drstart[L];
+    long cells = 0;
+    for (int b = 0; b < A.nBlocks; ++b)
+      if (A.hblocks[b].level == L) {
+        const DevParams& P = g.hcls[A.hblocks[b].cls];
+        cells += (long)(P.nxb / 2) * (P.ndim > 1 ? P.nyb / 2 : 1) * (P.ndim > 2 ? P.nzb / 2 : 1);
+      }
+    hipLaunchKernelGGL(k_amr_restrict, dim3(grid_for(cells)), dim3(kThreads), 0, g.stream, R);
+  }
+}
+
+// dt = cfl / max((|v| + c) / del) over every level, into g.dtBuf[0]; t += dt
+int amr_launch_dt(double cfl, fx_status* st) {
+  const int grid = grid_for(g.amr.cells);
+  if (g.partialCap < grid) {
+    if (g.partial) HIPCALL(hipFree(g.partial), st);
+    g.partial = nullptr;
+    HIPCALL(hipMalloc((void**)&g.partial, sizeof(double) * grid), st);
+    g.partialCap = grid;
+  }
+  AmrLaunch L = amr_launch();
+  L.start = g.amr.dcellStart;
+  hipLaunchKernelGGL(k_amr_dt, dim3(grid), dim3(kThreads), 0, g.stream, L, g.partial);
+  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kThreads), 0, g.stream, g.partial, grid, cfl,
+                     g.dtBuf, g.dtBuf + 1);
+  return 0;
+}
+
+// one step of every level: guard fill, fluxes, flux correction, update,
+// restriction (launches only; dt from the argument or, when dtDev is set,
+// from the device)
+int amr_launch_step(double dt) {
+  hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
+  amr_launch_fill();
+  if (launch_flux_stages(g.amr.nBlocks, dt, g.amr.TX, g.amr.TY)) return 4;
+  amr_launch_register();
+  launch_update(g.amr.nBlocks, dt, nullptr);
+  amr_launch_restrict();
+  return 0;
+}
+
 extern "C" {
 
 int fx_hydro_init(const fx_params* p) {
@@ -1508,6 +1652,7 @@ int fx_hydro_init(const fx_params* p) {
   if (!g.init || memcmp(&g.prm, p, sizeof *p) != 0) {
     g.batchValid = false;
     g.hcls.clear();  // classes carry the knobs of the init they were set under
+    g.amr.ready = false;
   }
   g.prm = *p;
   g.dp = make_dev_params(*p);
@@ -1532,6 +1677,7 @@ void fx_hydro_finalize(void) {
   for (hipEvent_t e : g.ev)
     if (e) (void)hipEventDestroy(e);
   if (g.capStream) (void)hipStreamDestroy(g.capStream);
+  amr_free();
   g = Lib{};
 }
 
@@ -1829,10 +1975,361 @@ int fx_hydro_set_block_classes(const int* nb, int nClasses) {
         hipSuccess)
       return 4;
   }
+  bool same = cls.size() == g.hcls.size();
+  for (size_t c = 0; same && c < cls.size(); ++c)
+    same = cls[c].nxb == g.hcls[c].nxb && cls[c].nyb == g.hcls[c].nyb && cls[c].nzb == g.hcls[c].nzb;
   g.hcls = cls;
-  g.nItems = 0;
-  g.batchValid = false;
+  if (!same) {  // work items and a grid name classes by index
+    g.nItems = 0;
+    g.batchValid = false;
+    g.amr.ready = false;
+  }
   return 0;
+}
+
+int fx_amr_setup(const fx_tile* tiles, const fx_amr_block* blocks, int nBlocks, int nLevels,
+                 const int* levelExtent, const int* periodic, fx_status* st) {
+  if (st) memset(st, 0, sizeof *st);
+  amr_free();
+  auto bad = [&]() {
+    if (st) st->code = 3;
+    return 3;
+  };
+  if (!g.init || g.hcls.empty() || !tiles || !blocks || nBlocks <= 0 || nLevels < 1 ||
+      nLevels > 16 || !levelExtent || g.prm.nunk > 64)
+    return bad();
+  const int ndim = g.dp.ndim;
+  // every level has one block shape, its cell width, and twice the
+  // resolution of the one below
+  std::vector<int> lcls(nLevels, -1);
+  std::vector<std::array<double, 3>> ldel(nLevels);
+  for (int b = 0; b < nBlocks; ++b) {
+    const int L = blocks[b].level - 1, c = tiles[b].block_class;
+    if (L < 0 || L >= nLevels || c < 0 || c >= (int)g.hcls.size()) return bad();
+    if (lcls[L] < 0) {
+      lcls[L] = c;
+      for (int a = 0; a < 3; ++a) ldel[L][a] = tiles[b].del[a];
+    } else if (g.hcls[lcls[L]].nxb != g.hcls[c].nxb || g.hcls[lcls[L]].nyb != g.hcls[c].nyb ||
+               g.hcls[lcls[L]].nzb != g.hcls[c].nzb) {
+      return bad();
+    }
+  }
+  std::vector<AmrLevelDev> lev(nLevels);
+  std::vector<std::vector<int>> index(nLevels);
+  for (int L = 0; L < nLevels; ++L) {
+    if (lcls[L] < 0) return bad();
+    const DevParams& P = g.hcls[lcls[L]];
+    const int nb[3] = {P.nxb, P.nyb, P.nzb};
+    for (int a = 0; a < 3; ++a) {
+      lev[L].nb[a] = nb[a];
+      lev[L].ext[a] = a < ndim ? levelExtent[3 * L + a] : 1;
+      lev[L].del[a] = ldel[L][a];
+      if (lev[L].ext[a] < 1) return bad();
+      if (L > 0 && a < ndim &&
+          (long)lev[L].ext[a] * nb[a] != 2L * lev[L - 1].ext[a] * lev[L - 1].nb[a])
+        return bad();
+    }
+    index[L].assign((size_t)lev[L].ext[0] * lev[L].ext[1] * lev[L].ext[2], -1);
+  }
+  Lib::Amr& A = g.amr;
+  A.hblocks.resize(nBlocks);
+  for (int b = 0; b < nBlocks; ++b) {
+    const int L = blocks[b].level - 1;
+    AmrBlock& B = A.hblocks[b];
+    B.U = tiles[b].U;
+    B.level = L;
+    B.cls = tiles[b].block_class;
+    for (int a = 0; a < 3; ++a) {
+      B.pos[a] = a < ndim ? blocks[b].pos[a] : 0;
+      if (B.pos[a] < 0 || B.pos[a] >= lev[L].ext[a]) return bad();
+    }
+    int& slot = index[L][((size_t)B.pos[2] * lev[L].ext[1] + B.pos[1]) * lev[L].ext[0] + B.pos[0]];
+    if (slot >= 0) return bad();  // two blocks at one position
+    slot = b;
+  }
+  A.nBlocks = nBlocks;
+  A.nLevels = nLevels;
+  for (int a = 0; a < 3; ++a) A.periodic[a] = periodic ? periodic[a] != 0 : 0;
+  for (int v : {FX_VELX, FX_VELY, FX_VELZ, FX_ENER, FX_EINT}) A.perMass |= 1ULL << g.prm.var[v];
+  // the grid's tiles, with flux and pavg arrays of the grid's own where absent
+  // (the library's shared ones may be regrown by other calls)
+  {
+    std::vector<fx_tile> own(tiles, tiles + nBlocks);
+    size_t fluxNeed = 0, pavgNeed = 0;
+    for (int b = 0; b < nBlocks; ++b) {
+      const DevParams& P = g.hcls[own[b].block_class];
+      for (int d = 0; d < ndim; ++d)
+        if (!own[b].flux[d]) fluxNeed += (size_t)P.nface[d] * FX_NFLUX;
+      if (!own[b].pavg) pavgNeed += (size_t)P.icells;
+    }
+    if (fluxNeed && hipMalloc((void**)&A.flux, sizeof(double) * fluxNeed) != hipSuccess) {
+      amr_free();
+      return 4;
+    }
+    if (pavgNeed && hipMalloc((void**)&A.pavg, sizeof(double) * pavgNeed) != hipSuccess) {
+      amr_free();
+      return 4;
+    }
+    double *fp = A.flux, *pp = A.pavg;
+    for (int b = 0; b < nBlocks; ++b) {
+      const DevParams& P = g.hcls[own[b].block_class];
+      for (int d = 0; d < ndim; ++d)
+        if (!own[b].flux[d]) {
+          own[b].flux[d] = fp;
+          fp += (size_t)P.nface[d] * FX_NFLUX;
+        }
+      if (!own[b].pavg) {
+        own[b].pavg = pp;
+        pp += (size_t)P.icells;
+      }
+    }
+    const int rc = upload_tiles(own.data(), nBlocks, true, st);
+    if (rc) {
+      amr_free();
+      return rc;
+    }
+  }
+  A.tiles = g.htiles;
+  A.gen = g.uploadGen;
+  if (g.multi) {
+    if (!multi_march_shape(&A.TX)) return bad();
+    A.TY = 8;
+  } else {
+    int NT = 0;
+    if (!march_shape(g.cur, &A.TX, &A.TY, &NT)) return bad();
+  }
+  if (g.multi && build_march_items(nBlocks, A.TX, st)) return 4;
+  if (ensure_step_scratch(nBlocks, st)) return 4;
+  // prefixes: guard-inclusive box cells and interior cells per block
+  std::vector<long> box(nBlocks + 1, 0);
+  for (int b = 0; b < nBlocks; ++b) box[b + 1] = box[b] + g.hcls[A.hblocks[b].cls].ncell;
+  A.boxCells = box[nBlocks];
+  A.cells = g.hcellStart[nBlocks];
+  int rc = upload_vec(&A.dboxStart, box, st);
+  if (!rc) rc = upload_vec(&A.dcellStart, g.hcellStart, st);
+  for (int L = 0; !rc && L < nLevels; ++L) {
+    int* d = nullptr;
+    rc = upload_vec(&d, index[L], st);
+    A.dindex.push_back(d);
+    lev[L].index = d;
+  }
+  if (!rc) rc = upload_vec(&A.dlevels, lev, st);
+  if (!rc) rc = upload_vec(&A.dblocks, A.hblocks, st);
+  // restriction lists: the blocks of each level above the first, with their
+  // covered parent cells
+  A.dlist.assign(nLevels, nullptr);
+  A.drstart.assign(nLevels, nullptr);
+  A.nlist.assign(nLevels, 0);
+  for (int L = 1; !rc && L < nLevels; ++L) {
+    std::vector<int> list;
+    std::vector<long> start(1, 0);
+    for (int b = 0; b < nBlocks; ++b)
+      if (A.hblocks[b].level == L) {
+        const DevParams& P = g.hcls[A.hblocks[b].cls];
+        list.push_back(b);
+        start.push_back(start.back() + (long)(P.nxb / 2) * (ndim > 1 ? P.nyb / 2 : 1) *
+                                           (ndim > 2 ? P.nzb / 2 : 1));
+      }
+    A.nlist[L] = (int)list.size();
+    rc = upload_vec(&A.dlist[L], list, st);
+    if (!rc) rc = upload_vec(&A.drstart[L], start, st);
+  }
+  // flux-register faces of each level pair: every coarse copy of each face
+  // on the boundary of the finer level (periodic wrap included, none on a
+  // non-periodic domain boundary)
+  A.dfaces.assign(nLevels, nullptr);
+  A.dreg.assign(nLevels, nullptr);
+  A.nfaces.assign(nLevels, 0);
+  for (int L = 0; !rc && L + 1 < nLevels; ++L) {
+    const AmrLevelDev &C = lev[L], &F = lev[L + 1];
+    std::vector<fx_fluxreg_face> faces;
+    auto at = [&](int Lv, const int p[3]) {
+      const AmrLevelDev& V = lev[Lv];
+      return index[Lv][((size_t)p[2] * V.ext[1] + p[1]) * V.ext[0] + p[0]];
+    };
+    for (int fb = 0; fb < nBlocks; ++fb) {
+      const AmrBlock& B = A.hblocks[fb];
+      if (B.level != L + 1) continue;
+      for (int d = 0; d < ndim; ++d)
+        for (int side = 0; side < 2; ++side) {
+          int np[3] = {B.pos[0], B.pos[1], B.pos[2]};
+          np[d] += side ? 1 : -1;
+          if (np[d] < 0 || np[d] >= F.ext[d]) {
+            if (!A.periodic[d]) continue;
+            np[d] = (np[d] + F.ext[d]) % F.ext[d];
+          }
+          if (at(L + 1, np) >= 0) continue;  // a fine neighbour: no register face
+          const int ncell = C.ext[d] * C.nb[d];
+          const int gcf = (B.pos[d] * F.nb[d] + (side ? F.nb[d] : 0)) / 2;  // coarse face
+          const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+          const int n1 = t1 < ndim ? F.nb[t1] / 2 : 1, n2 = t2 < ndim ? F.nb[t2] / 2 : 1;
+          for (int c2 = 0; c2 < n2; ++c2)
+            for (int c1 = 0; c1 < n1; ++c1) {
+              int gct[3] = {0, 0, 0};  // coarse transverse cells
+              if (t1 < ndim) gct[t1] = B.pos[t1] * (F.nb[t1] / 2) + c1;
+              if (t2 < ndim) gct[t2] = B.pos[t2] * (F.nb[t2] / 2) + c2;
+              int fi[3] = {0, 0, 0};
+              fi[d] = side ? F.nb[d] : 0;
+              if (t1 < ndim) fi[t1] = 2 * c1;
+              if (t2 < ndim) fi[t2] = 2 * c2;
+              // the coarse cells on either side of the face and their blocks
+              int last = -1;
+              for (int k = 0; k < 2; ++k) {
+                int gc[3] = {gct[0], gct[1], gct[2]};
+                gc[d] = k ? gcf : gcf - 1;
+                if (gc[d] < 0 || gc[d] >= ncell) {
+                  if (!A.periodic[d]) continue;
+                  gc[d] = (gc[d] + ncell) % ncell;
+                }
+                int bp[3];
+                for (int a = 0; a < 3; ++a) bp[a] = gc[a] / C.nb[a];
+                const int cb = at(L, bp);
+                if (cb < 0) { rc = 3; break; }  // the coarse level does not cover the face
+                if (cb == last) continue;       // both cells in one block: one copy
+                last = cb;
+                fx_fluxreg_face f{};
+                f.coarse_tile = cb;
+                f.fine_tile = fb;
+                f.dir = d;
+                int ci[3];
+                for (int a = 0; a < 3; ++a) ci[a] = gc[a] - bp[a] * C.nb[a];
+                if (!k) ci[d] += 1;  // the face is the high face of the low cell
+                f.ci = ci[0];
+                f.cj = ci[1];
+                f.ck = ci[2];
+                f.fi = fi[0];
+                f.fj = fi[1];
+                f.fk = fi[2];
+                faces.push_back(f);
+              }
+            }
+        }
+    }
+    if (rc) break;
+    if (check_reg_faces(A.tiles.data(), nBlocks, faces.data(), (int)faces.size(), true) ||
+        check_reg_faces(A.tiles.data(), nBlocks, faces.data(), (int)faces.size(), false)) {
+      rc = 3;
+      break;
+    }
+    A.nfaces[L] = (int)faces.size();
+    rc = upload_vec(&A.dfaces[L], faces, st);
+    if (!rc && !faces.empty())
+      if (hipMalloc((void**)&A.dreg[L], sizeof(double) * FX_NFLUX * faces.size()) != hipSuccess)
+        rc = 4;
+  }
+  if (rc) {
+    amr_free();
+    if (st) st->code = rc;
+    return rc;
+  }
+  if (!g.dtBuf) HIPCALL(hipMalloc((void**)&g.dtBuf, 2 * sizeof(double)), st);
+  A.ready = true;
+  return 0;
+}
+
+int fx_amr_fill_guards(fx_status* st) {
+  if (st) memset(st, 0, sizeof *st);
+  if (!g.amr.ready) return 3;
+  if (amr_ensure_uploaded(st)) return 4;
+  hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
+  amr_launch_fill();
+  HIPCALL(hipGetLastError(), st);
+  return finish(st);
+}
+
+int fx_amr_restrict(fx_status* st) {
+  if (st) memset(st, 0, sizeof *st);
+  if (!g.amr.ready) return 3;
+  if (amr_ensure_uploaded(st)) return 4;
+  hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
+  amr_launch_restrict();
+  HIPCALL(hipGetLastError(), st);
+  return finish(st);
+}
+
+int fx_amr_compute_dt(double cfl, double* dtOut) {
+  if (!g.amr.ready || !(cfl > 0.) || !dtOut) return 3;
+  if (amr_ensure_uploaded(nullptr)) return 4;
+  if (amr_launch_dt(cfl, nullptr)) return 4;
+  double td[2];
+  if (hipMemcpyAsync(td, g.dtBuf, sizeof td, hipMemcpyDeviceToHost, g.stream) != hipSuccess ||
+      hipStreamSynchronize(g.stream) != hipSuccess)
+    return 4;
+  *dtOut = td[0];
+  return 0;
+}
+
+int fx_amr_step(double dt, fx_status* st) {
+  if (st) memset(st, 0, sizeof *st);
+  if (!g.amr.ready || !(dt > 0.)) {
+    if (st) st->code = 3;
+    return 3;
+  }
+  if (g.cur.learned && !g.seedReady) {
+    if (st) st->code = 3;
+    return 3;
+  }
+  if (amr_ensure_uploaded(st)) return 4;
+  if (amr_launch_step(dt)) return 4;
+  HIPCALL(hipGetLastError(), st);
+  return finish(st);
+}
+
+int fx_amr_run_graph(double cfl, int stepsPerGraph, int nReplays, double* tOut, fx_status* st) {
+  if (st) memset(st, 0, sizeof *st);
+  if (!g.amr.ready || stepsPerGraph < 1 || nReplays < 1 || !(cfl > 0.) ||
+      (g.dp.learned && !g.seedReady)) {
+    if (st) st->code = 3;
+    return 3;
+  }
+  if (amr_ensure_uploaded(st)) return 4;
+  {  // partial maxima of the dt reduction, sized before capture
+    const int grid = grid_for(g.amr.cells);
+    if (g.partialCap < grid) {
+      if (g.partial) HIPCALL(hipFree(g.partial), st);
+      g.partial = nullptr;
+      HIPCALL(hipMalloc((void**)&g.partial, sizeof(double) * grid), st);
+      g.partialCap = grid;
+    }
+  }
+  if (!g.capStream) HIPCALL(hipStreamCreateWithFlags(&g.capStream, hipStreamNonBlocking), st);
+  const hipStream_t saved = g.stream;
+  const int savedAsync = g.async, savedTiming = g.timing;
+  if (saved) HIPCALL(hipStreamSynchronize(saved), st);
+  HIPCALL(hipDeviceSynchronize(), st);
+  g.stream = g.capStream;
+  double zero2[2] = {0.0, 0.0};
+  HIPCALL(hipMemcpy(g.dtBuf, zero2, sizeof zero2, hipMemcpyHostToDevice), st);
+  g.async = 1;
+  g.timing = 0;
+  g.dtDev = g.dtBuf;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int err = 0;
+  if (hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) err = 4;
+  for (int s = 0; !err && s < stepsPerGraph; ++s) {
+    if (amr_launch_dt(cfl, nullptr)) err = 4;
+    if (!err && amr_launch_step(0.0)) err = 4;
+  }
+  if (hipStreamEndCapture(g.stream, &graph) != hipSuccess) err = 4;
+  if (!err && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) err = 4;
+  for (int r = 0; !err && r < nReplays; ++r)
+    if (hipGraphLaunch(exec, g.stream) != hipSuccess) err = 4;
+  if (!err && hipStreamSynchronize(g.stream) != hipSuccess) err = 4;
+  double td[2] = {0.0, 0.0};
+  if (!err && hipMemcpy(td, g.dtBuf, sizeof td, hipMemcpyDeviceToHost) != hipSuccess) err = 4;
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  g.dtDev = nullptr;
+  g.async = savedAsync;
+  g.timing = savedTiming;
+  g.stream = saved;
+  if (err) {
+    if (st) st->code = 4;
+    return 4;
+  }
+  if (tOut) *tOut = td[1];
+  return finish(st);
 }
 const char* fx_hydro_version(void) { return "flash5_amd fx_hydro 0.1 (gfx950)"; }
 

@@ -61,6 +61,10 @@ class FxTile(C.Structure):
                 ("level", C.c_int), ("block_class", C.c_int)]
 
 
+class FxAmrBlock(C.Structure):
+    _fields_ = [("level", C.c_int), ("pos", C.c_int * 3)]
+
+
 class FxStatus(C.Structure):
     _fields_ = [("code", C.c_int), ("first_tile", C.c_int), ("first_i", C.c_int),
                 ("first_j", C.c_int), ("first_k", C.c_int), ("first_dir", C.c_int),
@@ -111,6 +115,14 @@ EXPORTS = {
     "fx_hydro_set_timing": ([C.c_int], None),
     "fx_hydro_set_kernel_path": ([C.c_int], None),
     "fx_hydro_set_block_classes": ([C.POINTER(C.c_int), C.c_int], C.c_int),
+    "fx_amr_setup": ([C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
+                      C.POINTER(C.c_int), C.POINTER(FxStatus)], C.c_int),
+    "fx_amr_fill_guards": ([C.POINTER(FxStatus)], C.c_int),
+    "fx_amr_restrict": ([C.POINTER(FxStatus)], C.c_int),
+    "fx_amr_compute_dt": ([C.c_double, C.POINTER(C.c_double)], C.c_int),
+    "fx_amr_step": ([C.c_double, C.POINTER(FxStatus)], C.c_int),
+    "fx_amr_run_graph": ([C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double),
+                          C.POINTER(FxStatus)], C.c_int),
     "fx_hydro_stage_times": ([C.POINTER(C.c_float), C.c_int], C.c_int),
     "fx_hydro_version": ([], C.c_char_p),
 }
@@ -289,6 +301,59 @@ class Hydro:
         """HydroParams of block class c (the params with that class's shape)."""
         nx, ny, nz = self._classes[c]
         return dataclasses.replace(self.params, nxb=nx, nyb=ny, nzb=nz)
+
+    # ---- multi-level grids held on the device (fx_amr_*) ----
+    def amr_setup(self, tiles: Sequence[Tile], levels: Sequence[int], positions,
+                  level_extent, periodic=(0, 0, 0)) -> None:
+        """Hands the library a multi-level grid: tiles (with block_class and
+        their level's deltas), each block's 1-based level and position on its
+        level, the blocks per axis covering the domain on each level."""
+        self._amr_tiles = self._tiles(tiles)          # keeps the ctypes array alive
+        blk = (FxAmrBlock * len(tiles))()
+        for n, (L, pos) in enumerate(zip(levels, positions)):
+            blk[n].level = int(L)
+            for a in range(3):
+                blk[n].pos[a] = int(pos[a]) if a < len(pos) else 0
+        ext = (C.c_int * (3 * len(level_extent)))(
+            *[int(e[a]) if a < len(e) else 1 for e in level_extent for a in range(3)])
+        per = (C.c_int * 3)(*[int(bool(p)) for p in periodic])
+        st = FxStatus()
+        rc = self.lib.fx_amr_setup(C.cast(self._amr_tiles, C.c_void_p), C.cast(blk, C.c_void_p),
+                                   len(tiles), len(level_extent), ext, per, C.byref(st))
+        if rc:
+            raise FxError("fx_amr_setup", st, rc)
+        self._amr_blocks = blk
+
+    def _amr_call(self, name, *args) -> FxStatus:
+        st = FxStatus()
+        rc = getattr(self.lib, name)(*args, C.byref(st))
+        self.last_status = st
+        if rc:
+            raise FxError(name, st, rc)
+        return st
+
+    def amr_fill_guards(self) -> FxStatus:
+        return self._amr_call("fx_amr_fill_guards")
+
+    def amr_restrict(self) -> FxStatus:
+        return self._amr_call("fx_amr_restrict")
+
+    def amr_compute_dt(self, cfl: float) -> float:
+        out = C.c_double()
+        rc = self.lib.fx_amr_compute_dt(cfl, C.byref(out))
+        if rc:
+            raise FxError("fx_amr_compute_dt", rc=rc)
+        return out.value
+
+    def amr_step(self, dt: float) -> FxStatus:
+        return self._amr_call("fx_amr_step", C.c_double(dt))
+
+    def amr_run_graph(self, cfl: float, steps_per_graph: int, replays: int) -> float:
+        """Runs replays x steps_per_graph graph-captured steps; returns t."""
+        t = C.c_double()
+        self._amr_call("fx_amr_run_graph", C.c_double(cfl), int(steps_per_graph), int(replays),
+                       C.byref(t))
+        return t.value
 
     def set_kernel_path(self, path: int) -> None:
         """0 = fused column-march kernel (default), 1 = three-kernel path."""

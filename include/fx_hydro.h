@@ -254,6 +254,46 @@ void fx_hydro_set_kernel_path(int path);
  * Returns 0, or 3 for a bad shape or count (at most FX_MAX_CLASSES). */
 #define FX_MAX_CLASSES 8
 int fx_hydro_set_block_classes(const int* nb, int nClasses);
+
+/* ---- multi-level block grids held on the device ----
+ * The Grid operations a multi-level hydro step needs between the Hydro
+ * calls, for grids whose levels differ in block size: guard fill
+ * (Grid_fillGuardCells(CENTER, ALLDIR), Amrex/Grid_fillGuardCells.F90:287-369:
+ * same-level copies, conservative linear prolongation from the next coarser
+ * level, EOS on the prolonged cells), the flux registers of every level pair
+ * (Grid_putFluxData / Grid_conserveFluxes, finest pair first) and restriction
+ * (Grid_restrictAllLevels -> Amrex/gr_restrictAllLevels.F90:86-130).  The
+ * refinement ratio is 2; each level has one block class; no subcycling (one
+ * dt for all levels).  Tiles carry block_class (fx_hydro_set_block_classes
+ * first) and the level's cell widths; null flux / pavg arrays are supplied
+ * by the library.  fx_hydro_set_block_classes with other shapes, or
+ * fx_hydro_init with other knobs, drops the grid (set it up again). */
+typedef struct {
+  int level;                /* 1-based refinement level */
+  int pos[3];               /* block position on its level, in blocks */
+} fx_amr_block;
+/* levelExtent[3*L + a]: blocks along axis a that would cover the domain at
+ * level L+1 (each level doubles the cells across the domain).  periodic[a]:
+ * 1 wraps axis a, 0 is outflow (zero gradient).  Returns 3 for an
+ * inconsistent grid (levels not nested by ratio 2, two blocks at one
+ * position, a level with two block shapes). */
+int fx_amr_setup(const fx_tile* tiles, const fx_amr_block* blocks, int nBlocks, int nLevels,
+                 const int* levelExtent, const int* periodic, fx_status* st);
+/* Guard cells of every block (code 3 if a fine block's coarse neighbourhood
+ * is missing: levels not properly nested). */
+int fx_amr_fill_guards(fx_status* st);
+/* Every covered cell from its children, finest level first. */
+int fx_amr_restrict(fx_status* st);
+/* dt = cfl / max over all levels' interior cells of (|v_d| + c) / del_d
+ * (Hydro_computeDt.F90:186-270 without the reductions over ranks). */
+int fx_amr_compute_dt(double cfl, double* dtOut);
+/* One step of every level: guard fill, fluxes of all blocks (one persistent
+ * launch per stage), flux correction at fine-coarse faces, update, then
+ * restriction. */
+int fx_amr_step(double dt, fx_status* st);
+/* nReplays of a HIP graph of stepsPerGraph steps (dt on the device each
+ * step); *tOut = simulated time. */
+int fx_amr_run_graph(double cfl, int stepsPerGraph, int nReplays, double* tOut, fx_status* st);
 /* Library version string. */
 const char* fx_hydro_version(void);
 
