@@ -138,6 +138,8 @@ def main() -> None:
                          "(untimed) on face states of this workload")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run plain 1 GiB copies (PMC byte-counter calibration)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in launch shapes instead of timing the candidates")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -169,6 +171,9 @@ def main() -> None:
         hy.set_seed_params(train_info.pop("params"))
     stream = torch.cuda.current_stream(dev)
     hy.set_stream(stream)
+    tune = None
+    if not args.no_autotune:   # untimed: candidate launch shapes timed on this batch
+        tune = hy.autotune_batch(U, dt, (dx,) * 3, reps=3)
     for _ in range(args.warmup):
         hy.advance_batch(U, dt, (dx,) * 3)
     torch.cuda.synchronize()
@@ -235,7 +240,7 @@ def main() -> None:
                    "newton_iters_per_face": (int(hy.last_status.newton_iters) /
                                              (3 * args.blocks * 17 * 16 * 16)),
                    "newton_fallbacks": int(hy.last_status.newton_fallbacks),
-                   "learned_seed": train_info},
+                   "learned_seed": train_info, "autotune": tune},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic("k_march", workload),

@@ -42,6 +42,7 @@ struct AmrLaunch {
   const int* list;          // block subset of the launch (restriction: one level), or null
   int nList;                // blocks in the launch
   const long* start;        // prefix over the launch's blocks of its work items
+  const int* wtab;          // block of every 64th work item (find_tile_w), or null
   int periodic[3];
   unsigned long long perMass;  // bit v: variable slot v is PER_MASS
   DevStatus* status;
@@ -92,15 +93,18 @@ __device__ __forceinline__ double amr_mc(double dl, double dr) {
 
 // Guard cells of every block: same-level copy where the level has a block,
 // else conservative linear prolongation from the next coarser level followed
-// by the EOS (MODE_DENS_EI).  One thread per cell of the guard-inclusive box;
-// interior cells return at once.
-__global__ void __launch_bounds__(kThreads) k_amr_fill(AmrLaunch A) {
-  const long total = A.start[A.nBlocks];
-  for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
-       gid += (long)gridDim.x * blockDim.x) {
-    const int b = find_tile(A.start, A.nBlocks, gid);
-    const int r = (int)(gid - A.start[b]);
+// by the EOS (MODE_DENS_EI).  Workgroup w takes the kThreads cells of
+// chunks[w] = (block, first cell) of the guard-inclusive box, so the block
+// and its parameters are uniform over the workgroup; interior cells return
+// at once.
+__global__ void __launch_bounds__(kThreads) k_amr_fill(AmrLaunch A,
+                                                        const int2* __restrict__ chunks) {
+  {
+    const int2 ch = chunks[blockIdx.x];
+    const int b = ch.x;
     const AmrBlock& B = A.blocks[b];
+    const int r = ch.y + (int)threadIdx.x;
+    if (r >= A.cls[B.cls].ncell) return;
     const DevParams& P = A.cls[B.cls];
     const int i = r % P.nx, j = (r / P.nx) % P.ny, k = r / (P.nx * P.ny);
     const int loc[3] = {i, j, k};
@@ -108,7 +112,7 @@ __global__ void __launch_bounds__(kThreads) k_amr_fill(AmrLaunch A) {
 #pragma unroll
     for (int a = 0; a < 3; ++a)
       if (a < P.ndim && (loc[a] < P.lo[a] || loc[a] > P.hi[a])) inside = false;
-    if (inside) continue;
+    if (inside) return;
     const int lev = B.level;
     const AmrLevelDev& Lf = A.levels[lev];
     int g[3] = {0, 0, 0};
@@ -124,12 +128,12 @@ __global__ void __launch_bounds__(kThreads) k_amr_fill(AmrLaunch A) {
       const AmrBlock& S = A.blocks[sb];
       const DevParams& PS = A.cls[S.cls];
       for (int v = 0; v < P.nunk; ++v) Ud[(long)v * P.ncell + cd] = S.U[(long)v * PS.ncell + sc];
-      continue;
+      return;
     }
     // prolongation from level lev - 1 (the coarse cell and its neighbours)
     if (lev == 0) {
       report_error(A.status, 3, b, i, j, k, -1);
-      continue;
+      return;
     }
     int gc[3] = {g[0] / 2, g[1] / 2, g[2] / 2};
     int off[3] = {g[0] - 2 * gc[0], g[1] - 2 * gc[1], g[2] - 2 * gc[2]};
@@ -137,7 +141,7 @@ __global__ void __launch_bounds__(kThreads) k_amr_fill(AmrLaunch A) {
     long cc;
     if (!amr_find(A, lev - 1, gc, cb, cc)) {  // the levels are not properly nested
       report_error(A.status, 3, b, i, j, k, -1);
-      continue;
+      return;
     }
     const AmrBlock& Cb = A.blocks[cb];
     const DevParams& PC = A.cls[Cb.cls];
@@ -156,7 +160,7 @@ __global__ void __launch_bounds__(kThreads) k_amr_fill(AmrLaunch A) {
     }
     if (!ok) {
       report_error(A.status, 3, b, i, j, k, -1);
-      continue;
+      return;
     }
     const double dens = [&] {
       const int v = (int)(P.voff[FX_DENS] / P.ncell);
@@ -256,7 +260,7 @@ __global__ void __launch_bounds__(kThreads) k_amr_dt(AmrLaunch A, double* __rest
   double m = 0.;
   for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
        gid += (long)gridDim.x * blockDim.x) {
-    const int b = find_tile(A.start, A.nBlocks, gid);
+    const int b = find_tile_w(A.start, A.wtab, A.nBlocks, gid);
     const int r = (int)(gid - A.start[b]);
     const AmrBlock& B = A.blocks[b];
     const DevParams& P = A.cls[B.cls];

@@ -61,6 +61,7 @@ struct ClassMap {
   const DevParams* cls;        // parameters per class
   const unsigned char* tcls;   // class of each tile
   const long* start;           // items before tile t; start[nTiles] = all items
+  const int* wtab;             // tile of every 64th item (find_tile_w)
 };
 
 // t with start[t] <= gid < start[t + 1] (start non-decreasing, start[0] = 0)
@@ -72,6 +73,18 @@ __device__ __forceinline__ int find_tile(const long* __restrict__ start, int n, 
     else hi = mid - 1;
   }
   return lo;
+}
+
+// The same through a table of 64-item chunks: wtab[c] is the tile holding
+// item 64 c.  Every tile holds at least 64 items, so an item lies in that
+// tile or the next; one load and one compare instead of a search.  wtab
+// null: the search.
+__device__ __forceinline__ int find_tile_w(const long* __restrict__ start,
+                                          const int* __restrict__ wtab, int n, long gid) {
+  if (!wtab) return find_tile(start, n, gid);
+  int t = wtab[gid >> 6];
+  if (t + 1 < n && start[t + 1] <= gid) ++t;
+  return t;
 }
 
 // the step's dt: from device memory when the host does not know it
@@ -587,7 +600,7 @@ __global__ void __launch_bounds__(kThreads) k_update_multi(Launch L, ClassMap M,
   const long total = M.start[L.nTiles];
   for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
        gid += (long)gridDim.x * blockDim.x) {
-    const int t = find_tile(M.start, L.nTiles, gid);
+    const int t = find_tile_w(M.start, M.wtab, L.nTiles, gid);
     update_cell<false, false>(M.cls[M.tcls[t]], L, t, gid - M.start[t], inPlace);
   }
 }
@@ -837,11 +850,19 @@ struct Lib {
   size_t dcellStartCap = 0;
   long* dfaceStart = nullptr;
   size_t dfaceStartCap = 0;
+  std::vector<int> hwCell, hwFace;     // find_tile_w tables of the cell and face prefixes
+  int* dwCell = nullptr;
+  size_t dwCellCap = 0;
+  int* dwFace = nullptr;
+  size_t dwFaceCap = 0;
   MarchItem* ditems = nullptr;         // column tiles of a multi-class march
   size_t ditemsCap = 0;
   int nItems = 0;
   int itemsTX = 0;                     // column-tile width the items were built for
-  int persist = 0;                     // workgroups per CU of the multi-class march (0: occupancy)
+  int itemsOrder = 0;                  // item order they were built in
+  int persist = 0;                     // multi-class march: multiple of the resident workgroups (0 = 1)
+  int tuneTX = 0;                      // autotuned column-tile width (0: built-in choice)
+  int itemOrder = 0;                   // multi-class items: 0 longest first, 1 tile by tile
   unsigned long uploadGen = 0;         // counts tile-list uploads
   // multi-level grid (fx_amr_*)
   struct Amr {
@@ -856,6 +877,10 @@ struct Lib {
     std::vector<int*> dindex;                 // per level
     long* dboxStart = nullptr;                // guard-inclusive cells before each block
     long* dcellStart = nullptr;               // interior cells before each block
+    int* dwBox = nullptr;                     // find_tile_w tables of the two prefixes
+    int* dwCell = nullptr;
+    int2* dchunks = nullptr;                  // guard-fill work: (block, first box cell)
+    long nChunks = 0;
     std::vector<int*> dlist;                  // per fine level: its blocks
     std::vector<long*> drstart;               // per fine level: covered parent cells before each
     std::vector<int> nlist;
@@ -974,7 +999,7 @@ DevParams make_dev_params(const fx_params& p) {
 bool march_shape(const DevParams& P, int* TX, int* TY, int* NT) {
   if (P.ndim < 2) return false;
   const int nx = P.nxb, ny = P.nyb;
-  if (nx % 16 == 0 && ny % 8 == 0) {
+  if (nx % 16 == 0 && ny % 8 == 0 && g.tuneTX != 8) {
     *TX = 16;
     *TY = 8;
   } else if (nx % 8 == 0 && ny % 8 == 0) {
@@ -1057,6 +1082,21 @@ int classify_tiles(const fx_tile* tiles, int nTiles, bool useClasses) {
   return 0;
 }
 
+// tile of every 64th item of a prefix (find_tile_w); every tile must hold
+// at least 64 items
+std::vector<int> wave_table(const std::vector<long>& start) {
+  const int n = (int)start.size() - 1;
+  const long total = start[n];
+  std::vector<int> w((size_t)((total + 63) / 64));
+  int t = 0;
+  for (size_t c = 0; c < w.size(); ++c) {
+    const long item = (long)c * 64;
+    while (t + 1 < n && start[t + 1] <= item) ++t;
+    w[c] = t;
+  }
+  return w;
+}
+
 // Uploads the tile list, substituting library-owned flux arrays where a
 // tile has none.
 int upload_tiles(const fx_tile* tiles, int nTiles, bool needFlux, fx_status* st,
@@ -1110,26 +1150,37 @@ int upload_tiles(const fx_tile* tiles, int nTiles, bool needFlux, fx_status* st,
     HIPCALL(hipMemcpyAsync(g.dfaceStart, g.hfaceStart.data(), sizeof(long) * (nTiles + 1),
                            hipMemcpyHostToDevice, g.stream),
             st);
+    g.hwCell = wave_table(g.hcellStart);
+    g.hwFace = wave_table(g.hfaceStart);
+    if (grow(&g.dwCell, &g.dwCellCap, g.hwCell.size(), st)) return 4;
+    if (grow(&g.dwFace, &g.dwFaceCap, g.hwFace.size(), st)) return 4;
+    HIPCALL(hipMemcpyAsync(g.dwCell, g.hwCell.data(), sizeof(int) * g.hwCell.size(),
+                           hipMemcpyHostToDevice, g.stream),
+            st);
+    HIPCALL(hipMemcpyAsync(g.dwFace, g.hwFace.data(), sizeof(int) * g.hwFace.size(),
+                           hipMemcpyHostToDevice, g.stream),
+            st);
     g.nItems = 0;  // march items are built for the tile shape of the call
   }
   return 0;
 }
 
-// Column tile shared by every class of a multi-class march.
+// Column tile shared by every class of a multi-class march (16 columns
+// across x where every class allows it, unless the autotuner chose 8).
 bool multi_march_shape(int* TX) {
   bool all16 = true;
   for (const DevParams& P : g.hcls) {
     if (P.ndim < 2 || P.nxb % 8 || P.nyb % 8) return false;
     if (P.nxb % 16) all16 = false;
   }
-  *TX = all16 ? 16 : 8;
+  *TX = all16 && g.tuneTX != 8 ? 16 : 8;
   return true;
 }
 
 // Column-tile work items of the uploaded multi-class list, longest columns
 // first (stable in tile order within a class).
 int build_march_items(int nTiles, int TX, fx_status* st) {
-  if (g.nItems > 0 && g.itemsTX == TX) return 0;
+  if (g.nItems > 0 && g.itemsTX == TX && g.itemsOrder == g.itemOrder) return 0;
   const int TY = 8;
   std::vector<MarchItem> items;
   for (int t = 0; t < nTiles; ++t) {
@@ -1138,15 +1189,17 @@ int build_march_items(int nTiles, int TX, fx_status* st) {
       for (int tx = 0; tx < P.nxb / TX; ++tx)
         items.push_back(MarchItem{t, (int)g.htcls[t], P.lo[0] + tx * TX, P.lo[1] + ty * TY});
   }
-  std::stable_sort(items.begin(), items.end(), [](const MarchItem& a, const MarchItem& b) {
-    return g.hcls[a.cls].nzb > g.hcls[b.cls].nzb;
-  });
+  if (g.itemOrder == 0)
+    std::stable_sort(items.begin(), items.end(), [](const MarchItem& a, const MarchItem& b) {
+      return g.hcls[a.cls].nzb > g.hcls[b.cls].nzb;
+    });
   if (grow(&g.ditems, &g.ditemsCap, items.size(), st)) return 4;
   HIPCALL(hipMemcpyAsync(g.ditems, items.data(), sizeof(MarchItem) * items.size(),
                          hipMemcpyHostToDevice, g.stream),
           st);
   g.nItems = (int)items.size();
   g.itemsTX = TX;
+  g.itemsOrder = g.itemOrder;
   return 0;
 }
 
@@ -1179,7 +1232,7 @@ void launch_march_multi_k(const MarchLaunch& M) {
       per = 1;
     resident = per;
   }
-  long G = (long)g.numCU * (g.persist > 0 ? g.persist : resident);
+  long G = (long)g.numCU * resident * (g.persist > 0 ? g.persist : 1);
   if (G > g.nItems) G = g.nItems;
   if (G < 1) G = 1;
   hipLaunchKernelGGL((k_march_multi<NDIM, TX, TY, NT, RS, R>), dim3((unsigned)G), dim3(NT), 0,
@@ -1227,7 +1280,7 @@ int launch_flux_stages(int nTiles, double dt, int TX, int TY) {
     const long faces = g.hfaceStart[nTiles];
     FaceLaunch F{g.dtiles, nTiles, g.fstates, P.learned ? g.seedY : nullptr, g.status, g.seedGate,
                  g.multi ? g.dcls : nullptr, g.multi ? g.dtcls : nullptr,
-                 g.multi ? g.dfaceStart : nullptr};
+                 g.multi ? g.dfaceStart : nullptr, g.multi ? g.dwFace : nullptr};
     g.lastFaces = faces;
     g.lastTiles = nTiles;
     g.lastMulti = g.multi;
@@ -1260,7 +1313,7 @@ int launch_flux_stages(int nTiles, double dt, int TX, int TY) {
 void launch_update(int nTiles, double dt, const unsigned char* dIn) {
   Launch L{g.dtiles, nTiles, nullptr, g.status, dt, g.dtDev};
   if (g.multi) {
-    const ClassMap C{g.dcls, g.dtcls, g.dcellStart};
+    const ClassMap C{g.dcls, g.dtcls, g.dcellStart, g.dwCell};
     hipLaunchKernelGGL(k_update_multi, dim3(grid_for(g.hcellStart[nTiles])), dim3(kThreads), 0,
                        g.stream, L, C, dIn);
   } else {
@@ -1523,11 +1576,105 @@ void launch_fill(double* U, long nBlocks, long blockStride) {
 }  // namespace
 
 // ---------------------------------------------------------------------
+// autotuner: launch shapes of the flux stages timed on the caller's batch
+// ---------------------------------------------------------------------
+int tune_flux_stages(int nTiles, double dt, int reps, fx_tune_result* out, int cap, int* n,
+                     int* chosenTX, fx_status* st) {
+  struct Cand {
+    int tx, persist, order;
+  };
+  const DevParams& P = g.cur;
+  bool can16 = true;
+  if (g.multi) {
+    for (const DevParams& C : g.hcls)
+      if (C.nxb % 16 || C.nyb % 8) can16 = false;
+  } else {
+    can16 = P.nxb % 16 == 0 && P.nyb % 8 == 0;
+  }
+  std::vector<Cand> cands;
+  for (int tx : {16, 8}) {
+    if (tx == 16 && !can16) continue;
+    if (g.multi) {
+      for (int persist : {1, 2, 4})
+        for (int order : {0, 1}) cands.push_back(Cand{tx, persist, order});
+    } else {
+      cands.push_back(Cand{tx, 0, 0});
+    }
+  }
+  if (ensure_step_scratch(nTiles, st)) return 4;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCALL(hipEventCreate(&e0), st);
+  HIPCALL(hipEventCreate(&e1), st);
+  const int savedTiming = g.timing;
+  g.timing = 0;
+  int best = -1, rc = 0;
+  float bestMs = 0.f;
+  std::vector<float> ms(cands.size(), -1.f);
+  for (size_t c = 0; c < cands.size() && !rc; ++c) {
+    g.tuneTX = cands[c].tx;
+    g.persist = cands[c].persist;
+    g.itemOrder = cands[c].order;
+    int TX = 8, TY = 8, NT = 0;
+    const bool ok = g.multi ? multi_march_shape(&TX) : march_shape(P, &TX, &TY, &NT);
+    if (!ok) continue;
+    if (g.multi && build_march_items(nTiles, TX, st)) {
+      rc = 4;
+      break;
+    }
+    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
+    if (launch_flux_stages(nTiles, dt, TX, TY)) rc = 4;  // warm-up
+    for (int r = 0; r < reps && !rc; ++r) {
+      float t = 0.f;
+      if (hipEventRecord(e0, g.stream) != hipSuccess || launch_flux_stages(nTiles, dt, TX, TY) ||
+          hipEventRecord(e1, g.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+        rc = 4;
+        break;
+      }
+      if (ms[c] < 0.f || t < ms[c]) ms[c] = t;
+    }
+    if (!rc && ms[c] >= 0.f && (best < 0 || ms[c] < bestMs)) {
+      best = (int)c;
+      bestMs = ms[c];
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  g.timing = savedTiming;
+  if (rc || best < 0) {
+    g.tuneTX = g.persist = g.itemOrder = 0;
+    if (st) st->code = rc ? rc : 3;
+    return rc ? rc : 3;
+  }
+  g.tuneTX = cands[best].tx;
+  g.persist = cands[best].persist;
+  g.itemOrder = cands[best].order;
+  int TX = 8;
+  if (g.multi) {
+    multi_march_shape(&TX);
+    if (build_march_items(nTiles, TX, st)) return 4;
+  } else {
+    TX = cands[best].tx;
+  }
+  if (chosenTX) *chosenTX = TX;
+  int k = 0;
+  for (size_t c = 0; c < cands.size(); ++c) {
+    if (ms[c] < 0.f) continue;
+    if (out && k < cap)
+      out[k] = fx_tune_result{cands[c].tx, cands[c].persist, cands[c].order, (int)c == best, ms[c]};
+    ++k;
+  }
+  if (n) *n = k;
+  return finish(st);
+}
+
+// ---------------------------------------------------------------------
 // multi-level grids (fx_amr_*)
 // ---------------------------------------------------------------------
 void amr_free() {
   Lib::Amr& A = g.amr;
-  void* bufs[] = {A.dblocks, A.dlevels, A.dboxStart, A.dcellStart, A.flux, A.pavg};
+  void* bufs[] = {A.dblocks, A.dlevels, A.dboxStart, A.dcellStart, A.dwBox, A.dwCell, A.flux,
+                  A.pavg, A.dchunks};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (int* q : A.dindex) if (q) (void)hipFree(q);
@@ -1557,6 +1704,7 @@ AmrLaunch amr_launch() {
   L.list = nullptr;
   L.nList = A.nBlocks;
   L.start = A.dboxStart;
+  L.wtab = A.dwBox;
   for (int a = 0; a < 3; ++a) L.periodic[a] = A.periodic[a];
   L.perMass = A.perMass;
   L.status = g.status;
@@ -1576,7 +1724,8 @@ int amr_ensure_uploaded(fx_status* st) {
 
 void amr_launch_fill() {
   const AmrLaunch L = amr_launch();
-  hipLaunchKernelGGL(k_amr_fill, dim3(grid_for(g.amr.boxCells)), dim3(kThreads), 0, g.stream, L);
+  hipLaunchKernelGGL(k_amr_fill, dim3((unsigned)g.amr.nChunks), dim3(kThreads), 0, g.stream, L,
+                     g.amr.dchunks);
 }
 
 // fine-to-coarse flux correction, finest level pair first
@@ -1604,6 +1753,7 @@ void amr_launch_restrict() {
     R.list = A.dlist[L];
     R.nList = A.nlist[L];
     R.start = A.drstart[L];
+    R.wtab = nullptr;
     long cells = 0;
     for (int b = 0; b < A.nBlocks; ++b)
       if (A.hblocks[b].level == L) {
@@ -1625,6 +1775,7 @@ int amr_launch_dt(double cfl, fx_status* st) {
   }
   AmrLaunch L = amr_launch();
   L.start = g.amr.dcellStart;
+  L.wtab = g.amr.dwCell;
   hipLaunchKernelGGL(k_amr_dt, dim3(grid), dim3(kThreads), 0, g.stream, L, g.partial);
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kThreads), 0, g.stream, g.partial, grid, cfl,
                      g.dtBuf, g.dtBuf + 1);
@@ -1668,7 +1819,7 @@ int fx_hydro_init(const fx_params* p) {
 }
 
 void fx_hydro_finalize(void) {
-  void* bufs[] = {g.states, g.flux, g.pavg, g.dtiles, g.dinplace, g.status, g.partial,
+  void* bufs[] = {g.dwCell, g.dwFace, g.states, g.flux, g.pavg, g.dtiles, g.dinplace, g.status, g.partial,
                   g.dregTiles, g.dfaces, g.mlpPacked, g.seedPacked, g.seedY, g.fstates,
                   g.dnbr, g.dtBuf, g.dcls, g.dtcls, g.dcellStart, g.dfaceStart,
                   g.ditems, g.dregCls};
@@ -1706,7 +1857,7 @@ int fx_hydro_update_solution(const fx_tile* tiles, int nTiles, double dt, double
   Launch L{g.dtiles, nTiles, nullptr, g.status, dt, g.dtDev};
   hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
   if (g.multi) {
-    const ClassMap C{g.dcls, g.dtcls, g.dcellStart};
+    const ClassMap C{g.dcls, g.dtcls, g.dcellStart, g.dwCell};
     hipLaunchKernelGGL(k_update_multi, dim3(grid_for(g.hcellStart[nTiles])), dim3(kThreads), 0,
                        g.stream, L, C, (const unsigned char*)nullptr);
   } else {
@@ -1917,7 +2068,7 @@ int fx_hydro_face_states(double* S, long cap, long* n) {
   *n = g.lastFaces;
   if (!g.init || g.lastFaces <= 0 || !S || cap < g.lastFaces) return 3;
   FaceLaunch F{g.dtiles, g.lastTiles, g.fstates, nullptr, g.status, g.seedGate,
-               g.dcls, g.dtcls, g.dfaceStart};
+               g.dcls, g.dtcls, g.dfaceStart, g.dwFace};
   if (g.lastMulti)
     hipLaunchKernelGGL(k_collect_faces<true>, dim3(grid_for(g.lastFaces)), dim3(kThreads), 0,
                        g.stream, g.cur, F, g.lastFaces, S);
@@ -2107,6 +2258,15 @@ int fx_amr_setup(const fx_tile* tiles, const fx_amr_block* blocks, int nBlocks, 
   A.cells = g.hcellStart[nBlocks];
   int rc = upload_vec(&A.dboxStart, box, st);
   if (!rc) rc = upload_vec(&A.dcellStart, g.hcellStart, st);
+  if (!rc) rc = upload_vec(&A.dwBox, wave_table(box), st);
+  {  // guard-fill chunks: kThreads consecutive box cells of one block each
+    std::vector<int2> ch;
+    for (int b = 0; b < nBlocks; ++b)
+      for (long c = 0; c < box[b + 1] - box[b]; c += kThreads) ch.push_back(make_int2(b, (int)c));
+    A.nChunks = (long)ch.size();
+    if (!rc) rc = upload_vec(&A.dchunks, ch, st);
+  }
+  if (!rc) rc = upload_vec(&A.dwCell, wave_table(g.hcellStart), st);
   for (int L = 0; !rc && L < nLevels; ++L) {
     int* d = nullptr;
     rc = upload_vec(&d, index[L], st);
@@ -2225,6 +2385,43 @@ int fx_amr_setup(const fx_tile* tiles, const fx_amr_block* blocks, int nBlocks, 
   if (!g.dtBuf) HIPCALL(hipMalloc((void**)&g.dtBuf, 2 * sizeof(double)), st);
   A.ready = true;
   return 0;
+}
+
+int fx_hydro_autotune(const fx_tile* tiles, int nTiles, double dt, int reps, fx_tune_result* out,
+                      int cap, int* n) {
+  fx_status st{};
+  if (n) *n = 0;
+  if (!g.init || !tiles || nTiles <= 0 || !(dt > 0.) || reps < 1) return 3;
+  if (g.cur.learned && !g.seedReady) return 3;
+  const int rc = upload_tiles(tiles, nTiles, true, &st);
+  if (rc) return rc;
+  int TX = 8, TY = 8, NT = 0;
+  if (g.multi ? !multi_march_shape(&TX) : (g.legacy || !march_shape(g.cur, &TX, &TY, &NT)))
+    return 3;  // the tuned stages are the column march's
+  if (g.cur.learned && !g.seedReady) return 3;
+  return tune_flux_stages(nTiles, dt, reps, out, cap, n, nullptr, &st);
+}
+
+int fx_amr_autotune(double dt, int reps, fx_tune_result* out, int cap, int* n) {
+  fx_status st{};
+  if (n) *n = 0;
+  if (!g.amr.ready || !(dt > 0.) || reps < 1) return 3;
+  if (g.cur.learned && !g.seedReady) return 3;
+  if (amr_ensure_uploaded(&st)) return 4;
+  int TX = g.amr.TX;
+  const int rc = tune_flux_stages(g.amr.nBlocks, dt, reps, out, cap, n, &TX, &st);
+  if (rc) return rc;
+  g.amr.TX = TX;
+  return 0;
+}
+
+void fx_hydro_autotune_reset(void) {
+  g.tuneTX = g.persist = g.itemOrder = 0;
+  g.nItems = 0;
+  if (g.amr.ready) {  // back to the built-in tile of the grid
+    int TX = 8, TY = 8, NT = 0;
+    if (g.multi ? multi_march_shape(&TX) : march_shape(g.cur, &TX, &TY, &NT)) g.amr.TX = TX;
+  }
 }
 
 int fx_amr_fill_guards(fx_status* st) {

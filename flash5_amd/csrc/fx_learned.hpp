@@ -215,6 +215,7 @@ struct FaceLaunch {
   const DevParams* cls;
   const unsigned char* tcls;
   const long* faceStart;
+  const int* wtab;          // tile of every 64th face (find_tile_w)
 };
 
 struct FaceRef {
@@ -236,7 +237,7 @@ template <bool MULTI>
 __device__ __forceinline__ const DevParams& face_tile(const DevParams& P0, const FaceLaunch& F,
                                                       long gid, int& t, long& r) {
   if constexpr (MULTI) {
-    t = find_tile(F.faceStart, F.nTiles, gid);
+    t = find_tile_w(F.faceStart, F.wtab, F.nTiles, gid);
     r = gid - F.faceStart[t];
     return F.cls[F.tcls[t]];
   } else {

@@ -266,11 +266,236 @@ def kh_precision_study(ncells: int = 2048, nb: int = 16, nsteps: int = 200):
             "mixed_zone_updates_per_s": zones / runs["mixed"][1]}
 
 
+# ---- three levels of uneven block sizes: the four-quadrant Riemann problem ----
+def quadrant_states(seed: int | None = None):
+    """Seeded quadrant states (dens, velx, vely, pres) for quadrants 0: x<1/2,
+    y<1/2; 1: x>1/2, y<1/2; 2: x<1/2, y>1/2; 3: x>1/2, y>1/2, drawn with dens
+    in [0.1, 2], pres in [0.03, 1.5], velocities in [-1.2, 1.2]; velz = 0."""
+    from .workloads import SEED_BASE
+    rng = np.random.default_rng(SEED_BASE + 5 if seed is None else seed)
+    dens = rng.uniform(0.1, 2.0, 4)
+    pres = rng.uniform(0.03, 1.5, 4)
+    vel = rng.uniform(-1.2, 1.2, (4, 2))
+    return dens, vel, pres
+
+
+class LevelStorage:
+    """Device storage of a LevelGrid: one contiguous float64 tensor per level
+    (blocks, nunk, nz, ny, nx); the tiles are views of its blocks."""
+
+    def __init__(self, grid, nunk: int = 10, device="cuda"):
+        import torch
+        self.grid = grid
+        self.index = []                  # per block: (level, slot)
+        self.levels = []
+        count = [0] * len(grid.nb)
+        for L, _ in grid.blocks:
+            self.index.append((L, count[L]))
+            count[L] += 1
+        for L in range(len(grid.nb)):
+            self.levels.append(torch.zeros((count[L],) + grid.block_shape(L, nunk),
+                                           dtype=torch.float64, device=device))
+
+    def block(self, b):
+        L, s = self.index[b]
+        return self.levels[L][s]
+
+    def clone(self):
+        other = LevelStorage.__new__(LevelStorage)
+        other.grid, other.index = self.grid, self.index
+        other.levels = [t.clone() for t in self.levels]
+        return other
+
+    def leaf_masks(self):
+        """Per level: which blocks are leaves (refined blocks are covered whole)."""
+        import torch
+        g = self.grid
+        refined = [set() for _ in g.nb]
+        for L, pos in g.blocks:
+            if L == 0:
+                continue
+            r = [g.ext[L][a] // g.ext[L - 1][a] for a in range(3)]
+            refined[L - 1].add(tuple(pos[a] // r[a] for a in range(3)))
+        masks = [torch.ones(t.shape[0], dtype=torch.bool, device=t.device) for t in self.levels]
+        for b, (L, pos) in enumerate(g.blocks):
+            if tuple(pos) in refined[L]:
+                masks[L][self.index[b][1]] = False
+        return masks
+
+
+def quadrant_init(grid, gamma: float = 1.4, states=None, nunk: int = 10) -> LevelStorage:
+    """The four-quadrant Riemann problem lifted to 3-D (uniform in z) on
+    every level, guard cells included, with the ideal-gas EOS."""
+    import torch
+    dens, vel, pres = quadrant_states() if states is None else states
+    S = LevelStorage(grid, nunk)
+    for L, U in enumerate(S.levels):
+        blocks = [b for b, (Lb, _) in enumerate(grid.blocks) if Lb == L]
+        pos = torch.tensor([grid.blocks[b][1] for b in blocks], dtype=torch.float64,
+                           device=U.device)
+        d = grid.dels(L)
+        nb = grid.nb[L]
+        g = grid.ng
+        ix = torch.arange(-g, nb[0] + g, dtype=torch.float64, device=U.device)
+        iy = torch.arange(-g, nb[1] + g, dtype=torch.float64, device=U.device)
+        x = (pos[:, 0, None] * nb[0] + ix[None, :] + 0.5) * d[0]      # (blocks, nx)
+        y = (pos[:, 1, None] * nb[1] + iy[None, :] + 0.5) * d[1]      # (blocks, ny)
+        q = (x[:, None, :] > 0.5).long() + 2 * (y[:, :, None] > 0.5).long()   # (blocks, ny, nx)
+        q = q[:, None, :, :].expand(-1, U.shape[2], -1, -1)
+        tab = lambda v: torch.tensor(v, dtype=torch.float64, device=U.device)[q]
+        U[:, DENS] = tab(dens)
+        U[:, VELX] = tab(vel[:, 0])
+        U[:, VELY] = tab(vel[:, 1])
+        U[:, VELZ] = 0.0
+        U[:, PRES] = tab(pres)
+        eos_fill(U, gamma)
+    return S
+
+
+class AmrRun:
+    """A LevelGrid stepped on the device (fx_amr_*): one block class per
+    level shape, the grid handed to the library once."""
+
+    def __init__(self, grid, params: HydroParams, storage: LevelStorage, seed_params=None):
+        from .hydro import Tile
+        self.grid, self.S = grid, storage
+        shapes = []
+        for n in grid.nb:
+            if n not in shapes:
+                shapes.append(n)
+        self.hydro = Hydro(params)
+        if seed_params is not None:
+            self.hydro.set_seed_params(seed_params)
+        self.hydro.set_block_classes(shapes)
+        tiles = [Tile(U=storage.block(b), deltas=grid.dels(L), level=L + 1,
+                      block_class=shapes.index(grid.nb[L]))
+                 for b, (L, _) in enumerate(grid.blocks)]
+        self.hydro.amr_setup(tiles, [L + 1 for L, _ in grid.blocks], [p for _, p in grid.blocks],
+                             grid.ext, grid.periodic)
+        self.t = 0.0
+
+    def step(self, dt: float) -> FxStatus:
+        st = self.hydro.amr_step(dt)
+        self.t += dt
+        return st
+
+    def dt(self, cfl: float = 0.8) -> float:
+        return self.hydro.amr_compute_dt(cfl)
+
+    def run_graph(self, cfl: float, steps_per_graph: int, replays: int) -> float:
+        self.t += self.hydro.amr_run_graph(cfl, steps_per_graph, replays)
+        return self.t
+
+
+def leaf_l1(a: LevelStorage, b: LevelStorage, names) -> dict:
+    """sfocu L1 (sum |a - b| / sum |b|) per variable over the leaf interior cells."""
+    out = {}
+    masks = b.leaf_masks()
+    g = a.grid
+    for name, v in names.items():
+        num = den = 0.0
+        for L, (Ua, Ub) in enumerate(zip(a.levels, b.levels)):
+            nb = g.nb[L]
+            sl = (masks[L], v, slice(4, 4 + nb[2]), slice(4, 4 + nb[1]), slice(4, 4 + nb[0]))
+            num += float((Ua[sl] - Ub[sl]).abs().sum())
+            den += float(Ub[sl].abs().sum())
+        out[name] = num / max(den, 1e-99)
+    return out
+
+
+def amr_study(steps: int = 100, steps_per_graph: int = 10, l1_steps: int = 20, grid=None,
+              only: str | None = None):
+    """Config-5 run: three levels (512 x 32^3, 2048 x 16^3, 8192 x 8^3 by
+    default) with the four-quadrant data, 100 graph-captured steps in each of
+    plain FP64 HLLC, FP64 exact, and mixed precision with the exact solver
+    seeded by the learned network (trained on this grid's own face states);
+    then l1_steps eager steps of one fixed dt in FP64 exact and in the mixed
+    learned form, compared per variable over the leaf cells."""
+    import dataclasses
+    import torch
+    from . import learned as Lr
+    from .amr import quadrant_grid
+    from .hydro import PREC_MIXED
+    grid = quadrant_grid() if grid is None else grid
+    gamma = 1.4
+    S0 = quadrant_init(grid, gamma)
+    zones = grid.zones()
+    base = HydroParams(ndim=3, nxb=grid.nb[0][0], nyb=grid.nb[0][1], nzb=grid.nb[0][2],
+                       gamma=gamma)
+    out = {"grid": {"levels": [sum(1 for L, _ in grid.blocks if L == lev) for lev in range(3)],
+                    "block_cells": [list(n) for n in grid.nb], "zones": zones},
+           "steps": steps, "steps_per_graph": steps_per_graph}
+    # the seed network: one FP64 exact step of the grid, its face states solved
+    # and learned (the exact solver's own states)
+    t0 = time.perf_counter()
+    exact64 = dataclasses.replace(base, RiemannSolver="exact")
+    run = AmrRun(grid, exact64, S0.clone())
+    run.step(run.dt(0.8))
+    faces = run.hydro.face_states()
+    del run
+    gen = torch.Generator(device=faces.device).manual_seed(7)
+    pick = torch.randperm(faces.shape[1], generator=gen, device=faces.device)[:1 << 22]
+    Sf = faces[:, pick].contiguous()
+    del faces
+    ps, _, _ = Lr.solve(Sf, gamma)
+    net = Lr.SeedNet(seed=3)
+    net.train(net.features(Sf, gamma), Lr.targets(Sf, ps, gamma), steps=1500, batch=8192,
+              lr=3e-3, log_every=500)
+    strength = Lr.strength(Sf, gamma)
+    out["seed"] = {"train_states": int(Sf.shape[1]), "train_s": time.perf_counter() - t0,
+                   "faces_at_or_above_gate": float((strength >= 0.2).double().mean())}
+    del Sf, ps
+    modes = {"fp64_hllc": base, "fp64_exact": exact64,
+             "mixed_exact_learned": dataclasses.replace(base, RiemannSolver="exact",
+                                                        precision=PREC_MIXED, learnedSeed=True)}
+    for name, hp in modes.items():
+        if only is not None and name != only:
+            continue
+        run = AmrRun(grid, hp, S0.clone(), seed_params=net.params if hp.learnedSeed else None)
+        tune = run.hydro.amr_autotune(run.dt(0.8), reps=2)   # launch shapes for this grid
+        run.run_graph(0.8, 1, 1)                       # warm-up step (captures once)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run.run_graph(0.8, steps_per_graph, steps // steps_per_graph)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        st = run.hydro.last_status
+        out[name] = {"zone_updates_per_s": zones * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                     "t_end": run.t, "code": st.code,
+                     "last_step_newton_iters": int(st.newton_iters),
+                     "last_step_newton_fallbacks": int(st.newton_fallbacks),
+                     "autotune": tune}
+        run.hydro.autotune_reset()
+        del run
+        torch.cuda.empty_cache()
+    if l1_steps <= 0:
+        return out
+    # precision: the same fixed dt in FP64 exact and in the mixed learned form
+    run = AmrRun(grid, exact64, S0.clone())
+    dt = run.dt(0.8)
+    for _ in range(l1_steps):
+        run.step(dt)
+    ref = run.S
+    del run
+    run = AmrRun(grid, modes["mixed_exact_learned"], S0.clone(), seed_params=net.params)
+    for _ in range(l1_steps):
+        run.step(dt)
+    names = {"dens": DENS, "velx": VELX, "vely": VELY, "velz": VELZ, "pres": PRES, "ener": ENER,
+             "eint": EINT}
+    out["l1_mixed_learned_vs_fp64_exact"] = {"steps": l1_steps, "dt": dt,
+                                            "l1": leaf_l1(run.S, ref, names)}
+    return out
+
+
 def main() -> None:
     import argparse
     import json
-    ap = argparse.ArgumentParser(description="multi-step runs: Sod (config 1), KH precision study")
-    ap.add_argument("case", choices=["sod", "kh"])
+    ap = argparse.ArgumentParser(description="multi-step runs: Sod (config 1), KH precision "
+                                             "study, three-level quadrant problem")
+    ap.add_argument("case", choices=["sod", "kh", "amr"])
+    ap.add_argument("--mode", choices=["fp64_hllc", "fp64_exact", "mixed_exact_learned"],
+                    default=None, help="amr: run only this mode")
+    ap.add_argument("--l1-steps", type=int, default=20, help="amr: steps of the precision study")
     ap.add_argument("--ncells", type=int, default=2048)
     ap.add_argument("--steps", type=int, default=200)
     a = ap.parse_args()
@@ -296,6 +521,9 @@ def main() -> None:
                           "t": grid.t, "zone_updates_per_s": st.zone_updates_per_s,
                           "fallbacks": st.fallbacks,
                           "first_60_steps_zone_updates_per_s": rates}))
+    elif a.case == "amr":
+        print(json.dumps(amr_study(a.steps if a.steps != 200 else 100, l1_steps=a.l1_steps,
+                                   only=a.mode)))
     else:
         print(json.dumps(kh_precision_study(a.ncells, 16, a.steps)))
 

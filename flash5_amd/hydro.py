@@ -65,6 +65,11 @@ class FxAmrBlock(C.Structure):
     _fields_ = [("level", C.c_int), ("pos", C.c_int * 3)]
 
 
+class FxTuneResult(C.Structure):
+    _fields_ = [("tile_x", C.c_int), ("persist", C.c_int), ("order", C.c_int),
+                ("chosen", C.c_int), ("ms", C.c_float)]
+
+
 class FxStatus(C.Structure):
     _fields_ = [("code", C.c_int), ("first_tile", C.c_int), ("first_i", C.c_int),
                 ("first_j", C.c_int), ("first_k", C.c_int), ("first_dir", C.c_int),
@@ -115,6 +120,10 @@ EXPORTS = {
     "fx_hydro_set_timing": ([C.c_int], None),
     "fx_hydro_set_kernel_path": ([C.c_int], None),
     "fx_hydro_set_block_classes": ([C.POINTER(C.c_int), C.c_int], C.c_int),
+    "fx_hydro_autotune": ([C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_int,
+                           C.POINTER(C.c_int)], C.c_int),
+    "fx_amr_autotune": ([C.c_double, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
+    "fx_hydro_autotune_reset": ([], None),
     "fx_amr_setup": ([C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
                       C.POINTER(C.c_int), C.POINTER(FxStatus)], C.c_int),
     "fx_amr_fill_guards": ([C.POINTER(FxStatus)], C.c_int),
@@ -301,6 +310,41 @@ class Hydro:
         """HydroParams of block class c (the params with that class's shape)."""
         nx, ny, nz = self._classes[c]
         return dataclasses.replace(self.params, nxb=nx, nyb=ny, nzb=nz)
+
+    # ---- autotuner ----
+    @staticmethod
+    def _tune_list(buf, n):
+        return [{"tile_x": buf[k].tile_x, "persist": buf[k].persist, "order": buf[k].order,
+                 "chosen": bool(buf[k].chosen), "ms": float(buf[k].ms)} for k in range(n)]
+
+    def autotune(self, tiles: Sequence[Tile], dt: float, reps: int = 3):
+        """Times the flux stages' candidate launch shapes on these tiles and
+        keeps the fastest for later calls; returns every candidate's time."""
+        self._reinit()
+        arr = self._tiles(tiles)
+        buf = (FxTuneResult * 32)()
+        n = C.c_int(0)
+        rc = self.lib.fx_hydro_autotune(C.cast(arr, C.c_void_p), len(tiles), dt, reps,
+                                        C.cast(buf, C.c_void_p), 32, C.byref(n))
+        if rc:
+            raise FxError("fx_hydro_autotune", rc=rc)
+        return self._tune_list(buf, n.value)
+
+    def autotune_batch(self, U, dt: float, deltas: Sequence[float], reps: int = 3):
+        """autotune on a batch tensor (nBlocks, nunk, nz, ny, nx) of blocks."""
+        tiles = [Tile(U=U[b], deltas=deltas) for b in range(U.shape[0])]
+        return self.autotune(tiles, dt, reps)
+
+    def amr_autotune(self, dt: float, reps: int = 3):
+        buf = (FxTuneResult * 32)()
+        n = C.c_int(0)
+        rc = self.lib.fx_amr_autotune(dt, reps, C.cast(buf, C.c_void_p), 32, C.byref(n))
+        if rc:
+            raise FxError("fx_amr_autotune", rc=rc)
+        return self._tune_list(buf, n.value)
+
+    def autotune_reset(self) -> None:
+        self.lib.fx_hydro_autotune_reset()
 
     # ---- multi-level grids held on the device (fx_amr_*) ----
     def amr_setup(self, tiles: Sequence[Tile], levels: Sequence[int], positions,

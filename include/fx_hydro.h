@@ -255,6 +255,32 @@ void fx_hydro_set_kernel_path(int path);
 #define FX_MAX_CLASSES 8
 int fx_hydro_set_block_classes(const int* nb, int nClasses);
 
+/* ---- autotuner (no counterpart in the reference) ----
+ * Times every candidate launch shape of the flux stages (column march, and
+ * the exact solver's face stage) on the caller's own batch and keeps the
+ * fastest for later calls.  Candidates: the column-tile width (16 or 8
+ * columns across x, where every tile shape divides it), and for tiles of
+ * several classes the persistent workgroups per CU (1, 2 or 4 times what
+ * stays resident) and the order of the column work items (longest columns
+ * first, or tile by tile).  Each candidate runs reps times after one
+ * warm-up, timed with HIP events; the minimum counts.  The flux stages read
+ * U and write only the tiles' flux arrays and library scratch, so U is left
+ * as it was.  out[0 .. *n-1] receives every candidate's time (up to cap);
+ * the chosen one has chosen = 1. */
+typedef struct {
+  int tile_x;               /* columns across x per workgroup (8 or 16) */
+  int persist;              /* workgroups per CU multiple (multi-class only; 0 = n/a) */
+  int order;                /* multi-class items: 0 longest first, 1 tile by tile */
+  int chosen;
+  float ms;                 /* fastest of reps */
+} fx_tune_result;
+int fx_hydro_autotune(const fx_tile* tiles, int nTiles, double dt, int reps, fx_tune_result* out,
+                      int cap, int* n);
+/* The same on the multi-level grid of fx_amr_setup. */
+int fx_amr_autotune(double dt, int reps, fx_tune_result* out, int cap, int* n);
+/* Forget the tuned choice (back to the built-in one). */
+void fx_hydro_autotune_reset(void);
+
 /* ---- multi-level block grids held on the device ----
  * The Grid operations a multi-level hydro step needs between the Hydro
  * calls, for grids whose levels differ in block size: guard fill

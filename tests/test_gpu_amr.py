@@ -190,3 +190,17 @@ def test_amr_mixed_precision_learned_seed_runs_under_graph():
     den = sum(np.abs(r[O.DENS]).sum() for r in ref)
     print(f"L1 rel dens mixed+learned vs FP64: {num / den:.2e}")
     assert 0 < num / den <= 1e-6, num / den
+
+
+def test_quadrant_study_small_grid():
+    """The three-level quadrant driver end to end on a reduced grid (16^3,
+    8^3, 8^3 blocks): every mode runs graph-captured, and the mixed learned
+    form stays within 1e-6 (L1 density) of FP64 exact over the leaves."""
+    from flash5_amd import drivers as D
+    from flash5_amd.amr import quadrant_grid
+    grid = quadrant_grid(nb=((16, 16, 16), (8, 8, 8), (8, 8, 8)), base=(2, 2, 2))
+    out = D.amr_study(steps=4, steps_per_graph=2, l1_steps=3, grid=grid)
+    print(out)
+    for name in ("fp64_hllc", "fp64_exact", "mixed_exact_learned"):
+        assert out[name]["code"] == 0 and out[name]["zone_updates_per_s"] > 0
+    assert 0 < out["l1_mixed_learned_vs_fp64_exact"]["l1"]["dens"] <= 1e-6

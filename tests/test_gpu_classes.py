@@ -114,3 +114,30 @@ def test_mixed_classes_equal_single_class_runs(learned):
         assert np.array_equal(together[i], alone[i]), f"tile {i} differs"
     if learned:
         assert st.newton_iters > 0
+
+
+def test_autotune_picks_a_candidate_and_keeps_results():
+    """The autotuner times every candidate on the batch, marks one chosen,
+    and the step after tuning gives the same bits as before (launch shapes
+    change the schedule, not the arithmetic)."""
+    ndim, gamma = 3, 1.4
+    batch = _batch(ndim, gamma, W.SEED_BASE + 41)
+    dt = _dt(batch, ndim)
+    hp = HydroParams(ndim=ndim, nxb=16, nyb=16, nzb=16, gamma=gamma)
+    before, _ = _run(hp, batch, dt)
+    hy = Hydro(hp)
+    hy.set_block_classes([(n, n, n) for n in SIZES])
+    tiles = [Tile(U=torch.from_numpy(U.copy()).cuda(), deltas=(dx,) * 3, block_class=c)
+             for c, U, dx in batch]
+    res = hy.autotune(tiles, dt, reps=2)
+    print(res)
+    assert len(res) >= 6 and sum(r["chosen"] for r in res) == 1
+    assert all(r["ms"] > 0 for r in res)
+    for t, (c, U, dx) in zip(tiles, batch):      # tuning left U alone
+        assert np.array_equal(t.U.cpu().numpy(), U)
+    try:
+        after, _ = _run(hp, batch, dt)
+    finally:
+        hy.autotune_reset()
+    for i in before:
+        assert np.array_equal(before[i], after[i]), f"tile {i}"
