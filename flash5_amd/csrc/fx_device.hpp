@@ -673,6 +673,31 @@ __device__ __forceinline__ int exact_face_flux(const DevParams& P, int d, const 
   return 0;
 }
 
+// Gate of the learned seed: an interface is strong when its largest
+// feature magnitude, max(|ln(rl/rr)|, |ln(pl/pr)|, |ur - ul| / c0) / 2 with
+// c0^2 = gamma sqrt(pl pr) / sqrt(rl rr), reaches the gate.  The same test
+// without logarithms or roots: with T = exp(2 gate) and K = 16 gate^4,
+//   rl >= T rr  or  rr >= T rl  or  pl >= T pr  or  pr >= T pl
+//   or  (ur - ul)^4 rl rr >= K gamma^2 pl pr.
+struct SeedGate {
+  double T, K;
+};
+
+__host__ __device__ inline SeedGate make_seed_gate(double gate) {
+  const double g2 = gate * gate;
+  return SeedGate{exp(2.0 * gate), 16.0 * g2 * g2};
+}
+
+__device__ __forceinline__ bool seed_gate_pass(const SeedGate& G, double g, double rl, double ul,
+                                               double pl, double rr, double ur, double pr) {
+  const double du = ur - ul, du2 = du * du;
+  return rl >= G.T * rr || rr >= G.T * rl || pl >= G.T * pr || pr >= G.T * pl ||
+         du2 * du2 * (rl * rr) >= G.K * (g * g) * (pl * pr);
+}
+
+// seed value of a face whose Newton iteration keeps the two-shock start
+constexpr float kNoSeed = 3.0e38f;
+
 // ---- gamma-law EOS, MODE_DENS_EI on one cell (Eos_getData.F90:189-215,
 // eos_idealGamma.F90:323-333, Eos_putData.F90:197-210) ----
 struct EosOut {

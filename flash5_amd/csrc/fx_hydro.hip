@@ -817,6 +817,9 @@ struct Lib {
   double seedGate = 0.2;               // fx_mlp_set_gate
   float* seedY = nullptr;              // network outputs per interface / face
   size_t seedYCap = 0;
+  unsigned* strongList = nullptr;      // faces at or above the gate, listed by the march
+  size_t strongListCap = 0;
+  unsigned* strongCount = nullptr;
   double* fstates = nullptr;           // face states for the separate Riemann stage
   size_t fstatesCap = 0;
   const double* dtDev = nullptr;       // set while capturing steps whose dt lives on the device
@@ -862,6 +865,7 @@ struct Lib {
   int itemsOrder = 0;                  // item order they were built in
   int persist = 0;                     // multi-class march: multiple of the resident workgroups (0 = 1)
   int tuneTX = 0;                      // autotuned column-tile width (0: built-in choice)
+  int tuneOcc = 0;                     // autotuned waves-per-SIMD hint of the FP32 3-D march (0: none)
   int itemOrder = 0;                   // multi-class items: 0 longest first, 1 tile by tile
   unsigned long uploadGen = 0;         // counts tile-list uploads
   // multi-level grid (fx_amr_*)
@@ -1205,6 +1209,15 @@ int build_march_items(int nTiles, int TX, fx_status* st) {
 
 template <int RS, typename R>
 void launch_march_r(const DevParams& P, const MarchLaunch& M, int TX, dim3 grid) {
+  if constexpr (sizeof(R) == 4) {  // FP32 3-D: the autotuner may ask for 3 waves per SIMD
+    if (P.ndim == 3 && g.tuneOcc == 3) {
+      if (TX == 16)
+        hipLaunchKernelGGL((k_march<3, 16, 8, 512, RS, R, 3>), grid, dim3(512), 0, g.stream, P, M);
+      else
+        hipLaunchKernelGGL((k_march<3, 8, 8, 256, RS, R, 3>), grid, dim3(256), 0, g.stream, P, M);
+      return;
+    }
+  }
   if (P.ndim == 3 && TX == 16) hipLaunchKernelGGL((k_march<3, 16, 8, 512, RS, R>), grid, dim3(512), 0, g.stream, P, M);
   else if (P.ndim == 3) hipLaunchKernelGGL((k_march<3, 8, 8, 256, RS, R>), grid, dim3(256), 0, g.stream, P, M);
   else if (TX == 16) hipLaunchKernelGGL((k_march<2, 16, 8, 256, RS, R>), grid, dim3(256), 0, g.stream, P, M);
@@ -1221,13 +1234,13 @@ void launch_march(const DevParams& P, const MarchLaunch& M, int TX, dim3 grid) {
 
 // Persistent multi-class march: as many workgroups as stay resident (or
 // g.persist per CU), each taking every G-th column tile.
-template <int NDIM, int TX, int TY, int NT, int RS, typename R>
+template <int NDIM, int TX, int TY, int NT, int RS, typename R, int OCC = 0>
 void launch_march_multi_k(const MarchLaunch& M) {
   static int resident = -1;
   if (resident < 0) {
     int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_march_multi<NDIM, TX, TY, NT, RS, R>,
-                                                     NT, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per, k_march_multi<NDIM, TX, TY, NT, RS, R, OCC>, NT, 0) != hipSuccess ||
         per < 1)
       per = 1;
     resident = per;
@@ -1235,12 +1248,19 @@ void launch_march_multi_k(const MarchLaunch& M) {
   long G = (long)g.numCU * resident * (g.persist > 0 ? g.persist : 1);
   if (G > g.nItems) G = g.nItems;
   if (G < 1) G = 1;
-  hipLaunchKernelGGL((k_march_multi<NDIM, TX, TY, NT, RS, R>), dim3((unsigned)G), dim3(NT), 0,
+  hipLaunchKernelGGL((k_march_multi<NDIM, TX, TY, NT, RS, R, OCC>), dim3((unsigned)G), dim3(NT), 0,
                      g.stream, g.dcls, M, g.ditems, g.nItems, g.dfaceStart);
 }
 
 template <int RS, typename R>
 void launch_march_multi_r(int ndim, int TX, const MarchLaunch& M) {
+  if constexpr (sizeof(R) == 4) {  // FP32 3-D: the autotuner may ask for 3 waves per SIMD
+    if (ndim == 3 && g.tuneOcc == 3) {
+      if (TX == 16) launch_march_multi_k<3, 16, 8, 512, RS, R, 3>(M);
+      else launch_march_multi_k<3, 8, 8, 256, RS, R, 3>(M);
+      return;
+    }
+  }
   if (ndim == 3 && TX == 16) launch_march_multi_k<3, 16, 8, 512, RS, R>(M);
   else if (ndim == 3) launch_march_multi_k<3, 8, 8, 256, RS, R>(M);
   else if (TX == 16) launch_march_multi_k<2, 16, 8, 256, RS, R>(M);
@@ -1269,16 +1289,25 @@ int record(int n) {
 int launch_flux_stages(int nTiles, double dt, int TX, int TY) {
   const DevParams& P = g.cur;
   MarchLaunch M{g.dtiles, nTiles, g.multi ? 0 : P.nxb / TX,
-                g.multi ? 0 : (P.ndim > 1 ? P.nyb : 1) / TY, g.status, dt, nullptr, g.dtDev};
+                g.multi ? 0 : (P.ndim > 1 ? P.nyb : 1) / TY, g.status, dt, nullptr, g.dtDev,
+                nullptr, nullptr, nullptr, make_seed_gate(g.seedGate), 0u};
   const dim3 grid((unsigned)(g.multi ? 1 : nTiles * M.tilesX * M.tilesY));
   if (P.riemann == FX_EXACT) {
-    // exact solver: the march stores face states, a separate stage solves
+    // exact solver: the march stores face states (and, with the learned
+    // seed, lists the strong faces), a separate stage solves
     M.fstates = g.fstates;
+    if (P.learned) {
+      M.seedY = g.seedY;
+      M.strongCount = g.strongCount;
+      M.strongList = g.strongList;
+      M.strongCap = (unsigned)g.hfaceStart[nTiles];
+      hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, g.stream, g.strongCount);
+    }
     if (g.multi) launch_march_multi<RS_STATES>(P.ndim, P.precision, TX, M);
     else launch_march<RS_STATES>(P, M, TX, grid);
     if (record(1)) return 4;
     const long faces = g.hfaceStart[nTiles];
-    FaceLaunch F{g.dtiles, nTiles, g.fstates, P.learned ? g.seedY : nullptr, g.status, g.seedGate,
+    FaceLaunch F{g.dtiles, nTiles, g.fstates, P.learned ? g.seedY : nullptr, g.status,
                  g.multi ? g.dcls : nullptr, g.multi ? g.dtcls : nullptr,
                  g.multi ? g.dfaceStart : nullptr, g.multi ? g.dwFace : nullptr};
     g.lastFaces = faces;
@@ -1287,13 +1316,13 @@ int launch_flux_stages(int nTiles, double dt, int TX, int TY) {
     const dim3 gf(grid_for(faces));
     if (g.multi && P.learned) {
       hipLaunchKernelGGL(k_face_seed<true>, wave_grid(faces), dim3(kLearnThreads), 0, g.stream, P,
-                         F, g.seedPacked, g.seedY);
+                         F, g.seedPacked, g.seedY, g.strongList, g.strongCount, faces);
       hipLaunchKernelGGL((k_face_solve<true, true>), gf, dim3(kThreads), 0, g.stream, P, F);
     } else if (g.multi) {
       hipLaunchKernelGGL((k_face_solve<false, true>), gf, dim3(kThreads), 0, g.stream, P, F);
     } else if (P.learned) {
       hipLaunchKernelGGL(k_face_seed<false>, wave_grid(faces), dim3(kLearnThreads), 0, g.stream,
-                         P, F, g.seedPacked, g.seedY);
+                         P, F, g.seedPacked, g.seedY, g.strongList, g.strongCount, faces);
       hipLaunchKernelGGL((k_face_solve<true, false>), gf, dim3(kThreads), 0, g.stream, P, F);
     } else {
       hipLaunchKernelGGL((k_face_solve<false, false>), gf, dim3(kThreads), 0, g.stream, P, F);
@@ -1329,7 +1358,13 @@ int ensure_step_scratch(int nTiles, fx_status* st) {
   const long faces = (long)nTiles <= (long)g.hfaceStart.size() - 1 ? g.hfaceStart[nTiles]
                                                                      : 0;
   if (grow(&g.fstates, &g.fstatesCap, (size_t)kFaceComps * faces, st)) return 4;
+  if (P.learned && faces > 0xffffffffL) {  // the strong-face list holds 32-bit face numbers
+    if (st) st->code = 3;
+    return 3;
+  }
   if (P.learned && grow(&g.seedY, &g.seedYCap, (size_t)faces, st)) return 4;
+  if (P.learned && grow(&g.strongList, &g.strongListCap, (size_t)faces, st)) return 4;
+  if (P.learned && !g.strongCount) HIPCALL(hipMalloc((void**)&g.strongCount, sizeof(unsigned)), st);
   return 0;
 }
 
@@ -1581,7 +1616,7 @@ void launch_fill(double* U, long nBlocks, long blockStride) {
 int tune_flux_stages(int nTiles, double dt, int reps, fx_tune_result* out, int cap, int* n,
                      int* chosenTX, fx_status* st) {
   struct Cand {
-    int tx, persist, order;
+    int tx, persist, order, occ;
   };
   const DevParams& P = g.cur;
   bool can16 = true;
@@ -1592,13 +1627,17 @@ int tune_flux_stages(int nTiles, double dt, int reps, fx_tune_result* out, int c
     can16 = P.nxb % 16 == 0 && P.nyb % 8 == 0;
   }
   std::vector<Cand> cands;
-  for (int tx : {16, 8}) {
-    if (tx == 16 && !can16) continue;
-    if (g.multi) {
-      for (int persist : {1, 2, 4})
-        for (int order : {0, 1}) cands.push_back(Cand{tx, persist, order});
-    } else {
-      cands.push_back(Cand{tx, 0, 0});
+  const bool occHint = P.precision == FX_PREC_MIXED && P.ndim == 3;
+  for (int occ : {0, 3}) {
+    if (occ && !occHint) continue;
+    for (int tx : {16, 8}) {
+      if (tx == 16 && !can16) continue;
+      if (g.multi) {
+        for (int persist : {1, 2, 4})
+          for (int order : {0, 1}) cands.push_back(Cand{tx, persist, order, occ});
+      } else {
+        cands.push_back(Cand{tx, 0, 0, occ});
+      }
     }
   }
   if (ensure_step_scratch(nTiles, st)) return 4;
@@ -1614,6 +1653,7 @@ int tune_flux_stages(int nTiles, double dt, int reps, fx_tune_result* out, int c
     g.tuneTX = cands[c].tx;
     g.persist = cands[c].persist;
     g.itemOrder = cands[c].order;
+    g.tuneOcc = cands[c].occ;
     int TX = 8, TY = 8, NT = 0;
     const bool ok = g.multi ? multi_march_shape(&TX) : march_shape(P, &TX, &TY, &NT);
     if (!ok) continue;
@@ -1642,13 +1682,14 @@ int tune_flux_stages(int nTiles, double dt, int reps, fx_tune_result* out, int c
   (void)hipEventDestroy(e1);
   g.timing = savedTiming;
   if (rc || best < 0) {
-    g.tuneTX = g.persist = g.itemOrder = 0;
+    g.tuneTX = g.persist = g.itemOrder = g.tuneOcc = 0;
     if (st) st->code = rc ? rc : 3;
     return rc ? rc : 3;
   }
   g.tuneTX = cands[best].tx;
   g.persist = cands[best].persist;
   g.itemOrder = cands[best].order;
+  g.tuneOcc = cands[best].occ;
   int TX = 8;
   if (g.multi) {
     multi_march_shape(&TX);
@@ -1661,7 +1702,8 @@ int tune_flux_stages(int nTiles, double dt, int reps, fx_tune_result* out, int c
   for (size_t c = 0; c < cands.size(); ++c) {
     if (ms[c] < 0.f) continue;
     if (out && k < cap)
-      out[k] = fx_tune_result{cands[c].tx, cands[c].persist, cands[c].order, (int)c == best, ms[c]};
+      out[k] = fx_tune_result{cands[c].tx, cands[c].persist, cands[c].order, cands[c].occ,
+                              (int)c == best, ms[c]};
     ++k;
   }
   if (n) *n = k;
@@ -1819,7 +1861,7 @@ int fx_hydro_init(const fx_params* p) {
 }
 
 void fx_hydro_finalize(void) {
-  void* bufs[] = {g.dwCell, g.dwFace, g.states, g.flux, g.pavg, g.dtiles, g.dinplace, g.status, g.partial,
+  void* bufs[] = {g.strongList, g.strongCount, g.dwCell, g.dwFace, g.states, g.flux, g.pavg, g.dtiles, g.dinplace, g.status, g.partial,
                   g.dregTiles, g.dfaces, g.mlpPacked, g.seedPacked, g.seedY, g.fstates,
                   g.dnbr, g.dtBuf, g.dcls, g.dtcls, g.dcellStart, g.dfaceStart,
                   g.ditems, g.dregCls};
@@ -1961,16 +2003,16 @@ int fx_riemann_batch(const double* states, long n, double gamma, const float* pa
     if (grow(&g.seedY, &g.seedYCap, (size_t)n, st)) return 4;
     if (record(0)) return 4;
     hipLaunchKernelGGL(k_mlp_seed_states, wave_grid(n), dim3(kLearnThreads), 0, g.stream, states,
-                       n, gamma, g.mlpPacked, g.seedY, g.seedGate);
+                       n, gamma, g.mlpPacked, g.seedY, make_seed_gate(g.seedGate));
     if (record(1)) return 4;
     hipLaunchKernelGGL(k_riemann_batch<true>, dim3(grid_for(n)), dim3(kThreads), 0, g.stream,
-                       states, n, gamma, g.seedY, tol, maxit, pstar, iters, g.status, g.seedGate);
+                       states, n, gamma, g.seedY, tol, maxit, pstar, iters, g.status, make_seed_gate(g.seedGate));
     if (record(2)) return 4;
   } else {
     if (record(0) || record(1)) return 4;
     hipLaunchKernelGGL(k_riemann_batch<false>, dim3(grid_for(n)), dim3(kThreads), 0, g.stream,
                        states, n, gamma, (const float*)nullptr, tol, maxit, pstar, iters, g.status,
-                       g.seedGate);
+                       make_seed_gate(g.seedGate));
     if (record(2)) return 4;
   }
   if (record(3)) return 4;
@@ -2067,7 +2109,7 @@ int fx_hydro_face_states(double* S, long cap, long* n) {
   if (!n) return 3;
   *n = g.lastFaces;
   if (!g.init || g.lastFaces <= 0 || !S || cap < g.lastFaces) return 3;
-  FaceLaunch F{g.dtiles, g.lastTiles, g.fstates, nullptr, g.status, g.seedGate,
+  FaceLaunch F{g.dtiles, g.lastTiles, g.fstates, nullptr, g.status,
                g.dcls, g.dtcls, g.dfaceStart, g.dwFace};
   if (g.lastMulti)
     hipLaunchKernelGGL(k_collect_faces<true>, dim3(grid_for(g.lastFaces)), dim3(kThreads), 0,
@@ -2416,7 +2458,7 @@ int fx_amr_autotune(double dt, int reps, fx_tune_result* out, int cap, int* n) {
 }
 
 void fx_hydro_autotune_reset(void) {
-  g.tuneTX = g.persist = g.itemOrder = 0;
+  g.tuneTX = g.persist = g.itemOrder = g.tuneOcc = 0;
   g.nItems = 0;
   if (g.amr.ready) {  // back to the built-in tile of the grid
     int TX = 8, TY = 8, NT = 0;

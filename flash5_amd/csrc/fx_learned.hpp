@@ -106,21 +106,10 @@ __global__ void __launch_bounds__(kThreads) k_mlp_adam(float* __restrict__ prm,
 }
 
 // The network predicts y = ln(p* / P0), P0 = sqrt(pL pR).  Weak interfaces
-// (every |feature| below the gate, fx_mlp_set_gate) keep the two-shock guess, which is
+// (every |feature| below the gate, fx_mlp_set_gate; seed_gate_pass) keep the two-shock guess, which is
 // already accurate to high order there; for the others Newton starts from
 // whichever of P0 * exp(y) and the two-shock guess has the smaller residual
 // of the pressure function.
-// largest feature magnitude of an interface
-__device__ __forceinline__ double seed_strength(double g, double rl, double ul, double pl,
-                                                double rr, double ur, double pr) {
-  const double P0 = sqrt(pl * pr), R0 = sqrt(rl * rr), C0 = sqrt(g * P0 / R0);
-  return fmax(fmax(fabs(0.5 * log(rl / rr)), fabs(0.5 * log(pl / pr))),
-              fabs(0.5 * (ur - ul) / C0));
-}
-
-// seed value of a face whose Newton iteration keeps the two-shock start
-constexpr float kNoSeed = 3.0e38f;
-
 __device__ __forceinline__ double seed_from_y(float y, double pl, double pr) {
   return sqrt(pl * pr) * exp((double)fminf(fmaxf(y, -60.f), 60.f));
 }
@@ -129,7 +118,7 @@ __device__ __forceinline__ double seed_from_y(float y, double pl, double pr) {
 // the Newton kernel below reads it, so each keeps its own register budget).
 __global__ void __launch_bounds__(kLearnThreads) k_mlp_seed_states(
     const double* __restrict__ S, long n, double gamma, const MlpPacked* __restrict__ W,
-    float* __restrict__ y, double gate) {
+    float* __restrict__ y, SeedGate gate) {
   __shared__ float xs[kLearnWaves][64 * 8];
   const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
   const long nw = (long)gridDim.x * kLearnWaves;
@@ -138,7 +127,7 @@ __global__ void __launch_bounds__(kLearnThreads) k_mlp_seed_states(
     const bool act = i < n;
     const IfaceState s = load_iface(S, n, i, act);
     // wavefronts whose interfaces are all weak skip the network
-    const bool need = act && seed_strength(gamma, s.rl, s.ul, s.pl, s.rr, s.ur, s.pr) >= gate;
+    const bool need = act && seed_gate_pass(gate, gamma, s.rl, s.ul, s.pl, s.rr, s.ur, s.pr);
     if (__ballot(need) == 0) continue;
     float f[6];
     double P0;
@@ -165,7 +154,7 @@ template <bool SEEDED>
 __global__ void __launch_bounds__(kThreads) k_riemann_batch(
     const double* __restrict__ S, long n, double gamma, const float* __restrict__ yseed,
     double tol, int maxit, double* __restrict__ pstar, int* __restrict__ iters, DevStatus* st,
-    double gate) {
+    SeedGate gate) {
   ExactStats es{0ULL, 0ULL};
   const double g = gamma;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -181,7 +170,7 @@ __global__ void __launch_bounds__(kThreads) k_riemann_batch(
     }
     const double du = s.ur - s.ul;
     const double ts = two_shock_guess(g, s.rl, s.ul, s.pl, cl, s.rr, s.ur, s.pr, cr);
-    const bool use = SEEDED && seed_strength(g, s.rl, s.ul, s.pl, s.rr, s.ur, s.pr) >= gate;
+    const bool use = SEEDED && seed_gate_pass(gate, g, s.rl, s.ul, s.pl, s.rr, s.ur, s.pr);
     double pm, pb = use ? seed_from_y(yseed[i], s.pl, s.pr) : -1.0;
     int it, total = 0;
     // one Newton loop for the seeded start and its two-shock restart
@@ -209,7 +198,6 @@ struct FaceLaunch {
   const double* fstates;   // [tile][dir][comp][face], from the march's RS_STATES stage
   const float* y;          // network seed per face (flattened like gid), or null
   DevStatus* status;
-  double gate;             // faces weaker than this keep the two-shock start
   // tiles of several block classes (the MULTI kernels): parameters per class,
   // class of each tile, faces before each tile (faceStart[nTiles] = all)
   const DevParams* cls;
@@ -316,18 +304,24 @@ k_face_solve(DevParams P0, FaceLaunch F) {
   flush_exact(F.status, es);
 }
 
-// network seed ln(p*/P0) of every face, from the stored face states
+// Network seed ln(p*/P0) of the faces the march listed as strong (at or
+// above the gate); every other face already holds "no seed".  The list
+// length is read on the device, so the launch needs no host sync.
 template <bool MULTI>
 __global__ void __launch_bounds__(kLearnThreads) k_face_seed(DevParams P0, FaceLaunch F,
                                                              const MlpPacked* __restrict__ W,
-                                                             float* __restrict__ y) {
+                                                             float* __restrict__ y,
+                                                             const unsigned* __restrict__ list,
+                                                             const unsigned* __restrict__ count,
+                                                             long cap) {
   __shared__ float xs[kLearnWaves][64 * 8];
   const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
-  const long total = face_total<MULTI>(P0, F);
+  const long total = (long)*count < cap ? (long)*count : cap;
   const long nw = (long)gridDim.x * kLearnWaves;
   for (long base = ((long)blockIdx.x * kLearnWaves + wv) * 64; base < total; base += nw * 64) {
-    const long gid = base + lane;
-    const bool act = gid < total;
+    const long i = base + lane;
+    const bool act = i < total;
+    const long gid = act ? (long)list[i] : 0;
     double rl = 1.0, ul = 0.0, pl = 1.0, rr = 1.0, ur = 0.0, pr = 1.0;
     if (act) {
       int t;
@@ -343,14 +337,6 @@ __global__ void __launch_bounds__(kLearnThreads) k_face_seed(DevParams P0, FaceL
       ur = b[(7 + f.d) * nf + f.r];
       pr = b[10 * nf + f.r];
     }
-    // weak (or invalid) faces get no seed; wavefronts of such faces only
-    // skip the network
-    const bool need = act && rl > 0. && rr > 0. && pl > 0. && pr > 0. &&
-                      seed_strength(P0.gamma, rl, ul, pl, rr, ur, pr) >= F.gate;
-    if (__ballot(need) == 0) {
-      if (act) y[gid] = kNoSeed;
-      continue;
-    }
     float x[6];
     double pref;
     mlp_features(P0.gamma, rl, ul, pl, rr, ur, pr, x, pref);
@@ -362,10 +348,12 @@ __global__ void __launch_bounds__(kLearnThreads) k_face_seed(DevParams P0, FaceL
     MlpActs A;
     float yo;
     mlp_forward_wave(*W, xs[wv], lane, A, yo);
-    if (act) y[gid] = need ? yo : kNoSeed;
+    if (act) y[gid] = yo;
     wave_lds_sync();
   }
 }
+
+__global__ void k_zero_count(unsigned* c) { c[threadIdx.x] = 0u; }
 
 // face states of the last exact-solver step as Riemann problems, SoA [6][n]
 // (rhoL, uL, pL, rhoR, uR, pR with u the face-normal velocity)

@@ -46,6 +46,14 @@ struct MarchLaunch {
   double dt;
   double* fstates;     // RS_STATES: per-face states for the separate Riemann stage
   const double* dtp;   // device-resident dt (graph-captured steps), or null
+  // RS_STATES with the learned seed: every face's seed slot is set to
+  // "no seed", and faces whose strength reaches the gate are appended to
+  // strongList (their flat face numbers) for the seed kernel
+  float* seedY;
+  unsigned* strongCount;
+  unsigned* strongList;
+  SeedGate gate;
+  unsigned strongCap;  // capacity of strongList (the launch's face count)
 };
 
 // Riemann-stage variant of the march: stage C stores each face's left and
@@ -71,6 +79,33 @@ __device__ __forceinline__ void write_face_states(const DevParams& P, double* ba
   for (int q = 0; q < 6; ++q) {
     base[q * nf + r] = double(VL[comp_slot(q)]);
     base[(6 + q) * nf + r] = double(VR[comp_slot(q)]);
+  }
+}
+
+// The learned seed's gate on face r (direction D) of the tile whose faces
+// start at flat number faceBase: the face's seed slot becomes "no seed", and
+// a face at or above the gate joins the strong list (one atomic per
+// wavefront, ranks by ballot).
+template <int D, typename R>
+__device__ __forceinline__ void seed_gate(const DevParams& P, const MarchLaunch& L, long faceBase,
+                                          long r, const R VL[8], const R VR[8]) {
+  long gid = faceBase + r;
+#pragma unroll
+  for (int e = 0; e < D; ++e) gid += P.nface[e];
+  const double rl = VL[P_DENS], pl = VL[P_PRES], rr = VR[P_DENS], pr = VR[P_PRES];
+  const bool strong = rl > 0. && rr > 0. && pl > 0. && pr > 0. &&
+                      seed_gate_pass(L.gate, P.gamma, rl, double(VL[P_VELX + D]), pl, rr,
+                                     double(VR[P_VELX + D]), pr);
+  L.seedY[gid] = kNoSeed;
+  const unsigned long long mask = __ballot(strong);
+  if (mask) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int leader = __builtin_ctzll(mask);
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(L.strongCount, (unsigned)__popcll(mask));
+    base = __shfl(base, leader);
+    const unsigned slot = base + (unsigned)__popcll(mask & ((1ULL << lane) - 1ULL));
+    if (strong && slot < L.strongCap) L.strongList[slot] = (unsigned)gid;
   }
 }
 
@@ -189,7 +224,7 @@ __device__ __forceinline__ R flat_tilde_vals(R pm2, R pm1, R pp1, R pp2, R vm1, 
 // tile t's face-state scratch (RS_STATES).
 template <int NDIM, int TX, int TY, int NT, int RS, typename R>
 __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch& L, int t, int ox,
-                                           int oy, double* fsb, ExactStats& es) {
+                                           int oy, double* fsb, long faceBase, ExactStats& es) {
   using I = TileIdx<TX, TY>;
   constexpr int NR = NDIM == 3 ? 3 : 1;        // ring depth
   constexpr int NCOL = TX * TY;
@@ -476,8 +511,9 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           VR[P_GAMC] = R(U[P.voff[FX_GAMC] + c]);
           VR[P_GAME] = R(U[P.voff[FX_GAME] + c]);
           if constexpr (RS == RS_STATES) {
-            write_face_states<0>(P, fsb + fstate_dir_offset(P, 0),
-                                 face_index(P, 0, ox + a, oy + b, k), VL, VR);
+            const long fr = face_index(P, 0, ox + a, oy + b, k);
+            write_face_states<0>(P, fsb + fstate_dir_offset(P, 0), fr, VL, VR);
+            if (L.seedY) seed_gate<0>(P, L, faceBase, fr, VL, VR);
             continue;
           } else {
             if (face_flux<0, RS>(P, VL, VR, F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 0); continue; }
@@ -494,8 +530,9 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           VR[P_GAMC] = R(U[P.voff[FX_GAMC] + c]);
           VR[P_GAME] = R(U[P.voff[FX_GAME] + c]);
           if constexpr (RS == RS_STATES) {
-            write_face_states<1>(P, fsb + fstate_dir_offset(P, 1),
-                                 face_index(P, 1, ox + a, oy + b, k), VL, VR);
+            const long fr = face_index(P, 1, ox + a, oy + b, k);
+            write_face_states<1>(P, fsb + fstate_dir_offset(P, 1), fr, VL, VR);
+            if (L.seedY) seed_gate<1>(P, L, faceBase, fr, VL, VR);
             continue;
           } else {
             if (face_flux<1, RS>(P, VL, VR, F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 1); continue; }
@@ -514,9 +551,9 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
             VR[P_GAMC] = R(U[P.voff[FX_GAMC] + c]);
             VR[P_GAME] = R(U[P.voff[FX_GAME] + c]);
             if constexpr (RS == RS_STATES) {
-              write_face_states<2>(P,
-                                   fsb + fstate_dir_offset(P, 2),
-                                   face_index(P, 2, ox + a, oy + b, k), VL, VR);
+              const long fr = face_index(P, 2, ox + a, oy + b, k);
+              write_face_states<2>(P, fsb + fstate_dir_offset(P, 2), fr, VL, VR);
+              if (L.seedY) seed_gate<2>(P, L, faceBase, fr, VL, VR);
             } else {
               if (face_flux<2, RS>(P, VL, VR, F, es)) report_error(st, 2, t, ox + a, oy + b, k, 2);
               else write_flux<2>(P, T.flux[2], face_index(P, 2, ox + a, oy + b, k), F);
@@ -543,9 +580,12 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
   if (bad) report_error(st, 1, t, ox, oy, -1, -1);
 }
 
-// one workgroup per column tile of a batch of equal-shape tiles
-template <int NDIM, int TX, int TY, int NT, int RS, typename R>
-__global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
+// One workgroup per column tile of a batch of equal-shape tiles.  OCC > 0
+// asks the compiler for OCC waves per SIMD (fewer registers, maybe spills);
+// the autotuner decides whether that pays.
+template <int NDIM, int TX, int TY, int NT, int RS, typename R, int OCC = 0>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1)))
+k_march(DevParams P, MarchLaunch L) {
   const int tilesPerBlock = L.tilesX * L.tilesY;
   const int t = blockIdx.x / tilesPerBlock;
   const int tr = blockIdx.x - t * tilesPerBlock;
@@ -553,7 +593,8 @@ __global__ void __launch_bounds__(NT) k_march(DevParams P, MarchLaunch L) {
   const int oy = P.lo[1] + (tr / L.tilesX) * TY;
   ExactStats es{0ULL, 0ULL};
   double* fsb = RS == RS_STATES ? L.fstates + t * fstate_tile_stride(P) : nullptr;
-  march_tile<NDIM, TX, TY, NT, RS, R>(P, L, t, ox, oy, fsb, es);
+  const long faceBase = (long)t * (P.nface[0] + P.nface[1] + P.nface[2]);
+  march_tile<NDIM, TX, TY, NT, RS, R>(P, L, t, ox, oy, fsb, faceBase, es);
   flush_exact(L.status, es);
 }
 
@@ -565,8 +606,9 @@ struct MarchItem {
   int t, cls, ox, oy;  // tile, its class, column-tile origin (guard-inclusive)
 };
 
-template <int NDIM, int TX, int TY, int NT, int RS, typename R>
-__global__ void __launch_bounds__(NT) k_march_multi(const DevParams* __restrict__ cls,
+template <int NDIM, int TX, int TY, int NT, int RS, typename R, int OCC = 0>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1)))
+k_march_multi(const DevParams* __restrict__ cls,
                                                     MarchLaunch L,
                                                     const MarchItem* __restrict__ items,
                                                     int nItems, const long* __restrict__ faceStart) {
@@ -574,8 +616,9 @@ __global__ void __launch_bounds__(NT) k_march_multi(const DevParams* __restrict_
   for (int it = blockIdx.x; it < nItems; it += gridDim.x) {
     const MarchItem w = items[it];
     const DevParams& P = cls[w.cls];
-    double* fsb = RS == RS_STATES ? L.fstates + kFaceComps * faceStart[w.t] : nullptr;
-    march_tile<NDIM, TX, TY, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, es);
+    const long faceBase = RS == RS_STATES ? faceStart[w.t] : 0;
+    double* fsb = RS == RS_STATES ? L.fstates + kFaceComps * faceBase : nullptr;
+    march_tile<NDIM, TX, TY, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, faceBase, es);
   }
   flush_exact(L.status, es);
 }

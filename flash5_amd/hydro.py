@@ -67,7 +67,7 @@ class FxAmrBlock(C.Structure):
 
 class FxTuneResult(C.Structure):
     _fields_ = [("tile_x", C.c_int), ("persist", C.c_int), ("order", C.c_int),
-                ("chosen", C.c_int), ("ms", C.c_float)]
+                ("occupancy", C.c_int), ("chosen", C.c_int), ("ms", C.c_float)]
 
 
 class FxStatus(C.Structure):
@@ -315,7 +315,8 @@ class Hydro:
     @staticmethod
     def _tune_list(buf, n):
         return [{"tile_x": buf[k].tile_x, "persist": buf[k].persist, "order": buf[k].order,
-                 "chosen": bool(buf[k].chosen), "ms": float(buf[k].ms)} for k in range(n)]
+                 "occupancy": buf[k].occupancy, "chosen": bool(buf[k].chosen),
+                 "ms": float(buf[k].ms)} for k in range(n)]
 
     def autotune(self, tiles: Sequence[Tile], dt: float, reps: int = 3):
         """Times the flux stages' candidate launch shapes on these tiles and
