@@ -2,7 +2,11 @@
 
 ``python -m flash5_amd.build`` compiles ``csrc/fx_hydro.hip`` into
 ``flash5_amd/libfx_hydro.so`` in-tree with hipcc.  All HIP code is built
-with packed-FP32 instructions disabled.
+with packed-FP32 instructions disabled.  FP32 division and square root use
+the hardware reciprocal and root (about 2.5 ulp) instead of correctly rounded
+sequences: only the mixed-precision reconstruction and the seed network
+compute in FP32, and their error bars are set against the FP64 path.  FP64
+code is not affected (its machine code is the same with or without it).
 """
 from __future__ import annotations
 
@@ -18,7 +22,7 @@ LIB = os.path.join(HERE, "libfx_hydro.so")
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
-    "-ffp-contract=off",
+    "-ffp-contract=off", "-fno-hip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
 ]
