@@ -90,6 +90,63 @@ def train_seed(hp, U, dt, dx, gamma, n_train: int = 1 << 22, steps: int = 1500):
             "train_s": time.perf_counter() - t0}
 
 
+def mixed_learned_leg(U0, dt, dx, gamma, steps: int, warmup: int, plain_value: float):
+    """The headline comparison on the same batch: FP32 reconstruction and
+    fluxes with the FP64 update, the exact solver seeded by the network
+    (trained untimed on this batch's own face states), launch shapes
+    autotuned; K timed steps from the initial state.  Its density L1 after
+    one step is taken against the FP64 exact-solver step of the same dt."""
+    import dataclasses
+    import torch
+    from flash5_amd.hydro import PREC_MIXED, Hydro, HydroParams
+    n = 16
+    base = HydroParams(ndim=3, nxb=n, nyb=n, nzb=n, gamma=gamma, RiemannSolver="exact")
+    info = train_seed(base, U0, dt, dx, gamma)
+    hp = dataclasses.replace(base, precision=PREC_MIXED, learnedSeed=True)
+    hy = Hydro(hp)
+    hy.set_seed_params(info.pop("params"))
+    stream = torch.cuda.current_stream()
+    hy.set_stream(stream)
+    # precision: one step of each from the initial state, same dt
+    U = U0.clone()
+    hy.advance_batch(U, dt, (dx,) * 3)
+    ref = U0.clone()
+    Hydro(base).advance_batch(ref, dt, (dx,) * 3)
+    g = 4
+    sl = (slice(None), slice(None), slice(g, g + n), slice(g, g + n), slice(g, g + n))
+    l1 = {}
+    for name, v in (("dens", 0), ("velx", 1), ("pres", 4), ("ener", 5), ("eint", 9)):
+        a, b = U[sl][:, v], ref[sl][:, v]
+        l1[name] = float((a - b).abs().sum() / b.abs().sum())
+    del ref
+    hy = Hydro(hp)
+    hy.set_stream(stream)
+    U.copy_(U0)
+    tune = hy.autotune_batch(U, dt, (dx,) * 3, reps=3)
+    for _ in range(warmup):
+        hy.advance_batch(U, dt, (dx,) * 3)
+    U.copy_(U0)
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(steps):
+        hy.advance_batch(U, dt, (dx,) * 3)
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / steps
+    value = U.shape[0] * n ** 3 / (ms * 1e-3)
+    st = hy.last_status
+    hy.autotune_reset()
+    return {"value": value, "unit": "zone-updates/s", "ms_per_step": ms,
+            "vs_plain_fp64": value / plain_value,
+            "l1_rel_vs_fp64_exact_one_step": l1,
+            "dtype": "f32+f64", "riemann": "exact", "learned_seed": info,
+            "newton_iters_per_face": int(st.newton_iters) / (3 * U.shape[0] * 17 * 16 * 16),
+            "newton_fallbacks": int(st.newton_fallbacks),
+            "autotune": [t for t in tune if t["chosen"]]}
+
+
 def cpu_baseline(U_gpu, dt, dx, target_s: float = 12.0):
     """Time the FP64 oracle (OpenMP over blocks) on a bounded sample of the
     batch: as many whole rounds of the sample as fit in about target_s, the
@@ -140,6 +197,8 @@ def main() -> None:
                     help="also run plain 1 GiB copies (PMC byte-counter calibration)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in launch shapes instead of timing the candidates")
+    ap.add_argument("--no-mixed-leg", action="store_true",
+                    help="skip the mixed-precision + learned-seed leg of the default run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -159,6 +218,8 @@ def main() -> None:
                      precision=prec)
     U = W.fourier_blocks(args.blocks, n=n, ng=g, gamma=gamma, device=dev)
     dt = W.cfl_dt(U, 3, g, (dx,) * 3, 0.8)
+    default_run = (args.precision == "fp64" and args.riemann == "hllc" and not args.learned)
+    U0 = U.clone() if default_run and not args.no_mixed_leg else None
     train_info = None
     if args.learned:
         if args.riemann != "exact":
@@ -209,6 +270,13 @@ def main() -> None:
                 + (" learned" if args.learned else ""))
     if args.copy_probe:
         copy_probe()
+    mixed = None
+    if U0 is not None:
+        hy.autotune_reset()
+        del U
+        torch.cuda.empty_cache()
+        mixed = mixed_learned_leg(U0, dt, dx, gamma, args.steps, args.warmup, value)
+        U = U0
 
     cpu = None
     if not args.no_cpu:
@@ -251,6 +319,7 @@ def main() -> None:
                      "step_bytes_per_zone": bz,
                      "step_frac": bz * zones / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
+        "mixed_learned": mixed,
         "fallbacks": fallbacks,
         "wall_s": wall,
     }
