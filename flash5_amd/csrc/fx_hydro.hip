@@ -497,66 +497,48 @@ __device__ __forceinline__ void update_cell(const DevParams& P, const Launch& L,
     const long c = cidx(P, i, j, k);
     const double dx = T.del[0], dy = P.ndim > 1 ? T.del[1] : 1., dz = P.ndim > 2 ? T.del[2] : 1.;
     const double dt = step_dt(L.dt, L.dtp);
-    double FL[8], FR[8], GL[8], GR[8], HL[8], HR[8];
-    {
-      const long nf = P.nface[0];
-      const long f0 = face_index(P, 0, i, j, k);
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        FL[q] = T.flux[0][q * nf + f0];
-        FR[q] = T.flux[0][q * nf + f0 + 1];
-        GL[q] = GR[q] = HL[q] = HR[q] = 0.0;
-      }
-    }
-    if (P.ndim >= 2) {
-      const long nf = P.nface[1];
-      const long g0 = face_index(P, 1, i, j, k);
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        GL[q] = T.flux[1][q * nf + g0] * 1;
-        GR[q] = T.flux[1][q * nf + g0 + P.fext[1][0]] * 1;
-      }
-    }
-    if (P.ndim == 3) {
-      const long nf = P.nface[2];
-      const long h0 = face_index(P, 2, i, j, k);
-      const long hs = (long)P.fext[2][0] * P.fext[2][1];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        HL[q] = T.flux[2][q * nf + h0] * 1;
-        HR[q] = T.flux[2][q * nf + h0 + hs] * 1;
-      }
-    }
-    double Sphys[5] = {0., 0., 0., 0., 0.};
-    Sphys[1] = (FL[5] - FR[5]) * (1.0 - P.fP) / dx;
-    if (P.ndim >= 2) Sphys[2] = (GL[5] - GR[5]) * (1.0 - P.fP) / dy;
-    if (P.ndim == 3) Sphys[3] = (HL[5] - HR[5]) * (1.0 - P.fP) / dz;
+    // One direction at a time: its face pair, the conserved-variable update,
+    // the aux e_int term and the pressure source.  Each quantity still takes
+    // its x, y, z terms in that order, with only one face pair live.
     const double dens0 = U[P.voff[FX_DENS] + c];
-    double IntEner = 0.;
-    if (P.aux) {
-      IntEner = dens0 * U[P.voff[FX_EINT] + c];
-      IntEner = IntEner + dt / dx * (FL[6] - FR[6] + pres * (FL[7] - FR[7]));
-      if (P.ndim > 1) {
-        IntEner = IntEner + dt / dy * (GL[6] - GR[6] + pres * (GL[7] - GR[7]));
-        if (P.ndim > 2) IntEner = IntEner + dt / dz * (HL[6] - HR[6] + pres * (HL[7] - HR[7]));
-      }
-      IntEner = IntEner + dt * 0.0;
-    }
     double U0[5];
     U0[0] = dens0;
     U0[1] = U[P.voff[FX_VELX] + c] * dens0;
     U0[2] = U[P.voff[FX_VELY] + c] * dens0;
     U0[3] = U[P.voff[FX_VELZ] + c] * dens0;
     U0[4] = dens0 * U[P.voff[FX_ENER] + c];
+    double IntEner = 0.;
+    if (P.aux) IntEner = dens0 * U[P.voff[FX_EINT] + c];
+    double Sphys[5] = {0., 0., 0., 0., 0.};
+    auto apply = [&](int d, double del, long fL, long fR) {
+      const double* Fd = T.flux[d];
+      const long nf = P.nface[d];
+      double FL[8], FR[8];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      U0[q] = U0[q] - dt / dx * (FR[q] - FL[q]);
-      if (P.ndim > 1) {
-        U0[q] = U0[q] - dt / dy * (GR[q] - GL[q]);
-        if (P.ndim > 2) U0[q] = U0[q] - dt / dz * (HR[q] - HL[q]);
+      for (int q = 0; q < 8; ++q) {
+        FL[q] = Fd[q * nf + fL];
+        FR[q] = Fd[q * nf + fR];
       }
-      U0[q] = U0[q] + dt * (0. + Sphys[q]);
+      Sphys[1 + d] = (FL[5] - FR[5]) * (1.0 - P.fP) / del;
+      if (P.aux) IntEner = IntEner + dt / del * (FL[6] - FR[6] + pres * (FL[7] - FR[7]));
+#pragma unroll
+      for (int q = 0; q < 5; ++q) U0[q] = U0[q] - dt / del * (FR[q] - FL[q]);
+    };
+    {
+      const long f0 = face_index(P, 0, i, j, k);
+      apply(0, dx, f0, f0 + 1);
     }
+    if (P.ndim >= 2) {
+      const long g0 = face_index(P, 1, i, j, k);
+      apply(1, dy, g0, g0 + P.fext[1][0]);
+    }
+    if (P.ndim == 3) {
+      const long h0 = face_index(P, 2, i, j, k);
+      apply(2, dz, h0, h0 + (long)P.fext[2][0] * P.fext[2][1]);
+    }
+    if (P.aux) IntEner = IntEner + dt * 0.0;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) U0[q] = U0[q] + dt * (0. + Sphys[q]);
     const double densNew = fmax(U0[0], P.smlrho);
     const double vx = U0[1] / densNew, vy = U0[2] / densNew, vz = U0[3] / densNew;
     const double ener = U0[4] / densNew;
