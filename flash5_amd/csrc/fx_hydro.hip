@@ -1151,15 +1151,20 @@ int upload_tiles(const fx_tile* tiles, int nTiles, bool needFlux, fx_status* st,
   return 0;
 }
 
-// Column tile shared by every class of a multi-class march (16 columns
-// across x where every class allows it, unless the autotuner chose 8).
+// Column-tile width of a multi-class march: 16 across x where every class
+// allows it, else 16 for the classes that allow it and 8 for the others
+// (returned as kPerClassTX), unless the autotuner chose one width for all.
+constexpr int kPerClassTX = 1;
 bool multi_march_shape(int* TX) {
-  bool all16 = true;
+  bool all16 = true, any16 = false;
   for (const DevParams& P : g.hcls) {
     if (P.ndim < 2 || P.nxb % 8 || P.nyb % 8) return false;
     if (P.nxb % 16) all16 = false;
+    else any16 = true;
   }
-  *TX = all16 && g.tuneTX != 8 ? 16 : 8;
+  if (g.tuneTX == 8 || !any16) *TX = 8;
+  else if (all16) *TX = 16;
+  else *TX = kPerClassTX;
   return true;
 }
 
@@ -1171,9 +1176,10 @@ int build_march_items(int nTiles, int TX, fx_status* st) {
   std::vector<MarchItem> items;
   for (int t = 0; t < nTiles; ++t) {
     const DevParams& P = tile_params(t);
+    const int W = TX == kPerClassTX ? (P.nxb % 16 ? 8 : 16) : TX;
     for (int ty = 0; ty < P.nyb / TY; ++ty)
-      for (int tx = 0; tx < P.nxb / TX; ++tx)
-        items.push_back(MarchItem{t, (int)g.htcls[t], P.lo[0] + tx * TX, P.lo[1] + ty * TY});
+      for (int tx = 0; tx < P.nxb / W; ++tx)
+        items.push_back(MarchItem{t, (int)g.htcls[t], P.lo[0] + tx * W, P.lo[1] + ty * TY, W});
   }
   if (g.itemOrder == 0)
     std::stable_sort(items.begin(), items.end(), [](const MarchItem& a, const MarchItem& b) {
@@ -1234,8 +1240,37 @@ void launch_march_multi_k(const MarchLaunch& M) {
                      g.stream, g.dcls, M, g.ditems, g.nItems, g.dfaceStart);
 }
 
+template <int NDIM, int NT, int RS, typename R, int OCC = 0>
+void launch_march_mixw_k(const MarchLaunch& M) {
+  static int resident = -1;
+  if (resident < 0) {
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_march_mixw<NDIM, NT, RS, R, OCC>, NT,
+                                                     0) != hipSuccess ||
+        per < 1)
+      per = 1;
+    resident = per;
+  }
+  long G = (long)g.numCU * resident * (g.persist > 0 ? g.persist : 1);
+  if (G > g.nItems) G = g.nItems;
+  if (G < 1) G = 1;
+  hipLaunchKernelGGL((k_march_mixw<NDIM, NT, RS, R, OCC>), dim3((unsigned)G), dim3(NT), 0,
+                     g.stream, g.dcls, M, g.ditems, g.nItems, g.dfaceStart);
+}
+
 template <int RS, typename R>
 void launch_march_multi_r(int ndim, int TX, const MarchLaunch& M) {
+  if (TX == kPerClassTX) {  // 16- and 8-wide column tiles in one launch
+    if constexpr (sizeof(R) == 4) {
+      if (ndim == 3 && g.tuneOcc == 3) {
+        launch_march_mixw_k<3, 512, RS, R, 3>(M);
+        return;
+      }
+    }
+    if (ndim == 3) launch_march_mixw_k<3, 512, RS, R>(M);
+    else launch_march_mixw_k<2, 256, RS, R>(M);
+    return;
+  }
   if constexpr (sizeof(R) == 4) {  // FP32 3-D: the autotuner may ask for 3 waves per SIMD
     if (ndim == 3 && g.tuneOcc == 3) {
       if (TX == 16) launch_march_multi_k<3, 16, 8, 512, RS, R, 3>(M);
@@ -1601,10 +1636,12 @@ int tune_flux_stages(int nTiles, double dt, int reps, fx_tune_result* out, int c
     int tx, persist, order, occ;
   };
   const DevParams& P = g.cur;
-  bool can16 = true;
+  bool can16 = true, any16 = false;
   if (g.multi) {
-    for (const DevParams& C : g.hcls)
+    for (const DevParams& C : g.hcls) {
       if (C.nxb % 16 || C.nyb % 8) can16 = false;
+      else any16 = true;
+    }
   } else {
     can16 = P.nxb % 16 == 0 && P.nyb % 8 == 0;
   }
@@ -1612,8 +1649,9 @@ int tune_flux_stages(int nTiles, double dt, int reps, fx_tune_result* out, int c
   const bool occHint = P.precision == FX_PREC_MIXED && P.ndim == 3;
   for (int occ : {0, 3}) {
     if (occ && !occHint) continue;
-    for (int tx : {16, 8}) {
+    for (int tx : {16, kPerClassTX, 8}) {
       if (tx == 16 && !can16) continue;
+      if (tx == kPerClassTX && (!g.multi || can16 || !any16)) continue;
       if (g.multi) {
         for (int persist : {1, 2, 4})
           for (int order : {0, 1}) cands.push_back(Cand{tx, persist, order, occ});

@@ -35,6 +35,24 @@ template <int TX, int TY> struct TileIdx {
   static constexpr int NLX = (TX + 4) * TY, NLY = TX * (TY + 4);
 };
 
+// LDS of one column tile (march_tile), as offsets into one array of R
+template <int NDIM, int TX, int TY> struct MarchLds {
+  using I = TileIdx<TX, TY>;
+  static constexpr int NR = NDIM == 3 ? 3 : 1, NCOL = TX * TY, Z = NDIM == 3 ? 1 : 0;
+  static constexpr int SX = 0;                                // sigma_x ring
+  static constexpr int SY = SX + NR * 6 * I::NRX;             // sigma_y ring
+  static constexpr int SZ = SY + NR * 6 * I::NRY;             // sigma_z of plane k
+  static constexpr int LX = SZ + (Z ? 6 * I::NC : 1);         // slopes along x, plane k
+  static constexpr int LY = LX + 6 * I::NLX;                  // slopes along y, plane k
+  static constexpr int LZ = LY + 6 * I::NLY;                  // slopes along z, ring
+  static constexpr int FZ = LZ + (Z ? 3 * 6 * NCOL : 1);      // flattening tilde along z, ring
+  static constexpr int WX = FZ + (Z ? 3 * NCOL : 1);          // x-face states (Wp, Wn)
+  static constexpr int WY = WX + 12 * I::NRY;                 // y-face states
+  static constexpr int WZ = WY + 12 * I::NRX;                 // z-face states of plane k
+  static constexpr int WZP = WZ + (Z ? 12 * NCOL : 1);        // z-face Wp of plane k-1
+  static constexpr int total = WZP + (Z ? 6 * NCOL : 1);
+};
+
 // components stored for sigma and slopes: dens, vx, vy, vz, pres, eint
 __device__ __forceinline__ int comp_slot(int q) { return q < 5 ? q : P_EINT; }
 
@@ -224,22 +242,24 @@ __device__ __forceinline__ R flat_tilde_vals(R pm2, R pm1, R pp1, R pp2, R vm1, 
 // tile t's face-state scratch (RS_STATES).
 template <int NDIM, int TX, int TY, int NT, int RS, typename R>
 __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch& L, int t, int ox,
-                                           int oy, double* fsb, long faceBase, ExactStats& es) {
+                                           int oy, double* fsb, long faceBase, ExactStats& es,
+                                           R* smem) {
   using I = TileIdx<TX, TY>;
+  using M = MarchLds<NDIM, TX, TY>;
   constexpr int NR = NDIM == 3 ? 3 : 1;        // ring depth
   constexpr int NCOL = TX * TY;
   constexpr int Z = NDIM == 3 ? 1 : 0;
-  __shared__ R SX[NR * 6 * I::NRX];        // sigma_x ring
-  __shared__ R SY[NR * 6 * I::NRY];        // sigma_y ring
-  __shared__ R SZ[Z ? 6 * I::NC : 1];      // sigma_z of plane k
-  __shared__ R LX[6 * I::NLX];             // slopes along x, plane k
-  __shared__ R LY[6 * I::NLY];             // slopes along y, plane k
-  __shared__ R LZ[Z ? 3 * 6 * NCOL : 1];   // slopes along z, ring
-  __shared__ R FZ[Z ? 3 * NCOL : 1];       // flattening tilde along z, ring
-  __shared__ R WX[12 * I::NRY];            // x-face states (Wp, Wn)
-  __shared__ R WY[12 * I::NRX];            // y-face states
-  __shared__ R WZ[Z ? 12 * NCOL : 1];      // z-face states of plane k
-  __shared__ R WZP[Z ? 6 * NCOL : 1];      // z-face Wp of plane k-1
+  R* const SX = smem + M::SX;
+  R* const SY = smem + M::SY;
+  R* const SZ = smem + M::SZ;
+  R* const LX = smem + M::LX;
+  R* const LY = smem + M::LY;
+  R* const LZ = smem + M::LZ;
+  R* const FZ = smem + M::FZ;
+  R* const WX = smem + M::WX;
+  R* const WY = smem + M::WY;
+  R* const WZ = smem + M::WZ;
+  R* const WZP = smem + M::WZP;
 
   const int tid = threadIdx.x;
   // A tile owns its high x (y) face row only at the block's high side;
@@ -594,7 +614,8 @@ k_march(DevParams P, MarchLaunch L) {
   ExactStats es{0ULL, 0ULL};
   double* fsb = RS == RS_STATES ? L.fstates + t * fstate_tile_stride(P) : nullptr;
   const long faceBase = (long)t * (P.nface[0] + P.nface[1] + P.nface[2]);
-  march_tile<NDIM, TX, TY, NT, RS, R>(P, L, t, ox, oy, fsb, faceBase, es);
+  __shared__ R smem[MarchLds<NDIM, TX, TY>::total];
+  march_tile<NDIM, TX, TY, NT, RS, R>(P, L, t, ox, oy, fsb, faceBase, es, smem);
   flush_exact(L.status, es);
 }
 
@@ -604,6 +625,7 @@ k_march(DevParams P, MarchLaunch L) {
 // deals the long columns first and the short ones fill in behind them.
 struct MarchItem {
   int t, cls, ox, oy;  // tile, its class, column-tile origin (guard-inclusive)
+  int tx;              // column-tile width of the item (mixed-width launches)
 };
 
 template <int NDIM, int TX, int TY, int NT, int RS, typename R, int OCC = 0>
@@ -613,12 +635,34 @@ k_march_multi(const DevParams* __restrict__ cls,
                                                     const MarchItem* __restrict__ items,
                                                     int nItems, const long* __restrict__ faceStart) {
   ExactStats es{0ULL, 0ULL};
+  __shared__ R smem[MarchLds<NDIM, TX, TY>::total];
   for (int it = blockIdx.x; it < nItems; it += gridDim.x) {
     const MarchItem w = items[it];
     const DevParams& P = cls[w.cls];
     const long faceBase = RS == RS_STATES ? faceStart[w.t] : 0;
     double* fsb = RS == RS_STATES ? L.fstates + kFaceComps * faceBase : nullptr;
-    march_tile<NDIM, TX, TY, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, faceBase, es);
+    march_tile<NDIM, TX, TY, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, faceBase, es, smem);
+  }
+  flush_exact(L.status, es);
+}
+
+// The same with the column-tile width chosen per item: 16 columns across x
+// for classes whose x extent allows it, 8 for the others.  One LDS array of
+// the larger layout serves both; NT is the 16-wide tile's thread count.
+template <int NDIM, int NT, int RS, typename R, int OCC = 0>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1)))
+k_march_mixw(const DevParams* __restrict__ cls, MarchLaunch L,
+             const MarchItem* __restrict__ items, int nItems, const long* __restrict__ faceStart) {
+  ExactStats es{0ULL, 0ULL};
+  constexpr int n16 = MarchLds<NDIM, 16, 8>::total, n8 = MarchLds<NDIM, 8, 8>::total;
+  __shared__ R smem[n16 > n8 ? n16 : n8];
+  for (int it = blockIdx.x; it < nItems; it += gridDim.x) {
+    const MarchItem w = items[it];
+    const DevParams& P = cls[w.cls];
+    const long faceBase = RS == RS_STATES ? faceStart[w.t] : 0;
+    double* fsb = RS == RS_STATES ? L.fstates + kFaceComps * faceBase : nullptr;
+    if (w.tx == 16) march_tile<NDIM, 16, 8, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, faceBase, es, smem);
+    else march_tile<NDIM, 8, 8, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, faceBase, es, smem);
   }
   flush_exact(L.status, es);
 }

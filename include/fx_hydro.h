@@ -259,7 +259,8 @@ int fx_hydro_set_block_classes(const int* nb, int nClasses);
  * Times every candidate launch shape of the flux stages (column march, and
  * the exact solver's face stage) on the caller's own batch and keeps the
  * fastest for later calls.  Candidates: the column-tile width (16 or 8
- * columns across x, where every tile shape divides it); in the mixed mode in
+ * columns across x, where every tile shape divides it, or per class for
+ * tiles of several classes); in the mixed mode in
  * 3-D, a register budget for 3 waves per SIMD (occupancy against spills);
  * and for tiles of several classes the persistent workgroups per CU (1, 2
  * or 4 times what stays resident) and the order of the column work items
@@ -269,7 +270,8 @@ int fx_hydro_set_block_classes(const int* nb, int nClasses);
  * as it was.  out[0 .. *n-1] receives every candidate's time (up to cap);
  * the chosen one has chosen = 1. */
 typedef struct {
-  int tile_x;               /* columns across x per workgroup (8 or 16) */
+  int tile_x;               /* columns across x per workgroup (8 or 16; 1 = per
+                               class: 16 where the class allows it, else 8) */
   int persist;              /* workgroups per CU multiple (multi-class only; 0 = n/a) */
   int order;                /* multi-class items: 0 longest first, 1 tile by tile */
   int occupancy;            /* waves-per-SIMD hint of the FP32 3-D march (0 = none) */
