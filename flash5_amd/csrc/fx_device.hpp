@@ -446,13 +446,17 @@ __device__ __forceinline__ int hllc(const DevParams& P, const R* Vm, const R* Vp
 // exact pressure function (Toro, Riemann Solvers and Numerical Methods, 3rd
 // ed., eqs. 4.6-4.9, derivative 4.37), seeded by a caller guess or by the
 // two-shock guess (Toro eq. 9.42), sampled at x/t = 0 (Toro sec. 4.5).  The
-// operation order follows the CPU oracle's restatement.
+// operation order follows the CPU oracle's restatement, except that the
+// rarefaction derivative reuses the value's power (one pow, not two).
 __device__ __forceinline__ void exact_fk(double p, double rk, double pk, double ck, double g,
                                          double& f, double& fd) {
   if (p <= pk) {
+    // rarefaction: the derivative's power is the value's divided by pratio,
+    // since (g - 1)/(2g) + (g + 1)/(2g) = 1 (one pow instead of two)
     const double pratio = p / pk;
-    f = 2. / (g - 1.) * ck * (pow(pratio, (g - 1.) / (2. * g)) - 1.);
-    fd = (1. / (rk * ck)) * pow(pratio, -(g + 1.) / (2. * g));
+    const double q = pow(pratio, (g - 1.) / (2. * g));
+    f = 2. / (g - 1.) * ck * (q - 1.);
+    fd = (1. / (rk * ck)) * (q / pratio);
   } else {
     const double ak = 2. / ((g + 1.) * rk);
     const double bk = (g - 1.) / (g + 1.) * pk;
