@@ -446,8 +446,10 @@ __device__ __forceinline__ int hllc(const DevParams& P, const R* Vm, const R* Vp
 // exact pressure function (Toro, Riemann Solvers and Numerical Methods, 3rd
 // ed., eqs. 4.6-4.9, derivative 4.37), seeded by a caller guess or by the
 // two-shock guess (Toro eq. 9.42), sampled at x/t = 0 (Toro sec. 4.5).  The
-// operation order follows the CPU oracle's restatement, except that the
-// rarefaction derivative reuses the value's power (one pow, not two).
+// operation order follows the CPU oracle's restatement, except where a power
+// follows from one already taken: the rarefaction derivative reuses the
+// value's power, the density behind a rarefaction is gamma p / c^2, and the
+// fan pressure's power is the density's times (c/ck)^2.
 __device__ __forceinline__ void exact_fk(double p, double rk, double pk, double ck, double g,
                                          double& f, double& fd) {
   if (p <= pk) {
@@ -541,11 +543,14 @@ __device__ __forceinline__ void exact_sample(double g, double rl, double ul, dou
       if (0. <= shl) { r = rl; u = ul; pp = pl; return; }
       const double cml = cl * pow(pm / pl, (g - 1.) / (2. * g));
       const double stl = um - cml;
-      if (0. > stl) { r = rl * pow(pm / pl, 1. / g); u = um; pp = pm; return; }
+      // isentropic: rho = gamma p / c^2 behind the rarefaction
+      if (0. > stl) { r = g * pm / (cml * cml); u = um; pp = pm; return; }
       const double c = 2. / (g + 1.) * (cl + (g - 1.) / 2. * (ul - 0.));
       u = 2. / (g + 1.) * (cl + (g - 1.) / 2. * ul + 0.);
-      r = rl * pow(c / cl, 2. / (g - 1.));
-      pp = pl * pow(c / cl, 2. * g / (g - 1.));
+      // in the fan, (c/cl)^(2g/(g-1)) = (c/cl)^(2/(g-1)) (c/cl)^2
+      const double q = pow(c / cl, 2. / (g - 1.)), s2 = (c / cl) * (c / cl);
+      r = rl * q;
+      pp = pl * (q * s2);
       return;
     }
     const double pml = pm / pl;
@@ -568,11 +573,12 @@ __device__ __forceinline__ void exact_sample(double g, double rl, double ul, dou
   if (0. >= shr) { r = rr; u = ur; pp = pr; return; }
   const double cmr = cr * pow(pm / pr, (g - 1.) / (2. * g));
   const double str = um + cmr;
-  if (0. <= str) { r = rr * pow(pm / pr, 1. / g); u = um; pp = pm; return; }
+  if (0. <= str) { r = g * pm / (cmr * cmr); u = um; pp = pm; return; }
   const double c = 2. / (g + 1.) * (cr - (g - 1.) / 2. * (ur - 0.));
   u = 2. / (g + 1.) * (-cr + (g - 1.) / 2. * ur + 0.);
-  r = rr * pow(c / cr, 2. / (g - 1.));
-  pp = pr * pow(c / cr, 2. * g / (g - 1.));
+  const double q = pow(c / cr, 2. / (g - 1.)), s2 = (c / cr) * (c / cr);
+  r = rr * q;
+  pp = pr * (q * s2);
 }
 
 // Exact solution at x/t = 0 of the face problem (rl, ul, pl | rr, ur, pr)
