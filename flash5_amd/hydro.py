@@ -289,10 +289,11 @@ class Hydro:
         if self._classes is not None:
             self._apply_classes()
 
-    def _apply_classes(self):
-        flat = [int(v) for shape in self._classes for v in shape]
+    def _apply_classes(self, classes=None):
+        classes = self._classes if classes is None else classes
+        flat = [int(v) for shape in classes for v in shape]
         arr = (C.c_int * max(1, len(flat)))(*flat)
-        rc = self.lib.fx_hydro_set_block_classes(arr, len(self._classes))
+        rc = self.lib.fx_hydro_set_block_classes(arr, len(classes))
         if rc:
             raise FxError("fx_hydro_set_block_classes", rc=rc)
 
@@ -300,11 +301,10 @@ class Hydro:
         """Interior shapes (nxb, nyb, nzb) of the block classes; a Tile picks
         its shape by block_class.  Tiles of several classes go through one
         persistent launch per stage.  An empty list returns to the params
-        shape."""
-        self._classes = [tuple(int(v) for v in sh) + (1,) * (3 - len(sh)) for sh in shapes]
-        self._apply_classes()
-        if not self._classes:
-            self._classes = None
+        shape.  A rejected list leaves the classes as they were."""
+        classes = [tuple(int(v) for v in sh) + (1,) * (3 - len(sh)) for sh in shapes]
+        self._apply_classes(classes)
+        self._classes = classes or None
 
     def class_params(self, c: int) -> HydroParams:
         """HydroParams of block class c (the params with that class's shape)."""

@@ -24,7 +24,8 @@ NVAR_USED = 10
 F_DENS, F_XMOM, F_YMOM, F_ZMOM, F_ENER, F_PRES, F_EINT, F_VOLU = range(8)
 NFLUX = 8
 MINMOD, MC, HYBRID, VANLEER, LIMITED = 1, 2, 3, 4, 5
-ROE, HLL, HLLC, EXACT = 1, 3, 4, 20
+ROE, HLL, HLLC, HYBR, EXACT = 1, 3, 4, 8, 20
+SHOK = 10        # slot of SHOK_VAR when a layout carries it after the ten used variables
 
 
 class Params(C.Structure):
@@ -38,6 +39,7 @@ class Params(C.Structure):
         ("smallE", C.c_double), ("eintSwitch", C.c_double), ("tiny", C.c_double),
         ("gasConstant", C.c_double), ("singleSpeciesA", C.c_double),
         ("exactMaxIter", C.c_int), ("exactTol", C.c_double),
+        ("shockDetect", C.c_int), ("shokVar", C.c_int),
     ]
 
 
@@ -84,6 +86,9 @@ def lib():
         L.ox_ppm.argtypes = [C.POINTER(Params), C.c_int, C.c_double, C.c_double, _DP, C.c_double,
                              C.c_int, _DP, _DP, _DP, _DP, _DP, _DP]
         L.ox_hllc.argtypes = [C.POINTER(Params), C.c_int, _DP, _DP, _DP, _DP]
+        L.ox_hll.argtypes = [C.POINTER(Params), C.c_int, _DP, _DP, _DP, _DP]
+        L.ox_shock_detect.argtypes = [C.POINTER(Params), _DP]
+        L.ox_shock_detect.restype = C.c_long
         L.ox_exact_riemann.argtypes = [C.POINTER(Params), C.c_int, _DP, _DP, _DP, _DP,
                                        C.POINTER(Stats)]
         L.ox_exact_pstar.argtypes = [C.c_double] * 8 + [C.c_double, C.c_int, _DP, _DP]
@@ -218,6 +223,19 @@ def hllc(p: Params, d: int, VL, VR):
     f = lambda a: _p(np.ascontiguousarray(a, dtype=np.float64))
     rc = lib().ox_hllc(C.byref(p), d, f(VL), f(VR), _p(F), C.byref(sp))
     return rc, F, sp.value
+
+
+def hll(p: Params, d: int, VL, VR):
+    F, sp = np.zeros(6), C.c_double(0.0)
+    f = lambda a: _p(np.ascontiguousarray(a, dtype=np.float64))
+    rc = lib().ox_hll(C.byref(p), d, f(VL), f(VR), _p(F), C.byref(sp))
+    return rc, F, sp.value
+
+
+def shock_detect(p: Params, U: np.ndarray) -> int:
+    """hy_shockDetectBlk on one block in place; returns the flagged cells
+    (-1 without a SHOK slot)."""
+    return lib().ox_shock_detect(C.byref(p), _p(U))
 
 
 def exact_flux(p: Params, d: int, VL, VR):

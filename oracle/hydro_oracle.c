@@ -7,7 +7,8 @@
  *   hy_DataReconstructNormalDir_PPM.F90, hy_eigenParameters.F90,
  *   hy_eigenValue.F90, OnlyHydro/hy_eigenVector.F90, hy_TVDslope.F90,
  *   hy_slopeLimiters.F90, hy_upwindTransverseFlux.F90, hy_getFaceFlux.F90,
- *   hy_HLLC.F90, hy_prim2con.F90, hy_prim2flx.F90, hy_unsplitUpdate.F90,
+ *   hy_HLLC.F90, hy_HLL.F90, hy_shockDetectBlk.F90, hy_prim2con.F90,
+ *   hy_prim2flx.F90, hy_unsplitUpdate.F90,
  *   hy_energyFix.F90, hy_updateSolution.F90, Hydro_computeDt.F90;
  *   physics/Eos/EosMain/{Eos_getData,Eos_putData}.F90,
  *   physics/Eos/EosMain/Gamma/{eos_idealGamma,eos_initGamma}.F90.
@@ -61,6 +62,8 @@ void ox_default_params(ox_params* p, int ndim, int nb, double gamma) {
   p->singleSpeciesA = 1.0;
   p->exactMaxIter = 20;
   p->exactTol = 1.e-12;
+  p->shockDetect = 0;
+  p->shokVar = -1;
 }
 
 /* ------------------------------------------------------------------ */
@@ -488,6 +491,93 @@ int ox_hllc(const ox_params* p, int dir, const double Vm[9], const double Vp[9],
     for (int f = 0; f < 6; ++f) Fstar[f] = FR[f];
   }
   return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* HLL: hy_HLL.F90:82-165 (hydro; Davis speeds as in HLLC)              */
+/* ------------------------------------------------------------------ */
+int ox_hll(const ox_params* p, int dir, const double Vm[9], const double Vp[9],
+           double Fstar[6], double* speed) {
+  double velNL = Vm[H_VELX + dir], velNR = Vp[H_VELX + dir];        /* :97-98 */
+  double aL2 = Vm[H_GAMC] * Vm[H_PRES] / Vm[H_DENS];               /* :101-102 */
+  double aR2 = Vp[H_GAMC] * Vp[H_PRES] / Vp[H_DENS];
+  *speed = 0.;
+  if ((Vm[H_DENS] < p->tiny && Vm[H_DENS] > 0.) || (Vp[H_DENS] < p->tiny && Vp[H_DENS] > 0.) ||
+      (Vm[H_PRES] < p->tiny && Vm[H_PRES] > 0.) || (Vp[H_PRES] < p->tiny && Vp[H_PRES] > 0.)) {
+    for (int f = 0; f < 6; ++f) Fstar[f] = 0.;                      /* :112-118 */
+    return 0;
+  } else if (aL2 < 0. || aR2 < 0.) {
+    return 1;                                                       /* :119-121 */
+  }
+  double cfL = sqrt(aL2), cfR = sqrt(aR2);
+  double SL = fmin(velNL - cfL, velNR - cfR);                        /* :136-137 */
+  double SR = fmax(velNL + cfL, velNR + cfR);
+  *speed = fmax(fabs(SL), fabs(SR));                                 /* :141 */
+  double UL[6], UR[6], FL[6], FR[6];
+  prim2con(Vm, UL); prim2con(Vp, UR);                                /* :144-146 */
+  UL[5] = 0.0; UR[5] = 0.0;
+  prim2flx(dir, Vm, p->fP, FL); prim2flx(dir, Vp, p->fP, FR);
+  if (SL > 0.) {                                                     /* :151-164 */
+    for (int f = 0; f < 6; ++f) Fstar[f] = FL[f];
+  } else if (SR < 0.) {
+    for (int f = 0; f < 6; ++f) Fstar[f] = FR[f];
+  } else {
+    for (int f = 0; f < 6; ++f)
+      Fstar[f] = (SR * FL[f] - SL * FR[f] + SR * SL * (UR[f] - UL[f])) / (SR - SL);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Shock flag: hy_shockDetectBlk.F90:93-229 (Balsara & Spicer, JCP      */
+/* 149:270, 1999)                                                       */
+/* ------------------------------------------------------------------ */
+long ox_shock_detect(const ox_params* p, double* U) {
+  if (p->shokVar < 0 || p->shokVar >= p->nunk) return -1;
+  geom_t g = geom_of(p);
+  double* S = U + (long)p->shokVar * g.ncell;
+  for (long c = 0; c < g.ncell; ++c) S[c] = 0.;                      /* :117 */
+  const int k2 = p->ndim > 1, k3 = p->ndim > 2;
+  const long sx = 1, sy = g.nx, sz = (long)g.nx * g.ny;
+  double* Vc = malloc(sizeof(double) * g.ncell);
+  if (!Vc) return -1;
+  for (int k = g.lo[2] - k3; k <= g.hi[2] + k3; ++k)                 /* :132-138 */
+    for (int j = g.lo[1] - k2; j <= g.hi[1] + k2; ++j)
+      for (int i = g.lo[0] - 1; i <= g.hi[0] + 1; ++i) {
+        long c = CIDX(g, i, j, k);
+        Vc[c] = sqrt(UV(U, p, g, OX_GAMC, c) * UV(U, p, g, OX_PRES, c) / UV(U, p, g, OX_DENS, c));
+      }
+  const double beta = 0.5, delta = 0.1;                              /* :94-95 */
+  long n = 0;
+  for (int k = g.lo[2]; k <= g.hi[2]; ++k)
+    for (int j = g.lo[1]; j <= g.hi[1]; ++j)
+      for (int i = g.lo[0]; i <= g.hi[0]; ++i) {
+        long c = CIDX(g, i, j, k);
+        double minP = UV(U, p, g, OX_PRES, c), minC = Vc[c];         /* :152-153 */
+        for (int c3 = -k3; c3 <= k3; ++c3)
+          for (int c2 = -k2; c2 <= k2; ++c2)
+            for (int c1 = -1; c1 <= 1; ++c1) {
+              long q = c + c1 * sx + c2 * sy + c3 * sz;
+              minP = fmin(minP, UV(U, p, g, OX_PRES, q));
+              minC = fmin(minC, Vc[q]);
+            }
+        double divv = UV(U, p, g, OX_VELX, c + sx) - UV(U, p, g, OX_VELX, c - sx);  /* :157-164 */
+        if (k2) divv = divv + UV(U, p, g, OX_VELY, c + sy) - UV(U, p, g, OX_VELY, c - sy);
+        if (k3) divv = divv + UV(U, p, g, OX_VELZ, c + sz) - UV(U, p, g, OX_VELZ, c - sz);
+        divv = 0.5 * divv;
+        double gradPx = 0.5 * (UV(U, p, g, OX_PRES, c + sx) - UV(U, p, g, OX_PRES, c - sx));
+        double gradPy = 0., gradPz = 0.;                             /* :167-174 */
+        if (k2) gradPy = 0.5 * (UV(U, p, g, OX_PRES, c + sy) - UV(U, p, g, OX_PRES, c - sy));
+        if (k3) gradPz = 0.5 * (UV(U, p, g, OX_PRES, c + sz) - UV(U, p, g, OX_PRES, c - sz));
+        int sw1 = fabs(gradPx) + fabs(gradPy) + fabs(gradPz) >= beta * minP;   /* :175-178 */
+        int sw2 = -(delta * minC) >= divv;                                     /* :180-182 */
+        if (sw1 && sw2) {                                                      /* :185-191 */
+          S[c] = 1.;
+          ++n;
+        }
+      }
+  free(Vc);
+  return n;
 }
 
 /* ------------------------------------------------------------------ */
@@ -921,7 +1011,7 @@ static int get_riemann_state(const ox_params* p, const double* U, const geom_t* 
 /* ------------------------------------------------------------------ */
 /* hy_getFaceFlux.F90:53-1275 (Cartesian, hydro)                        */
 /* ------------------------------------------------------------------ */
-static int get_face_flux(const ox_params* p, const geom_t* g, scratch_t* s,
+static int get_face_flux(const ox_params* p, const double* U, const geom_t* g, scratch_t* s,
                          double* flux[3], ox_stats* st) {
   const int nd = p->ndim;
   const int* lo = g->lo;
@@ -944,6 +1034,10 @@ static int get_face_flux(const ox_params* p, const geom_t* g, scratch_t* s,
           double F[8], speed;
           int ierr;
           if (p->riemann == OX_EXACT) ierr = ox_exact_riemann(p, d, VL, VR, F, &speed, st);
+          else if (p->riemann == OX_HLL) ierr = ox_hll(p, d, VL, VR, F, &speed);
+          else if (p->riemann == OX_HYBR &&                            /* :353-358 */
+                   U[(long)p->shokVar * ncell + cm] + U[(long)p->shokVar * ncell + c] > 0.)
+            ierr = ox_hll(p, d, VL, VR, F, &speed);
           else ierr = ox_hllc(p, d, VL, VR, F, &speed);
           if (ierr) { st->riemannErrors++; return 1; }                /* :1217 do_error */
           F[OX_F_EINT] = 0.; F[OX_F_VOLU] = 0.;
@@ -1074,6 +1168,8 @@ static void unsplit_update(const ox_params* p, double* U, const geom_t* g,
 
 int ox_compute_fluxes(const ox_params* p, double* U, double* flux[3], double* pavg,
                       double dt, const double del[3], int update, ox_stats* st) {
+  if (p->riemann == OX_HYBR && (p->shokVar < 0 || p->shokVar >= p->nunk))
+    return -1;  /* hy_shockDetectBlk.F90:119-122: HYBR needs SHOK_VAR */
   geom_t g = geom_of(p);
   scratch_t s;
   ox_stats local;
@@ -1085,7 +1181,7 @@ int ox_compute_fluxes(const ox_params* p, double* U, double* flux[3], double* pa
   for (int d = 0; d < p->ndim; ++d)
     if (!fl[d] && update) { own[d] = malloc(sizeof(double) * OX_NFLUX * fgeom_of(p, d).nf); fl[d] = own[d]; }
   int rc = get_riemann_state(p, U, &g, dt, del, &s, st);
-  if (!rc) rc = get_face_flux(p, &g, &s, fl, st);
+  if (!rc) rc = get_face_flux(p, U, &g, &s, fl, st);
   if (!rc && pavg) {
     long n = 0;
     for (int k = g.lo[2]; k <= g.hi[2]; ++k)
@@ -1128,7 +1224,10 @@ int ox_compute_fluxes_batch(const ox_params* p, double* U, long nblocks, double 
     ox_stats s;
     memset(&s, 0, sizeof s);
     double* fl[3] = {NULL, NULL, NULL};
-    rc |= ox_compute_fluxes(p, U + b * ncell * p->nunk, fl, NULL, dt, del, 1, &s);
+    double* Ub = U + b * ncell * p->nunk;
+    /* Hydro_init.F90:387-395 turns detection on for HYBR */
+    if ((p->shockDetect || p->riemann == OX_HYBR) && ox_shock_detect(p, Ub) < 0) rc |= 1;
+    rc |= ox_compute_fluxes(p, Ub, fl, NULL, dt, del, 1, &s);
     fb += s.fallbacks; re += s.riemannErrors; ee += s.eigenErrors;
     xi += s.exactIters; xf += s.exactFallbacks;
   }

@@ -34,9 +34,10 @@ enum { OX_F_DENS = 0, OX_F_XMOM, OX_F_YMOM, OX_F_ZMOM, OX_F_ENER, OX_F_PRES,
 /* Slope limiters, numbered as UHD.h:15-19. */
 enum { OX_MINMOD = 1, OX_MC = 2, OX_HYBRID = 3, OX_VANLEER = 4, OX_LIMITED = 5 };
 
-/* Riemann solvers: HLLC as UHD.h:369; OX_EXACT is the exact two-shock
+/* Riemann solvers, numbered as UHD.h:59-66 (HLL, HLLC, HYBR = HLL at
+ * flagged shocks and HLLC elsewhere); OX_EXACT is the exact two-shock
  * iterative solver, which the reference does not contain. */
-enum { OX_ROE = 1, OX_HLL = 3, OX_HLLC = 4, OX_EXACT = 20 };
+enum { OX_ROE = 1, OX_HLL = 3, OX_HLLC = 4, OX_HYBR = 8, OX_EXACT = 20 };
 
 typedef struct {
   int ndim;             /* 1, 2 or 3 */
@@ -52,7 +53,7 @@ typedef struct {
   int steepening;       /* use_steepening */
   int use3dFullCTU;     /* use_3dFullCTU */
   int useAuxEint;       /* use_auxEintEqn */
-  int riemann;          /* OX_HLLC or OX_EXACT */
+  int riemann;          /* OX_HLL, OX_HLLC, OX_HYBR or OX_EXACT */
   double fP;            /* hy_fPresInMomFlux */
   double smlrho, smallp, smallE, eintSwitch;
   double tiny;          /* hy_tiny (Hydro_data.F90:65) */
@@ -61,6 +62,10 @@ typedef struct {
   /* exact solver controls */
   int exactMaxIter;
   double exactTol;
+  /* shock detection (hy_shockDetectBlk.F90): shockDetect as the runtime
+   * parameter; shokVar = slot of SHOK_VAR in U's var axis, -1 if absent */
+  int shockDetect;
+  int shokVar;
 } ox_params;
 
 typedef struct {
@@ -106,7 +111,9 @@ int ox_update_solution(const ox_params* p, double* U, double* const flux[3],
                        ox_stats* st);
 
 /* Batched driver over independent blocks, OpenMP over blocks
- * (nthreads <= 0 uses the OpenMP default). */
+ * (nthreads <= 0 uses the OpenMP default).  With p->shockDetect each block
+ * first runs ox_shock_detect (Hydro.F90:94-110; the blocks have no guard
+ * fill, so their SHOK guard cells stay 0). */
 int ox_compute_fluxes_batch(const ox_params* p, double* U, long nblocks,
                             double dt, const double del[3], int nthreads,
                             ox_stats* st);
@@ -136,6 +143,17 @@ int ox_ppm(const ox_params* p, int dir, double dt, double delta,
 /* HLLC flux (hy_HLLC.F90:53-324): Fstar[0..5]; returns ierr. */
 int ox_hllc(const ox_params* p, int dir, const double VL[9], const double VR[9],
             double Fstar[6], double* speed);
+/* HLL flux (hy_HLL.F90:82-165): Fstar[0..5]; returns ierr. */
+int ox_hll(const ox_params* p, int dir, const double VL[9], const double VR[9],
+           double Fstar[6], double* speed);
+/* Shock flag of one block (hy_shockDetectBlk.F90:93-229, Balsara & Spicer
+ * 1999): SHOK_VAR := 0 over the guard-inclusive box, then 1 on interior
+ * cells where the undivided pressure jump reaches 0.5 of the local minimum
+ * pressure and the undivided velocity divergence is at most -0.1 of the
+ * local minimum sound speed.  Reads dens, pres, gamc and velocities on the
+ * interior and one guard layer.  Returns the number of flagged cells, or
+ * -1 when p->shokVar < 0. */
+long ox_shock_detect(const ox_params* p, double* U);
 /* Exact two-shock Riemann solver with the same contract. */
 int ox_exact_riemann(const ox_params* p, int dir, const double VL[9],
                      const double VR[9], double Fstar[6], double* speed,

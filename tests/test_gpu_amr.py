@@ -204,3 +204,50 @@ def test_quadrant_study_small_grid():
     for name in ("fp64_hllc", "fp64_exact", "mixed_exact_learned"):
         assert out[name]["code"] == 0 and out[name]["zone_updates_per_s"] > 0
     assert 0 < out["l1_mixed_learned_vs_fp64_exact"]["l1"]["dens"] <= 1e-6
+
+
+def test_amr_rejects_inconsistent_grids():
+    """Host checks return 3 before anything runs: a level with two block
+    shapes, a block outside its level, two blocks at one position, a fine
+    block with no coarse blocks around it."""
+    from flash5_amd.hydro import FxError
+    grid = _grid2d()
+    U = _state(grid, 31)
+    hp = HydroParams(ndim=2, nxb=16, nyb=16, nzb=1, gamma=GAMMA)
+    hy, tiles = _setup(grid, U, hp)
+    levels = [L + 1 for L, _ in grid.blocks]
+    pos = [p for _, p in grid.blocks]
+    bad_tiles = list(tiles)
+    bad_tiles[-1] = Tile(U=tiles[-1].U, deltas=tiles[-1].deltas, level=3, block_class=0)
+    with pytest.raises(FxError):           # level 3 given a second block shape
+        hy.amr_setup(bad_tiles, levels, pos, grid.ext, grid.periodic)
+    with pytest.raises(FxError):           # a block outside its level
+        hy.amr_setup(tiles, levels, pos[:-1] + [(99, 0, 0)], grid.ext, grid.periodic)
+    with pytest.raises(FxError):           # two blocks at one position
+        hy.amr_setup(tiles, levels, pos[:-1] + [pos[-2]], grid.ext, grid.periodic)
+    # a level-2 block over level-1 cells that no block covers: its register
+    # faces have no coarse side, so setup refuses the grid
+    lone = LevelGrid(2, grid.nb, grid.ext, list(grid.blocks) + [(2, (0, 12, 0))])
+    Ul = _state(lone, 33)
+    with pytest.raises(FxError) as e:
+        _setup(lone, Ul, hp)
+    assert e.value.status.code == 3
+    # the library is usable after the refusals
+    hy3, tiles3 = _setup(grid, U, hp)
+    assert hy3.amr_step(_dt(grid, U)).code == 0
+
+
+def test_block_class_checks():
+    from flash5_amd.hydro import FxError
+    hp = HydroParams(ndim=3, nxb=16, nyb=16, nzb=16, gamma=GAMMA)
+    hy = Hydro(hp)
+    with pytest.raises(FxError):
+        hy.set_block_classes([(12, 16, 16)])          # x extent not a multiple of 8
+    with pytest.raises(FxError):
+        hy.set_block_classes([(16, 16, 16)] * 9)      # more than FX_MAX_CLASSES
+    hy.set_block_classes([(16, 16, 16), (8, 8, 8)])
+    U = _state(_grid3d(), 35)
+    t = Tile(U=torch.from_numpy(U[0].copy()).cuda(), deltas=(1 / 16,) * 3, block_class=2)
+    with pytest.raises(FxError):                      # class out of range
+        hy.compute_fluxes([t], 1e-4)
+    hy.set_block_classes([])

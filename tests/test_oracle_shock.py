@@ -1,0 +1,441 @@
+"""Pinning tests for the oracle's HLL solver, shock detector and hybrid
+Riemann switch (CPU only).
+
+* HLL (hy_HLL.F90:82-165): the intermediate state U* built here from the
+  integral form of the conservation law must satisfy the Rankine-Hugoniot
+  conditions across both outer waves, F = F_L + S_L (U* - U_L) =
+  F_R + S_R (U* - U_R) (Toro, Riemann Solvers, 3rd ed., sec. 10.3); outside
+  the fan the upwind flux comes back bit for bit; consistency F(V,V) = F(V);
+  mirror symmetry; the pressure slot averages with zero state jump; at a
+  stationary contact HLL diffuses mass where HLLC does not.
+* Shock detector (hy_shockDetectBlk.F90:93-229): closed-form thresholds on
+  linear pressure and velocity fields (0.5 of the minimum pressure, -0.1 of
+  the minimum sound speed, undivided centred differences); a pressure ramp
+  alone or a compression alone flags nothing; axis permutations permute
+  the flags; only one guard layer is read; on a Sod tube the flags sit at
+  the shock and not in the rarefaction or at the contact.
+* Hybrid switch (hy_getFaceFlux.F90:353-367): with no flags the block
+  update is HLLC's bit for bit, with every flag HLL's; one flagged cell
+  changes exactly the faces that touch it; the batch driver detects first.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+G = 1.4
+RNG = np.random.default_rng(80808)
+
+
+def rand_prim(umax=1.0):
+    rho = np.exp(RNG.uniform(np.log(0.1), np.log(10.0)))
+    p = np.exp(RNG.uniform(np.log(0.1), np.log(10.0)))
+    v = RNG.uniform(-umax, umax, 3)
+    return O.prim(rho, *v, p, G)
+
+
+def phys(V, d, fP=1.0):
+    """Physical flux (6 slots, the last the face pressure) and conserved
+    state of a primitive vector, from the Euler equations."""
+    rho, vel, p, ge = V[0], V[1:4], V[4], V[6]
+    E = 0.5 * rho * vel @ vel + p / (ge - 1)
+    un = vel[d]
+    F = np.array([rho * un, rho * un * vel[0], rho * un * vel[1], rho * un * vel[2],
+                  (E + p) * un, p])
+    F[1 + d] += fP * p
+    return F, np.array([rho, *(rho * vel), E])
+
+
+def davis(VL, VR, d):
+    cl, cr = np.sqrt(G * VL[4] / VL[0]), np.sqrt(G * VR[4] / VR[0])
+    return min(VL[1 + d] - cl, VR[1 + d] - cr), max(VL[1 + d] + cl, VR[1 + d] + cr)
+
+
+# ---------------------------------------------------------------- HLL
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_hll_satisfies_rankine_hugoniot_across_both_waves(d):
+    p = O.params(3, 8, G)
+    regions = set()
+    for _ in range(400):
+        VL, VR = rand_prim(umax=2.5), rand_prim(umax=2.5)
+        SL, SR = davis(VL, VR, d)
+        FL, UL = phys(VL, d)
+        FR, UR = phys(VR, d)
+        rc, F, speed = O.hll(p, d, VL, VR)
+        assert rc == 0
+        assert speed == pytest.approx(max(abs(SL), abs(SR)), rel=1e-15)
+        if SL > 0:
+            regions.add(0)
+            np.testing.assert_allclose(F, FL, rtol=1e-15, atol=1e-15 * np.abs(FL).max())
+            continue
+        if SR < 0:
+            regions.add(2)
+            np.testing.assert_allclose(F, FR, rtol=1e-15, atol=1e-15 * np.abs(FR).max())
+            continue
+        regions.add(1)
+        # the state that conserves the integral over the fan [SL, SR]
+        Us = (SR * UR - SL * UL - FR[:5] + FL[:5]) / (SR - SL)
+        scale = np.abs(FL).max() + np.abs(FR).max()
+        np.testing.assert_allclose(F[:5], FL[:5] + SL * (Us - UL), atol=1e-13 * scale, rtol=0)
+        np.testing.assert_allclose(F[:5], FR[:5] + SR * (Us - UR), atol=1e-13 * scale, rtol=0)
+        # face pressure: the same average with no state jump
+        assert F[5] == pytest.approx((SR * VL[4] - SL * VR[4]) / (SR - SL), rel=1e-14)
+    assert regions == {0, 1, 2}
+
+
+def test_hll_upwind_regions_are_exact_copies():
+    p = O.params(3, 8, G)
+    for d in range(3):
+        VL, VR = O.prim(1.0, 0.0, 0.0, 0.0, 1.0, G), O.prim(0.5, 0.0, 0.0, 0.0, 0.7, G)
+        VL[1 + d] = VR[1 + d] = 5.0           # supersonic to the right
+        rc, F, _ = O.hll(p, d, VL, VR)
+        rcc, Fc, _ = O.hllc(p, d, VL, VR)
+        assert rc == rcc == 0
+        np.testing.assert_array_equal(F, Fc)  # both return the left physical flux
+        VL[1 + d] = VR[1 + d] = -5.0
+        np.testing.assert_array_equal(O.hll(p, d, VL, VR)[1], O.hllc(p, d, VL, VR)[1])
+
+
+def test_hll_consistency_and_mirror_symmetry():
+    p = O.params(3, 8, G)
+    for _ in range(100):
+        d = int(RNG.integers(3))
+        V = rand_prim()
+        _, F, _ = O.hll(p, d, V, V)
+        np.testing.assert_allclose(F, phys(V, d)[0], rtol=1e-14, atol=1e-14 * np.abs(F).max())
+        VL, VR = rand_prim(umax=1.5), rand_prim(umax=1.5)
+        flip = np.ones(9)
+        flip[1 + d] = -1
+        _, Fa, _ = O.hll(p, d, VL, VR)
+        _, Fb, _ = O.hll(p, d, VR * flip, VL * flip)
+        sgn = -np.ones(6)
+        sgn[1 + d] = 1
+        sgn[5] = 1
+        np.testing.assert_allclose(Fb, sgn * Fa, atol=1e-13 * np.abs(Fa).max())
+
+
+def test_hll_diffuses_a_stationary_contact_that_hllc_keeps():
+    p = O.params(3, 8, G)
+    VL, VR = O.prim(1.0, 0, 0, 0, 1.0, G), O.prim(0.25, 0, 0, 0, 1.0, G)
+    _, Fc, _ = O.hllc(p, 0, VL, VR)
+    _, Fh, _ = O.hll(p, 0, VL, VR)
+    assert Fc[0] == 0.0
+    SL, SR = davis(VL, VR, 0)
+    assert Fh[0] == pytest.approx(SR * SL * (0.25 - 1.0) / (SR - SL), rel=1e-14)
+    assert Fh[0] > 0.0                         # mass leaks toward the light side
+
+
+def test_hll_left_state_only_for_strictly_positive_sl():
+    """hy_HLL.F90:151 takes the left flux for SL > 0 only; at SL = 0 the fan
+    formula applies, and with SL = 0 it reduces to (SR F_L) / SR, whose
+    rounding can differ from F_L.  gamc = 2, rho = 1, p = 2 gives c_L = 2
+    exactly, so u_L = 2 puts SL at 0; HLLC (whose branch is SL >= 0) returns
+    the oracle's own F_L for the same states."""
+    p = O.params(3, 8, G)
+    r = np.random.default_rng(1)
+    SR = max(2.0 + 2.0, 3.0 + np.sqrt(2.0))
+    for _ in range(200):
+        VL = np.array([1.0, 2.0, *r.uniform(-1, 1, 2), 2.0, 2.0, 2.0, 2.0, 0.0])
+        VR = np.array([2.0, 3.0, *r.uniform(-1, 1, 2), 2.0, 2.0, 2.0, 1.0, 0.0])
+        FL = O.hllc(p, 0, VL, VR)[1]
+        fan = (SR * FL) / SR
+        if np.array_equal(fan, FL):
+            continue
+        _, F, speed = O.hll(p, 0, VL, VR)
+        assert speed == SR
+        np.testing.assert_array_equal(F, fan)
+        return
+    pytest.fail("no SL = 0 case separated the branches")
+
+
+def test_hll_vacuum_and_imaginary_sound_speed():
+    p = O.params(3, 8, G)
+    V = O.prim(1.0, 0.1, 0, 0, 1.0, G)
+    tiny = V.copy()
+    tiny[4] = 1e-40
+    rc, F, _ = O.hll(p, 0, V, tiny)
+    assert rc == 0 and not F.any()
+    bad = V.copy()
+    bad[5] = -G
+    assert O.hll(p, 0, V, bad)[0] == 1
+
+
+# ---------------------------------------------------------------- detector
+def shock_params(ndim, nb=8, **kw):
+    kw.setdefault("nunk", 11)
+    kw.setdefault("shokVar", O.SHOK)
+    return O.params(ndim, nb, G, **kw)
+
+
+def block(p, rho, vel, pres):
+    nunk, nz, ny, nx = O.block_shape(p)
+    U = np.zeros((nunk, nz, ny, nx))
+    U[O.DENS], U[O.PRES] = rho, pres
+    U[O.VELX], U[O.VELY], U[O.VELZ] = vel
+    O.eos(p, U, 1)
+    return U
+
+
+def cell_x(p):
+    """Cell indices along x (guard-inclusive), 0 at the first interior cell."""
+    return np.arange(p.nxb + 2 * p.ng) - p.ng
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_detector_needs_both_jump_and_compression(ndim):
+    p = shock_params(ndim)
+    _, nz, ny, nx = O.block_shape(p)
+    x = np.broadcast_to(cell_x(p), (nz, ny, nx)).astype(float)
+    zero = np.zeros((nz, ny, nx))
+    # a steep pressure ramp at rest: no compression
+    U = block(p, 1.0, (zero, zero, zero), 1.0 + 0.4 * (x + p.ng))
+    U[O.SHOK] = 7.0
+    assert O.shock_detect(p, U) == 0
+    assert not U[O.SHOK].any()                 # guard cells are cleared too
+    # a strong compression at uniform pressure: no jump
+    U = block(p, 1.0, (-0.5 * x, zero, zero), 1.0 + zero)
+    assert O.shock_detect(p, U) == 0
+    # without a SHOK slot the detector refuses
+    assert O.shock_detect(O.params(ndim, 8, G), block(O.params(ndim, 8, G), 1.0,
+                                                       (zero, zero, zero), 1.0 + zero)) == -1
+
+
+def test_detector_thresholds_are_inclusive():
+    """Both switches compare with >= (hy_shockDetectBlk.F90:175, 180).  At
+    the middle cell, pressures 2, 3, 4 give the centred jump 1 = 0.5 * 2
+    exactly, and velocities +a, -a around it give divv = -a with a set to
+    the double 0.1 * c_min that the detector itself forms."""
+    p = shock_params(1)
+    _, nz, ny, nx = O.block_shape(p)
+    g = p.ng
+    i = g + 3
+    pres = np.full((nz, ny, nx), 50.0)
+    pres[..., i - 1:i + 2] = (2.0, 3.0, 4.0)
+    rho = np.ones((nz, ny, nx))
+    c = np.sqrt(G * pres / rho)                 # gamc * pres / dens, as the detector forms it
+    cmin = c[0, 0, i - 1:i + 2].min()
+    a = 0.1 * cmin
+    vx = np.zeros((nz, ny, nx))
+    vx[..., i - 1], vx[..., i + 1] = a, -a
+    U = block(p, rho, (vx, 0 * vx, 0 * vx), pres)
+    assert np.array_equal(np.sqrt(U[O.GAMC] * U[O.PRES] / U[O.DENS]), c)
+    O.shock_detect(p, U)
+    assert U[O.SHOK][0, 0, i] == 1.0
+
+
+def _ramp_block(p, r, a, c0=1.5):
+    """p = r^x along x (x the undivided cell index); rho = gamma p / c0^2 so
+    that the sound speed is c0 everywhere; isotropic compression v = -a x_i
+    on every axis of the block."""
+    _, nz, ny, nx = O.block_shape(p)
+    g = p.ng
+    zz, yy, xx = np.meshgrid(np.arange(nz) - (g if p.ndim > 2 else 0),
+                             np.arange(ny) - (g if p.ndim > 1 else 0),
+                             np.arange(nx) - g, indexing="ij")
+    pres = r ** xx.astype(float)
+    zero = np.zeros_like(pres)
+    vel = [-a * xx, -a * yy if p.ndim > 1 else zero, -a * zz if p.ndim > 2 else zero]
+    return block(p, G * pres / c0 ** 2, vel, pres)
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+@pytest.mark.parametrize("above", [True, False])
+def test_detector_pressure_threshold_is_half_the_local_minimum(ndim, above):
+    """The undivided centred jump 0.5 (r^(x+1) - r^(x-1)) over the 3-cell
+    minimum r^(x-1) is 0.5 (r^2 - 1): every cell is flagged iff r >= sqrt(2)
+    (compression far above its threshold)."""
+    p = shock_params(ndim)
+    r = np.sqrt(2.0) * (1 + 1e-9 if above else 1 - 1e-9)
+    U = _ramp_block(p, r, a=2.0)
+    n = O.shock_detect(p, U)
+    assert n == (p.nxb ** ndim if above else 0)
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+@pytest.mark.parametrize("above", [True, False])
+def test_detector_compression_threshold_is_a_tenth_of_the_sound_speed(ndim, above):
+    """v = -a x_i on each of ndim axes gives the undivided divergence -ndim a;
+    with the sound speed c0 everywhere a cell is flagged iff ndim a >= 0.1 c0
+    (pressure jump far above its threshold)."""
+    p = shock_params(ndim)
+    c0 = 1.5
+    a = 0.1 * c0 / ndim * (1 + 1e-9 if above else 1 - 1e-9)
+    U = _ramp_block(p, 2.0, a, c0)
+    n = O.shock_detect(p, U)
+    assert n == (p.nxb ** ndim if above else 0)
+
+
+def test_detector_flags_permute_with_the_axes():
+    p = shock_params(3)
+    _, nz, ny, nx = O.block_shape(p)
+    r = np.random.default_rng(5)
+    rho = np.exp(r.normal(0, 0.3, (nz, ny, nx)))
+    pres = np.exp(r.normal(0, 0.8, (nz, ny, nx)))
+    vel = [r.normal(0, 1.0, (nz, ny, nx)) for _ in range(3)]
+    U = block(p, rho, vel, pres)
+    n0 = O.shock_detect(p, U)
+    assert 0 < n0 < p.nxb ** 3
+    base = U[O.SHOK].copy()
+    # array axes are (z, y, x); velocity components follow their axes
+    for perm in itertools.permutations(range(3)):
+        arr_perm = tuple(2 - perm[2 - a] for a in range(3))
+        t = lambda A: np.transpose(A, arr_perm)
+        vp = [None] * 3
+        for a in range(3):
+            vp[a] = t(vel[perm[a]])
+        V = block(p, t(rho), vp, t(pres))
+        assert O.shock_detect(p, V) == n0
+        np.testing.assert_array_equal(V[O.SHOK], t(base))
+
+
+def test_detector_reads_one_guard_layer():
+    p = shock_params(2, nb=8)
+    _, nz, ny, nx = O.block_shape(p)
+    r = np.random.default_rng(9)
+    U = block(p, np.exp(r.normal(0, 0.3, (nz, ny, nx))), [r.normal(0, 1, (nz, ny, nx))
+                                                          for _ in range(3)],
+              np.exp(r.normal(0, 0.8, (nz, ny, nx))))
+    O.shock_detect(p, U)
+    want = U[O.SHOK].copy()
+    V = U.copy()
+    g = p.ng
+    outer = np.ones((ny, nx), bool)
+    outer[g - 1:g + p.nyb + 1, g - 1:g + p.nxb + 1] = False
+    for v in (O.DENS, O.PRES, O.VELX, O.VELY, O.VELZ, O.GAMC):
+        V[v][0][outer] = 1e6 if v != O.GAMC else 0.5
+    O.shock_detect(p, V)
+    np.testing.assert_array_equal(V[O.SHOK], want)
+    # the first guard layer matters: a strong compression planted there
+    W = U.copy()
+    W[O.VELX][0, g:g + p.nyb, g - 1] = 50.0
+    W[O.PRES][0, g:g + p.nyb, g - 1] = 1e3
+    O.shock_detect(p, W)
+    assert W[O.SHOK][0, g:g + p.nyb, g].all()
+
+
+def _sod_1d(n=128, t_end=0.1):
+    """1-D Sod tube in one block with zero-gradient guard cells."""
+    p = shock_params(1, nb=n)
+    _, nz, ny, nx = O.block_shape(p)
+    xc = (cell_x(p) + 0.5) / n
+    left = xc < 0.5
+    z = np.zeros((1, 1, nx))
+    U = block(p, np.where(left, 1.0, 0.125)[None, None], (z, z, z),
+              np.where(left, 1.0, 0.1)[None, None])
+    dx, t = 1.0 / n, 0.0
+    g = p.ng
+    while t < t_end:
+        U[:, :, :, :g] = U[:, :, :, g:g + 1]
+        U[:, :, :, g + n:] = U[:, :, :, g + n - 1:g + n]
+        dt = min(O.block_dt(p, U, (dx,) * 3, 0.8), t_end - t)
+        rc, _, _, _ = O.compute_fluxes(p, U, dt, (dx,) * 3, update=True, want_flux=False)
+        assert rc == 0
+        t += dt
+    U[:, :, :, :g] = U[:, :, :, g:g + 1]
+    U[:, :, :, g + n:] = U[:, :, :, g + n - 1:g + n]
+    return p, U, xc, t
+
+
+def test_detector_on_sod_flags_the_shock_only():
+    p, U, xc, t = _sod_1d()
+    n = O.shock_detect(p, U)
+    g = p.ng
+    flags = U[O.SHOK][0, 0, g:g + p.nxb] > 0
+    x = xc[g:g + p.nxb]
+    dx = 1.0 / p.nxb
+    # Toro (2009) test 1: shock speed 1.75216, contact 0.92745, head of the
+    # rarefaction -1.18322, tail -0.07027
+    xs, xcon = 0.5 + 1.75216 * t, 0.5 + 0.92745 * t
+    assert n > 0
+    assert np.all(np.abs(x[flags] - xs) <= 3 * dx)
+    fan = (x > 0.5 - 1.18322 * t - 2 * dx) & (x < 0.5 - 0.07027 * t + 2 * dx)
+    assert not flags[fan].any()
+    assert not flags[np.abs(x - xcon) <= 3 * dx].any()
+
+
+# ---------------------------------------------------------------- hybrid
+def _random_block(p, seed):
+    _, nz, ny, nx = O.block_shape(p)
+    r = np.random.default_rng(seed)
+    sh = (nz, ny, nx)
+    return block(p, np.exp(r.normal(0, 0.2, sh)), [r.normal(0, 0.4, sh) for _ in range(3)],
+                 np.exp(r.normal(0, 0.3, sh)))
+
+
+def _fluxes(p, U, dt=2e-3):
+    V = U.copy()
+    rc, fl, pavg, _ = O.compute_fluxes(p, V, dt, (1 / p.nxb,) * 3, update=True, want_pavg=True)
+    assert rc == 0
+    return V, fl, pavg
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_hybrid_reduces_to_hllc_and_to_hll(ndim):
+    ph = shock_params(ndim, riemann=O.HYBR)
+    U = _random_block(ph, 100 + ndim)
+    U[O.SHOK] = 0.0
+    Vh, fh, _ = _fluxes(ph, U)
+    Vc, fc, _ = _fluxes(shock_params(ndim, riemann=O.HLLC), U)
+    np.testing.assert_array_equal(Vh, Vc)
+    for d in range(ndim):
+        np.testing.assert_array_equal(fh[d], fc[d])
+    U[O.SHOK] = 1.0
+    Vh, fh, _ = _fluxes(ph, U)
+    Vl, fl_, _ = _fluxes(shock_params(ndim, riemann=O.HLL), U)
+    np.testing.assert_array_equal(Vh, Vl)
+    for d in range(ndim):
+        np.testing.assert_array_equal(fh[d], fl_[d])
+    assert not np.array_equal(Vl, Vc)
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_one_flagged_cell_switches_exactly_its_faces(ndim):
+    ph = shock_params(ndim, riemann=O.HYBR)
+    U = _random_block(ph, 7 + ndim)
+    U[O.SHOK] = 0.0
+    _, fc, _ = _fluxes(ph, U)
+    _, fl_, _ = _fluxes(shock_params(ndim, riemann=O.HLL), U)
+    g = ph.ng
+    cell = [g + 2, g + 5, g + 1][:ndim]         # interior cell (x, y, z)
+    idx = (0,) * (3 - ndim) + tuple(reversed(cell))
+    U[O.SHOK][idx] = 1.0
+    _, fh, _ = _fluxes(ph, U)
+    for d in range(ndim):
+        want = fc[d].copy()
+        # faces of the flagged cell in direction d: its low face and high face
+        loc = [c - g for c in cell]
+        for off in (0, 1):
+            f = list(loc)
+            f[d] += off
+            fi = (slice(None),) + (0,) * (3 - ndim) + tuple(reversed(f))
+            want[fi] = fl_[d][fi]
+        np.testing.assert_array_equal(fh[d], want)
+    # a flag in a guard cell reaches the boundary face
+    U[O.SHOK] = 0.0
+    U[O.SHOK][(0,) * (3 - ndim) + tuple(reversed([g - 1] + [g + 3] * (ndim - 1)))] = 1.0
+    _, fh, _ = _fluxes(ph, U)
+    fi = (slice(None),) + (0,) * (3 - ndim) + tuple(reversed([0] + [3] * (ndim - 1)))
+    np.testing.assert_array_equal(fh[0][fi], fl_[0][fi])
+    fh[0][fi] = fc[0][fi]
+    np.testing.assert_array_equal(fh[0], fc[0])
+
+
+def test_batch_driver_detects_before_the_update():
+    ph = shock_params(2, nb=8, riemann=O.HYBR)
+    blocks = np.stack([_random_block(ph, s) for s in range(3)])
+    blocks[1][O.PRES][:, :, :8] *= 20.0       # a pressure wall...
+    blocks[1][O.VELX][:, :, :8] += 3.0        # ...running into the rest
+    O.eos(ph, blocks[1], 1)
+    want = blocks.copy()
+    for b in range(3):
+        assert O.shock_detect(ph, want[b]) >= 0
+        rc, _, _, _ = O.compute_fluxes(ph, want[b], 2e-3, (1 / 8,) * 3, update=True,
+                                       want_flux=False)
+        assert rc == 0
+    got = blocks.copy()
+    rc, _ = O.compute_fluxes_batch(ph, got, 2e-3, (1 / 8,) * 3)
+    assert rc == 0
+    np.testing.assert_array_equal(got, want)
+    assert want[1][O.SHOK].any()
