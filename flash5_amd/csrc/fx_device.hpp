@@ -45,6 +45,7 @@ struct DevParams {
   double gammam1, gam1inv, ggprodinv, abar;
   int exactMaxIter;
   double exactTol;
+  long shokOff;              // SHOK_VAR slot * ncell, or -1 without shock detection
   int bext[3][3];            // [dir][axis] extent of the dir-band of state cells
   long bcells[3];            // cells per dir-band
   long boff[3];              // offset of each band in a tile's state scratch
@@ -438,6 +439,57 @@ __device__ __forceinline__ int hllc(const DevParams& P, const R* Vm, const R* Vp
 #pragma unroll
   for (int f = 0; f < 5; ++f) F[f] = Fs[f] + S * (Us[f] - Uc[f]);
   F[5] = Fs[5];
+  return 0;
+}
+
+// ---- HLL (hy_HLL.F90:82-165) ----
+// conserved state (hy_prim2con.F90): dens, three momenta, total energy
+template <typename R>
+__device__ __forceinline__ void prim2con(const R* V, R Uc[5]) {
+  const R u2 = V[P_VELX] * V[P_VELX] + V[P_VELY] * V[P_VELY] + V[P_VELZ] * V[P_VELZ];
+  Uc[0] = V[P_DENS];
+  Uc[1] = V[P_DENS] * V[P_VELX];
+  Uc[2] = V[P_DENS] * V[P_VELY];
+  Uc[3] = V[P_DENS] * V[P_VELZ];
+  Uc[4] = R(0.5) * V[P_DENS] * u2 + V[P_PRES] / (V[P_GAME] - R(1.)) + R(0.5) * R(0.);
+}
+
+// returns 0 ok, 1 imaginary sound speed.  The fan average runs over all six
+// slots; the pressure slot's state jump is zero (hy_HLL.F90:146).
+template <int DIR, typename R>
+__device__ __forceinline__ int hll(const DevParams& P, const R* Vm, const R* Vp, R F[6]) {
+  const R tiny = R(P.tiny), fP = R(P.fP);
+  const R velNL = Vm[P_VELX + DIR], velNR = Vp[P_VELX + DIR];
+  const R aL2 = Vm[P_GAMC] * Vm[P_PRES] / Vm[P_DENS];
+  const R aR2 = Vp[P_GAMC] * Vp[P_PRES] / Vp[P_DENS];
+  if ((Vm[P_DENS] < tiny && Vm[P_DENS] > R(0.)) || (Vp[P_DENS] < tiny && Vp[P_DENS] > R(0.)) ||
+      (Vm[P_PRES] < tiny && Vm[P_PRES] > R(0.)) || (Vp[P_PRES] < tiny && Vp[P_PRES] > R(0.))) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) F[f] = R(0.);
+    return 0;
+  }
+  if (aL2 < R(0.) || aR2 < R(0.)) return 1;
+  const R cfL = sqrt(aL2), cfR = sqrt(aR2);
+  const R SL = fmin(velNL - cfL, velNR - cfR);
+  const R SR = fmax(velNL + cfL, velNR + cfR);
+  if (SL > R(0.)) {
+    prim2flx<DIR>(Vm, fP, F);
+    return 0;
+  }
+  if (SR < R(0.)) {
+    prim2flx<DIR>(Vp, fP, F);
+    return 0;
+  }
+  R FL[6], FR[6], UL[5], UR[5];
+  prim2flx<DIR>(Vm, fP, FL);
+  prim2flx<DIR>(Vp, fP, FR);
+  prim2con(Vm, UL);
+  prim2con(Vp, UR);
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const R dU = f < 5 ? UR[f] - UL[f] : R(0.0) - R(0.0);
+    F[f] = (SR * FL[f] - SL * FR[f] + SR * SL * dU) / (SR - SL);
+  }
   return 0;
 }
 

@@ -43,6 +43,7 @@ struct DevStatus {
   unsigned long long fallbacks;
   unsigned long long newtonIters;
   unsigned long long newtonFallbacks;
+  unsigned long long shockTiles;  // tiles with a flagged cell (k_shock_detect)
 };
 
 struct Launch {
@@ -354,21 +355,30 @@ __device__ __forceinline__ void aux_fluxes(const DevParams& P, R f0, const R* VL
   }
 }
 
-// Face flux densities F[0..7] (hy_getFaceFlux.F90:324-442): HLLC on the
-// face states VL, VR in the reconstruction type R, then the aux
+// Whether the face between cells cl and cr takes HLL: RiemannSolver = HLL,
+// or HYBR with a flagged cell on either side (hy_getFaceFlux.F90:353-367).
+__device__ __forceinline__ bool hll_face(const DevParams& P, const double* __restrict__ U, long cl,
+                                         long cr) {
+  if (P.riemann == FX_HLL) return true;
+  if (P.riemann != FX_HYBR) return false;
+  return U[P.shokOff + cl] + U[P.shokOff + cr] > 0.;
+}
+
+// Face flux densities F[0..7] (hy_getFaceFlux.F90:324-442): HLL (useHll) or
+// HLLC on the face states VL, VR in the reconstruction type R, then the aux
 // internal-energy fluxes.  The exact solver takes exact_face_flux.
 template <int D, int RS, typename R>
-__device__ __forceinline__ int face_flux(const DevParams& P, const R* VL, const R* VR, double F[8],
-                                         ExactStats& es) {
-  static_assert(RS == FX_HLLC, "face_flux serves HLLC");
+__device__ __forceinline__ int face_flux(const DevParams& P, const R* VL, const R* VR, bool useHll,
+                                         double F[8], ExactStats& es) {
+  static_assert(RS == FX_HLLC, "face_flux serves HLL, HLLC and the hybrid switch");
   R f0;
   if constexpr (sizeof(R) == sizeof(double)) {
-    const int ierr = hllc<D>(P, VL, VR, F);
+    const int ierr = useHll ? hll<D>(P, VL, VR, F) : hllc<D>(P, VL, VR, F);
     if (ierr) return ierr;
     f0 = F[0];
   } else {
     R Fr[6];
-    const int ierr = hllc<D>(P, VL, VR, Fr);
+    const int ierr = useHll ? hll<D>(P, VL, VR, Fr) : hllc<D>(P, VL, VR, Fr);
     if (ierr) return ierr;
 #pragma unroll
     for (int q = 0; q < 6; ++q) F[q] = double(Fr[q]);
@@ -435,9 +445,10 @@ k_fluxes(DevParams P, Launch L) {
       VL[P_GAME] = U[P.voff[FX_GAME] + cl];
       VR[P_GAMC] = U[P.voff[FX_GAMC] + cr];
       VR[P_GAME] = U[P.voff[FX_GAME] + cr];
-      if (d == 0) ierr = face_flux<0, RS>(P, VL, VR, F, es);
-      else if (d == 1) ierr = face_flux<1, RS>(P, VL, VR, F, es);
-      else ierr = face_flux<2, RS>(P, VL, VR, F, es);
+      const bool useHll = hll_face(P, U, cl, cr);
+      if (d == 0) ierr = face_flux<0, RS>(P, VL, VR, useHll, F, es);
+      else if (d == 1) ierr = face_flux<1, RS>(P, VL, VR, useHll, F, es);
+      else ierr = face_flux<2, RS>(P, VL, VR, useHll, F, es);
     }
     if (ierr) {
       report_error(L.status, 2, t, i, j, k, d);
@@ -449,6 +460,63 @@ k_fluxes(DevParams P, Launch L) {
     for (int q = 0; q < FX_NFLUX; ++q) Fo[q * nf + r] = F[q];
   }
   flush_exact(L.status, es);
+}
+
+// sound speed of cell c as hy_shockDetectBlk.F90:135 forms it
+__device__ __forceinline__ double shock_sound(const DevParams& P, const double* __restrict__ U,
+                                              long c) {
+  return sqrt(U[P.voff[FX_GAMC] + c] * U[P.voff[FX_PRES] + c] / U[P.voff[FX_DENS] + c]);
+}
+
+// hy_shockDetectBlk.F90:93-229, one workgroup per tile over its
+// guard-inclusive box: SHOK_VAR := 0 on guard cells and the Balsara-Spicer
+// flag on interior cells (undivided pressure jump >= 0.5 of the 3^d minimum
+// pressure and undivided velocity divergence <= -0.1 of the 3^d minimum
+// sound speed).  Reads the interior and the first guard layer only; the
+// SHOK_VAR slot is distinct from the slots read.  cls == nullptr: every tile
+// has the parameters P0.
+__global__ void __launch_bounds__(kThreads) k_shock_detect(Launch L, DevParams P0,
+                                                           const DevParams* __restrict__ cls,
+                                                           const unsigned char* __restrict__ tcls) {
+  const int t = blockIdx.x;
+  const DevParams& P = cls ? cls[tcls[t]] : P0;
+  double* __restrict__ U = L.tiles[t].U;
+  const int k2 = P.ndim > 1, k3 = P.ndim > 2;
+  const long sy = P.nx, sz = (long)P.nx * P.ny;
+  const long pr = P.voff[FX_PRES];
+  int any = 0;
+  for (long c = threadIdx.x; c < P.ncell; c += blockDim.x) {
+    const int i = (int)(c % P.nx), j = (int)((c / P.nx) % P.ny), k = (int)(c / sz);
+    double flag = 0.;
+    if (i >= P.lo[0] && i <= P.hi[0] && j >= P.lo[1] && j <= P.hi[1] && k >= P.lo[2] &&
+        k <= P.hi[2]) {
+      double minP = U[pr + c], minC = shock_sound(P, U, c);
+      for (int c3 = -k3; c3 <= k3; ++c3)
+        for (int c2 = -k2; c2 <= k2; ++c2)
+          for (int c1 = -1; c1 <= 1; ++c1) {
+            const long q = c + c1 + c2 * sy + c3 * sz;
+            minP = fmin(minP, U[pr + q]);
+            minC = fmin(minC, shock_sound(P, U, q));
+          }
+      const long vx = P.voff[FX_VELX], vy = P.voff[FX_VELY], vz = P.voff[FX_VELZ];
+      double divv = U[vx + c + 1] - U[vx + c - 1];
+      if (k2) divv = divv + U[vy + c + sy] - U[vy + c - sy];
+      if (k3) divv = divv + U[vz + c + sz] - U[vz + c - sz];
+      divv = 0.5 * divv;
+      const double gradPx = 0.5 * (U[pr + c + 1] - U[pr + c - 1]);
+      double gradPy = 0., gradPz = 0.;
+      if (k2) gradPy = 0.5 * (U[pr + c + sy] - U[pr + c - sy]);
+      if (k3) gradPz = 0.5 * (U[pr + c + sz] - U[pr + c - sz]);
+      const bool sw1 = fabs(gradPx) + fabs(gradPy) + fabs(gradPz) >= 0.5 * minP;
+      const bool sw2 = -(0.1 * minC) >= divv;
+      if (sw1 && sw2) {
+        flag = 1.;
+        any = 1;
+      }
+    }
+    U[P.shokOff + c] = flag;
+  }
+  if (__syncthreads_or(any) && threadIdx.x == 0) atomicAdd(&L.status->shockTiles, 1ULL);
 }
 
 __device__ __forceinline__ long face_index(const DevParams& P, int d, int i, int j, int k) {
@@ -646,6 +714,7 @@ __global__ void k_reset_status(DevStatus* s) {
   s->fallbacks = 0;
   s->newtonIters = 0;
   s->newtonFallbacks = 0;
+  s->shockTiles = 0;
 }
 
 // ---------------------------------------------------------------------
@@ -915,6 +984,11 @@ int grow(T** ptr, size_t* cap, size_t need, fx_status* st) {
   return 0;
 }
 
+// Hydro_init.F90:117-124, 387-395: HYBR and shockLowerCFL turn detection on
+bool shock_on(const fx_params& p) {
+  return p.shockDetect || p.shockLowerCFL || p.riemann == FX_HYBR;
+}
+
 DevParams make_dev_params(const fx_params& p) {
   DevParams d{};
   d.ndim = p.ndim;
@@ -958,6 +1032,7 @@ DevParams make_dev_params(const fx_params& p) {
   d.abar = p.singleSpeciesA;
   d.exactMaxIter = p.exactMaxIter;
   d.exactTol = p.exactTol;
+  d.shokOff = shock_on(p) ? (long)p.shokVar * d.ncell : -1;
   const int n[3] = {d.nxb, d.nyb, d.nzb};
   long off = 0;
   for (int dir = 0; dir < 3; ++dir) {
@@ -1014,9 +1089,16 @@ int validate(const fx_params* p) {
   for (int v = 0; v < FX_NVAR_USED; ++v)
     if (p->var[v] < 0 || p->var[v] >= p->nunk) return 3;
   if (!(p->gamma > 1.0)) return 3;
-  if (p->riemann != FX_HLLC && p->riemann != FX_EXACT) return 3;  // HLLC or the exact solver
+  if (p->riemann != FX_HLL && p->riemann != FX_HLLC && p->riemann != FX_HYBR &&
+      p->riemann != FX_EXACT)
+    return 3;
   if (p->precision != FX_PREC_FP64 && p->precision != FX_PREC_MIXED) return 3;
   if (p->learnedSeed && p->riemann != FX_EXACT) return 3;
+  if (shock_on(*p)) {  // SHOK_VAR: its own slot of U (hy_shockDetectBlk.F90:119-122)
+    if (p->shokVar < 0 || p->shokVar >= p->nunk) return 3;
+    for (int v = 0; v < FX_NVAR_USED; ++v)
+      if (p->var[v] == p->shokVar) return 3;
+  }
   return 0;
 }
 
@@ -1035,8 +1117,22 @@ int finish(fx_status* st) {
     st->fallbacks = (long long)h.fallbacks;
     st->newton_iters = (long long)h.newtonIters;
     st->newton_fallbacks = (long long)h.newtonFallbacks;
+    // hy_shockDetectBlk.F90:211-227: the CFL cap of a step with a flagged cell
+    if (g.prm.shockLowerCFL && h.shockTiles)
+      st->min_cell_cfl = g.prm.ndim == 1 ? 0.60 : (g.prm.ndim == 2 ? 0.45 : 0.25);
   }
   return h.code;
+}
+
+// k_shock_detect over the uploaded tiles, one workgroup each
+void launch_shock_detect(int nTiles) {
+  Launch L{g.dtiles, nTiles, nullptr, g.status, 0.0, nullptr};
+  if (g.multi)
+    hipLaunchKernelGGL(k_shock_detect, dim3(nTiles), dim3(kThreads), 0, g.stream, L, g.cur, g.dcls,
+                       g.dtcls);
+  else
+    hipLaunchKernelGGL(k_shock_detect, dim3(nTiles), dim3(kThreads), 0, g.stream, L, g.cur,
+                       (const DevParams*)nullptr, (const unsigned char*)nullptr);
 }
 
 // parameters of tile t of the uploaded list
@@ -1850,6 +1946,10 @@ int amr_launch_dt(double cfl, fx_status* st) {
 int amr_launch_step(double dt) {
   hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
   amr_launch_fill();
+  if (g.dp.shokOff >= 0) {  // Hydro.F90:94-123: flag, then fill again (SHOK_VAR included)
+    launch_shock_detect(g.amr.nBlocks);
+    amr_launch_fill();
+  }
   if (launch_flux_stages(g.amr.nBlocks, dt, g.amr.TX, g.amr.TY)) return 4;
   amr_launch_register();
   launch_update(g.amr.nBlocks, dt, nullptr);
@@ -1935,6 +2035,37 @@ int fx_hydro_advance_batch(double* U, long nBlocks, long blockStride, const doub
   const int rc = prepare_batch(U, nBlocks, blockStride, del, st);
   if (rc) return rc;
   return run_compute(nullptr, (int)nBlocks, dt, nullptr, st, true);
+}
+
+int fx_hydro_shock_detect_batch(double* U, long nBlocks, long blockStride, const double del[3],
+                                fx_status* st) {
+  if (st) memset(st, 0, sizeof *st);
+  if (!g.init || g.dp.shokOff < 0 || !del) {
+    if (st) st->code = 3;
+    return 3;
+  }
+  const int rc = prepare_batch(U, nBlocks, blockStride, del, st);
+  if (rc) return rc;
+  if (nBlocks == 0) return 0;
+  hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
+  launch_shock_detect((int)nBlocks);
+  HIPCALL(hipGetLastError(), st);
+  return finish(st);
+}
+
+int fx_hydro_shock_detect(const fx_tile* tiles, int nTiles, fx_status* st) {
+  if (st) memset(st, 0, sizeof *st);
+  if (!g.init || !tiles || nTiles < 0 || g.dp.shokOff < 0) {
+    if (st) st->code = 3;
+    return 3;
+  }
+  if (nTiles == 0) return 0;
+  const int rc = upload_tiles(tiles, nTiles, true, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
+  launch_shock_detect(nTiles);
+  HIPCALL(hipGetLastError(), st);
+  return finish(st);
 }
 
 
@@ -2098,6 +2229,10 @@ int fx_grid_run_graph(double* U, long nBlocks, long blockStride, const int* nbr,
   if (hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) err = 4;
   for (int s = 0; !err && s < stepsPerGraph; ++s) {
     launch_fill(U, nBlocks, blockStride);
+    if (P.shokOff >= 0) {  // Hydro.F90:94-123: flag, then fill again (SHOK_VAR included)
+      launch_shock_detect((int)nBlocks);
+      launch_fill(U, nBlocks, blockStride);
+    }
     hipLaunchKernelGGL(k_dt, dim3(dtGrid), dim3(kThreads), 0, g.stream, P, U, nBlocks, blockStride,
                        1. / del[0], 1. / del[1], 1. / del[2], g.partial);
     hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kThreads), 0, g.stream, g.partial, dtGrid, cfl,

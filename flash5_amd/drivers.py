@@ -147,6 +147,8 @@ class UniformGrid:
         self.cfl = cfl
         self.t = 0.0
         self.nsteps = 0
+        self.detect = bool(params.shockDetect or params.shockLowerCFL or
+                           params.RiemannSolver.lower() == "hybrid")
 
     def fill_guards(self) -> None:
         import ctypes as C
@@ -159,8 +161,13 @@ class UniformGrid:
             raise FxError("fx_grid_fill_guards", st, rc)
 
     def step(self, dt_max: float = math.inf) -> float:
-        """Guard fill, CFL dt (capped by dt_max), unsplit update in place."""
+        """Guard fill, CFL dt (capped by dt_max), unsplit update in place.
+        With shock detection on: fill, flag, fill again (SHOK_VAR's guard
+        cells too), as Hydro.F90:94-123 orders them."""
         self.fill_guards()
+        if self.detect:
+            self.hydro.shock_detect_batch(self.U, self.deltas)
+            self.fill_guards()
         dt = min(self.hydro.compute_dt(self.U, self.deltas, self.cfl), dt_max)
         self.hydro.advance_batch(self.U, dt, self.deltas)
         self.t += dt

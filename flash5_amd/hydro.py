@@ -30,12 +30,12 @@ NVAR_USED = 10
 NFLUX = 8
 DENS, VELX, VELY, VELZ, PRES, ENER, GAMC, GAME, TEMP, EINT = range(10)
 MINMOD, MC, HYBRID, VANLEER, LIMITED = 1, 2, 3, 4, 5
-ROE, HLL, HLLC, EXACT = 1, 3, 4, 20
+ROE, HLL, HLLC, HYBR, EXACT = 1, 3, 4, 8, 20
 PREC_FP64, PREC_FP64_FAST, PREC_MIXED = 0, 1, 2
 
 _LIMITERS = {"minmod": MINMOD, "mc": MC, "hybrid": HYBRID, "vanleer": VANLEER,
              "limited": LIMITED}
-_SOLVERS = {"roe": ROE, "hll": HLL, "hllc": HLLC, "exact": EXACT}
+_SOLVERS = {"roe": ROE, "hll": HLL, "hllc": HLLC, "hybrid": HYBR, "exact": EXACT}
 
 
 class FxParams(C.Structure):
@@ -50,6 +50,7 @@ class FxParams(C.Structure):
         ("gasConstant", C.c_double), ("singleSpeciesA", C.c_double),
         ("exactMaxIter", C.c_int), ("exactTol", C.c_double),
         ("precision", C.c_int), ("learnedSeed", C.c_int),
+        ("shockDetect", C.c_int), ("shockLowerCFL", C.c_int), ("shokVar", C.c_int),
     ]
 
 
@@ -94,6 +95,9 @@ EXPORTS = {
                                   C.POINTER(FxStatus)], C.c_int),
     "fx_hydro_advance_batch": ([_DP, C.c_long, C.c_long, C.c_double * 3, C.c_double,
                                 C.POINTER(FxStatus)], C.c_int),
+    "fx_hydro_shock_detect": ([C.POINTER(FxTile), C.c_int, C.POINTER(FxStatus)], C.c_int),
+    "fx_hydro_shock_detect_batch": ([_DP, C.c_long, C.c_long, C.c_double * 3,
+                                     C.POINTER(FxStatus)], C.c_int),
     "fx_hydro_compute_dt": ([_DP, C.c_long, C.c_long, C.c_double * 3, C.c_double, _DP], C.c_int),
     "fx_fluxreg_put": ([C.POINTER(FxTile), C.c_int, C.POINTER(FxFluxRegFace), C.c_int, _DP,
                         C.POINTER(FxStatus)], C.c_int),
@@ -201,6 +205,9 @@ class HydroParams:
     exactTol: float = 1.e-12
     precision: int = PREC_FP64
     learnedSeed: bool = False
+    shockDetect: bool = False
+    shockLowerCFL: bool = False
+    shokVar: int = -1            # slot of SHOK_VAR in U's var axis (-1: none)
 
     def to_c(self) -> FxParams:
         p = FxParams()
@@ -225,6 +232,8 @@ class HydroParams:
         p.gasConstant, p.singleSpeciesA = self.gasConstant, self.eos_singleSpeciesA
         p.exactMaxIter, p.exactTol = self.exactMaxIter, self.exactTol
         p.precision, p.learnedSeed = self.precision, int(self.learnedSeed)
+        p.shockDetect, p.shockLowerCFL = int(self.shockDetect), int(self.shockLowerCFL)
+        p.shokVar = self.shokVar
         return p
 
     def block_shape(self):
@@ -508,6 +517,31 @@ class Hydro:
         if rc:
             raise FxError("fx_hydro_face_states", rc=rc)
         return S
+
+    def shock_detect(self, tiles: Sequence[Tile]) -> FxStatus:
+        """hy_shockDetect over the tiles: writes SHOK_VAR in place; the
+        status's min_cell_cfl is the CFL cap under shockLowerCFL."""
+        self._reinit()
+        arr = self._tiles(tiles)
+        st = FxStatus()
+        rc = self.lib.fx_hydro_shock_detect(arr, len(tiles), C.byref(st))
+        self.last_status = st
+        if rc:
+            raise FxError("fx_hydro_shock_detect", st, rc)
+        return st
+
+    def shock_detect_batch(self, U, deltas: Sequence[float]) -> FxStatus:
+        """shock_detect over a (nblocks, nunk, nz, ny, nx) batch."""
+        self._reinit()
+        nb = U.shape[0]
+        st = FxStatus()
+        rc = self.lib.fx_hydro_shock_detect_batch(_ptr(U), nb, U[0].numel() if nb else 0,
+                                                  (C.c_double * 3)(*[float(x) for x in deltas]),
+                                                  C.byref(st))
+        self.last_status = st
+        if rc:
+            raise FxError("fx_hydro_shock_detect_batch", st, rc)
+        return st
 
     def advance_batch(self, U, dt: float, deltas: Sequence[float]) -> FxStatus:
         """In-place step of a (nblocks, nunk, nz, ny, nx) batch of independent blocks."""

@@ -47,8 +47,10 @@ enum { FX_NFLUX = 8 };
 
 /* slope limiters (UHD.h:15-19) */
 enum { FX_MINMOD = 1, FX_MC = 2, FX_HYBRID = 3, FX_VANLEER = 4, FX_LIMITED = 5 };
-/* Riemann solvers: HLLC (UHD.h:369); FX_EXACT = exact iterative solver */
-enum { FX_ROE = 1, FX_HLL = 3, FX_HLLC = 4, FX_EXACT = 20 };
+/* Riemann solvers numbered as UHD.h:59-66: FX_HLL, FX_HLLC and FX_HYBR (HLL
+ * at faces next to a flagged shock cell, HLLC elsewhere,
+ * hy_getFaceFlux.F90:353-367); FX_EXACT = exact iterative solver */
+enum { FX_ROE = 1, FX_HLL = 3, FX_HLLC = 4, FX_HYBR = 8, FX_EXACT = 20 };
 /* precision modes */
 enum { FX_PREC_FP64 = 0,      /* plain FP64, IEEE ops without contraction */
        FX_PREC_FP64_FAST = 1, /* reserved (rejected by fx_hydro_init) */
@@ -72,7 +74,7 @@ typedef struct {
   int steepening;           /* use_steepening */
   int use3dFullCTU;         /* use_3dFullCTU */
   int useAuxEint;           /* use_auxEintEqn */
-  int riemann;              /* RiemannSolver: FX_HLLC or FX_EXACT */
+  int riemann;              /* RiemannSolver: FX_HLL, FX_HLLC, FX_HYBR, FX_EXACT */
   double fP;                /* hy_fPresInMomFlux */
   double smlrho, smallp, smallE, eintSwitch;
   double tiny;              /* hy_tiny */
@@ -82,6 +84,12 @@ typedef struct {
   double exactTol;          /* relative p* convergence tolerance */
   int precision;            /* FX_PREC_* */
   int learnedSeed;          /* 1: seed exact-solver Newton from the MLP */
+  /* shock detection (Hydro_init.F90:117-124, 387-395) */
+  int shockDetect;          /* shockDetect; on whenever riemann = FX_HYBR */
+  int shockLowerCFL;        /* shockLowerCFL; turns shockDetect on */
+  int shokVar;              /* 0-based slot of SHOK_VAR in U's var axis (the
+                               unsplit Config's "shok"); read only while
+                               shock detection is on */
 } fx_params;
 
 /* One tile as the reference's tile iterator yields it (Grid_tile.F90). */
@@ -104,7 +112,11 @@ typedef struct {
   long long fallbacks;      /* cells reset to first-order states */
   long long newton_iters;   /* exact solver: total Newton iterations */
   long long newton_fallbacks; /* exact solver: learned seeds that failed */
-  double min_cell_cfl;      /* reserved (hy_cfl side effect) */
+  double min_cell_cfl;      /* fx_hydro_shock_detect: the CFL cap 0.60,
+                               0.45, 0.25 (NDIM 1, 2, 3) when shockLowerCFL
+                               is on and a cell was flagged (the reference
+                               sets hy_cfl = min(hy_cfl, cap),
+                               hy_shockDetectBlk.F90:211-227); else 0 */
 } fx_status;
 
 /* Hydro_init: set knobs, allocate nothing (scratch grows on first use). */
@@ -125,6 +137,22 @@ int fx_hydro_compute_fluxes(const fx_tile* tiles, int nTiles, double dt,
  * pavg, then energy fix and EOS. */
 int fx_hydro_update_solution(const fx_tile* tiles, int nTiles, double dt,
                              double dtOld, fx_status* st);
+
+/* hy_shockDetect (hy_shockDetect.F90, hy_shockDetectBlk.F90:93-229) over
+ * the tiles: SHOK_VAR := 0 over each guard-inclusive box, then 1 on the
+ * interior cells where the undivided pressure jump reaches 0.5 of the 3^d
+ * minimum pressure and the undivided velocity divergence is at most -0.1 of
+ * the 3^d minimum sound speed.  Reads dens, pres, gamc and the velocities on
+ * the interior and the first guard layer, which the host has filled (the
+ * reference's first guard fill, Hydro.F90:102-104).  The host then fills the
+ * guard cells (SHOK_VAR included) before fx_hydro_compute_fluxes.  Returns 3
+ * when shock detection is off. */
+int fx_hydro_shock_detect(const fx_tile* tiles, int nTiles, fx_status* st);
+
+/* fx_hydro_shock_detect over a strided batch (block b's U starts at
+ * U + b*blockStride doubles), no host-side tile list. */
+int fx_hydro_shock_detect_batch(double* U, long nBlocks, long blockStride,
+                                const double del[3], fx_status* st);
 
 /* Strided batch of independent blocks: block b's U starts at
  * U + b*blockStride (doubles).  Equivalent to fx_hydro_compute_fluxes with

@@ -295,10 +295,18 @@ def level_params(grid, L, gamma, **kw):
 
 def step(grid, U, dt, gamma, **kw):
     """One step of every level: guard fill, per-block fluxes, flux register
-    (finest pair first), per-block update, restriction."""
-    fill_guards(grid, U, gamma)
+    (finest pair first), per-block update, restriction.  With shock
+    detection (kw shockDetect, or riemann = HYBR): fill, flag every block,
+    fill again so that SHOK_VAR's guard cells hold their neighbours' flags
+    (Hydro.F90:94-123)."""
+    nunk = kw.get("nunk", O.NVAR_USED)
+    fill_guards(grid, U, gamma, nunk)
     nlev = len(grid["nb"])
     params = [level_params(grid, L, gamma, **kw) for L in range(nlev)]
+    if kw.get("shockDetect") or kw.get("riemann") == O.HYBR:
+        for b, (L, _) in enumerate(grid["blocks"]):
+            assert O.shock_detect(params[L], U[b]) >= 0
+        fill_guards(grid, U, gamma, nunk)
     fluxes, pavgs = [], []
     for b, (L, _) in enumerate(grid["blocks"]):
         rc, fl, pavg, _ = O.compute_fluxes(params[L], U[b], dt, grid["del"][L], update=False,
@@ -313,4 +321,4 @@ def step(grid, U, dt, gamma, **kw):
             O.fluxreg_conserve(params[L], fluxes, faces, reg)
     for b, (L, _) in enumerate(grid["blocks"]):
         assert O.update_solution(params[L], U[b], fluxes[b], pavgs[b], dt, grid["del"][L]) == 0
-    restrict(grid, U)
+    restrict(grid, U, nunk)
