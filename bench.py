@@ -147,14 +147,19 @@ def mixed_learned_leg(U0, dt, dx, gamma, steps: int, warmup: int, plain_value: f
             "autotune": [t for t in tune if t["chosen"]]}
 
 
-def cpu_baseline(U_gpu, dt, dx, target_s: float = 12.0):
+def cpu_baseline(U_gpu, dt, dx, riemann: str = "hllc", target_s: float = 12.0):
     """Time the FP64 oracle (OpenMP over blocks) on a bounded sample of the
     batch: as many whole rounds of the sample as fit in about target_s, the
-    median per-round rate reported."""
+    median per-round rate reported.  The hybrid solver's rounds include the
+    oracle's shock detection."""
     import numpy as np
     import oracle as O
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
-    p = O.params(3, 16, 5.0 / 3.0)
+    if riemann in ("hll", "hybrid"):
+        p = O.params(3, 16, 5.0 / 3.0, nunk=11, shokVar=O.SHOK,
+                     riemann={"hll": O.HLL, "hybrid": O.HYBR}[riemann])
+    else:
+        p = O.params(3, 16, 5.0 / 3.0)
     nb = nthreads                       # calibration: one block per thread
     U = U_gpu[:nb].cpu().numpy().copy()
     t0 = time.perf_counter()
@@ -186,8 +191,11 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--blocks", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--riemann", choices=["hllc", "exact"], default="hllc",
-                    help="face flux solver (HLLC, or the exact iterative solver)")
+    ap.add_argument("--riemann", choices=["hllc", "hll", "hybrid", "exact"], default="hllc",
+                    help="face flux solver: HLLC, HLL, the hybrid switch (HLL next to "
+                         "cells the shock detector flags, HLLC elsewhere; the blocks carry a "
+                         "shock variable and each step flags them first), or the exact "
+                         "iterative solver")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="fp64, or FP32 reconstruction/fluxes with the FP64 update")
     ap.add_argument("--learned", action="store_true",
@@ -217,7 +225,12 @@ def main() -> None:
     hp = HydroParams(ndim=3, nxb=n, nyb=n, nzb=n, gamma=gamma, RiemannSolver=args.riemann,
                      precision=prec)
     U = W.fourier_blocks(args.blocks, n=n, ng=g, gamma=gamma, device=dev)
-    dt = W.cfl_dt(U, 3, g, (dx,) * 3, 0.8)
+    cfl = 0.5 if args.riemann == "hll" else 0.8
+    dt = W.cfl_dt(U, 3, g, (dx,) * 3, cfl)
+    hybrid = args.riemann == "hybrid"
+    if args.riemann in ("hll", "hybrid"):      # a SHOK_VAR slot after the ten hydro variables
+        hp.nunk, hp.shokVar = 11, 10
+        U = torch.cat([U, torch.zeros_like(U[:, :1])], dim=1).contiguous()
     default_run = (args.precision == "fp64" and args.riemann == "hllc" and not args.learned)
     U0 = U.clone() if default_run and not args.no_mixed_leg else None
     train_info = None
@@ -236,19 +249,32 @@ def main() -> None:
     if not args.no_autotune:   # untimed: candidate launch shapes timed on this batch
         tune = hy.autotune_batch(U, dt, (dx,) * 3, reps=3)
     for _ in range(args.warmup):
+        if hybrid:
+            hy.shock_detect_batch(U, (dx,) * 3)
         hy.advance_batch(U, dt, (dx,) * 3)
     torch.cuda.synchronize()
 
-    # timed region: exactly K steps
+    # timed region: exactly K steps (with the hybrid solver, each step flags
+    # shocks first, as Hydro does)
     hy.set_timing(True)
     stage_sum = [0.0, 0.0, 0.0]
+    detect_ms = 0.0
+    d0 = torch.cuda.Event(enable_timing=True)
+    d1 = torch.cuda.Event(enable_timing=True)
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     start.record(stream)
     for _ in range(args.steps):
+        if hybrid:
+            d0.record(stream)
+            hy.shock_detect_batch(U, (dx,) * 3)
+            d1.record(stream)
         hy.advance_batch(U, dt, (dx,) * 3)
+        if hybrid:
+            d1.synchronize()
+            detect_ms += d0.elapsed_time(d1)
         st = hy.stage_times()
         for k in range(3):
             stage_sum[k] += st[k]
@@ -268,6 +294,7 @@ def main() -> None:
     fallbacks = int(hy.last_status.fallbacks)
     workload = (f"{args.blocks}x16^3+4g {args.precision} {args.riemann}"
                 + (" learned" if args.learned else ""))
+    flagged = float((U[:, 10, g:g + n, g:g + n, g:g + n] > 0).double().mean()) if hybrid else None
     if args.copy_probe:
         copy_probe()
     mixed = None
@@ -281,7 +308,7 @@ def main() -> None:
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(U, dt, dx)
+            cpu = cpu_baseline(U, dt, dx, args.riemann)
         except Exception as e:  # report, do not hide
             cpu = {"value": None, "unit": "zone-updates/s", "cores": 0, "kind": "port",
                    "sample": f"failed: {e}"}
@@ -304,10 +331,11 @@ def main() -> None:
                                f"PPM+{args.riemann.upper()} step ({args.precision}"
                                + (", learned Newton seed)" if args.learned else ")"),
                    "blocks": args.blocks, "block": [16, 16, 16], "nguard": 4,
-                   "precision": args.precision, "riemann": args.riemann, "dt": dt,
+                   "precision": args.precision, "riemann": args.riemann, "dt": dt, "cfl": cfl,
                    "newton_iters_per_face": (int(hy.last_status.newton_iters) /
                                              (3 * args.blocks * 17 * 16 * 16)),
                    "newton_fallbacks": int(hy.last_status.newton_fallbacks),
+                   "flagged_cell_fraction": flagged,
                    "learned_seed": train_info, "autotune": tune},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
@@ -315,7 +343,8 @@ def main() -> None:
                      "kernel": "k_march (face states + fluxes, fused column march)",
                      "kernel_ms": march_ms, "bytes_per_zone": bmz, "zones_per_launch": zones,
                      "stage_ms": {"k_march": stage_ms[0], "riemann_stage": stage_ms[1],
-                                  "k_update": stage_ms[2]},
+                                  "k_update": stage_ms[2],
+                                  "shock_detect": detect_ms / args.steps if hybrid else None},
                      "step_bytes_per_zone": bz,
                      "step_frac": bz * zones / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "cpu_baseline": cpu,

@@ -46,6 +46,7 @@ struct AmrLaunch {
   int periodic[3];
   unsigned long long perMass;  // bit v: variable slot v is PER_MASS
   DevStatus* status;
+  int onlyVar;              // k_amr_fill: >= 0 fills that slot alone, without the EOS
 };
 
 // global cell g (level lev) wrapped or clamped into the domain: periodic
@@ -127,7 +128,8 @@ __global__ void __launch_bounds__(kThreads) k_amr_fill(AmrLaunch A,
     if (amr_find(A, lev, g, sb, sc)) {
       const AmrBlock& S = A.blocks[sb];
       const DevParams& PS = A.cls[S.cls];
-      for (int v = 0; v < P.nunk; ++v) Ud[(long)v * P.ncell + cd] = S.U[(long)v * PS.ncell + sc];
+      const int v0 = A.onlyVar >= 0 ? A.onlyVar : 0, v1 = A.onlyVar >= 0 ? A.onlyVar + 1 : P.nunk;
+      for (int v = v0; v < v1; ++v) Ud[(long)v * P.ncell + cd] = S.U[(long)v * PS.ncell + sc];
       return;
     }
     // prolongation from level lev - 1 (the coarse cell and its neighbours)
@@ -177,7 +179,8 @@ __global__ void __launch_bounds__(kThreads) k_amr_fill(AmrLaunch A,
       }
       return q;
     }();
-    for (int v = 0; v < P.nunk; ++v) {
+    const int v0 = A.onlyVar >= 0 ? A.onlyVar : 0, v1 = A.onlyVar >= 0 ? A.onlyVar + 1 : P.nunk;
+    for (int v = v0; v < v1; ++v) {
       const double q0 = amr_q(A, PC, Cb.U, cc, v);
       double q = q0;
 #pragma unroll
@@ -191,6 +194,7 @@ __global__ void __launch_bounds__(kThreads) k_amr_fill(AmrLaunch A,
       }
       Ud[(long)v * P.ncell + cd] = (A.perMass >> v) & 1ULL ? q / dens : q;
     }
+    if (A.onlyVar >= 0) return;
     const EosOut eo = eos_dens_ei(P, dens, Ud[P.voff[FX_VELX] + cd], Ud[P.voff[FX_VELY] + cd],
                                   Ud[P.voff[FX_VELZ] + cd], Ud[P.voff[FX_EINT] + cd],
                                   Ud[P.voff[FX_ENER] + cd]);

@@ -462,59 +462,95 @@ k_fluxes(DevParams P, Launch L) {
   flush_exact(L.status, es);
 }
 
-// sound speed of cell c as hy_shockDetectBlk.F90:135 forms it
-__device__ __forceinline__ double shock_sound(const DevParams& P, const double* __restrict__ U,
-                                              long c) {
-  return sqrt(U[P.voff[FX_GAMC] + c] * U[P.voff[FX_PRES] + c] / U[P.voff[FX_DENS] + c]);
+// hy_shockDetectBlk.F90:93-229, one workgroup per tile.  SHOK_VAR := 0 on
+// guard cells; interior cells take the Balsara-Spicer flag (undivided
+// pressure jump >= 0.5 of the 3^d minimum pressure and undivided velocity
+// divergence <= -0.1 of the 3^d minimum sound speed).  The workgroup marches
+// along z: each plane's pressure and sound speed over the interior plus one
+// guard layer go to LDS once (one division and root per cell); each thread
+// takes the 3x3 minima of its interior columns and keeps those of the two
+// planes below in LDS slots only it touches, so a cell's 3^d minimum is the
+// minimum of three.  Minima are exact, so the result does not depend on the
+// order.  Reads the interior and the first guard layer only; the SHOK_VAR
+// slot is distinct from the slots read.  cls == nullptr: every tile has the
+// parameters P0.  Dynamic LDS: shock_lds_doubles() of the largest class.
+__host__ __device__ inline long shock_lds_doubles(const DevParams& P) {
+  const long nx2 = P.nxb + 2, ny2 = P.ndim > 1 ? P.nyb + 2 : 1;
+  const long ncol = (long)P.nxb * (P.ndim > 1 ? P.nyb : 1);
+  return 2 * nx2 * ny2 + (P.ndim > 2 ? 4 * ncol : 0);
 }
+constexpr long kShockLdsMax = 65536 / sizeof(double);   // dynamic LDS of one workgroup
 
-// hy_shockDetectBlk.F90:93-229, one workgroup per tile over its
-// guard-inclusive box: SHOK_VAR := 0 on guard cells and the Balsara-Spicer
-// flag on interior cells (undivided pressure jump >= 0.5 of the 3^d minimum
-// pressure and undivided velocity divergence <= -0.1 of the 3^d minimum
-// sound speed).  Reads the interior and the first guard layer only; the
-// SHOK_VAR slot is distinct from the slots read.  cls == nullptr: every tile
-// has the parameters P0.
 __global__ void __launch_bounds__(kThreads) k_shock_detect(Launch L, DevParams P0,
                                                            const DevParams* __restrict__ cls,
                                                            const unsigned char* __restrict__ tcls) {
+  extern __shared__ double sm[];
   const int t = blockIdx.x;
   const DevParams& P = cls ? cls[tcls[t]] : P0;
   double* __restrict__ U = L.tiles[t].U;
   const int k2 = P.ndim > 1, k3 = P.ndim > 2;
   const long sy = P.nx, sz = (long)P.nx * P.ny;
-  const long pr = P.voff[FX_PRES];
-  int any = 0;
+  const long pr = P.voff[FX_PRES], de = P.voff[FX_DENS], gc = P.voff[FX_GAMC];
+  const long vx = P.voff[FX_VELX], vy = P.voff[FX_VELY], vz = P.voff[FX_VELZ];
+  const int nxb = P.nxb, nyb = k2 ? P.nyb : 1;
+  const int nx2 = nxb + 2, ny2 = k2 ? nyb + 2 : 1;
+  const int ncol = nxb * nyb, nplane = nx2 * ny2;
+  double* __restrict__ plane = sm;                 // [2][nplane]: p, then c
+  double* __restrict__ hist = sm + 2 * nplane;     // 3-D: [2 planes][2][ncol] 3x3 minima
+  // guard cells of SHOK_VAR (hy_shockDetectBlk.F90:117)
   for (long c = threadIdx.x; c < P.ncell; c += blockDim.x) {
     const int i = (int)(c % P.nx), j = (int)((c / P.nx) % P.ny), k = (int)(c / sz);
-    double flag = 0.;
-    if (i >= P.lo[0] && i <= P.hi[0] && j >= P.lo[1] && j <= P.hi[1] && k >= P.lo[2] &&
-        k <= P.hi[2]) {
-      double minP = U[pr + c], minC = shock_sound(P, U, c);
-      for (int c3 = -k3; c3 <= k3; ++c3)
-        for (int c2 = -k2; c2 <= k2; ++c2)
-          for (int c1 = -1; c1 <= 1; ++c1) {
-            const long q = c + c1 + c2 * sy + c3 * sz;
-            minP = fmin(minP, U[pr + q]);
-            minC = fmin(minC, shock_sound(P, U, q));
-          }
-      const long vx = P.voff[FX_VELX], vy = P.voff[FX_VELY], vz = P.voff[FX_VELZ];
-      double divv = U[vx + c + 1] - U[vx + c - 1];
+    if (i < P.lo[0] || i > P.hi[0] || j < P.lo[1] || j > P.hi[1] || k < P.lo[2] || k > P.hi[2])
+      U[P.shokOff + c] = 0.;
+  }
+  int any = 0;
+  const int klo = P.lo[2] - k3, khi = P.hi[2] + k3;
+  for (int k = klo; k <= khi; ++k) {
+    for (int q = threadIdx.x; q < nplane; q += blockDim.x) {
+      const int i = P.lo[0] - 1 + q % nx2, j = P.lo[1] - k2 + q / nx2;
+      const long c = ((long)k * P.ny + j) * P.nx + i;
+      const double p = U[pr + c];
+      plane[q] = p;
+      plane[nplane + q] = sqrt(U[gc + c] * p / U[de + c]);   // :135
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < ncol; q += blockDim.x) {
+      const int a = q % nxb, b = q / nxb;
+      double minP = plane[(b + k2) * nx2 + a + 1], minC = plane[nplane + (b + k2) * nx2 + a + 1];
+      for (int y = 0; y <= 2 * k2; ++y)
+        for (int x = 0; x <= 2; ++x) {
+          minP = fmin(minP, plane[(b + y) * nx2 + a + x]);
+          minC = fmin(minC, plane[nplane + (b + y) * nx2 + a + x]);
+        }
+      int kc = k;
+      if (k3) {   // the cell below: planes k-2, k-1 from the history, k from above
+        const double p1 = hist[q], c1 = hist[ncol + q];
+        const double p2 = hist[2 * ncol + q], c2 = hist[3 * ncol + q];
+        hist[2 * ncol + q] = p1;
+        hist[3 * ncol + q] = c1;
+        hist[q] = minP;
+        hist[ncol + q] = minC;
+        if (k < klo + 2) continue;
+        minP = fmin(fmin(minP, p1), p2);
+        minC = fmin(fmin(minC, c1), c2);
+        kc = k - 1;
+      }
+      const long c = ((long)kc * P.ny + P.lo[1] + b) * P.nx + P.lo[0] + a;
+      double divv = U[vx + c + 1] - U[vx + c - 1];               // :157-164
       if (k2) divv = divv + U[vy + c + sy] - U[vy + c - sy];
       if (k3) divv = divv + U[vz + c + sz] - U[vz + c - sz];
       divv = 0.5 * divv;
-      const double gradPx = 0.5 * (U[pr + c + 1] - U[pr + c - 1]);
+      const double gradPx = 0.5 * (U[pr + c + 1] - U[pr + c - 1]);  // :167-174
       double gradPy = 0., gradPz = 0.;
       if (k2) gradPy = 0.5 * (U[pr + c + sy] - U[pr + c - sy]);
       if (k3) gradPz = 0.5 * (U[pr + c + sz] - U[pr + c - sz]);
-      const bool sw1 = fabs(gradPx) + fabs(gradPy) + fabs(gradPz) >= 0.5 * minP;
-      const bool sw2 = -(0.1 * minC) >= divv;
-      if (sw1 && sw2) {
-        flag = 1.;
-        any = 1;
-      }
+      const bool sw1 = fabs(gradPx) + fabs(gradPy) + fabs(gradPz) >= 0.5 * minP;  // :175-178
+      const bool sw2 = -(0.1 * minC) >= divv;                                     // :180-182
+      const bool f = sw1 && sw2;
+      any |= f;
+      U[P.shokOff + c] = f ? 1. : 0.;
     }
-    U[P.shokOff + c] = flag;
+    __syncthreads();   // the next plane reuses `plane`
   }
   if (__syncthreads_or(any) && threadIdx.x == 0) atomicAdd(&L.status->shockTiles, 1ULL);
 }
@@ -780,10 +816,12 @@ __global__ void __launch_bounds__(kThreads) k_fluxreg_conserve(DevParams P, RegL
 // later axes and guard-inclusive in the earlier ones, so three passes also
 // fill edges and corners from the neighbours' already filled guards.
 // nbr[b*6 + 2*axis + side]: neighbour block, or FX_BC_OUTFLOW (zero
-// gradient) or FX_BC_REFLECT (mirror, normal velocity negated).
+// gradient) or FX_BC_REFLECT (mirror, normal velocity negated).  onlyVar >= 0
+// fills that variable slot alone.
 __global__ void __launch_bounds__(kThreads) k_fill_guards(DevParams P, double* __restrict__ U,
                                                           long nBlocks, long stride,
-                                                          const int* __restrict__ nbr, int axis) {
+                                                          const int* __restrict__ nbr, int axis,
+                                                          int onlyVar) {
   const int ng = P.ng;
   const int ext[3] = {P.nx, P.ny, P.nz};
   // extent of the slab along the two other axes
@@ -832,7 +870,8 @@ __global__ void __launch_bounds__(kThreads) k_fill_guards(DevParams P, double* _
       }
     }
     double* dstU = U + b * stride;
-    for (int v = 0; v < P.nunk; ++v) {
+    const int v0 = onlyVar >= 0 ? onlyVar : 0, v1 = onlyVar >= 0 ? onlyVar + 1 : P.nunk;
+    for (int v = v0; v < v1; ++v) {
       const long off = (long)v * P.ncell;
       double val = src[off + sidx];
       if (v == P.velSlot[axis]) val *= vsign;
@@ -1098,6 +1137,8 @@ int validate(const fx_params* p) {
     if (p->shokVar < 0 || p->shokVar >= p->nunk) return 3;
     for (int v = 0; v < FX_NVAR_USED; ++v)
       if (p->var[v] == p->shokVar) return 3;
+    // the detector's planes fit one workgroup's LDS (x-y extents up to 36 x 36)
+    if (shock_lds_doubles(make_dev_params(*p)) > kShockLdsMax) return 3;
   }
   return 0;
 }
@@ -1124,15 +1165,20 @@ int finish(fx_status* st) {
   return h.code;
 }
 
-// k_shock_detect over the uploaded tiles, one workgroup each
+// k_shock_detect over the uploaded tiles, one workgroup each, with the LDS
+// of the largest class present
 void launch_shock_detect(int nTiles) {
   Launch L{g.dtiles, nTiles, nullptr, g.status, 0.0, nullptr};
-  if (g.multi)
-    hipLaunchKernelGGL(k_shock_detect, dim3(nTiles), dim3(kThreads), 0, g.stream, L, g.cur, g.dcls,
-                       g.dtcls);
-  else
-    hipLaunchKernelGGL(k_shock_detect, dim3(nTiles), dim3(kThreads), 0, g.stream, L, g.cur,
+  if (g.multi) {
+    long lds = 0;
+    for (const DevParams& P : g.hcls) lds = std::max(lds, shock_lds_doubles(P));
+    hipLaunchKernelGGL(k_shock_detect, dim3(nTiles), dim3(kThreads), sizeof(double) * lds,
+                       g.stream, L, g.cur, g.dcls, g.dtcls);
+  } else {
+    hipLaunchKernelGGL(k_shock_detect, dim3(nTiles), dim3(kThreads),
+                       sizeof(double) * shock_lds_doubles(g.cur), g.stream, L, g.cur,
                        (const DevParams*)nullptr, (const unsigned char*)nullptr);
+  }
 }
 
 // parameters of tile t of the uploaded list
@@ -1712,12 +1758,13 @@ int upload_neighbors(double* U, long nBlocks, long blockStride, const int* nbr, 
   return 0;
 }
 
-void launch_fill(double* U, long nBlocks, long blockStride) {
+// onlyVar >= 0: that variable slot alone (the shock flags after detection)
+void launch_fill(double* U, long nBlocks, long blockStride, int onlyVar = -1) {
   const DevParams& P = g.dp;
   for (int axis = 0; axis < P.ndim; ++axis) {
     const long slab = 2L * P.ng * (P.nx + P.ny + P.nz) * (P.nx + P.ny + P.nz);
     hipLaunchKernelGGL(k_fill_guards, dim3(grid_for(slab * nBlocks)), dim3(kThreads), 0, g.stream,
-                       P, U, nBlocks, blockStride, g.dnbr, axis);
+                       P, U, nBlocks, blockStride, g.dnbr, axis, onlyVar);
   }
 }
 
@@ -1866,6 +1913,7 @@ AmrLaunch amr_launch() {
   for (int a = 0; a < 3; ++a) L.periodic[a] = A.periodic[a];
   L.perMass = A.perMass;
   L.status = g.status;
+  L.onlyVar = -1;
   return L;
 }
 
@@ -1880,8 +1928,11 @@ int amr_ensure_uploaded(fx_status* st) {
   return ensure_step_scratch(A.nBlocks, st);
 }
 
-void amr_launch_fill() {
-  const AmrLaunch L = amr_launch();
+// onlyVar >= 0: that variable slot alone, no EOS (the shock flags after
+// detection; the other variables are unchanged since the full fill)
+void amr_launch_fill(int onlyVar = -1) {
+  AmrLaunch L = amr_launch();
+  L.onlyVar = onlyVar;
   hipLaunchKernelGGL(k_amr_fill, dim3((unsigned)g.amr.nChunks), dim3(kThreads), 0, g.stream, L,
                      g.amr.dchunks);
 }
@@ -1946,9 +1997,9 @@ int amr_launch_dt(double cfl, fx_status* st) {
 int amr_launch_step(double dt) {
   hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
   amr_launch_fill();
-  if (g.dp.shokOff >= 0) {  // Hydro.F90:94-123: flag, then fill again (SHOK_VAR included)
+  if (g.dp.shokOff >= 0) {  // Hydro.F90:94-123: flag, then fill SHOK_VAR's guard cells
     launch_shock_detect(g.amr.nBlocks);
-    amr_launch_fill();
+    amr_launch_fill(g.prm.shokVar);
   }
   if (launch_flux_stages(g.amr.nBlocks, dt, g.amr.TX, g.amr.TY)) return 4;
   amr_launch_register();
@@ -2229,9 +2280,9 @@ int fx_grid_run_graph(double* U, long nBlocks, long blockStride, const int* nbr,
   if (hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) err = 4;
   for (int s = 0; !err && s < stepsPerGraph; ++s) {
     launch_fill(U, nBlocks, blockStride);
-    if (P.shokOff >= 0) {  // Hydro.F90:94-123: flag, then fill again (SHOK_VAR included)
+    if (P.shokOff >= 0) {  // Hydro.F90:94-123: flag, then fill SHOK_VAR's guard cells
       launch_shock_detect((int)nBlocks);
-      launch_fill(U, nBlocks, blockStride);
+      launch_fill(U, nBlocks, blockStride, g.prm.shokVar);
     }
     hipLaunchKernelGGL(k_dt, dim3(dtGrid), dim3(kThreads), 0, g.stream, P, U, nBlocks, blockStride,
                        1. / del[0], 1. / del[1], 1. / del[2], g.partial);

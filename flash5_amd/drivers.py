@@ -414,7 +414,9 @@ def amr_study(steps: int = 100, steps_per_graph: int = 10, l1_steps: int = 20, g
               only: str | None = None):
     """Config-5 run: three levels (512 x 32^3, 2048 x 16^3, 8192 x 8^3 by
     default) with the four-quadrant data, 100 graph-captured steps in each of
-    plain FP64 HLLC, FP64 exact, and mixed precision with the exact solver
+    plain FP64 HLLC, FP64 exact, FP64 with the hybrid switch (fill, flag,
+    fill, then HLL next to flagged cells and HLLC elsewhere; the blocks carry
+    a shock variable), and mixed precision with the exact solver
     seeded by the learned network (trained on this grid's own face states);
     then l1_steps eager steps of one fixed dt in FP64 exact and in the mixed
     learned form, compared per variable over the leaf cells."""
@@ -453,12 +455,19 @@ def amr_study(steps: int = 100, steps_per_graph: int = 10, l1_steps: int = 20, g
                    "faces_at_or_above_gate": float((strength >= 0.2).double().mean())}
     del Sf, ps
     modes = {"fp64_hllc": base, "fp64_exact": exact64,
+             "fp64_hybrid": dataclasses.replace(base, RiemannSolver="hybrid", nunk=11, shokVar=10),
              "mixed_exact_learned": dataclasses.replace(base, RiemannSolver="exact",
                                                         precision=PREC_MIXED, learnedSeed=True)}
     for name, hp in modes.items():
         if only is not None and name != only:
             continue
-        run = AmrRun(grid, hp, S0.clone(), seed_params=net.params if hp.learnedSeed else None)
+        S = S0.clone()
+        if hp.nunk == 11:                  # the same data with a SHOK_VAR slot
+            S11 = LevelStorage(grid, 11)
+            for a, b in zip(S11.levels, S.levels):
+                a[:, :10] = b
+            S = S11
+        run = AmrRun(grid, hp, S, seed_params=net.params if hp.learnedSeed else None)
         tune = run.hydro.amr_autotune(run.dt(0.8), reps=2)   # launch shapes for this grid
         run.run_graph(0.8, 1, 1)                       # warm-up step (captures once)
         torch.cuda.synchronize()
@@ -467,8 +476,14 @@ def amr_study(steps: int = 100, steps_per_graph: int = 10, l1_steps: int = 20, g
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         st = run.hydro.last_status
+        flagged = None
+        if hp.nunk == 11:                  # flagged interior cells after the last step's fill
+            n_in = sum(t.shape[0] * grid.nb[L][0] * grid.nb[L][1] * grid.nb[L][2]
+                       for L, t in enumerate(run.S.levels))
+            flagged = sum(float((t[:, 10, 4:-4, 4:-4, 4:-4] > 0).sum())
+                          for t in run.S.levels) / n_in
         out[name] = {"zone_updates_per_s": zones * steps / wall, "ms_per_step": 1e3 * wall / steps,
-                     "t_end": run.t, "code": st.code,
+                     "t_end": run.t, "code": st.code, "flagged_cell_fraction": flagged,
                      "last_step_newton_iters": int(st.newton_iters),
                      "last_step_newton_fallbacks": int(st.newton_fallbacks),
                      "autotune": tune}
@@ -500,7 +515,8 @@ def main() -> None:
     ap = argparse.ArgumentParser(description="multi-step runs: Sod (config 1), KH precision "
                                              "study, three-level quadrant problem")
     ap.add_argument("case", choices=["sod", "kh", "amr"])
-    ap.add_argument("--mode", choices=["fp64_hllc", "fp64_exact", "mixed_exact_learned"],
+    ap.add_argument("--mode", choices=["fp64_hllc", "fp64_exact", "fp64_hybrid",
+                                       "mixed_exact_learned"],
                     default=None, help="amr: run only this mode")
     ap.add_argument("--l1-steps", type=int, default=20, help="amr: steps of the precision study")
     ap.add_argument("--ncells", type=int, default=2048)

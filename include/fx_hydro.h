@@ -146,7 +146,10 @@ int fx_hydro_update_solution(const fx_tile* tiles, int nTiles, double dt,
  * the interior and the first guard layer, which the host has filled (the
  * reference's first guard fill, Hydro.F90:102-104).  The host then fills the
  * guard cells (SHOK_VAR included) before fx_hydro_compute_fluxes.  Returns 3
- * when shock detection is off. */
+ * when shock detection is off.  Shock detection needs
+ * 2 (nxb+2)(nyb+2) + 4 nxb nyb doubles of LDS in 3-D (2 (nxb+2)(nyb+2) in
+ * 1-D and 2-D) within 64 KiB, i.e. x-y block extents up to 36 x 36;
+ * fx_hydro_init and fx_hydro_set_block_classes refuse larger ones. */
 int fx_hydro_shock_detect(const fx_tile* tiles, int nTiles, fx_status* st);
 
 /* fx_hydro_shock_detect over a strided batch (block b's U starts at
