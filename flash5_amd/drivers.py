@@ -411,7 +411,7 @@ def leaf_l1(a: LevelStorage, b: LevelStorage, names) -> dict:
 
 
 def amr_study(steps: int = 100, steps_per_graph: int = 10, l1_steps: int = 20, grid=None,
-              only: str | None = None):
+              only: str | None = None, checkpoint_dir: str | None = None):
     """Config-5 run: three levels (512 x 32^3, 2048 x 16^3, 8192 x 8^3 by
     default) with the four-quadrant data, 100 graph-captured steps in each of
     plain FP64 HLLC, FP64 exact, FP64 with the hybrid switch (fill, flag,
@@ -506,6 +506,26 @@ def amr_study(steps: int = 100, steps_per_graph: int = 10, l1_steps: int = 20, g
              "eint": EINT}
     out["l1_mixed_learned_vs_fp64_exact"] = {"steps": l1_steps, "dt": dt,
                                             "l1": leaf_l1(run.S, ref, names)}
+    # the same two states as FLASH-layout checkpoints through the sfocu
+    # comparison (benchmark: FP64 exact); its L1 is the leaf L1 above
+    from . import checkpoint as CK
+    from .sfocu import compare_checkpoints
+    slots = list(names.values())
+    ck_mixed = CK.from_level_storage(grid, run.S, slots, list(names), time=l1_steps * dt, dt=dt,
+                                     nstep=l1_steps)
+    ck_ref = CK.from_level_storage(grid, ref, slots, list(names), time=l1_steps * dt, dt=dt,
+                                   nstep=l1_steps)
+    rep = compare_checkpoints(ck_mixed, ck_ref, err_tol=1e-6)
+    out["l1_mixed_learned_vs_fp64_exact"]["sfocu"] = {
+        "blocks_compared": rep["blocks_compared"], "result": "FAILURE" if rep["fail"] else "SUCCESS",
+        "err_tol": 1e-6,
+        "vars": {n: {"l1": v["ln_norm"], "mag_error": v["mag_error"], "bad_blocks": v["bad_blocks"]}
+                 for n, v in rep["vars"].items()}}
+    if checkpoint_dir:
+        import os
+        os.makedirs(checkpoint_dir, exist_ok=True)
+        CK.write(os.path.join(checkpoint_dir, "quadrant_mixed_learned_chk.npz"), ck_mixed)
+        CK.write(os.path.join(checkpoint_dir, "quadrant_fp64_exact_chk.npz"), ck_ref)
     return out
 
 
@@ -519,6 +539,9 @@ def main() -> None:
                                        "mixed_exact_learned"],
                     default=None, help="amr: run only this mode")
     ap.add_argument("--l1-steps", type=int, default=20, help="amr: steps of the precision study")
+    ap.add_argument("--checkpoint-dir", default=None,
+                    help="amr: write the precision study's two end states as FLASH-layout "
+                         "checkpoints (python -m flash5_amd.sfocu compares them)")
     ap.add_argument("--ncells", type=int, default=2048)
     ap.add_argument("--steps", type=int, default=200)
     a = ap.parse_args()
@@ -546,7 +569,7 @@ def main() -> None:
                           "first_60_steps_zone_updates_per_s": rates}))
     elif a.case == "amr":
         print(json.dumps(amr_study(a.steps if a.steps != 200 else 100, l1_steps=a.l1_steps,
-                                   only=a.mode)))
+                                   only=a.mode, checkpoint_dir=a.checkpoint_dir)))
     else:
         print(json.dumps(kh_precision_study(a.ncells, 16, a.steps)))
 
