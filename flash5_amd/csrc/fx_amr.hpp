@@ -80,8 +80,9 @@ __device__ __forceinline__ bool amr_find(const AmrLaunch& A, int lev, const int 
 // conserved-form value of variable v at cell c of U
 __device__ __forceinline__ double amr_q(const AmrLaunch& A, const DevParams& P, const double* U,
                                         long c, int v) {
-  const double x = U[(long)v * P.ncell + c];
-  return (A.perMass >> v) & 1ULL ? U[P.voff[FX_DENS] + c] * x : x;
+  const gdouble* G = as_global(U);
+  const double x = G[(long)v * P.ncell + c];
+  return (A.perMass >> v) & 1ULL ? G[P.voff[FX_DENS] + c] * x : x;
 }
 
 // MC-limited slope from the left and right differences
@@ -121,7 +122,7 @@ __global__ void __launch_bounds__(kThreads) k_amr_fill(AmrLaunch A,
     for (int a = 0; a < 3; ++a)
       if (a < P.ndim) g[a] = B.pos[a] * Lf.nb[a] + loc[a] - P.lo[a];
     amr_domain(A, lev, P.ndim, g);
-    double* Ud = B.U;
+    gdouble* Ud = as_global(B.U);
     const long cd = (long)r;
     int sb;
     long sc;
@@ -129,7 +130,8 @@ __global__ void __launch_bounds__(kThreads) k_amr_fill(AmrLaunch A,
       const AmrBlock& S = A.blocks[sb];
       const DevParams& PS = A.cls[S.cls];
       const int v0 = A.onlyVar >= 0 ? A.onlyVar : 0, v1 = A.onlyVar >= 0 ? A.onlyVar + 1 : P.nunk;
-      for (int v = v0; v < v1; ++v) Ud[(long)v * P.ncell + cd] = S.U[(long)v * PS.ncell + sc];
+      const gdouble* Us = as_global(S.U);
+      for (int v = v0; v < v1; ++v) Ud[(long)v * P.ncell + cd] = Us[(long)v * PS.ncell + sc];
       return;
     }
     // prolongation from level lev - 1 (the coarse cell and its neighbours)
@@ -251,7 +253,7 @@ __global__ void __launch_bounds__(kThreads) k_amr_restrict(AmrLaunch A) {
     const double dens = child_sum((int)(P.voff[FX_DENS] / P.ncell)) * frac;
     for (int v = 0; v < P.nunk; ++v) {
       const double avg = child_sum(v) * frac;
-      Pb.U[(long)v * PP.ncell + pc] = (A.perMass >> v) & 1ULL ? avg / dens : avg;
+      as_global(Pb.U)[(long)v * PP.ncell + pc] = (A.perMass >> v) & 1ULL ? avg / dens : avg;
     }
   }
 }
@@ -271,7 +273,7 @@ __global__ void __launch_bounds__(kThreads) k_amr_dt(AmrLaunch A, double* __rest
     const AmrLevelDev& L = A.levels[B.level];
     const int nxi = P.nxb, nyi = P.ndim > 1 ? P.nyb : 1;
     const long c = cidx(P, r % nxi + P.lo[0], (r / nxi) % nyi + P.lo[1], r / (nxi * nyi) + P.lo[2]);
-    const double* U = B.U;
+    const gdouble* U = as_global(B.U);
     const double cs = sqrt(U[P.voff[FX_GAMC] + c] * U[P.voff[FX_PRES] + c] / U[P.voff[FX_DENS] + c]);
     double dtl = (fabs(U[P.voff[FX_VELX] + c] - 0.) + cs) * (1. / L.del[0]);
     if (P.ndim > 1) dtl = fmax(dtl, (fabs(U[P.voff[FX_VELY] + c] - 0.) + cs) * (1. / L.del[1]));

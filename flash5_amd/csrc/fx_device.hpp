@@ -30,6 +30,18 @@ namespace fx {
 enum { P_DENS = 0, P_VELX = 1, P_VELY = 2, P_VELZ = 3, P_PRES = 4, P_GAMC = 5,
        P_GAME = 6, P_EINT = 7 };
 
+// Pointers read from device memory (a tile's U, flux and pavg arrays) are
+// generic, and accesses through generic pointers compile to flat
+// instructions, which also count against the LDS wait counter.  The arrays
+// are device memory, so the kernels hold them as global-address-space
+// pointers (gdouble*) from the point they are read; loads and stores through
+// them are global instructions, which also take a scalar base address.
+typedef __attribute__((address_space(1))) double gdouble;
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* as_global(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
 struct DevParams {
   int ndim, nxb, nyb, nzb, ng, nunk;
   int nx, ny, nz;            // guard-inclusive extents

@@ -113,7 +113,7 @@ __device__ __forceinline__ long cidx(const DevParams& P, int i, int j, int k) {
 // primitive vector of cell c in the reconstruction type R (a conversion
 // from the FP64 cell data when R is float)
 template <typename R>
-__device__ __forceinline__ void load_prim(const DevParams& P, const double* __restrict__ U, long c,
+__device__ __forceinline__ void load_prim(const DevParams& P, const gdouble* __restrict__ U, long c,
                                           R V[8]) {
   V[P_DENS] = R(U[P.voff[FX_DENS] + c]);
   V[P_VELX] = R(U[P.voff[FX_VELX] + c]);
@@ -127,10 +127,10 @@ __device__ __forceinline__ void load_prim(const DevParams& P, const double* __re
 
 // FlatTilde of one cell along DIR (hy_getRiemannState.F90:365-404)
 template <int DIR, typename R = double>
-__device__ __forceinline__ R flat_tilde(const DevParams& P, const double* __restrict__ U, long c,
+__device__ __forceinline__ R flat_tilde(const DevParams& P, const gdouble* __restrict__ U, long c,
                                         long s) {
-  const double* pr = U + P.voff[FX_PRES];
-  const double* vn = U + P.voff[FX_VELX + DIR];
+  const gdouble* pr = U + P.voff[FX_PRES];
+  const gdouble* vn = U + P.voff[FX_VELX + DIR];
   const R dp1 = (R(pr[c + s]) - R(pr[c - s]));
   const R dp2 = (R(pr[c + 2 * s]) - R(pr[c - 2 * s]));
   const R dv1 = R(vn[c + s]) - R(vn[c - s]);
@@ -143,9 +143,9 @@ __device__ __forceinline__ R flat_tilde(const DevParams& P, const double* __rest
 
 // FlatCoeff (hy_getRiemannState.F90:408-453)
 template <int DIR, typename R = double>
-__device__ __forceinline__ R flat_coeff(const DevParams& P, const double* __restrict__ U, long c,
+__device__ __forceinline__ R flat_coeff(const DevParams& P, const gdouble* __restrict__ U, long c,
                                         long s) {
-  const double* pr = U + P.voff[FX_PRES];
+  const gdouble* pr = U + P.voff[FX_PRES];
   const R dp1 = (R(pr[c + s]) - R(pr[c - s]));
   const R f0 = flat_tilde<DIR, R>(P, U, c, s);
   if (dp1 < R(0.0)) return fmax(f0, flat_tilde<DIR, R>(P, U, c + s, s));
@@ -161,7 +161,7 @@ __device__ __forceinline__ long axis_stride(const DevParams& P, int a) {
 template <class T> struct NoDeduce { using type = T; };
 
 template <int SD, typename R>
-__device__ __forceinline__ bool sigma_at(const DevParams& P, const double* __restrict__ U, long c,
+__device__ __forceinline__ bool sigma_at(const DevParams& P, const gdouble* __restrict__ U, long c,
                                          R sig[8], typename NoDeduce<EigT<R>>::type* eout) {
   const long s = axis_stride(P, SD);
   R Vm[8], Vc[8], Vp[8];
@@ -179,7 +179,7 @@ __device__ __forceinline__ bool sigma_at(const DevParams& P, const double* __res
 // for that direction, then :952-1327 in 3-D).  Returns false on an
 // imaginary sound speed.
 template <int D, int NDIM>
-__device__ __forceinline__ bool state_cell(const DevParams& P, const double* __restrict__ U,
+__device__ __forceinline__ bool state_cell(const DevParams& P, const gdouble* __restrict__ U,
                                            const double del[3], double dt, long c, double Wp[8],
                                            double Wn[8], DevStatus* status) {
   constexpr int O1 = (D + 1) % 3, O2 = (D + 2) % 3;
@@ -303,9 +303,10 @@ __global__ void __launch_bounds__(kThreads) k_states(DevParams P, Launch L) {
     double Wp[8], Wn[8];
     bool ok;
     const double dt = step_dt(L.dt, L.dtp);
-    if (d == 0) ok = state_cell<0, NDIM>(P, T.U, T.del, dt, c, Wp, Wn, L.status);
-    else if (d == 1) ok = state_cell<1, NDIM>(P, T.U, T.del, dt, c, Wp, Wn, L.status);
-    else ok = state_cell<2, NDIM>(P, T.U, T.del, dt, c, Wp, Wn, L.status);
+    const gdouble* Ut = as_global(T.U);
+    if (d == 0) ok = state_cell<0, NDIM>(P, Ut, T.del, dt, c, Wp, Wn, L.status);
+    else if (d == 1) ok = state_cell<1, NDIM>(P, Ut, T.del, dt, c, Wp, Wn, L.status);
+    else ok = state_cell<2, NDIM>(P, Ut, T.del, dt, c, Wp, Wn, L.status);
     if (!ok) {
       report_error(L.status, 1, t, ci[0], ci[1], ci[2], d);
       continue;
@@ -357,7 +358,7 @@ __device__ __forceinline__ void aux_fluxes(const DevParams& P, R f0, const R* VL
 
 // Whether the face between cells cl and cr takes HLL: RiemannSolver = HLL,
 // or HYBR with a flagged cell on either side (hy_getFaceFlux.F90:353-367).
-__device__ __forceinline__ bool hll_face(const DevParams& P, const double* __restrict__ U, long cl,
+__device__ __forceinline__ bool hll_face(const DevParams& P, const gdouble* __restrict__ U, long cl,
                                          long cr) {
   if (P.riemann == FX_HLL) return true;
   if (P.riemann != FX_HYBR) return false;
@@ -413,7 +414,7 @@ k_fluxes(DevParams P, Launch L) {
     else if (d == 1) jm -= 1;
     else km -= 1;
     const fx_tile& T = L.tiles[t];
-    const double* __restrict__ U = T.U;
+    const gdouble* __restrict__ U = as_global(T.U);
     const double* S = L.states + (long)t * P.scellsPerTile * kStateComps + P.boff[d] * kStateComps;
     const long bc = P.bcells[d];
     const long bl = band_index(P, d, im, jm, km);
@@ -454,7 +455,7 @@ k_fluxes(DevParams P, Launch L) {
       report_error(L.status, 2, t, i, j, k, d);
       continue;
     }
-    double* Fo = T.flux[d];
+    gdouble* Fo = as_global(T.flux[d]);
     const long nf = P.nface[d];
 #pragma unroll
     for (int q = 0; q < FX_NFLUX; ++q) Fo[q * nf + r] = F[q];
@@ -487,7 +488,7 @@ __global__ void __launch_bounds__(kThreads) k_shock_detect(Launch L, DevParams P
   extern __shared__ double sm[];
   const int t = blockIdx.x;
   const DevParams& P = cls ? cls[tcls[t]] : P0;
-  double* __restrict__ U = L.tiles[t].U;
+  gdouble* __restrict__ U = as_global(L.tiles[t].U);
   const int k2 = P.ndim > 1, k3 = P.ndim > 2;
   const long sy = P.nx, sz = (long)P.nx * P.ny;
   const long pr = P.voff[FX_PRES], de = P.voff[FX_DENS], gc = P.voff[FX_GAMC];
@@ -591,13 +592,13 @@ __device__ __forceinline__ void update_cell(const DevParams& P, const Launch& L,
           else if (d == 1) pres = pres * 0.5 + (pN + pP) * 0.25;
           else pres = (pres * 4.0 + (pN + pP)) / 6.0;
         }
-        if (!doUpdate && T.pavg) T.pavg[r] = pres;
+        if (!doUpdate && T.pavg) as_global(T.pavg)[r] = pres;
       } else {
-        pres = T.pavg[r];
+        pres = as_global(T.pavg)[r];
       }
     }
     if (!doUpdate) return;
-    double* U = T.U;
+    gdouble* U = as_global(T.U);
     const long c = cidx(P, i, j, k);
     const double dx = T.del[0], dy = P.ndim > 1 ? T.del[1] : 1., dz = P.ndim > 2 ? T.del[2] : 1.;
     const double dt = step_dt(L.dt, L.dtp);
@@ -615,7 +616,7 @@ __device__ __forceinline__ void update_cell(const DevParams& P, const Launch& L,
     if (P.aux) IntEner = dens0 * U[P.voff[FX_EINT] + c];
     double Sphys[5] = {0., 0., 0., 0., 0.};
     auto apply = [&](int d, double del, long fL, long fR) {
-      const double* Fd = T.flux[d];
+      const gdouble* Fd = as_global(T.flux[d]);
       const long nf = P.nface[d];
       double FL[8], FR[8];
 #pragma unroll
@@ -781,7 +782,7 @@ __global__ void __launch_bounds__(kThreads) k_fluxreg_put(DevParams P, RegLaunch
     const fx_fluxreg_face f = R.faces[e];
     const int d = f.dir;
     const DevParams& Pt = R.cls ? R.cls[R.tcls[f.fine_tile]] : P;
-    const double* F = R.tiles[f.fine_tile].flux[d] + (long)q * Pt.nface[d];
+    const gdouble* F = as_global(R.tiles[f.fine_tile].flux[d]) + (long)q * Pt.nface[d];
     const int inner = d == 0 ? 1 : 0, outer = d == 2 ? 1 : 2;
     const int nIn = P.ndim > 1 ? 2 : 1, nOut = P.ndim > 2 ? 2 : 1;
     double acc = 0.0;
@@ -805,7 +806,7 @@ __global__ void __launch_bounds__(kThreads) k_fluxreg_conserve(DevParams P, RegL
     const fx_fluxreg_face f = R.faces[e];
     const int d = f.dir;
     const DevParams& Pt = R.cls ? R.cls[R.tcls[f.coarse_tile]] : P;
-    double* F = R.tiles[f.coarse_tile].flux[d] + (long)q * Pt.nface[d];
+    gdouble* F = as_global(R.tiles[f.coarse_tile].flux[d]) + (long)q * Pt.nface[d];
     F[reg_face(Pt, d, f.ci, f.cj, f.ck)] = R.reg[gid] * 1.0;
   }
 }

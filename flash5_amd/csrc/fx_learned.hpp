@@ -287,7 +287,7 @@ k_face_solve(DevParams P0, FaceLaunch F) {
       const float y = F.y[gid];
       if (y != kNoSeed) guess = seed_from_y(y, Lb[4 * nf], Rb[4 * nf]);
     }
-    const double* U = F.tiles[f.t].U;
+    const gdouble* U = as_global(F.tiles[f.t].U);
     double Fo[8];
     const int ierr = exact_face_flux<SEEDED>(P, d, Lb, Rb, nf, U[P.voff[FX_GAMC] + f.cL],
                                              U[P.voff[FX_GAME] + f.cL], U[P.voff[FX_GAME] + f.cR],
@@ -297,7 +297,7 @@ k_face_solve(DevParams P0, FaceLaunch F) {
                    (int)(f.cR / ((long)P.nx * P.ny)), d);
       continue;
     }
-    double* out = F.tiles[f.t].flux[d];
+    gdouble* out = as_global(F.tiles[f.t].flux[d]);
 #pragma unroll
     for (int q = 0; q < FX_NFLUX; ++q) out[q * nf + f.r] = Fo[q];
   }

@@ -129,13 +129,13 @@ __device__ __forceinline__ void seed_gate(const DevParams& P, const MarchLaunch&
 }
 
 template <int SD, typename R>
-__device__ __forceinline__ bool sigma_cell(const DevParams& P, const double* __restrict__ U, long c,
+__device__ __forceinline__ bool sigma_cell(const DevParams& P, const gdouble* __restrict__ U, long c,
                                            R sig[8]) {
   return sigma_at<SD>(P, U, c, sig, nullptr);
 }
 
 template <int DIR, typename R>
-__device__ __forceinline__ bool slope_cell(const DevParams& P, const double* __restrict__ U, long c,
+__device__ __forceinline__ bool slope_cell(const DevParams& P, const gdouble* __restrict__ U, long c,
                                            R db[8]) {
   const long s = axis_stride(P, DIR);
   R Vm[8], Vc[8], Vp[8];
@@ -210,7 +210,7 @@ __device__ __forceinline__ void put_state(R* base, int n, int idx, const R Wp[8]
 }
 
 template <int D>
-__device__ __forceinline__ void write_flux(const DevParams& P, double* Fo, long r, const double F[8]) {
+__device__ __forceinline__ void write_flux(const DevParams& P, gdouble* Fo, long r, const double F[8]) {
   const long nf = P.nface[D];
 #pragma unroll
   for (int q = 0; q < FX_NFLUX; ++q) Fo[q * nf + r] = F[q];
@@ -269,7 +269,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
   const int ex = (ox + TX > P.hi[0]) ? 1 : 0;
   const int ey = (oy + TY > P.hi[1]) ? 1 : 0;
   const fx_tile& T = L.tiles[t];
-  const double* __restrict__ U = T.U;
+  const gdouble* __restrict__ U = as_global(T.U);
   const double dt = step_dt(L.dt, L.dtp);
   const double dx = T.del[0], dy = T.del[1], dz = T.del[2];
   const long sx = 1, sy = P.nx, sz = (long)P.nx * P.ny;
@@ -300,8 +300,8 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
     R db[8];
     if (!slope_cell<2>(P, U, c, db)) { bad = true; return; }
     lds_put6(LZ + ring(m) * 6 * NCOL, NCOL, w, db);
-    const double* pr = U + P.voff[FX_PRES];
-    const double* vz = U + P.voff[FX_VELZ];
+    const gdouble* pr = U + P.voff[FX_PRES];
+    const gdouble* vz = U + P.voff[FX_VELZ];
     FZ[ring(m) * NCOL + w] = flat_tilde_vals(R(pr[c - 2 * sz]), R(pr[c - sz]), R(pr[c + sz]),
                                              R(pr[c + 2 * sz]), R(vz[c - sz]), R(vz[c + sz]));
   };
@@ -538,7 +538,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
             continue;
           } else {
             if (face_flux<0, RS>(P, VL, VR, hll_face(P, U, c - sx, c), F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 0); continue; }
-            write_flux<0>(P, T.flux[0], face_index(P, 0, ox + a, oy + b, k), F);
+            write_flux<0>(P, as_global(T.flux[0]), face_index(P, 0, ox + a, oy + b, k), F);
           }
         } else if (w < nFX + nFY) {
           const int w2 = w - nFX;
@@ -557,7 +557,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
             continue;
           } else {
             if (face_flux<1, RS>(P, VL, VR, hll_face(P, U, c - sy, c), F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 1); continue; }
-            write_flux<1>(P, T.flux[1], face_index(P, 1, ox + a, oy + b, k), F);
+            write_flux<1>(P, as_global(T.flux[1]), face_index(P, 1, ox + a, oy + b, k), F);
           }
         } else {
           // column item: z-face flux at face k, pavg of the cell, carry Wp
@@ -577,7 +577,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
               if (L.seedY) seed_gate<2>(P, L, faceBase, fr, VL, VR);
             } else {
               if (face_flux<2, RS>(P, VL, VR, hll_face(P, U, c - sz, c), F, es)) report_error(st, 2, t, ox + a, oy + b, k, 2);
-              else write_flux<2>(P, T.flux[2], face_index(P, 2, ox + a, oy + b, k), F);
+              else write_flux<2>(P, as_global(T.flux[2]), face_index(P, 2, ox + a, oy + b, k), F);
             }
           }
           if (interior && P.aux) {
@@ -587,7 +587,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
             pres = pres * R(0.5) + (WY[10 * I::NRX + I::rx(a, b)] + WY[4 * I::NRX + I::rx(a, b)]) * R(0.25);
             if (Z) pres = (pres * R(4.0) + (WZ[10 * NCOL + w2] + WZ[4 * NCOL + w2])) / R(6.0);
             const long r = ((long)(k - P.lo[2]) * P.nyb + (oy + b - P.lo[1])) * P.nxb + (ox + a - P.lo[0]);
-            T.pavg[r] = double(pres);
+            as_global(T.pavg)[r] = double(pres);
           }
           if (Z) {
 #pragma unroll
