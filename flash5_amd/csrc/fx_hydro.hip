@@ -365,21 +365,30 @@ __device__ __forceinline__ bool hll_face(const DevParams& P, const gdouble* __re
   return U[P.shokOff + cl] + U[P.shokOff + cr] > 0.;
 }
 
+// The kernels' solver family: RS = FX_HLLC compiles HLLC alone; RS = FX_HYBR
+// serves RiemannSolver = HLL and HYBR, choosing per face at run time.
+template <int RS>
+__device__ __forceinline__ bool use_hll(const DevParams& P, const gdouble* __restrict__ U, long cl,
+                                        long cr) {
+  if constexpr (RS == FX_HYBR) return hll_face(P, U, cl, cr);
+  else return false;
+}
+
 // Face flux densities F[0..7] (hy_getFaceFlux.F90:324-442): HLL (useHll) or
 // HLLC on the face states VL, VR in the reconstruction type R, then the aux
 // internal-energy fluxes.  The exact solver takes exact_face_flux.
 template <int D, int RS, typename R>
 __device__ __forceinline__ int face_flux(const DevParams& P, const R* VL, const R* VR, bool useHll,
                                          double F[8], ExactStats& es) {
-  static_assert(RS == FX_HLLC, "face_flux serves HLL, HLLC and the hybrid switch");
+  static_assert(RS == FX_HLLC || RS == FX_HYBR, "face_flux serves HLL, HLLC and the hybrid switch");
   R f0;
   if constexpr (sizeof(R) == sizeof(double)) {
-    const int ierr = useHll ? hll<D>(P, VL, VR, F) : hllc<D>(P, VL, VR, F);
+    const int ierr = (RS == FX_HYBR && useHll) ? hll<D>(P, VL, VR, F) : hllc<D>(P, VL, VR, F);
     if (ierr) return ierr;
     f0 = F[0];
   } else {
     R Fr[6];
-    const int ierr = useHll ? hll<D>(P, VL, VR, Fr) : hllc<D>(P, VL, VR, Fr);
+    const int ierr = (RS == FX_HYBR && useHll) ? hll<D>(P, VL, VR, Fr) : hllc<D>(P, VL, VR, Fr);
     if (ierr) return ierr;
 #pragma unroll
     for (int q = 0; q < 6; ++q) F[q] = double(Fr[q]);
@@ -446,7 +455,7 @@ k_fluxes(DevParams P, Launch L) {
       VL[P_GAME] = U[P.voff[FX_GAME] + cl];
       VR[P_GAMC] = U[P.voff[FX_GAMC] + cr];
       VR[P_GAME] = U[P.voff[FX_GAME] + cr];
-      const bool useHll = hll_face(P, U, cl, cr);
+      const bool useHll = use_hll<RS>(P, U, cl, cr);
       if (d == 0) ierr = face_flux<0, RS>(P, VL, VR, useHll, F, es);
       else if (d == 1) ierr = face_flux<1, RS>(P, VL, VR, useHll, F, es);
       else ierr = face_flux<2, RS>(P, VL, VR, useHll, F, es);
@@ -1488,9 +1497,14 @@ int launch_flux_stages(int nTiles, double dt, int TX, int TY) {
       hipLaunchKernelGGL((k_face_solve<false, false>), gf, dim3(kThreads), 0, g.stream, P, F);
     }
     if (record(2)) return 4;
-  } else {
+  } else if (P.riemann == FX_HLLC) {
     if (g.multi) launch_march_multi<FX_HLLC>(P.ndim, P.precision, TX, M);
     else launch_march<FX_HLLC>(P, M, TX, grid);
+    if (record(1)) return 4;
+    if (record(2)) return 4;
+  } else {  // HLL, or the hybrid switch
+    if (g.multi) launch_march_multi<FX_HYBR>(P.ndim, P.precision, TX, M);
+    else launch_march<FX_HYBR>(P, M, TX, grid);
     if (record(1)) return 4;
     if (record(2)) return 4;
   }
@@ -1589,7 +1603,8 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
   {
     const dim3 gf(grid_for((P.nface[0] + P.nface[1] + P.nface[2]) * nTiles));
     if (P.riemann == FX_EXACT) hipLaunchKernelGGL(k_fluxes<FX_EXACT>, gf, dim3(kThreads), 0, g.stream, P, L);
-    else hipLaunchKernelGGL(k_fluxes<FX_HLLC>, gf, dim3(kThreads), 0, g.stream, P, L);
+    else if (P.riemann == FX_HLLC) hipLaunchKernelGGL(k_fluxes<FX_HLLC>, gf, dim3(kThreads), 0, g.stream, P, L);
+    else hipLaunchKernelGGL(k_fluxes<FX_HYBR>, gf, dim3(kThreads), 0, g.stream, P, L);
   }
   if (record(2)) return 4;
   (void)anyFluxOnly;

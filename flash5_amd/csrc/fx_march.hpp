@@ -537,7 +537,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
             if (L.seedY) seed_gate<0>(P, L, faceBase, fr, VL, VR);
             continue;
           } else {
-            if (face_flux<0, RS>(P, VL, VR, hll_face(P, U, c - sx, c), F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 0); continue; }
+            if (face_flux<0, RS>(P, VL, VR, use_hll<RS>(P, U, c - sx, c), F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 0); continue; }
             write_flux<0>(P, as_global(T.flux[0]), face_index(P, 0, ox + a, oy + b, k), F);
           }
         } else if (w < nFX + nFY) {
@@ -556,7 +556,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
             if (L.seedY) seed_gate<1>(P, L, faceBase, fr, VL, VR);
             continue;
           } else {
-            if (face_flux<1, RS>(P, VL, VR, hll_face(P, U, c - sy, c), F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 1); continue; }
+            if (face_flux<1, RS>(P, VL, VR, use_hll<RS>(P, U, c - sy, c), F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 1); continue; }
             write_flux<1>(P, as_global(T.flux[1]), face_index(P, 1, ox + a, oy + b, k), F);
           }
         } else {
@@ -576,7 +576,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
               write_face_states<2>(P, fsb + fstate_dir_offset(P, 2), fr, VL, VR);
               if (L.seedY) seed_gate<2>(P, L, faceBase, fr, VL, VR);
             } else {
-              if (face_flux<2, RS>(P, VL, VR, hll_face(P, U, c - sz, c), F, es)) report_error(st, 2, t, ox + a, oy + b, k, 2);
+              if (face_flux<2, RS>(P, VL, VR, use_hll<RS>(P, U, c - sz, c), F, es)) report_error(st, 2, t, ox + a, oy + b, k, 2);
               else write_flux<2>(P, as_global(T.flux[2]), face_index(P, 2, ox + a, oy + b, k), F);
             }
           }
