@@ -237,6 +237,29 @@ __device__ __forceinline__ void trans_sigma(const DevParams& P, const R* vm1, co
   }
 }
 
+// x / C for C = 3, 6, 12 with the IEEE quotient's bits, in four FP64
+// instructions instead of the division sequence (a quarter-rate reciprocal
+// and about ten instructions).  q = x RN(1/C) is faithful, so rho = x - q C
+// is exact in one FMA; q + rho RN(1/C), rounded once by a second FMA, lies
+// within 2^-52 ulp of x/C.  As C = 3 * 2^k, a normal quotient x/C sits 0,
+// 1/3 or 2/3 of an ulp past a double, never within 1/6 ulp of a rounding
+// midpoint, so both round alike; copysign keeps the sign of a zero.  This
+// holds for every finite x whose quotient is normal or zero (checked against
+// IEEE division by tests/cdiv/const_div_check.c); for C = 6 and 12 a
+// quotient below 2^-1022 may differ in its last place.  FP32 divides.
+template <int C>
+__device__ __forceinline__ double div_const(double x) {
+  static_assert(C == 3 || C == 6 || C == 12, "constants of the form 3 * 2^k");
+  constexpr double r = 1.0 / C;
+  const double q = x * r;
+  const double rho = fma(-q, double(C), x);
+  return copysign(fma(rho, r, q), x);
+}
+template <int C>
+__device__ __forceinline__ float div_const(float x) {
+  return x / float(C);
+}
+
 // PPM interface values, limiting and characteristic tracing from the three
 // projected slopes of cells i-1, i, i+1 (PPM.F90:578-842).  Vm/Vc/Vp are the
 // primitive vectors of i-1, i, i+1; densMM/densPP the densities of i-2, i+2
@@ -254,8 +277,8 @@ __device__ __forceinline__ void ppm_trace(const DevParams& P, R dtn, R delta, co
     vL[v] = R(0.);
     vR[v] = R(0.);
     if (v <= endVar) {
-      vL[v] = R(0.5) * (Vc[v] + Vm[v]) - (db0[v] - dbN[v]) / R(6.);
-      vR[v] = R(0.5) * (Vc[v] + Vp[v]) - (dbP[v] - db0[v]) / R(6.);
+      vL[v] = R(0.5) * (Vc[v] + Vm[v]) - div_const<6>(db0[v] - dbN[v]);
+      vR[v] = R(0.5) * (Vc[v] + Vp[v]) - div_const<6>(dbP[v] - db0[v]);
     }
   }
   if (P.steep) {
@@ -314,8 +337,8 @@ __device__ __forceinline__ void ppm_trace(const DevParams& P, R dtn, R delta, co
       if (v > endVar) continue;
       const R a = v <= P_PRES ? lmax : emax;
       const R b = v <= P_PRES ? lmin : emin;
-      Wp[v] = vR[v] - a * hdtn * (dW[v] - (R(1.0) - a * hdtn * R(4.) / R(3.)) * W6[v]);
-      Wm[v] = vL[v] + b * hdtn * (dW[v] + (R(1.0) - b * hdtn * R(4.) / R(3.)) * W6[v]);
+      Wp[v] = vR[v] - a * hdtn * (dW[v] - (R(1.0) - div_const<3>(a * hdtn * R(4.))) * W6[v]);
+      Wm[v] = vL[v] + b * hdtn * (dW[v] + (R(1.0) - div_const<3>(b * hdtn * R(4.))) * W6[v]);
     }
   }
   Wp[P_GAMC] = Vc[P_GAMC];
@@ -356,8 +379,8 @@ __device__ __forceinline__ void ppm_trace(const DevParams& P, R dtn, R delta, co
 #undef FX_TRACE
 #pragma unroll
   for (int v = 0; v <= P_PRES; ++v) {
-    Wm[v] = Vc[v] + W6[v] / R(12.) + sL[v];
-    Wp[v] = Vc[v] + W6[v] / R(12.) + sR[v];
+    Wm[v] = Vc[v] + div_const<12>(W6[v]) + sL[v];
+    Wp[v] = Vc[v] + div_const<12>(W6[v]) + sR[v];
   }
 }
 

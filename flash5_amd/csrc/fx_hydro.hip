@@ -599,7 +599,7 @@ __device__ __forceinline__ void update_cell(const DevParams& P, const Launch& L,
           const double pN = S[10 * bc + bi], pP = S[4 * bc + bi];
           if (d == 0) pres = 0.5 * (pN + pP);
           else if (d == 1) pres = pres * 0.5 + (pN + pP) * 0.25;
-          else pres = (pres * 4.0 + (pN + pP)) / 6.0;
+          else pres = div_const<6>(pres * 4.0 + (pN + pP));
         }
         if (!doUpdate && T.pavg) as_global(T.pavg)[r] = pres;
       } else {

@@ -585,7 +585,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
             // y, z analogues)
             R pres = R(0.5) * (WX[10 * I::NRY + I::ry(a, b)] + WX[4 * I::NRY + I::ry(a, b)]);
             pres = pres * R(0.5) + (WY[10 * I::NRX + I::rx(a, b)] + WY[4 * I::NRX + I::rx(a, b)]) * R(0.25);
-            if (Z) pres = (pres * R(4.0) + (WZ[10 * NCOL + w2] + WZ[4 * NCOL + w2])) / R(6.0);
+            if (Z) pres = div_const<6>(pres * R(4.0) + (WZ[10 * NCOL + w2] + WZ[4 * NCOL + w2]));
             const long r = ((long)(k - P.lo[2]) * P.nyb + (oy + b - P.lo[1])) * P.nxb + (ox + a - P.lo[0]);
             as_global(T.pavg)[r] = double(pres);
           }
