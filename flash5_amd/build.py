@@ -4,9 +4,11 @@
 ``flash5_amd/libfx_hydro.so`` in-tree with hipcc.  All HIP code is built
 with packed-FP32 instructions disabled.  FP32 division and square root use
 the hardware reciprocal and root (about 2.5 ulp) instead of correctly rounded
-sequences: only the mixed-precision reconstruction and the seed network
-compute in FP32, and their error bars are set against the FP64 path.  FP64
-code is not affected (its machine code is the same with or without it).
+sequences, and FP32 subnormals are flushed to zero, which drops the
+frexp/ldexp scaling around each FP32 reciprocal and root: only the
+mixed-precision reconstruction and the seed network compute in FP32, and
+their error bars are set against the FP64 path.  FP64 code is not affected
+by either flag (its machine code is the same with or without them).
 """
 from __future__ import annotations
 
@@ -23,6 +25,7 @@ LIB = os.path.join(HERE, "libfx_hydro.so")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off", "-fno-hip-fp32-correctly-rounded-divide-sqrt",
+    "-fgpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
 ]
