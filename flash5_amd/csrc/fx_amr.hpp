@@ -47,6 +47,8 @@ struct AmrLaunch {
   unsigned long long perMass;  // bit v: variable slot v is PER_MASS
   DevStatus* status;
   int onlyVar;              // k_amr_fill: >= 0 fills that slot alone, without the EOS
+  const int* guard;         // k_amr_fill: guard-cell box indices of every class, class by class
+  const int* guardOff;      // class c's guard cells are guard[guardOff[c] .. guardOff[c+1])
 };
 
 // global cell g (level lev) wrapped or clamped into the domain: periodic
@@ -95,26 +97,22 @@ __device__ __forceinline__ double amr_mc(double dl, double dr) {
 
 // Guard cells of every block: same-level copy where the level has a block,
 // else conservative linear prolongation from the next coarser level followed
-// by the EOS (MODE_DENS_EI).  Workgroup w takes the kThreads cells of
-// chunks[w] = (block, first cell) of the guard-inclusive box, so the block
-// and its parameters are uniform over the workgroup; interior cells return
-// at once.
+// by the EOS (MODE_DENS_EI).  Workgroup w takes kThreads guard cells of one
+// block, chunks[w] = (block, first guard cell in its class's list), so the
+// block and its parameters are uniform over the workgroup and every lane
+// has a guard cell (the lists hold guard cells only, in box order).
 __global__ void __launch_bounds__(kThreads) k_amr_fill(AmrLaunch A,
                                                         const int2* __restrict__ chunks) {
   {
     const int2 ch = chunks[blockIdx.x];
     const int b = ch.x;
     const AmrBlock& B = A.blocks[b];
-    const int r = ch.y + (int)threadIdx.x;
-    if (r >= A.cls[B.cls].ncell) return;
+    const int o = A.guardOff[B.cls] + ch.y + (int)threadIdx.x;
+    if (o >= A.guardOff[B.cls + 1]) return;
+    const int r = A.guard[o];
     const DevParams& P = A.cls[B.cls];
     const int i = r % P.nx, j = (r / P.nx) % P.ny, k = r / (P.nx * P.ny);
     const int loc[3] = {i, j, k};
-    bool inside = true;
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-      if (a < P.ndim && (loc[a] < P.lo[a] || loc[a] > P.hi[a])) inside = false;
-    if (inside) return;
     const int lev = B.level;
     const AmrLevelDev& Lf = A.levels[lev];
     int g[3] = {0, 0, 0};

@@ -983,7 +983,9 @@ struct Lib {
     long* dcellStart = nullptr;               // interior cells before each block
     int* dwBox = nullptr;                     // find_tile_w tables of the two prefixes
     int* dwCell = nullptr;
-    int2* dchunks = nullptr;                  // guard-fill work: (block, first box cell)
+    int2* dchunks = nullptr;                  // guard-fill work: (block, first guard cell)
+    int* dguard = nullptr;                    // guard-cell box indices, class by class
+    int* dguardOff = nullptr;                 // per class: first entry in dguard (+ total)
     long nChunks = 0;
     std::vector<int*> dlist;                  // per fine level: its blocks
     std::vector<long*> drstart;               // per fine level: covered parent cells before each
@@ -1895,7 +1897,7 @@ int tune_flux_stages(int nTiles, double dt, int reps, fx_tune_result* out, int c
 void amr_free() {
   Lib::Amr& A = g.amr;
   void* bufs[] = {A.dblocks, A.dlevels, A.dboxStart, A.dcellStart, A.dwBox, A.dwCell, A.flux,
-                  A.pavg, A.dchunks};
+                  A.pavg, A.dchunks, A.dguard, A.dguardOff};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (int* q : A.dindex) if (q) (void)hipFree(q);
@@ -1930,6 +1932,8 @@ AmrLaunch amr_launch() {
   L.perMass = A.perMass;
   L.status = g.status;
   L.onlyVar = -1;
+  L.guard = A.dguard;
+  L.guardOff = A.dguardOff;
   return L;
 }
 
@@ -2523,12 +2527,29 @@ int fx_amr_setup(const fx_tile* tiles, const fx_amr_block* blocks, int nBlocks, 
   int rc = upload_vec(&A.dboxStart, box, st);
   if (!rc) rc = upload_vec(&A.dcellStart, g.hcellStart, st);
   if (!rc) rc = upload_vec(&A.dwBox, wave_table(box), st);
-  {  // guard-fill chunks: kThreads consecutive box cells of one block each
+  {  // guard-fill work: each class's guard cells in box order, and chunks of
+     // kThreads guard cells of one block
+    std::vector<int> glist, goff(1, 0);
+    for (const DevParams& P : g.hcls) {
+      for (long r = 0; r < P.ncell; ++r) {
+        const int loc[3] = {(int)(r % P.nx), (int)((r / P.nx) % P.ny),
+                            (int)(r / ((long)P.nx * P.ny))};
+        bool inside = true;
+        for (int a = 0; a < P.ndim; ++a)
+          if (loc[a] < P.lo[a] || loc[a] > P.hi[a]) inside = false;
+        if (!inside) glist.push_back((int)r);
+      }
+      goff.push_back((int)glist.size());
+    }
     std::vector<int2> ch;
-    for (int b = 0; b < nBlocks; ++b)
-      for (long c = 0; c < box[b + 1] - box[b]; c += kThreads) ch.push_back(make_int2(b, (int)c));
+    for (int b = 0; b < nBlocks; ++b) {
+      const int c = A.hblocks[b].cls;
+      for (int o = 0; o < goff[c + 1] - goff[c]; o += kThreads) ch.push_back(make_int2(b, o));
+    }
     A.nChunks = (long)ch.size();
     if (!rc) rc = upload_vec(&A.dchunks, ch, st);
+    if (!rc) rc = upload_vec(&A.dguard, glist, st);
+    if (!rc) rc = upload_vec(&A.dguardOff, goff, st);
   }
   if (!rc) rc = upload_vec(&A.dwCell, wave_table(g.hcellStart), st);
   for (int L = 0; !rc && L < nLevels; ++L) {
