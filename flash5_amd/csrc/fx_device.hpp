@@ -537,33 +537,54 @@ __device__ __forceinline__ int hll(const DevParams& P, const R* Vm, const R* Vp,
 // follows from one already taken: the rarefaction derivative reuses the
 // value's power, the density behind a rarefaction is gamma p / c^2, and the
 // fan pressure's power is the density's times (c/ck)^2.
-__device__ __forceinline__ void exact_fk(double p, double rk, double pk, double ck, double g,
-                                         double& f, double& fd) {
+
+// R = double is the FP64 solver.  R = float serves the mixed-precision
+// mode: the pressure function, sampling and flux run in FP32, while the
+// Newton iterate and its convergence test stay in FP64 (the iterate is
+// rounded to FP32 only to evaluate the function).  Every R = double
+// expression is the FP64 code's, in the same order.
+
+// x^e for x > 0 as exp2(e log2 x).  In FP64 this is about 130 instructions
+// against pow's 230 (pow also handles signs, integer exponents and extra
+// precision that positive pressure and sound-speed ratios do not need); its
+// error, a few ulp, is far below the solver's 1e-12 convergence tolerance.
+// In FP32 it is the hardware log2 and exp2.  x = 0 gives 0, as pow does for
+// e > 0.
+__device__ __forceinline__ double pow_pos(double x, double e) { return exp2(e * log2(x)); }
+__device__ __forceinline__ float pow_pos(float x, float e) { return exp2f(e * log2f(x)); }
+
+// Tolerance floor of the FP32 solve.  FP32 evaluations of the pressure
+// function leave rounding noise of up to a few 1e-7 in p*, where Newton's
+// relative change can stall; a change of 1e-5 already puts the next iterate
+// (quadratic convergence) below FP32 resolution.
+constexpr double kExactTolF32 = 1.e-5;
+
+template <typename R>
+__device__ __forceinline__ void exact_fk(R p, R rk, R pk, R ck, R g, R& f, R& fd) {
   if (p <= pk) {
     // rarefaction: the derivative's power is the value's divided by pratio,
     // since (g - 1)/(2g) + (g + 1)/(2g) = 1 (one pow instead of two)
-    const double pratio = p / pk;
-    const double q = pow(pratio, (g - 1.) / (2. * g));
-    f = 2. / (g - 1.) * ck * (q - 1.);
-    fd = (1. / (rk * ck)) * (q / pratio);
+    const R pratio = p / pk;
+    const R q = pow_pos(pratio, (g - R(1.)) / (R(2.) * g));
+    f = R(2.) / (g - R(1.)) * ck * (q - R(1.));
+    fd = (R(1.) / (rk * ck)) * (q / pratio);
   } else {
-    const double ak = 2. / ((g + 1.) * rk);
-    const double bk = (g - 1.) / (g + 1.) * pk;
-    const double qrt = sqrt(ak / (bk + p));
+    const R ak = R(2.) / ((g + R(1.)) * rk);
+    const R bk = (g - R(1.)) / (g + R(1.)) * pk;
+    const R qrt = sqrt(ak / (bk + p));
     f = (p - pk) * qrt;
-    fd = (1. - 0.5 * (p - pk) / (bk + p)) * qrt;
+    fd = (R(1.) - R(0.5) * (p - pk) / (bk + p)) * qrt;
   }
 }
 
-__device__ __forceinline__ double two_shock_guess(double g, double rl, double ul, double pl,
-                                                  double cl, double rr, double ur, double pr,
-                                                  double cr) {
-  const double ppv = 0.5 * (pl + pr) - 0.125 * (ur - ul) * (rl + rr) * (cl + cr);
-  const double p0 = fmax(1.e-30, ppv);
-  const double gl = sqrt((2. / ((g + 1.) * rl)) / ((g - 1.) / (g + 1.) * pl + p0));
-  const double gr = sqrt((2. / ((g + 1.) * rr)) / ((g - 1.) / (g + 1.) * pr + p0));
-  const double pts = (gl * pl + gr * pr - (ur - ul)) / (gl + gr);
-  return fmax(1.e-30, pts);
+template <typename R>
+__device__ __forceinline__ R two_shock_guess(R g, R rl, R ul, R pl, R cl, R rr, R ur, R pr, R cr) {
+  const R ppv = R(0.5) * (pl + pr) - R(0.125) * (ur - ul) * (rl + rr) * (cl + cr);
+  const R p0 = fmax(R(1.e-30), ppv);
+  const R gl = sqrt((R(2.) / ((g + R(1.)) * rl)) / ((g - R(1.)) / (g + R(1.)) * pl + p0));
+  const R gr = sqrt((R(2.) / ((g + R(1.)) * rr)) / ((g - R(1.)) / (g + R(1.)) * pr + p0));
+  const R pts = (gl * pl + gr * pr - (ur - ul)) / (gl + gr);
+  return fmax(R(1.e-30), pts);
 }
 
 // Newton from the better of two starts: the pressure function is evaluated
@@ -571,21 +592,22 @@ __device__ __forceinline__ double two_shock_guess(double g, double rl, double ul
 // |f(p)| (its f and f' serve the first step).  pb <= 0 means no second
 // start: plain Newton from pa, in the CPU oracle's operation order.
 // Returns iterations, negative when maxit was reached; pick = 0 when pa was
-// taken.
-__device__ __forceinline__ int exact_newton_pick(double g, double rl, double pl, double cl,
-                                                 double rr, double pr, double cr, double du,
+// taken.  The iterate p is FP64 whatever R is.
+template <typename R>
+__device__ __forceinline__ int exact_newton_pick(R g, R rl, R pl, R cl, R rr, R pr, R cr, R du,
                                                  double pa, double pb, double tol, int maxit,
                                                  double& pstar, int& pick) {
-  double fl, fld, fr, frd;
-  exact_fk(pa, rl, pl, cl, g, fl, fld);
-  exact_fk(pa, rr, pr, cr, g, fr, frd);
-  double p = pa, f = fl + fr + du, fd = fld + frd;
+  R fl, fld, fr, frd;
+  exact_fk(R(pa), rl, pl, cl, g, fl, fld);
+  exact_fk(R(pa), rr, pr, cr, g, fr, frd);
+  double p = pa;
+  R f = fl + fr + du, fd = fld + frd;
   pick = 0;
   if (pb > 0.) {
-    double gl, gld, gr, grd;
-    exact_fk(pb, rl, pl, cl, g, gl, gld);
-    exact_fk(pb, rr, pr, cr, g, gr, grd);
-    const double fb = gl + gr + du;
+    R gl, gld, gr, grd;
+    exact_fk(R(pb), rl, pl, cl, g, gl, gld);
+    exact_fk(R(pb), rr, pr, cr, g, gr, grd);
+    const R fb = gl + gr + du;
     if (!(fabs(f) <= fabs(fb))) {
       pick = 1;
       p = pb;
@@ -595,12 +617,12 @@ __device__ __forceinline__ int exact_newton_pick(double g, double rl, double pl,
   }
   for (int it = 1; it <= maxit; ++it) {
     if (it > 1) {
-      exact_fk(p, rl, pl, cl, g, fl, fld);
-      exact_fk(p, rr, pr, cr, g, fr, frd);
+      exact_fk(R(p), rl, pl, cl, g, fl, fld);
+      exact_fk(R(p), rr, pr, cr, g, fr, frd);
       f = fl + fr + du;
       fd = fld + frd;
     }
-    double pn = p - f / fd;
+    double pn = p - double(f / fd);
     if (!(pn > 0.)) pn = 1.e-30;
     const double change = 2. * fabs((pn - p) / (pn + p));
     p = pn;
@@ -619,51 +641,50 @@ struct ExactStats {
 
 // state at x/t = 0 (Toro sec. 4.5); leftside tells which input supplies the
 // tangential velocities and gammas
-__device__ __forceinline__ void exact_sample(double g, double rl, double ul, double pl, double cl,
-                                             double rr, double ur, double pr, double cr, double pm,
-                                             double um, double& r, double& u, double& pp,
-                                             bool& leftside) {
-  if (0. <= um) {
+template <typename R>
+__device__ __forceinline__ void exact_sample(R g, R rl, R ul, R pl, R cl, R rr, R ur, R pr, R cr,
+                                             R pm, R um, R& r, R& u, R& pp, bool& leftside) {
+  if (R(0.) <= um) {
     leftside = true;
     if (pm <= pl) {
-      const double shl = ul - cl;
-      if (0. <= shl) { r = rl; u = ul; pp = pl; return; }
-      const double cml = cl * pow(pm / pl, (g - 1.) / (2. * g));
-      const double stl = um - cml;
+      const R shl = ul - cl;
+      if (R(0.) <= shl) { r = rl; u = ul; pp = pl; return; }
+      const R cml = cl * pow_pos(pm / pl, (g - R(1.)) / (R(2.) * g));
+      const R stl = um - cml;
       // isentropic: rho = gamma p / c^2 behind the rarefaction
-      if (0. > stl) { r = g * pm / (cml * cml); u = um; pp = pm; return; }
-      const double c = 2. / (g + 1.) * (cl + (g - 1.) / 2. * (ul - 0.));
-      u = 2. / (g + 1.) * (cl + (g - 1.) / 2. * ul + 0.);
+      if (R(0.) > stl) { r = g * pm / (cml * cml); u = um; pp = pm; return; }
+      const R c = R(2.) / (g + R(1.)) * (cl + (g - R(1.)) / R(2.) * (ul - R(0.)));
+      u = R(2.) / (g + R(1.)) * (cl + (g - R(1.)) / R(2.) * ul + R(0.));
       // in the fan, (c/cl)^(2g/(g-1)) = (c/cl)^(2/(g-1)) (c/cl)^2
-      const double q = pow(c / cl, 2. / (g - 1.)), s2 = (c / cl) * (c / cl);
+      const R q = pow_pos(c / cl, R(2.) / (g - R(1.))), s2 = (c / cl) * (c / cl);
       r = rl * q;
       pp = pl * (q * s2);
       return;
     }
-    const double pml = pm / pl;
-    const double sl = ul - cl * sqrt((g + 1.) / (2. * g) * pml + (g - 1.) / (2. * g));
-    if (0. <= sl) { r = rl; u = ul; pp = pl; return; }
-    r = rl * (pml + (g - 1.) / (g + 1.)) / (pml * (g - 1.) / (g + 1.) + 1.);
+    const R pml = pm / pl;
+    const R sl = ul - cl * sqrt((g + R(1.)) / (R(2.) * g) * pml + (g - R(1.)) / (R(2.) * g));
+    if (R(0.) <= sl) { r = rl; u = ul; pp = pl; return; }
+    r = rl * (pml + (g - R(1.)) / (g + R(1.))) / (pml * (g - R(1.)) / (g + R(1.)) + R(1.));
     u = um; pp = pm;
     return;
   }
   leftside = false;
   if (pm > pr) {
-    const double pmr = pm / pr;
-    const double sr = ur + cr * sqrt((g + 1.) / (2. * g) * pmr + (g - 1.) / (2. * g));
-    if (0. >= sr) { r = rr; u = ur; pp = pr; return; }
-    r = rr * (pmr + (g - 1.) / (g + 1.)) / (pmr * (g - 1.) / (g + 1.) + 1.);
+    const R pmr = pm / pr;
+    const R sr = ur + cr * sqrt((g + R(1.)) / (R(2.) * g) * pmr + (g - R(1.)) / (R(2.) * g));
+    if (R(0.) >= sr) { r = rr; u = ur; pp = pr; return; }
+    r = rr * (pmr + (g - R(1.)) / (g + R(1.))) / (pmr * (g - R(1.)) / (g + R(1.)) + R(1.));
     u = um; pp = pm;
     return;
   }
-  const double shr = ur + cr;
-  if (0. >= shr) { r = rr; u = ur; pp = pr; return; }
-  const double cmr = cr * pow(pm / pr, (g - 1.) / (2. * g));
-  const double str = um + cmr;
-  if (0. <= str) { r = g * pm / (cmr * cmr); u = um; pp = pm; return; }
-  const double c = 2. / (g + 1.) * (cr - (g - 1.) / 2. * (ur - 0.));
-  u = 2. / (g + 1.) * (-cr + (g - 1.) / 2. * ur + 0.);
-  const double q = pow(c / cr, 2. / (g - 1.)), s2 = (c / cr) * (c / cr);
+  const R shr = ur + cr;
+  if (R(0.) >= shr) { r = rr; u = ur; pp = pr; return; }
+  const R cmr = cr * pow_pos(pm / pr, (g - R(1.)) / (R(2.) * g));
+  const R str = um + cmr;
+  if (R(0.) <= str) { r = g * pm / (cmr * cmr); u = um; pp = pm; return; }
+  const R c = R(2.) / (g + R(1.)) * (cr - (g - R(1.)) / R(2.) * (ur - R(0.)));
+  u = R(2.) / (g + R(1.)) * (-cr + (g - R(1.)) / R(2.) * ur + R(0.));
+  const R q = pow_pos(c / cr, R(2.) / (g - R(1.))), s2 = (c / cr) * (c / cr);
   r = rr * q;
   pp = pr * (q * s2);
 }
@@ -674,27 +695,30 @@ __device__ __forceinline__ void exact_sample(double g, double rl, double ul, dou
 // the seed and the two-shock guess, and a seeded start that does not
 // converge restarts from the two-shock guess (counted as fallback).  One
 // Newton loop serves both passes, which keeps a seeded kernel's register
-// budget that of the unseeded one.  SEEDED = false ignores guess.
-template <bool SEEDED>
-__device__ __forceinline__ int exact_riemann(const DevParams& P, double g, double rl, double ul,
-                                             double pl, double rr, double ur, double pr,
-                                             double guess, ExactStats& es, double& r, double& u,
-                                             double& pp, bool& left) {
-  if (!(rl > 0. && rr > 0. && pl > 0. && pr > 0.)) return 1;
-  const double cl = sqrt(g * pl / rl), cr = sqrt(g * pr / rr);
-  if (2. / (g - 1.) * (cl + cr) <= ur - ul) return 2;
-  const double du = ur - ul;
-  const double ts = two_shock_guess(g, rl, ul, pl, cl, rr, ur, pr, cr);
+// budget that of the unseeded one.  SEEDED = false ignores guess.  The FP32
+// solve (R = float) converges to max(exactTol, kExactTolF32), and a seed is
+// held at 1e-30 or above, inside FP32's normal range.
+template <bool SEEDED, typename R>
+__device__ __forceinline__ int exact_riemann(const DevParams& P, R g, R rl, R ul, R pl, R rr, R ur,
+                                             R pr, double guess, ExactStats& es, R& r, R& u,
+                                             R& pp, bool& left) {
+  constexpr bool kF32 = sizeof(R) == sizeof(float);
+  if (!(rl > R(0.) && rr > R(0.) && pl > R(0.) && pr > R(0.))) return 1;
+  const R cl = sqrt(g * pl / rl), cr = sqrt(g * pr / rr);
+  if (R(2.) / (g - R(1.)) * (cl + cr) <= ur - ul) return 2;
+  const R du = ur - ul;
+  const R ts = two_shock_guess(g, rl, ul, pl, cl, rr, ur, pr, cr);
+  const double tol = kF32 ? fmax(P.exactTol, kExactTolF32) : P.exactTol;
   double pm;
   int it, pick;
   if constexpr (!SEEDED) {
-    it = exact_newton_pick(g, rl, pl, cl, rr, pr, cr, du, ts, -1.0, P.exactTol, P.exactMaxIter, pm,
-                           pick);
+    it = exact_newton_pick(g, rl, pl, cl, rr, pr, cr, du, double(ts), -1.0, tol, P.exactMaxIter,
+                           pm, pick);
     if (it < 0) es.fallbacks += 1;
   } else {
-    double pb = guess;
+    double pb = (kF32 && guess > 0.) ? fmax(guess, 1.e-30) : guess;
     for (;;) {
-      it = exact_newton_pick(g, rl, pl, cl, rr, pr, cr, du, ts, pb, P.exactTol, P.exactMaxIter,
+      it = exact_newton_pick(g, rl, pl, cl, rr, pr, cr, du, double(ts), pb, tol, P.exactMaxIter,
                              pm, pick);
       if (it < 0 && (pb > 0. || guess <= 0.)) es.fallbacks += 1;   // a restart counts once
       if (it >= 0 || pick == 0) break;
@@ -703,27 +727,29 @@ __device__ __forceinline__ int exact_riemann(const DevParams& P, double g, doubl
     }
   }
   es.iters += (unsigned long long)(it < 0 ? -it : it);
-  double fl, fld, fr, frd;
-  exact_fk(pm, rl, pl, cl, g, fl, fld);
-  exact_fk(pm, rr, pr, cr, g, fr, frd);
-  const double um = 0.5 * (ul + ur) + 0.5 * (fr - fl);
-  exact_sample(g, rl, ul, pl, cl, rr, ur, pr, cr, pm, um, r, u, pp, left);
+  const R pmr = R(pm);
+  R fl, fld, fr, frd;
+  exact_fk(pmr, rl, pl, cl, g, fl, fld);
+  exact_fk(pmr, rr, pr, cr, g, fr, frd);
+  const R um = R(0.5) * (ul + ur) + R(0.5) * (fr - fl);
+  exact_sample(g, rl, ul, pl, cl, rr, ur, pr, cr, pmr, um, r, u, pp, left);
   return 0;
 }
 
 // prim2flx with the direction chosen at run time (same operations and
 // order as prim2flx<DIR>)
-__device__ __forceinline__ void prim2flx_dir(int d, double r, double vx, double vy, double vz,
-                                             double p, double game, double fP, double F[6]) {
-  const double u2 = vx * vx + vy * vy + vz * vz;
-  const double E = 0.5 * r * u2 + p / (game - 1.);
-  const double un = d == 0 ? vx : (d == 1 ? vy : vz);
+template <typename R>
+__device__ __forceinline__ void prim2flx_dir(int d, R r, R vx, R vy, R vz, R p, R game, R fP,
+                                             R F[6]) {
+  const R u2 = vx * vx + vy * vy + vz * vz;
+  const R E = R(0.5) * r * u2 + p / (game - R(1.));
+  const R un = d == 0 ? vx : (d == 1 ? vy : vz);
   F[5] = p;
   F[0] = r * un;
   F[1] = F[0] * vx;
   F[2] = F[0] * vy;
   F[3] = F[0] * vz;
-  const double fp = fP * p;
+  const R fp = fP * p;
   if (d == 0) F[1] = F[1] + fp;
   else if (d == 1) F[2] = F[2] + fp;
   else F[3] = F[3] + fp;
@@ -736,35 +762,40 @@ __device__ __forceinline__ void prim2flx_dir(int d, double r, double vx, double 
 // vely, velz, pres, eint at Lb[q * s] (left) and Rb[q * s] (right); reading
 // them with the run-time offset d keeps them out of private arrays, and one
 // copy of the solver serves all directions.  gamc is the left cell's, game
-// the cells' on either side.  Returns exact_riemann's codes.
-template <bool SEEDED>
+// the cells' on either side.  The solve runs in R; F is FP64 for the
+// update.  Returns exact_riemann's codes.
+template <bool SEEDED, typename R>
 __device__ __forceinline__ int exact_face_flux(const DevParams& P, int d, const double* Lb,
                                                const double* Rb, long s, double gamc,
                                                double gameL, double gameR, double guess,
                                                double F[8], ExactStats& es) {
-  const double rl = Lb[0], pl = Lb[4 * s], rr = Rb[0], pr = Rb[4 * s];
-  const double ul = Lb[(1 + d) * s], ur = Rb[(1 + d) * s];
-  double r, u, pp;
+  const R rl = R(Lb[0]), pl = R(Lb[4 * s]), rr = R(Rb[0]), pr = R(Rb[4 * s]);
+  const R ul = R(Lb[(1 + d) * s]), ur = R(Rb[(1 + d) * s]);
+  R r, u, pp;
   bool left;
-  const int e = exact_riemann<SEEDED>(P, gamc, rl, ul, pl, rr, ur, pr, guess, es, r, u, pp, left);
+  const int e = exact_riemann<SEEDED>(P, R(gamc), rl, ul, pl, rr, ur, pr, guess, es, r, u, pp,
+                                      left);
   if (e) return e;
   // the sampled state takes its tangential velocities and game from the side
   // of the contact the face lies on
   const double* Sb = left ? Lb : Rb;
-  double vx = Sb[s], vy = Sb[2 * s], vz = Sb[3 * s];
+  R vx = R(Sb[s]), vy = R(Sb[2 * s]), vz = R(Sb[3 * s]);
   if (d == 0) vx = u;
   else if (d == 1) vy = u;
   else vz = u;
-  prim2flx_dir(d, r, vx, vy, vz, pp, left ? gameL : gameR, P.fP, F);
+  R Fr[6];
+  prim2flx_dir(d, r, vx, vy, vz, pp, R(left ? gameL : gameR), R(P.fP), Fr);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) F[q] = double(Fr[q]);
   F[6] = 0.;
   F[7] = 0.;
   if (P.aux) {
-    if (F[0] > 0.) {
-      F[7] = F[0] / rl;
-      F[6] = F[0] * Lb[5 * s];
+    if (Fr[0] > R(0.)) {
+      F[7] = double(Fr[0] / rl);
+      F[6] = double(Fr[0] * R(Lb[5 * s]));
     } else {
-      F[7] = F[0] / rr;
-      F[6] = F[0] * Rb[5 * s];
+      F[7] = double(Fr[0] / rr);
+      F[6] = double(Fr[0] * R(Rb[5 * s]));
     }
   }
   return 0;

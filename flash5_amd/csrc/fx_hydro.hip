@@ -432,9 +432,9 @@ k_fluxes(DevParams P, Launch L) {
     double F[8];
     int ierr;
     if constexpr (RS == FX_EXACT) {
-      ierr = exact_face_flux<false>(P, d, S + bl, S + 6 * bc + br, bc, U[P.voff[FX_GAMC] + cl],
-                                    U[P.voff[FX_GAME] + cl], U[P.voff[FX_GAME] + cr], -1.0, F,
-                                    es);
+      ierr = exact_face_flux<false, double>(P, d, S + bl, S + 6 * bc + br, bc,
+                                            U[P.voff[FX_GAMC] + cl], U[P.voff[FX_GAME] + cl],
+                                            U[P.voff[FX_GAME] + cr], -1.0, F, es);
     } else {
       double VL[8], VR[8];
       VL[P_DENS] = S[0 * bc + bl];
@@ -1454,6 +1454,21 @@ int record(int n) {
   return hipEventRecord(g.ev[n], g.stream) == hipSuccess ? 0 : 4;
 }
 
+// The exact solver's face stage over `faces` faces, seeded when the learned
+// seed is on; R = float in the mixed-precision mode.
+template <typename R>
+void launch_face_solve(const DevParams& P, const FaceLaunch& F, long faces, bool multi) {
+  const dim3 gf(grid_for(faces));
+  if (multi && P.learned)
+    hipLaunchKernelGGL((k_face_solve<true, true, R>), gf, dim3(kThreads), 0, g.stream, P, F);
+  else if (multi)
+    hipLaunchKernelGGL((k_face_solve<false, true, R>), gf, dim3(kThreads), 0, g.stream, P, F);
+  else if (P.learned)
+    hipLaunchKernelGGL((k_face_solve<true, false, R>), gf, dim3(kThreads), 0, g.stream, P, F);
+  else
+    hipLaunchKernelGGL((k_face_solve<false, false, R>), gf, dim3(kThreads), 0, g.stream, P, F);
+}
+
 // Flux stages of a step on the uploaded tiles (march path): the column march,
 // then the exact solver's face stage (records events 1 and 2).  Launches
 // only: scratch and work items are in place.
@@ -1484,20 +1499,14 @@ int launch_flux_stages(int nTiles, double dt, int TX, int TY) {
     g.lastFaces = faces;
     g.lastTiles = nTiles;
     g.lastMulti = g.multi;
-    const dim3 gf(grid_for(faces));
-    if (g.multi && P.learned) {
+    if (g.multi && P.learned)
       hipLaunchKernelGGL(k_face_seed<true>, wave_grid(faces), dim3(kLearnThreads), 0, g.stream, P,
                          F, g.seedPacked, g.seedY, g.strongList, g.strongCount, faces);
-      hipLaunchKernelGGL((k_face_solve<true, true>), gf, dim3(kThreads), 0, g.stream, P, F);
-    } else if (g.multi) {
-      hipLaunchKernelGGL((k_face_solve<false, true>), gf, dim3(kThreads), 0, g.stream, P, F);
-    } else if (P.learned) {
+    else if (P.learned)
       hipLaunchKernelGGL(k_face_seed<false>, wave_grid(faces), dim3(kLearnThreads), 0, g.stream,
                          P, F, g.seedPacked, g.seedY, g.strongList, g.strongCount, faces);
-      hipLaunchKernelGGL((k_face_solve<true, false>), gf, dim3(kThreads), 0, g.stream, P, F);
-    } else {
-      hipLaunchKernelGGL((k_face_solve<false, false>), gf, dim3(kThreads), 0, g.stream, P, F);
-    }
+    if (P.precision == FX_PREC_MIXED) launch_face_solve<float>(P, F, faces, g.multi);
+    else launch_face_solve<double>(P, F, faces, g.multi);
     if (record(2)) return 4;
   } else if (P.riemann == FX_HLLC) {
     if (g.multi) launch_march_multi<FX_HLLC>(P.ndim, P.precision, TX, M);

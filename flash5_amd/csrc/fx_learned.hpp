@@ -263,9 +263,11 @@ __device__ __forceinline__ FaceRef face_ref(const DevParams& P, int t, long rt) 
 }
 
 // One code path serves the three directions: the face-normal velocity and
-// the flux component that takes the pressure are picked at run time.
+// the flux component that takes the pressure are picked at run time.  R is
+// the solver's arithmetic: double, or float in the mixed-precision mode
+// (FP64 Newton iterate and convergence test, FP64 fluxes out).
 // (3 waves per SIMD fit without spilling the solver's registers)
-template <bool SEEDED, bool MULTI>
+template <bool SEEDED, bool MULTI, typename R>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3)))
 k_face_solve(DevParams P0, FaceLaunch F) {
   const long total = face_total<MULTI>(P0, F);
@@ -289,9 +291,9 @@ k_face_solve(DevParams P0, FaceLaunch F) {
     }
     const gdouble* U = as_global(F.tiles[f.t].U);
     double Fo[8];
-    const int ierr = exact_face_flux<SEEDED>(P, d, Lb, Rb, nf, U[P.voff[FX_GAMC] + f.cL],
-                                             U[P.voff[FX_GAME] + f.cL], U[P.voff[FX_GAME] + f.cR],
-                                             guess, Fo, es);
+    const int ierr = exact_face_flux<SEEDED, R>(P, d, Lb, Rb, nf, U[P.voff[FX_GAMC] + f.cL],
+                                                U[P.voff[FX_GAME] + f.cL],
+                                                U[P.voff[FX_GAME] + f.cR], guess, Fo, es);
     if (ierr) {
       report_error(F.status, 2, f.t, (int)(f.cR % P.nx), (int)((f.cR / P.nx) % P.ny),
                    (int)(f.cR / ((long)P.nx * P.ny)), d);

@@ -152,9 +152,10 @@ def test_fluxes_then_update_solution_matches_oracle():
 @pytest.mark.parametrize("path", [0, 1], ids=["fused", "three-kernel"])
 @pytest.mark.parametrize("ndim,n", [(3, 8), (2, 16)])
 def test_fp64_exact_riemann_solver_matches_oracle(ndim, n, path):
-    """Exact solver in place of HLLC.  The device pow and the host libm pow
-    may round differently in the last place, so the bar is the 1e-12 L1 one,
-    not bitwise; iteration counts must agree closely."""
+    """Exact solver in place of HLLC.  The device takes powers as
+    exp2(e log2 x) and the oracle calls libm pow; the two round differently
+    in the last places, so the bar is the 1e-12 L1 one, not bitwise;
+    iteration counts must agree closely."""
     hp = HydroParams(ndim=ndim, nxb=n, nyb=n, nzb=n, gamma=1.4, RiemannSolver="exact")
     U = _blocks(hp, 4 if ndim == 3 else 8, seed=W.SEED_BASE + 13)
     dx = 1.0 / n
@@ -168,17 +169,24 @@ def test_fp64_exact_riemann_solver_matches_oracle(ndim, n, path):
     assert abs(g.newton_iters - o_iters) <= 1e-3 * o_iters, (g.newton_iters, o_iters)
 
 
+@pytest.mark.parametrize("riemann", ["hllc", "exact"])
 @pytest.mark.parametrize("ndim,n", [(3, 16), (2, 16)])
-def test_mixed_precision_l1_against_fp64_oracle(ndim, n):
-    """FP32 reconstruction and HLLC fluxes with the FP64 update: the L1
+def test_mixed_precision_l1_against_fp64_oracle(ndim, n, riemann):
+    """FP32 reconstruction and fluxes (HLLC, or the exact solver with its
+    FP64 Newton iterate and convergence test) with the FP64 update: the L1
     relative error of every variable against the FP64 oracle after one step
     of the batch workload, density at most 1e-6."""
     from flash5_amd.hydro import PREC_MIXED
-    hp = HydroParams(ndim=ndim, nxb=n, nyb=n, nzb=n, gamma=5.0 / 3.0, precision=PREC_MIXED)
+    hp = HydroParams(ndim=ndim, nxb=n, nyb=n, nzb=n, gamma=5.0 / 3.0, precision=PREC_MIXED,
+                     RiemannSolver=riemann)
     U = _blocks(hp, 6 if ndim == 3 else 12, seed=W.SEED_BASE + 2)
     dx = 1.0 / n
     dt = W.cfl_dt(torch.from_numpy(U), ndim, 4, (dx,) * 3, 0.8)
-    Uo, Ug = _run_both(hp, U, dt, dx)
+    stats = []
+    Uo, Ug = _run_both(hp, U, dt, dx, stats=stats)
+    if riemann == "exact":
+        g = stats[-1]
+        assert g.newton_iters > 0 and g.newton_fallbacks == 0
     norms = sfocu.compare(Ug, Uo, 1, NAMES, ndim, 4)
     print({k: f"{v[0]:.2e}" for k, v in norms.items()})
     assert norms["dens"][0] <= 1e-6, norms["dens"]
