@@ -762,15 +762,15 @@ __device__ __forceinline__ void prim2flx_dir(int d, R r, R vx, R vy, R vz, R p, 
 // vely, velz, pres, eint at Lb[q * s] (left) and Rb[q * s] (right); reading
 // them with the run-time offset d keeps them out of private arrays, and one
 // copy of the solver serves all directions.  gamc is the left cell's, game
-// the cells' on either side.  The solve runs in R; F is FP64 for the
-// update.  Returns exact_riemann's codes.
+// the cells' on either side.  The states and the solve are in R; F is FP64
+// for the update.  Returns exact_riemann's codes.
 template <bool SEEDED, typename R>
-__device__ __forceinline__ int exact_face_flux(const DevParams& P, int d, const double* Lb,
-                                               const double* Rb, long s, double gamc,
+__device__ __forceinline__ int exact_face_flux(const DevParams& P, int d, const R* Lb,
+                                               const R* Rb, long s, double gamc,
                                                double gameL, double gameR, double guess,
                                                double F[8], ExactStats& es) {
-  const R rl = R(Lb[0]), pl = R(Lb[4 * s]), rr = R(Rb[0]), pr = R(Rb[4 * s]);
-  const R ul = R(Lb[(1 + d) * s]), ur = R(Rb[(1 + d) * s]);
+  const R rl = Lb[0], pl = Lb[4 * s], rr = Rb[0], pr = Rb[4 * s];
+  const R ul = Lb[(1 + d) * s], ur = Rb[(1 + d) * s];
   R r, u, pp;
   bool left;
   const int e = exact_riemann<SEEDED>(P, R(gamc), rl, ul, pl, rr, ur, pr, guess, es, r, u, pp,
@@ -778,8 +778,8 @@ __device__ __forceinline__ int exact_face_flux(const DevParams& P, int d, const 
   if (e) return e;
   // the sampled state takes its tangential velocities and game from the side
   // of the contact the face lies on
-  const double* Sb = left ? Lb : Rb;
-  R vx = R(Sb[s]), vy = R(Sb[2 * s]), vz = R(Sb[3 * s]);
+  const R* Sb = left ? Lb : Rb;
+  R vx = Sb[s], vy = Sb[2 * s], vz = Sb[3 * s];
   if (d == 0) vx = u;
   else if (d == 1) vy = u;
   else vz = u;
@@ -792,10 +792,10 @@ __device__ __forceinline__ int exact_face_flux(const DevParams& P, int d, const 
   if (P.aux) {
     if (Fr[0] > R(0.)) {
       F[7] = double(Fr[0] / rl);
-      F[6] = double(Fr[0] * R(Lb[5 * s]));
+      F[6] = double(Fr[0] * Lb[5 * s]);
     } else {
       F[7] = double(Fr[0] / rr);
-      F[6] = double(Fr[0] * R(Rb[5 * s]));
+      F[6] = double(Fr[0] * Rb[5 * s]);
     }
   }
   return 0;

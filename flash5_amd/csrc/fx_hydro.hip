@@ -945,6 +945,7 @@ struct Lib {
   DevParams cur{};
   bool multi = false;
   bool lastMulti = false;              // multi state of the last exact-solver step
+  int lastPrec = FX_PREC_FP64;         // its precision (the face states' type)
   std::vector<unsigned char> htcls;
   std::vector<long> hcellStart, hfaceStart;
   unsigned char* dtcls = nullptr;
@@ -1454,11 +1455,18 @@ int record(int n) {
   return hipEventRecord(g.ev[n], g.stream) == hipSuccess ? 0 : 4;
 }
 
-// The exact solver's face stage over `faces` faces, seeded when the learned
-// seed is on; R = float in the mixed-precision mode.
+// The exact solver's face stage over `faces` faces: with the learned seed,
+// the network over the listed strong faces first.  R is the type of the
+// stored face states and of the solve: float in the mixed-precision mode.
 template <typename R>
 void launch_face_solve(const DevParams& P, const FaceLaunch& F, long faces, bool multi) {
   const dim3 gf(grid_for(faces));
+  if (multi && P.learned)
+    hipLaunchKernelGGL((k_face_seed<true, R>), wave_grid(faces), dim3(kLearnThreads), 0, g.stream,
+                       P, F, g.seedPacked, g.seedY, g.strongList, g.strongCount, faces);
+  else if (P.learned)
+    hipLaunchKernelGGL((k_face_seed<false, R>), wave_grid(faces), dim3(kLearnThreads), 0,
+                       g.stream, P, F, g.seedPacked, g.seedY, g.strongList, g.strongCount, faces);
   if (multi && P.learned)
     hipLaunchKernelGGL((k_face_solve<true, true, R>), gf, dim3(kThreads), 0, g.stream, P, F);
   else if (multi)
@@ -1499,12 +1507,7 @@ int launch_flux_stages(int nTiles, double dt, int TX, int TY) {
     g.lastFaces = faces;
     g.lastTiles = nTiles;
     g.lastMulti = g.multi;
-    if (g.multi && P.learned)
-      hipLaunchKernelGGL(k_face_seed<true>, wave_grid(faces), dim3(kLearnThreads), 0, g.stream, P,
-                         F, g.seedPacked, g.seedY, g.strongList, g.strongCount, faces);
-    else if (P.learned)
-      hipLaunchKernelGGL(k_face_seed<false>, wave_grid(faces), dim3(kLearnThreads), 0, g.stream,
-                         P, F, g.seedPacked, g.seedY, g.strongList, g.strongCount, faces);
+    g.lastPrec = P.precision;
     if (P.precision == FX_PREC_MIXED) launch_face_solve<float>(P, F, faces, g.multi);
     else launch_face_solve<double>(P, F, faces, g.multi);
     if (record(2)) return 4;
@@ -2346,12 +2349,21 @@ int fx_hydro_face_states(double* S, long cap, long* n) {
   if (!g.init || g.lastFaces <= 0 || !S || cap < g.lastFaces) return 3;
   FaceLaunch F{g.dtiles, g.lastTiles, g.fstates, nullptr, g.status,
                g.dcls, g.dtcls, g.dfaceStart, g.dwFace};
-  if (g.lastMulti)
-    hipLaunchKernelGGL(k_collect_faces<true>, dim3(grid_for(g.lastFaces)), dim3(kThreads), 0,
-                       g.stream, g.cur, F, g.lastFaces, S);
-  else
-    hipLaunchKernelGGL(k_collect_faces<false>, dim3(grid_for(g.lastFaces)), dim3(kThreads), 0,
-                       g.stream, g.cur, F, g.lastFaces, S);
+  const dim3 gc(grid_for(g.lastFaces));
+  if (g.lastPrec == FX_PREC_MIXED) {
+    if (g.lastMulti)
+      hipLaunchKernelGGL((k_collect_faces<true, float>), gc, dim3(kThreads), 0, g.stream, g.cur, F,
+                         g.lastFaces, S);
+    else
+      hipLaunchKernelGGL((k_collect_faces<false, float>), gc, dim3(kThreads), 0, g.stream, g.cur,
+                         F, g.lastFaces, S);
+  } else if (g.lastMulti) {
+    hipLaunchKernelGGL((k_collect_faces<true, double>), gc, dim3(kThreads), 0, g.stream, g.cur, F,
+                       g.lastFaces, S);
+  } else {
+    hipLaunchKernelGGL((k_collect_faces<false, double>), gc, dim3(kThreads), 0, g.stream, g.cur, F,
+                       g.lastFaces, S);
+  }
   return sync_or_async(nullptr);
 }
 

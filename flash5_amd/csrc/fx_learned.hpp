@@ -195,7 +195,8 @@ __global__ void __launch_bounds__(kThreads) k_riemann_batch(
 struct FaceLaunch {
   const fx_tile* tiles;
   int nTiles;
-  const double* fstates;   // [tile][dir][comp][face], from the march's RS_STATES stage
+  const void* fstates;     // [tile][dir][comp][face], from the march's RS_STATES stage;
+                           // double, or float in the mixed-precision mode
   const float* y;          // network seed per face (flattened like gid), or null
   DevStatus* status;
   // tiles of several block classes (the MULTI kernels): parameters per class,
@@ -236,11 +237,19 @@ __device__ __forceinline__ const DevParams& face_tile(const DevParams& P0, const
   }
 }
 
-// offset of tile t's face states in F.fstates
+// offset (in elements) of tile t's face states in F.fstates
 template <bool MULTI>
 __device__ __forceinline__ long face_state_offset(const DevParams& P, const FaceLaunch& F, int t) {
   if constexpr (MULTI) return kFaceComps * F.faceStart[t];
   else return t * fstate_tile_stride(P);
+}
+
+// tile t's face states of direction d, stored as S
+template <typename S, bool MULTI>
+__device__ __forceinline__ const S* face_states_at(const DevParams& P, const FaceLaunch& F, int t,
+                                                   int d) {
+  return static_cast<const S*>(F.fstates) + face_state_offset<MULTI>(P, F, t) +
+         fstate_dir_offset(P, d);
 }
 
 // face rt of tile t (index arithmetic in 32 bits: a tile holds < 2^31 faces)
@@ -279,11 +288,11 @@ k_face_solve(DevParams P0, FaceLaunch F) {
     const DevParams& P = face_tile<MULTI>(P0, F, gid, t, rt);
     const FaceRef f = face_ref(P, t, rt);
     const int d = f.d;
-    const double* b = F.fstates + face_state_offset<MULTI>(P, F, t) + fstate_dir_offset(P, d);
+    const R* b = face_states_at<R, MULTI>(P, F, t, d);
     const long nf = P.nface[d];
     // face states: comps 0..5 dens, velx, vely, velz, pres, eint; left, then right
-    const double* Lb = b + f.r;
-    const double* Rb = b + 6 * nf + f.r;
+    const R* Lb = b + f.r;
+    const R* Rb = b + 6 * nf + f.r;
     double guess = -1.0;
     if (SEEDED) {
       const float y = F.y[gid];
@@ -309,7 +318,7 @@ k_face_solve(DevParams P0, FaceLaunch F) {
 // Network seed ln(p*/P0) of the faces the march listed as strong (at or
 // above the gate); every other face already holds "no seed".  The list
 // length is read on the device, so the launch needs no host sync.
-template <bool MULTI>
+template <bool MULTI, typename S>
 __global__ void __launch_bounds__(kLearnThreads) k_face_seed(DevParams P0, FaceLaunch F,
                                                              const MlpPacked* __restrict__ W,
                                                              float* __restrict__ y,
@@ -330,7 +339,7 @@ __global__ void __launch_bounds__(kLearnThreads) k_face_seed(DevParams P0, FaceL
       long rt;
       const DevParams& P = face_tile<MULTI>(P0, F, gid, t, rt);
       const FaceRef f = face_ref(P, t, rt);
-      const double* b = F.fstates + face_state_offset<MULTI>(P, F, t) + fstate_dir_offset(P, f.d);
+      const S* b = face_states_at<S, MULTI>(P, F, t, f.d);
       const long nf = P.nface[f.d];
       rl = b[0 * nf + f.r];
       ul = b[(1 + f.d) * nf + f.r];
@@ -358,23 +367,24 @@ __global__ void __launch_bounds__(kLearnThreads) k_face_seed(DevParams P0, FaceL
 __global__ void k_zero_count(unsigned* c) { c[threadIdx.x] = 0u; }
 
 // face states of the last exact-solver step as Riemann problems, SoA [6][n]
-// (rhoL, uL, pL, rhoR, uR, pR with u the face-normal velocity)
-template <bool MULTI>
+// (rhoL, uL, pL, rhoR, uR, pR with u the face-normal velocity); S is the
+// type they were stored in
+template <bool MULTI, typename S>
 __global__ void __launch_bounds__(kThreads) k_collect_faces(DevParams P0, FaceLaunch F, long n,
-                                                            double* __restrict__ S) {
+                                                            double* __restrict__ out) {
   for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < n;
        gid += (long)gridDim.x * blockDim.x) {
     int t;
     long rt;
     const DevParams& P = face_tile<MULTI>(P0, F, gid, t, rt);
     const FaceRef f = face_ref(P, t, rt);
-    const double* b = F.fstates + face_state_offset<MULTI>(P, F, t) + fstate_dir_offset(P, f.d);
+    const S* b = face_states_at<S, MULTI>(P, F, t, f.d);
     const long nf = P.nface[f.d];
-    S[gid] = b[0 * nf + f.r];
-    S[n + gid] = b[(1 + f.d) * nf + f.r];
-    S[2 * n + gid] = b[4 * nf + f.r];
-    S[3 * n + gid] = b[6 * nf + f.r];
-    S[4 * n + gid] = b[(7 + f.d) * nf + f.r];
-    S[5 * n + gid] = b[10 * nf + f.r];
+    out[gid] = b[0 * nf + f.r];
+    out[n + gid] = b[(1 + f.d) * nf + f.r];
+    out[2 * n + gid] = b[4 * nf + f.r];
+    out[3 * n + gid] = b[6 * nf + f.r];
+    out[4 * n + gid] = b[(7 + f.d) * nf + f.r];
+    out[5 * n + gid] = b[10 * nf + f.r];
   }
 }

@@ -63,7 +63,7 @@ struct MarchLaunch {
   int tilesX, tilesY;  // column tiles per block
   DevStatus* status;
   double dt;
-  double* fstates;     // RS_STATES: per-face states for the separate Riemann stage
+  void* fstates;       // RS_STATES: per-face states for the separate Riemann stage, in R
   const double* dtp;   // device-resident dt (graph-captured steps), or null
   // RS_STATES with the learned seed: every face's seed slot is set to
   // "no seed", and faces whose strength reaches the gate are appended to
@@ -80,7 +80,8 @@ struct MarchLaunch {
 constexpr int RS_STATES = 99;
 constexpr int kFaceComps = 12;
 
-// face-state scratch of one tile: [dir][comp 0..11][face]
+// face-state scratch of one tile: [dir][comp 0..11][face], in the march's
+// real type R (FP32 in the mixed mode); offsets below count elements
 __device__ __forceinline__ long fstate_tile_stride(const DevParams& P) {
   return (long)kFaceComps * (P.nface[0] + P.nface[1] + P.nface[2]);
 }
@@ -91,13 +92,13 @@ __device__ __forceinline__ long fstate_dir_offset(const DevParams& P, int d) {
 }
 
 template <int D, typename R>
-__device__ __forceinline__ void write_face_states(const DevParams& P, double* base, long r,
+__device__ __forceinline__ void write_face_states(const DevParams& P, R* base, long r,
                                                   const R VL[8], const R VR[8]) {
   const long nf = P.nface[D];
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
-    base[q * nf + r] = double(VL[comp_slot(q)]);
-    base[(6 + q) * nf + r] = double(VR[comp_slot(q)]);
+    base[q * nf + r] = VL[comp_slot(q)];
+    base[(6 + q) * nf + r] = VR[comp_slot(q)];
   }
 }
 
@@ -243,7 +244,7 @@ __device__ __forceinline__ R flat_tilde_vals(R pm2, R pm1, R pp1, R pp2, R vm1, 
 // tile t's face-state scratch (RS_STATES).
 template <int NDIM, int TX, int TY, int NT, int RS, typename R>
 __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch& L, int t, int ox,
-                                           int oy, double* fsb, long faceBase, ExactStats& es,
+                                           int oy, R* fsb, long faceBase, ExactStats& es,
                                            R* smem) {
   using I = TileIdx<TX, TY>;
   using M = MarchLds<NDIM, TX, TY>;
@@ -613,7 +614,7 @@ k_march(DevParams P, MarchLaunch L) {
   const int ox = P.lo[0] + (tr % L.tilesX) * TX;
   const int oy = P.lo[1] + (tr / L.tilesX) * TY;
   ExactStats es{0ULL, 0ULL};
-  double* fsb = RS == RS_STATES ? L.fstates + t * fstate_tile_stride(P) : nullptr;
+  R* fsb = RS == RS_STATES ? static_cast<R*>(L.fstates) + t * fstate_tile_stride(P) : nullptr;
   const long faceBase = (long)t * (P.nface[0] + P.nface[1] + P.nface[2]);
   __shared__ R smem[MarchLds<NDIM, TX, TY>::total];
   march_tile<NDIM, TX, TY, NT, RS, R>(P, L, t, ox, oy, fsb, faceBase, es, smem);
@@ -641,7 +642,7 @@ k_march_multi(const DevParams* __restrict__ cls,
     const MarchItem w = items[it];
     const DevParams& P = cls[w.cls];
     const long faceBase = RS == RS_STATES ? faceStart[w.t] : 0;
-    double* fsb = RS == RS_STATES ? L.fstates + kFaceComps * faceBase : nullptr;
+    R* fsb = RS == RS_STATES ? static_cast<R*>(L.fstates) + kFaceComps * faceBase : nullptr;
     march_tile<NDIM, TX, TY, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, faceBase, es, smem);
   }
   flush_exact(L.status, es);
@@ -661,7 +662,7 @@ k_march_mixw(const DevParams* __restrict__ cls, MarchLaunch L,
     const MarchItem w = items[it];
     const DevParams& P = cls[w.cls];
     const long faceBase = RS == RS_STATES ? faceStart[w.t] : 0;
-    double* fsb = RS == RS_STATES ? L.fstates + kFaceComps * faceBase : nullptr;
+    R* fsb = RS == RS_STATES ? static_cast<R*>(L.fstates) + kFaceComps * faceBase : nullptr;
     if (w.tx == 16) march_tile<NDIM, 16, 8, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, faceBase, es, smem);
     else march_tile<NDIM, 8, 8, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, faceBase, es, smem);
   }

@@ -506,7 +506,8 @@ class Hydro:
             raise FxError("fx_hydro_set_seed_params", rc=rc)
 
     def face_states(self):
-        """Riemann problems of the last exact-solver step, (6, nfaces) float64."""
+        """Riemann problems of the last exact-solver step, (6, nfaces) float64
+        (FP32 values after a mixed-precision step, which stores them in FP32)."""
         import torch
         n = C.c_long(0)
         self.lib.fx_hydro_face_states(None, 0, C.byref(n))

@@ -229,7 +229,8 @@ int fx_mlp_set_gate(double gate);
 int fx_hydro_set_seed_params(const float* params);
 /* The face states of the last exact-solver step as Riemann problems in the
  * fx_riemann_batch layout (u = face-normal velocity), for training the seed
- * on states the solver itself produced.  *n receives the face count; returns
+ * on states the solver itself produced (FP32 values after an FX_PREC_MIXED
+ * step, whose face states are FP32).  *n receives the face count; returns
  * nonzero when cap < *n or no such step ran. */
 int fx_hydro_face_states(double* states, long cap, long* n);
 

@@ -119,3 +119,29 @@ def test_learned_seed_inside_hydro_step():
     # weak faces keep the two-shock start; the seed may only help the rest
     assert st1.newton_iters <= st0.newton_iters
     hy2.set_seed_params(None)
+
+
+def test_mixed_mode_face_states_are_fp32():
+    """The mixed-precision exact step stores its face states in FP32: the
+    collected states are FP32 numbers, within FP32 reconstruction error of the
+    FP64 step's states on the same blocks; the seeded mixed step converges
+    with no fallbacks."""
+    from flash5_amd import workloads as W
+    from flash5_amd.hydro import PREC_MIXED, Hydro, HydroParams
+    n = 8
+    U = W.fourier_blocks(8, n=n, gamma=G, seed=W.SEED_BASE + 4).numpy()
+    dx = 1.0 / n
+    dt = W.cfl_dt(torch.from_numpy(U), 3, 4, (dx,) * 3, 0.8)
+    states = {}
+    for prec in ("fp64", PREC_MIXED):
+        kw = {} if prec == "fp64" else {"precision": PREC_MIXED}
+        hy = Hydro(HydroParams(ndim=3, nxb=n, nyb=n, nzb=n, gamma=G, RiemannSolver="exact", **kw))
+        st = hy.advance_batch(torch.from_numpy(U.copy()).cuda(), dt, (dx,) * 3)
+        assert st.code == 0 and st.newton_fallbacks == 0
+        states[prec] = hy.face_states().cpu().numpy()
+    s64, s32 = states["fp64"], states[PREC_MIXED]
+    assert s32.shape == s64.shape == (6, 8 * 3 * 9 * 8 * 8)
+    np.testing.assert_array_equal(s32, s32.astype(np.float32).astype(np.float64))
+    assert not np.array_equal(s32, s64)
+    scale = np.abs(s64).max(axis=1, keepdims=True)
+    assert (np.abs(s32 - s64) <= 1e-5 * scale).all(), np.abs(s32 - s64).max()
