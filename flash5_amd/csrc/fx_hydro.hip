@@ -735,9 +735,11 @@ __global__ void __launch_bounds__(kThreads) k_dt(DevParams P, const double* __re
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// dt = cfl / max of the partial maxima, kept on the device; t += dt
+// dt = cfl / max of the partial maxima, kept on the device; t += dt.  With
+// cflDev set, the CFL is cflDev[0] (shockLowerCFL's CFL of this step).
 __global__ void __launch_bounds__(kThreads) k_dt_final(const double* __restrict__ partial, int n,
-                                                       double cfl, double* dtDev, double* tDev) {
+                                                       double cfl, const double* cflDev,
+                                                       double* dtDev, double* tDev) {
   __shared__ double red[kThreads];
   double m = 0.;
   for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, partial[i]);
@@ -748,11 +750,23 @@ __global__ void __launch_bounds__(kThreads) k_dt_final(const double* __restrict_
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    const double dt = red[0] != 0. ? cfl / red[0] : 1.e300;
+    const double c = cflDev ? cflDev[0] : cfl;
+    const double dt = red[0] != 0. ? c / red[0] : 1.e300;
     dtDev[0] = dt;
     tDev[0] = tDev[0] + dt;
   }
 }
+
+// shockLowerCFL (Hydro.F90:104-108 and 128-141, hy_shockDetectBlk.F90:
+// 228-240): each step resets the CFL to cfl before the detector, a flagged
+// cell lowers it to min(cfl, cflMax), and the recovery before the fluxes
+// raises it by 1.2 up to cfl, so the next dt's CFL is min(cfl, 1.2 cflMax)
+// after a step with a flagged cell and cfl otherwise.
+__global__ void k_shock_cfl(const DevStatus* s, double cfl, double cap, double* cflNext) {
+  cflNext[0] = s->shockTiles ? fmin(cfl, cap) : cfl;
+}
+
+__global__ void k_zero_shock_tiles(DevStatus* s) { s->shockTiles = 0; }
 
 __global__ void k_reset_status(DevStatus* s) {
   s->code = 0;
@@ -923,7 +937,8 @@ struct Lib {
   double* fstates = nullptr;           // face states for the separate Riemann stage
   size_t fstatesCap = 0;
   const double* dtDev = nullptr;       // set while capturing steps whose dt lives on the device
-  double* dtBuf = nullptr;             // [dt, t] of graph-captured runs
+  double* dtBuf = nullptr;             // [dt, t, next dt's CFL] of graph-captured runs
+  double runCfl = 0.;                  // the CFL argument of the graph-captured run
   hipStream_t capStream = nullptr;     // library stream for graph capture (default priority)
   std::vector<int> hnbr;               // neighbour table last uploaded (host copy)
   int* dnbr = nullptr;
@@ -1156,6 +1171,9 @@ int validate(const fx_params* p) {
   return 0;
 }
 
+// hy_shockDetectBlk.F90:228-235: the CFL cap of a step with a flagged cell
+double shock_cfl_max(int ndim) { return ndim == 1 ? 0.60 : (ndim == 2 ? 0.45 : 0.25); }
+
 int finish(fx_status* st) {
   if (g.async) return 0;
   HIPCALL(hipStreamSynchronize(g.stream), st);
@@ -1172,10 +1190,24 @@ int finish(fx_status* st) {
     st->newton_iters = (long long)h.newtonIters;
     st->newton_fallbacks = (long long)h.newtonFallbacks;
     // hy_shockDetectBlk.F90:211-227: the CFL cap of a step with a flagged cell
-    if (g.prm.shockLowerCFL && h.shockTiles)
-      st->min_cell_cfl = g.prm.ndim == 1 ? 0.60 : (g.prm.ndim == 2 ? 0.45 : 0.25);
+    if (g.prm.shockLowerCFL && h.shockTiles) st->min_cell_cfl = shock_cfl_max(g.prm.ndim);
   }
   return h.code;
+}
+
+// Inside a graph-captured run under shockLowerCFL: the device slot holding
+// the next dt's CFL, or null (dt takes the CFL argument).
+const double* shock_cfl_dev() {
+  return g.prm.shockLowerCFL && g.dtDev ? g.dtDev + 2 : nullptr;
+}
+
+// Inside a graph-captured run under shockLowerCFL: the next dt's CFL from
+// this step's flags (k_shock_cfl).  Eager callers get the cap in
+// fx_status.min_cell_cfl instead.
+void launch_shock_cfl() {
+  if (!g.prm.shockLowerCFL || !g.dtDev) return;   // g.dtDev == g.dtBuf while capturing
+  hipLaunchKernelGGL(k_shock_cfl, dim3(1), dim3(1), 0, g.stream, g.status, g.runCfl,
+                     1.2 * shock_cfl_max(g.prm.ndim), g.dtBuf + 2);
 }
 
 // k_shock_detect over the uploaded tiles, one workgroup each, with the LDS
@@ -2019,7 +2051,7 @@ int amr_launch_dt(double cfl, fx_status* st) {
   L.wtab = g.amr.dwCell;
   hipLaunchKernelGGL(k_amr_dt, dim3(grid), dim3(kThreads), 0, g.stream, L, g.partial);
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kThreads), 0, g.stream, g.partial, grid, cfl,
-                     g.dtBuf, g.dtBuf + 1);
+                     shock_cfl_dev(), g.dtBuf, g.dtBuf + 1);
   return 0;
 }
 
@@ -2031,6 +2063,7 @@ int amr_launch_step(double dt) {
   amr_launch_fill();
   if (g.dp.shokOff >= 0) {  // Hydro.F90:94-123: flag, then fill SHOK_VAR's guard cells
     launch_shock_detect(g.amr.nBlocks);
+    launch_shock_cfl();
     amr_launch_fill(g.prm.shokVar);
   }
   if (launch_flux_stages(g.amr.nBlocks, dt, g.amr.TX, g.amr.TY)) return 4;
@@ -2294,15 +2327,17 @@ int fx_grid_run_graph(double* U, long nBlocks, long blockStride, const int* nbr,
     HIPCALL(hipMalloc((void**)&g.partial, sizeof(double) * dtGrid), st);
     g.partialCap = dtGrid;
   }
-  if (!g.dtBuf) HIPCALL(hipMalloc((void**)&g.dtBuf, 2 * sizeof(double)), st);
+  if (!g.dtBuf) HIPCALL(hipMalloc((void**)&g.dtBuf, 3 * sizeof(double)), st);
   if (!g.capStream) HIPCALL(hipStreamCreateWithFlags(&g.capStream, hipStreamNonBlocking), st);
   const hipStream_t saved = g.stream;
   const int savedAsync = g.async, savedTiming = g.timing;
   // the clock starts at zero; dt and t stay on the device for the whole run
   if (saved) HIPCALL(hipStreamSynchronize(saved), st);
   g.stream = g.capStream;
-  double zero2[2] = {0.0, 0.0};
-  HIPCALL(hipMemcpy(g.dtBuf, zero2, sizeof zero2, hipMemcpyHostToDevice), st);
+  // t = 0, and a graph run starts at the original CFL
+  const double init3[3] = {0.0, 0.0, cfl};
+  HIPCALL(hipMemcpy(g.dtBuf, init3, sizeof init3, hipMemcpyHostToDevice), st);
+  g.runCfl = cfl;
   g.async = 1;
   g.timing = 0;
   g.dtDev = g.dtBuf;
@@ -2311,15 +2346,19 @@ int fx_grid_run_graph(double* U, long nBlocks, long blockStride, const int* nbr,
   int err = 0;
   if (hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) err = 4;
   for (int s = 0; !err && s < stepsPerGraph; ++s) {
-    launch_fill(U, nBlocks, blockStride);
-    if (P.shokOff >= 0) {  // Hydro.F90:94-123: flag, then fill SHOK_VAR's guard cells
-      launch_shock_detect((int)nBlocks);
-      launch_fill(U, nBlocks, blockStride, g.prm.shokVar);
-    }
+    // dt from the interior cells (the guard fill does not change them), with
+    // the CFL the previous step's flags left
     hipLaunchKernelGGL(k_dt, dim3(dtGrid), dim3(kThreads), 0, g.stream, P, U, nBlocks, blockStride,
                        1. / del[0], 1. / del[1], 1. / del[2], g.partial);
     hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kThreads), 0, g.stream, g.partial, dtGrid, cfl,
-                       g.dtBuf, g.dtBuf + 1);
+                       shock_cfl_dev(), g.dtBuf, g.dtBuf + 1);
+    launch_fill(U, nBlocks, blockStride);
+    if (P.shokOff >= 0) {  // Hydro.F90:94-123: flag, then fill SHOK_VAR's guard cells
+      hipLaunchKernelGGL(k_zero_shock_tiles, dim3(1), dim3(1), 0, g.stream, g.status);
+      launch_shock_detect((int)nBlocks);
+      launch_shock_cfl();
+      launch_fill(U, nBlocks, blockStride, g.prm.shokVar);
+    }
     if (run_compute(nullptr, (int)nBlocks, 0.0, nullptr, nullptr, true)) err = 4;
   }
   if (hipStreamEndCapture(g.stream, &graph) != hipSuccess) err = 4;
@@ -2688,7 +2727,7 @@ int fx_amr_setup(const fx_tile* tiles, const fx_amr_block* blocks, int nBlocks, 
     if (st) st->code = rc;
     return rc;
   }
-  if (!g.dtBuf) HIPCALL(hipMalloc((void**)&g.dtBuf, 2 * sizeof(double)), st);
+  if (!g.dtBuf) HIPCALL(hipMalloc((void**)&g.dtBuf, 3 * sizeof(double)), st);
   A.ready = true;
   return 0;
 }
@@ -2801,8 +2840,10 @@ int fx_amr_run_graph(double cfl, int stepsPerGraph, int nReplays, double* tOut, 
   if (saved) HIPCALL(hipStreamSynchronize(saved), st);
   HIPCALL(hipDeviceSynchronize(), st);
   g.stream = g.capStream;
-  double zero2[2] = {0.0, 0.0};
-  HIPCALL(hipMemcpy(g.dtBuf, zero2, sizeof zero2, hipMemcpyHostToDevice), st);
+  // t = 0, and a graph run starts at the original CFL
+  const double init3[3] = {0.0, 0.0, cfl};
+  HIPCALL(hipMemcpy(g.dtBuf, init3, sizeof init3, hipMemcpyHostToDevice), st);
+  g.runCfl = cfl;
   g.async = 1;
   g.timing = 0;
   g.dtDev = g.dtBuf;

@@ -132,6 +132,16 @@ class RunStats:
         return self.zone_updates / self.wall_s if self.wall_s > 0 else 0.0
 
 
+def shock_cfl(cfl: float, cap: float) -> float:
+    """CFL of the next dt under shockLowerCFL (Hydro.F90:104-108 and 128-141,
+    hy_shockDetectBlk.F90:228-240).  Each step resets the CFL to cfl before
+    the detector, a flagged cell lowers it to min(cfl, cap), and the
+    recovery before the fluxes raises it by 1.2 up to cfl: min(cfl, 1.2 cap)
+    after a step with a flagged cell.  cap is fx_status.min_cell_cfl, 0 when
+    nothing was flagged or shockLowerCFL is off."""
+    return min(cfl, 1.2 * cap) if cap > 0 else cfl
+
+
 class UniformGrid:
     """A single-level grid of equal blocks advanced by the hot path."""
 
@@ -149,6 +159,7 @@ class UniformGrid:
         self.nsteps = 0
         self.detect = bool(params.shockDetect or params.shockLowerCFL or
                            params.RiemannSolver.lower() == "hybrid")
+        self._cfl_next = cfl          # shockLowerCFL: the CFL the last step's flags left
 
     def fill_guards(self) -> None:
         import ctypes as C
@@ -163,12 +174,15 @@ class UniformGrid:
     def step(self, dt_max: float = math.inf) -> float:
         """Guard fill, CFL dt (capped by dt_max), unsplit update in place.
         With shock detection on: fill, flag, fill again (SHOK_VAR's guard
-        cells too), as Hydro.F90:94-123 orders them."""
+        cells too), as Hydro.F90:94-123 orders them.  Under shockLowerCFL the
+        dt takes the CFL that the previous step's flags left (shock_cfl)."""
+        cfl = self._cfl_next
         self.fill_guards()
         if self.detect:
-            self.hydro.shock_detect_batch(self.U, self.deltas)
+            st = self.hydro.shock_detect_batch(self.U, self.deltas)
             self.fill_guards()
-        dt = min(self.hydro.compute_dt(self.U, self.deltas, self.cfl), dt_max)
+            self._cfl_next = shock_cfl(self.cfl, st.min_cell_cfl)
+        dt = min(self.hydro.compute_dt(self.U, self.deltas, cfl), dt_max)
         self.hydro.advance_batch(self.U, dt, self.deltas)
         self.t += dt
         self.nsteps += 1
@@ -177,7 +191,8 @@ class UniformGrid:
     def run_graph(self, steps_per_graph: int, replays: int) -> float:
         """steps_per_graph whole steps (guard fill, device CFL dt, update)
         captured once in a HIP graph and replayed; returns the simulated time
-        the replays covered."""
+        the replays covered.  The run starts at the original CFL; under
+        shockLowerCFL the device carries the lowered CFL from step to step."""
         import ctypes as C
         st = FxStatus()
         t = C.c_double(0.0)
@@ -380,14 +395,18 @@ class AmrRun:
         self.hydro.amr_setup(tiles, [L + 1 for L, _ in grid.blocks], [p for _, p in grid.blocks],
                              grid.ext, grid.periodic)
         self.t = 0.0
+        self._flag_cap = 0.0          # shockLowerCFL: the cap the last step's flags set
 
     def step(self, dt: float) -> FxStatus:
         st = self.hydro.amr_step(dt)
         self.t += dt
+        self._flag_cap = st.min_cell_cfl
         return st
 
     def dt(self, cfl: float = 0.8) -> float:
-        return self.hydro.amr_compute_dt(cfl)
+        """CFL dt over every level; under shockLowerCFL with the CFL that the
+        last step's flags left (shock_cfl)."""
+        return self.hydro.amr_compute_dt(shock_cfl(cfl, self._flag_cap))
 
     def run_graph(self, cfl: float, steps_per_graph: int, replays: int) -> float:
         self.t += self.hydro.amr_run_graph(cfl, steps_per_graph, replays)

@@ -225,38 +225,49 @@ def _sod11(nblk, nb):
     return out
 
 
-def _hybrid_sod_grid(nblk=4, nb=8):
+def _hybrid_sod_grid(nblk=4, nb=8, lower=False):
     from flash5_amd import drivers as D
     dx = 1.0 / (nblk * nb)
     hp = HydroParams(ndim=2, nxb=nb, nyb=nb, nzb=1, gamma=1.4, slopeLimiter="mc",
-                     eintSwitch=1e-4, nunk=11, shokVar=SHOK, RiemannSolver="hybrid")
+                     eintSwitch=1e-4, nunk=11, shokVar=SHOK, RiemannSolver="hybrid",
+                     shockLowerCFL=lower)
     return D.UniformGrid(hp, _sod11(nblk, nb), D.uniform_neighbors((nblk, nblk),
                                                                    (False, False), 2),
                          (dx, dx, 1.0))
 
 
-def test_hybrid_sod_driver_matches_oracle_bitwise():
-    """Sod along the diagonal, 12 steps of fill, flag, fill, dt, update."""
+@pytest.mark.parametrize("lower", [False, True], ids=["cfl", "shockLowerCFL"])
+def test_hybrid_sod_driver_matches_oracle_bitwise(lower):
+    """Sod along the diagonal, 12 steps of fill, flag, fill, dt, update.
+    Under shockLowerCFL the dt after a step with a flagged cell takes
+    CFL min(0.8, 1.2 * 0.45) in 2-D: Hydro.F90:104-108 resets the CFL before
+    the detector, hy_shockDetectBlk.F90:228-240 lowers it to 0.45, and
+    Hydro.F90:128-141 raises it by 1.2 up to the original."""
     from test_drivers import pad_fill
     nblk, nb, nsteps = 4, 8, 12
     dx = 1.0 / (nblk * nb)
-    grid = _hybrid_sod_grid(nblk, nb)
+    grid = _hybrid_sod_grid(nblk, nb, lower)
     grid.run(nsteps=nsteps)
     p = O.params(2, nb, 1.4, limiter=O.MC, eintSwitch=1e-4, nunk=11, shokVar=SHOK,
                  riemann=O.HYBR)
-    U, t, flagged = _sod11(nblk, nb), 0.0, 0
+    U, t, flagged, cfl, lowered = _sod11(nblk, nb), 0.0, 0, 0.8, 0
     bc = ("outflow", "outflow")
     for _ in range(nsteps):
         U = pad_fill(U, (nblk, nblk), 2, 4, bc)
-        flagged += sum(O.shock_detect(p, U[b]) for b in range(U.shape[0]))
+        f = sum(O.shock_detect(p, U[b]) for b in range(U.shape[0]))
+        flagged += f
         U = pad_fill(U, (nblk, nblk), 2, 4, bc)
-        dt = min(O.block_dt(p, U[b], (dx, dx, 1.0), 0.8) for b in range(U.shape[0]))
+        dt = min(O.block_dt(p, U[b], (dx, dx, 1.0), cfl) for b in range(U.shape[0]))
+        lowered += cfl < 0.8
         for b in range(U.shape[0]):
             rc, _, _, _ = O.compute_fluxes(p, U[b], dt, (dx, dx, 1.0), update=True,
                                            want_flux=False)
             assert rc == 0
         t += dt
+        if lower:
+            cfl = 1.2 * 0.45 if f else 0.8
     assert flagged > 0 and grid.t == t
+    assert (lowered > 0) == lower
     g = 4
     _bitwise(grid.U.cpu().numpy()[:, :, :, g:g + nb, g:g + nb], U[:, :, :, g:g + nb, g:g + nb],
              "hybrid Sod")
@@ -272,6 +283,55 @@ def test_hybrid_graph_captured_steps_match_eager_steps():
         assert t > 0 and graph.t == pytest.approx(eager.t, rel=0, abs=1e-15)
         _bitwise(graph.U.cpu().numpy()[:, :, :, 4:12, 4:12],
                  eager.U.cpu().numpy()[:, :, :, 4:12, 4:12], f"replay round {rep}")
+
+
+def test_shock_lower_cfl_graph_matches_eager():
+    """Six graph-captured steps (three per graph, two replays) under
+    shockLowerCFL against six eager steps: the lowered CFL is carried on the
+    device from step to step and across replays."""
+    eager, graph = _hybrid_sod_grid(lower=True), _hybrid_sod_grid(lower=True)
+    t = graph.run_graph(steps_per_graph=3, replays=2)
+    dts = [eager.step() for _ in range(6)]
+    plain = _hybrid_sod_grid()
+    dts_plain = [plain.step() for _ in range(6)]
+    torch.cuda.synchronize()
+    # nothing is flagged before the flow compresses; later dts are lowered
+    assert dts[0] == dts_plain[0] and dts != dts_plain
+    assert t > 0 and graph.t == pytest.approx(eager.t, rel=0, abs=1e-15)
+    _bitwise(graph.U.cpu().numpy()[:, :, :, 4:12, 4:12],
+             eager.U.cpu().numpy()[:, :, :, 4:12, 4:12], "shockLowerCFL graph")
+
+
+def test_shock_lower_cfl_amr_graph_matches_eager():
+    """The blast on the three-level 2-D grid under shockLowerCFL: four eager
+    steps whose dt takes the CFL the previous step's flags left, against two
+    graphs of two steps."""
+    from flash5_amd.drivers import shock_cfl
+    from test_gpu_amr import _setup
+    from test_oracle_amr import GAMMA, _grid2d, _state
+
+    grid = _grid2d()
+    blast = lambda X, Y, Z: (1.0 + 0 * X, np.where(X < 0.4, 1.2, -1.2), 0 * X, 0 * X,
+                             np.where(X < 0.4, 6.0, 0.6))
+    U = [np.concatenate([u, np.zeros((1,) + u.shape[1:])]) for u in _state(grid, 41, fn=blast)]
+    hp = HydroParams(ndim=2, nxb=16, nyb=16, nzb=1, gamma=GAMMA, nunk=11, shokVar=SHOK,
+                     RiemannSolver="hybrid", shockLowerCFL=True)
+    hy, tiles = _setup(grid, U, hp)
+    t_eager, cap, caps = 0.0, 0.0, []
+    for _ in range(4):
+        dt = hy.amr_compute_dt(shock_cfl(0.8, cap))
+        st = hy.amr_step(dt)
+        assert st.code == 0
+        cap = st.min_cell_cfl
+        caps.append(cap)
+        t_eager += dt
+    assert caps[0] == 0.45
+    eager = [t.U.cpu().numpy() for t in tiles]
+    hy2, tiles2 = _setup(grid, U, hp)
+    t_graph = hy2.amr_run_graph(0.8, 2, 2)
+    assert t_graph == t_eager
+    for b, t in enumerate(tiles2):
+        _bitwise(t.U.cpu().numpy(), eager[b], f"block {b}")
 
 
 @pytest.mark.parametrize("ndim", [2, 3])
