@@ -275,9 +275,11 @@ __device__ __forceinline__ FaceRef face_ref(const DevParams& P, int t, long rt) 
 // the flux component that takes the pressure are picked at run time.  R is
 // the solver's arithmetic: double, or float in the mixed-precision mode
 // (FP64 Newton iterate and convergence test, FP64 fluxes out).
-// (3 waves per SIMD fit without spilling the solver's registers)
+// (3 waves per SIMD fit the FP64 solver's registers without spilling, 4 the
+// FP32 one's)
 template <bool SEEDED, bool MULTI, typename R>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == sizeof(float) ? 4 : 3)))
 k_face_solve(DevParams P0, FaceLaunch F) {
   const long total = face_total<MULTI>(P0, F);
   ExactStats es{0ULL, 0ULL};
