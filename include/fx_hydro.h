@@ -54,9 +54,12 @@ enum { FX_ROE = 1, FX_HLL = 3, FX_HLLC = 4, FX_HYBR = 8, FX_EXACT = 20 };
 /* precision modes */
 enum { FX_PREC_FP64 = 0,      /* plain FP64, IEEE ops without contraction */
        FX_PREC_FP64_FAST = 1, /* reserved (rejected by fx_hydro_init) */
-       FX_PREC_MIXED = 2 };   /* FP32 reconstruction and HLLC fluxes; FP64
-                                 update, e_int recovery, EOS and exact-solver
-                                 Newton iteration; 1-D blocks run FP64 */
+       FX_PREC_MIXED = 2 };   /* FP32 reconstruction and fluxes (HLLC, HLL
+                                 and the exact solver, whose face states are
+                                 stored in FP32); FP64 update, e_int
+                                 recovery, EOS and the exact solver's Newton
+                                 iterate and convergence test (tolerance
+                                 max(exactTol, 1e-5)); 1-D blocks run FP64 */
 
 /* Runtime knobs read by Hydro_init (Hydro_init.F90:60-420) and the EOS
  * (Eos_init.F90:66-73, eos_initGamma.F90:42). */
