@@ -38,6 +38,7 @@ struct AmrLaunch {
   const AmrBlock* blocks;
   int nBlocks;
   const AmrLevelDev* levels;
+  int nLevels;
   const DevParams* cls;
   const int* list;          // block subset of the launch (restriction: one level), or null
   int nList;                // blocks in the launch
@@ -256,8 +257,8 @@ __global__ void __launch_bounds__(kThreads) k_amr_restrict(AmrLaunch A) {
   }
 }
 
-// per-workgroup max of (|v_d| + c) / del_d over the interior cells of every
-// block of every level
+// per-workgroup max of (|v_d| + c) / del_d over the leaf cells of every
+// level: interior cells that no block of the next finer level covers
 __global__ void __launch_bounds__(kThreads) k_amr_dt(AmrLaunch A, double* __restrict__ partial) {
   __shared__ double red[kThreads];
   const long total = A.start[A.nBlocks];
@@ -270,7 +271,17 @@ __global__ void __launch_bounds__(kThreads) k_amr_dt(AmrLaunch A, double* __rest
     const DevParams& P = A.cls[B.cls];
     const AmrLevelDev& L = A.levels[B.level];
     const int nxi = P.nxb, nyi = P.ndim > 1 ? P.nyb : 1;
-    const long c = cidx(P, r % nxi + P.lo[0], (r / nxi) % nyi + P.lo[1], r / (nxi * nyi) + P.lo[2]);
+    const int ii = r % nxi, jj = (r / nxi) % nyi, kk = r / (nxi * nyi);
+    if (B.level + 1 < A.nLevels) {
+      // Driver_computeDt.F90:219 takes leaf blocks only: skip a cell that
+      // the next finer level covers (its first fine cell's fine block exists)
+      const AmrLevelDev& F = A.levels[B.level + 1];
+      const int fx = 2 * (B.pos[0] * nxi + ii) / F.nb[0];
+      const int fy = P.ndim > 1 ? 2 * (B.pos[1] * nyi + jj) / F.nb[1] : 0;
+      const int fz = P.ndim > 2 ? 2 * (B.pos[2] * P.nzb + kk) / F.nb[2] : 0;
+      if (F.index[((long)fz * F.ext[1] + fy) * F.ext[0] + fx] >= 0) continue;
+    }
+    const long c = cidx(P, ii + P.lo[0], jj + P.lo[1], kk + P.lo[2]);
     const gdouble* U = as_global(B.U);
     const double cs = sqrt(U[P.voff[FX_GAMC] + c] * U[P.voff[FX_PRES] + c] / U[P.voff[FX_DENS] + c]);
     double dtl = (fabs(U[P.voff[FX_VELX] + c] - 0.) + cs) * (1. / L.del[0]);

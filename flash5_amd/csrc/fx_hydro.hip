@@ -1967,6 +1967,7 @@ AmrLaunch amr_launch() {
   L.blocks = A.dblocks;
   L.nBlocks = A.nBlocks;
   L.levels = A.dlevels;
+  L.nLevels = A.nLevels;
   L.cls = g.dcls;
   L.list = nullptr;
   L.nList = A.nBlocks;
@@ -2037,7 +2038,8 @@ void amr_launch_restrict() {
   }
 }
 
-// dt = cfl / max((|v| + c) / del) over every level, into g.dtBuf[0]; t += dt
+// dt = cfl / max((|v| + c) / del) over the leaf cells of every level, into
+// g.dtBuf[0]; t += dt
 int amr_launch_dt(double cfl, fx_status* st) {
   const int grid = grid_for(g.amr.cells);
   if (g.partialCap < grid) {

@@ -139,6 +139,48 @@ def test_amr_graph_steps_match_eager_steps():
         assert np.array_equal(t.U.cpu().numpy(), eager[b]), f"block {b}"
 
 
+def _leaf_dt(grid, U, cfl):
+    """cfl / max over leaf cells (cells no finer block covers) of
+    (|v_a| + c) / del_a, in the device's operation order."""
+    idx = OA._index(grid.as_dict())
+    m = 0.0
+    for b, (L, pos) in enumerate(grid.blocks):
+        nb = grid.nb[L]
+        sl = tuple(slice(4, 4 + nb[a]) if a < grid.ndim else slice(0, 1) for a in (2, 1, 0))
+        u = U[b]
+        cs = np.sqrt(u[(O.GAMC,) + sl] * u[(O.PRES,) + sl] / u[(O.DENS,) + sl])
+        d = grid.dels(L)
+        s = (np.abs(u[(O.VELX,) + sl] - 0.0) + cs) * (1.0 / d[0])
+        for a, v in ((1, O.VELY), (2, O.VELZ))[:grid.ndim - 1]:
+            s = np.fmax(s, (np.abs(u[(v,) + sl] - 0.0) + cs) * (1.0 / d[a]))
+        if L + 1 < len(grid.nb):
+            k, j, i = np.indices(s.shape)
+            g = np.array([pos[0] * nb[0] + i, pos[1] * nb[1] + j,
+                          pos[2] * nb[2] + k if grid.ndim > 2 else 0 * k]).reshape(3, -1)
+            nbf = grid.nb[L + 1]
+            bp = [2 * g[a] // nbf[a] for a in range(3)]
+            s = np.where((idx[L + 1][bp[2], bp[1], bp[0]] >= 0).reshape(s.shape), 0.0, s)
+        m = max(m, s.max())
+    return cfl / m
+
+
+def test_amr_dt_takes_leaf_cells_only():
+    """Driver_computeDt.F90:219 iterates leaf blocks: the device dt is the
+    CFL minimum over the cells no finer block covers.  A parent block whose
+    pressure would otherwise set the step leaves it unchanged."""
+    grid = _grid2d()
+    U = _state(grid, 25)
+    parent = next(b for b, (L, pos) in enumerate(grid.blocks) if L == 0 and pos[1] == 0)
+    U[parent][O.PRES] *= 1e4
+    U[parent][O.ENER] = U[parent][O.PRES] / (U[parent][O.DENS] * (GAMMA - 1.0)) + 0.5 * (
+        U[parent][O.VELX] ** 2 + U[parent][O.VELY] ** 2 + U[parent][O.VELZ] ** 2)
+    hp = HydroParams(ndim=2, nxb=16, nyb=16, nzb=1, gamma=GAMMA)
+    hy, _ = _setup(grid, U, hp)
+    dt = hy.amr_compute_dt(0.8)
+    assert dt == _leaf_dt(grid, U, 0.8)
+    assert dt > _dt(grid, U)          # every block's minimum would take the parent's
+
+
 def test_amr_conserves_leaf_mass_on_a_periodic_grid():
     grid = _grid2d(periodic=(True, True, False))
     U = _state(grid, 27)
