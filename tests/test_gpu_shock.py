@@ -357,3 +357,34 @@ def test_hybrid_amr_step_matches_oracle(ndim):
     assert sum(u[SHOK].sum() for u in Uo) > 0
     for b, t in enumerate(tiles):
         _bitwise(t.U.cpu().numpy(), Uo[b], f"block {b} (level {grid.blocks[b][0]})")
+
+
+def test_shock_lower_cfl_amr_run_driver_matches_graph():
+    """AmrRun's eager dt()/step() loop under shockLowerCFL against its
+    graph-captured steps, bit for bit."""
+    from flash5_amd.drivers import AmrRun, LevelStorage
+    from test_oracle_amr import GAMMA, _grid2d, _state
+
+    grid = _grid2d()
+    blast = lambda X, Y, Z: (1.0 + 0 * X, np.where(X < 0.4, 1.2, -1.2), 0 * X, 0 * X,
+                             np.where(X < 0.4, 6.0, 0.6))
+    U = [np.concatenate([u, np.zeros((1,) + u.shape[1:])]) for u in _state(grid, 41, fn=blast)]
+    hp = HydroParams(ndim=2, nxb=16, nyb=16, nzb=1, gamma=GAMMA, nunk=11, shokVar=SHOK,
+                     RiemannSolver="hybrid", shockLowerCFL=True)
+
+    def run():
+        S = LevelStorage(grid, nunk=11)
+        for b in range(len(grid.blocks)):
+            S.block(b).copy_(torch.from_numpy(U[b]))
+        return S, AmrRun(grid, hp, S)
+
+    S1, eager = run()
+    for _ in range(4):
+        assert eager.step(eager.dt(0.8)).code == 0
+    assert eager._flag_cap == 0.45
+    e = [S1.block(b).cpu().numpy() for b in range(len(grid.blocks))]
+    S2, graph = run()
+    graph.run_graph(0.8, 2, 2)
+    assert graph.t == eager.t
+    for b in range(len(grid.blocks)):
+        _bitwise(S2.block(b).cpu().numpy(), e[b], f"block {b}")
