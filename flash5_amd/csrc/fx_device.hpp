@@ -536,13 +536,14 @@ __device__ __forceinline__ int hll(const DevParams& P, const R* Vm, const R* Vp,
 // operation order follows the CPU oracle's restatement, except where a power
 // follows from one already taken: the rarefaction derivative reuses the
 // value's power, the density behind a rarefaction is gamma p / c^2, and the
-// fan pressure's power is the density's times (c/ck)^2.
+// fan pressure's power is the density's times (c/ck)^2; and powers are taken
+// as exp2(e log2 x) (pow_pos) where the oracle calls pow.
 
 // R = double is the FP64 solver.  R = float serves the mixed-precision
 // mode: the pressure function, sampling and flux run in FP32, while the
 // Newton iterate and its convergence test stay in FP64 (the iterate is
-// rounded to FP32 only to evaluate the function).  Every R = double
-// expression is the FP64 code's, in the same order.
+// rounded to FP32 only to evaluate the function).  The R = double
+// instantiation performs the FP64 operations in the order described above.
 
 // x^e for x > 0 as exp2(e log2 x).  In FP64 this is about 130 instructions
 // against pow's 230 (pow also handles signs, integer exponents and extra
