@@ -255,7 +255,11 @@ int fx_grid_fill_guards(double* U, long nBlocks, long blockStride, const int* nb
  * the device (as fx_hydro_compute_dt, kept in device memory), and advances
  * every block (as fx_hydro_advance_batch with that dt).  stepsPerGraph steps
  * are captured once and the graph is replayed nReplays times; *tOut gets the
- * elapsed simulation time (the clock starts at 0).  Not with learnedSeed. */
+ * elapsed simulation time (the clock starts at 0).  With shock detection on,
+ * each step fills, flags and fills SHOK_VAR's guard cells (Hydro.F90:94-123);
+ * under shockLowerCFL a step after one with a flagged cell takes CFL
+ * min(cfl, 1.2 * cap) (Hydro.F90:104-141), carried on the device across
+ * steps and replays, starting at cfl.  Not with learnedSeed. */
 int fx_grid_run_graph(double* U, long nBlocks, long blockStride, const int* nbr,
                       const double del[3], double cfl, int stepsPerGraph, int nReplays,
                       double* tOut, fx_status* st);
@@ -349,15 +353,19 @@ int fx_amr_setup(const fx_tile* tiles, const fx_amr_block* blocks, int nBlocks, 
 int fx_amr_fill_guards(fx_status* st);
 /* Every covered cell from its children, finest level first. */
 int fx_amr_restrict(fx_status* st);
-/* dt = cfl / max over all levels' interior cells of (|v_d| + c) / del_d
- * (Hydro_computeDt.F90:186-270 without the reductions over ranks). */
+/* dt = cfl / max over all levels' leaf cells (interior cells no finer block
+ * covers, as Driver_computeDt.F90:219 iterates LEAF blocks) of
+ * (|v_d| + c) / del_d (Hydro_computeDt.F90:186-270 without the reductions
+ * over ranks).  Under shockLowerCFL the caller passes the lowered CFL: the
+ * last step's fx_status.min_cell_cfl gives the cap. */
 int fx_amr_compute_dt(double cfl, double* dtOut);
 /* One step of every level: guard fill, fluxes of all blocks (one persistent
  * launch per stage), flux correction at fine-coarse faces, update, then
  * restriction. */
 int fx_amr_step(double dt, fx_status* st);
 /* nReplays of a HIP graph of stepsPerGraph steps (dt on the device each
- * step); *tOut = simulated time. */
+ * step, as fx_amr_compute_dt; under shockLowerCFL the lowered CFL is carried
+ * on the device, as in fx_grid_run_graph); *tOut = simulated time. */
 int fx_amr_run_graph(double cfl, int stepsPerGraph, int nReplays, double* tOut, fx_status* st);
 /* Library version string. */
 const char* fx_hydro_version(void);
