@@ -115,10 +115,12 @@ typedef struct {
   long long fallbacks;      /* cells reset to first-order states */
   long long newton_iters;   /* exact solver: total Newton iterations */
   long long newton_fallbacks; /* exact solver: learned seeds that failed */
-  double min_cell_cfl;      /* fx_hydro_shock_detect: the CFL cap 0.60,
-                               0.45, 0.25 (NDIM 1, 2, 3) when shockLowerCFL
-                               is on and a cell was flagged (the reference
-                               sets hy_cfl = min(hy_cfl, cap),
+  double min_cell_cfl;      /* calls that run the detector
+                               (fx_hydro_shock_detect*, fx_amr_step, the
+                               graph-captured runs' last step): the CFL cap
+                               0.60, 0.45, 0.25 (NDIM 1, 2, 3) when
+                               shockLowerCFL is on and a cell was flagged
+                               (the reference sets hy_cfl = min(hy_cfl, cap),
                                hy_shockDetectBlk.F90:211-227); else 0 */
 } fx_status;
 
