@@ -195,6 +195,30 @@ def test_amr_conserves_leaf_mass_on_a_periodic_grid():
     assert abs(m1 - m0) <= 1e-13 * abs(m0), (m0, m1)
 
 
+def test_amr_face_states_per_precision():
+    """The face states of a multi-level exact step, collected through the
+    class tables: FP64 in the FP64 mode, FP32 numbers in the mixed mode,
+    the two within FP32 reconstruction error of each other, with one
+    Riemann problem per face of every block."""
+    grid = _grid2d()
+    U = _state(grid, 29)
+    dt = _dt(grid, U)
+    out = {}
+    for prec in (0, PREC_MIXED):
+        hp = HydroParams(ndim=2, nxb=16, nyb=16, nzb=1, gamma=GAMMA, RiemannSolver="exact",
+                         precision=prec)
+        hy, _ = _setup(grid, U, hp)
+        assert hy.amr_step(dt).code == 0
+        out[prec] = hy.face_states().cpu().numpy()
+    faces = sum(2 * (n[0] + 1) * n[1] for n in (grid.nb[L] for L, _ in grid.blocks))
+    s64, s32 = out[0], out[PREC_MIXED]
+    assert s64.shape == s32.shape == (6, faces)
+    np.testing.assert_array_equal(s32, s32.astype(np.float32).astype(np.float64))
+    assert not np.array_equal(s64, s64.astype(np.float32).astype(np.float64))
+    scale = np.abs(s64).max(axis=1, keepdims=True)
+    assert (np.abs(s32 - s64) <= 1e-5 * scale).all(), np.abs(s32 - s64).max()
+
+
 def test_amr_mixed_precision_learned_seed_runs_under_graph():
     """The mixed-precision exact solver with the learned seed on a
     three-level grid: graph replays equal eager steps, and after two steps of
