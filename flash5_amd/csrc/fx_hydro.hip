@@ -1529,8 +1529,13 @@ int launch_flux_stages(int nTiles, double dt, int TX, int TY) {
       M.strongCap = (unsigned)g.hfaceStart[nTiles];
       hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, g.stream, g.strongCount);
     }
-    if (g.multi) launch_march_multi<RS_STATES>(P.ndim, P.precision, TX, M);
-    else launch_march<RS_STATES>(P, M, TX, grid);
+    if (P.learned) {
+      if (g.multi) launch_march_multi<RS_STATES_GATED>(P.ndim, P.precision, TX, M);
+      else launch_march<RS_STATES_GATED>(P, M, TX, grid);
+    } else {
+      if (g.multi) launch_march_multi<RS_STATES>(P.ndim, P.precision, TX, M);
+      else launch_march<RS_STATES>(P, M, TX, grid);
+    }
     if (record(1)) return 4;
     const long faces = g.hfaceStart[nTiles];
     FaceLaunch F{g.dtiles, nTiles, g.fstates, P.learned ? g.seedY : nullptr, g.status,

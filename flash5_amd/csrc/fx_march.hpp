@@ -65,7 +65,7 @@ struct MarchLaunch {
   double dt;
   void* fstates;       // RS_STATES: per-face states for the separate Riemann stage, in R
   const double* dtp;   // device-resident dt (graph-captured steps), or null
-  // RS_STATES with the learned seed: every face's seed slot is set to
+  // RS_STATES_GATED (the learned seed): every face's seed slot is set to
   // "no seed", and faces whose strength reaches the gate are appended to
   // strongList (their flat face numbers) for the seed kernel
   float* seedY;
@@ -78,6 +78,10 @@ struct MarchLaunch {
 // Riemann-stage variant of the march: stage C stores each face's left and
 // right states (dens, vx, vy, vz, pres, eint) instead of solving there.
 constexpr int RS_STATES = 99;
+// RS_STATES with the learned seed's gate at every face (compiled apart, so
+// that the unseeded exact path carries none of the gate's registers)
+constexpr int RS_STATES_GATED = 98;
+constexpr bool is_states(int rs) { return rs == RS_STATES || rs == RS_STATES_GATED; }
 constexpr int kFaceComps = 12;
 
 // face-state scratch of one tile: [dir][comp 0..11][face], in the march's
@@ -532,10 +536,11 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           VL[P_GAME] = R(U[P.voff[FX_GAME] + c - sx]);
           VR[P_GAMC] = R(U[P.voff[FX_GAMC] + c]);
           VR[P_GAME] = R(U[P.voff[FX_GAME] + c]);
-          if constexpr (RS == RS_STATES) {
+          if constexpr (is_states(RS)) {
             const long fr = face_index(P, 0, ox + a, oy + b, k);
             write_face_states<0>(P, fsb + fstate_dir_offset(P, 0), fr, VL, VR);
-            if (L.seedY) seed_gate<0>(P, L, faceBase, fr, VL, VR);
+            if constexpr (RS == RS_STATES_GATED)
+              if (L.seedY) seed_gate<0>(P, L, faceBase, fr, VL, VR);
             continue;
           } else {
             if (face_flux<0, RS>(P, VL, VR, use_hll<RS>(P, U, c - sx, c), F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 0); continue; }
@@ -551,10 +556,11 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           VL[P_GAME] = R(U[P.voff[FX_GAME] + c - sy]);
           VR[P_GAMC] = R(U[P.voff[FX_GAMC] + c]);
           VR[P_GAME] = R(U[P.voff[FX_GAME] + c]);
-          if constexpr (RS == RS_STATES) {
+          if constexpr (is_states(RS)) {
             const long fr = face_index(P, 1, ox + a, oy + b, k);
             write_face_states<1>(P, fsb + fstate_dir_offset(P, 1), fr, VL, VR);
-            if (L.seedY) seed_gate<1>(P, L, faceBase, fr, VL, VR);
+            if constexpr (RS == RS_STATES_GATED)
+              if (L.seedY) seed_gate<1>(P, L, faceBase, fr, VL, VR);
             continue;
           } else {
             if (face_flux<1, RS>(P, VL, VR, use_hll<RS>(P, U, c - sy, c), F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 1); continue; }
@@ -572,10 +578,11 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
             VL[P_GAME] = R(U[P.voff[FX_GAME] + c - sz]);
             VR[P_GAMC] = R(U[P.voff[FX_GAMC] + c]);
             VR[P_GAME] = R(U[P.voff[FX_GAME] + c]);
-            if constexpr (RS == RS_STATES) {
+            if constexpr (is_states(RS)) {
               const long fr = face_index(P, 2, ox + a, oy + b, k);
               write_face_states<2>(P, fsb + fstate_dir_offset(P, 2), fr, VL, VR);
-              if (L.seedY) seed_gate<2>(P, L, faceBase, fr, VL, VR);
+              if constexpr (RS == RS_STATES_GATED)
+                if (L.seedY) seed_gate<2>(P, L, faceBase, fr, VL, VR);
             } else {
               if (face_flux<2, RS>(P, VL, VR, use_hll<RS>(P, U, c - sz, c), F, es)) report_error(st, 2, t, ox + a, oy + b, k, 2);
               else write_flux<2>(P, as_global(T.flux[2]), face_index(P, 2, ox + a, oy + b, k), F);
@@ -614,7 +621,7 @@ k_march(DevParams P, MarchLaunch L) {
   const int ox = P.lo[0] + (tr % L.tilesX) * TX;
   const int oy = P.lo[1] + (tr / L.tilesX) * TY;
   ExactStats es{0ULL, 0ULL};
-  R* fsb = RS == RS_STATES ? static_cast<R*>(L.fstates) + t * fstate_tile_stride(P) : nullptr;
+  R* fsb = is_states(RS) ? static_cast<R*>(L.fstates) + t * fstate_tile_stride(P) : nullptr;
   const long faceBase = (long)t * (P.nface[0] + P.nface[1] + P.nface[2]);
   __shared__ R smem[MarchLds<NDIM, TX, TY>::total];
   march_tile<NDIM, TX, TY, NT, RS, R>(P, L, t, ox, oy, fsb, faceBase, es, smem);
@@ -641,8 +648,8 @@ k_march_multi(const DevParams* __restrict__ cls,
   for (int it = blockIdx.x; it < nItems; it += gridDim.x) {
     const MarchItem w = items[it];
     const DevParams& P = cls[w.cls];
-    const long faceBase = RS == RS_STATES ? faceStart[w.t] : 0;
-    R* fsb = RS == RS_STATES ? static_cast<R*>(L.fstates) + kFaceComps * faceBase : nullptr;
+    const long faceBase = is_states(RS) ? faceStart[w.t] : 0;
+    R* fsb = is_states(RS) ? static_cast<R*>(L.fstates) + kFaceComps * faceBase : nullptr;
     march_tile<NDIM, TX, TY, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, faceBase, es, smem);
   }
   flush_exact(L.status, es);
@@ -661,8 +668,8 @@ k_march_mixw(const DevParams* __restrict__ cls, MarchLaunch L,
   for (int it = blockIdx.x; it < nItems; it += gridDim.x) {
     const MarchItem w = items[it];
     const DevParams& P = cls[w.cls];
-    const long faceBase = RS == RS_STATES ? faceStart[w.t] : 0;
-    R* fsb = RS == RS_STATES ? static_cast<R*>(L.fstates) + kFaceComps * faceBase : nullptr;
+    const long faceBase = is_states(RS) ? faceStart[w.t] : 0;
+    R* fsb = is_states(RS) ? static_cast<R*>(L.fstates) + kFaceComps * faceBase : nullptr;
     if (w.tx == 16) march_tile<NDIM, 16, 8, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, faceBase, es, smem);
     else march_tile<NDIM, 8, 8, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, faceBase, es, smem);
   }
