@@ -455,8 +455,9 @@ __device__ __forceinline__ int hllc(const DevParams& P, const R* Vm, const R* Vp
   const R tp = left ? tpL : tpR;
   const R numer = S - velN, denom = S - qStar;
   const R pStar = Vm[P_DENS] * (SL - velNL) * (qStar - velNL) + tpL;
-  R Fs[6];
-  prim2flx<DIR>(V, fP, Fs);
+  // the chosen side's flux, built in F itself (a second private array here
+  // ends up in scratch memory after inlining)
+  prim2flx<DIR>(V, fP, F);
   // conserved state of the chosen side (hy_prim2con.F90)
   const R u2 = V[P_VELX] * V[P_VELX] + V[P_VELY] * V[P_VELY] + V[P_VELZ] * V[P_VELZ];
   R Uc[5];
@@ -472,8 +473,7 @@ __device__ __forceinline__ int hllc(const DevParams& P, const R* Vm, const R* Vp
 #pragma unroll
   for (int m = 0; m < 3; ++m) Us[1 + m] = (m == DIR) ? dStar * qStar : Uc[1 + m] * numer / denom;
 #pragma unroll
-  for (int f = 0; f < 5; ++f) F[f] = Fs[f] + S * (Us[f] - Uc[f]);
-  F[5] = Fs[5];
+  for (int f = 0; f < 5; ++f) F[f] = F[f] + S * (Us[f] - Uc[f]);
   return 0;
 }
 
