@@ -1,0 +1,216 @@
+// fx_common.hpp -- device structures and helpers shared by the kernels of
+// every translation unit of the library (the main unit fx_hydro.hip and the
+// column-march units built from fx_march_inst.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fx_device.hpp"
+#include "fx_hydro.h"
+
+namespace fx {
+constexpr int kStateComps = 12;  // Wp and Wn of dens, vx, vy, vz, pres, eint
+constexpr int kThreads = 256;
+
+struct DevStatus {
+  int code;
+  int tile, i, j, k, dir;
+  unsigned long long fallbacks;
+  unsigned long long newtonIters;
+  unsigned long long newtonFallbacks;
+  unsigned long long shockTiles;  // tiles with a flagged cell (k_shock_detect)
+};
+
+struct Launch {
+  const fx_tile* tiles;  // device array
+  int nTiles;
+  double* states;        // scratch: [tile][band][comp][cell]
+  DevStatus* status;
+  double dt;
+  const double* dtp;     // device-resident dt (graph-captured steps), or null
+};
+
+// Tiles of several block classes in one launch: each tile's class selects its
+// parameters, and per-tile prefix offsets map a flat cell (or face) index to
+// its tile.
+struct ClassMap {
+  const DevParams* cls;        // parameters per class
+  const unsigned char* tcls;   // class of each tile
+  const long* start;           // items before tile t; start[nTiles] = all items
+  const int* wtab;             // tile of every 64th item (find_tile_w)
+};
+
+// t with start[t] <= gid < start[t + 1] (start non-decreasing, start[0] = 0)
+__device__ __forceinline__ int find_tile(const long* __restrict__ start, int n, long gid) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= gid) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// The same through a table of 64-item chunks: wtab[c] is the tile holding
+// item 64 c.  Every tile holds at least 64 items, so an item lies in that
+// tile or the next; one load and one compare instead of a search.  wtab
+// null: the search.
+__device__ __forceinline__ int find_tile_w(const long* __restrict__ start,
+                                          const int* __restrict__ wtab, int n, long gid) {
+  if (!wtab) return find_tile(start, n, gid);
+  int t = wtab[gid >> 6];
+  if (t + 1 < n && start[t + 1] <= gid) ++t;
+  return t;
+}
+
+// the step's dt: from device memory when the host does not know it
+__device__ __forceinline__ double step_dt(double dt, const double* dtp) { return dtp ? *dtp : dt; }
+
+__device__ __forceinline__ void report_error(DevStatus* s, int code, int tile, int i, int j, int k, int dir) {
+  if (atomicCAS(&s->code, 0, code) == 0) {
+    s->tile = tile;
+    s->i = i;
+    s->j = j;
+    s->k = k;
+    s->dir = dir;
+  }
+}
+
+__device__ __forceinline__ void flush_exact(DevStatus* s, const ExactStats& es) {
+  if (es.iters) atomicAdd(&s->newtonIters, es.iters);
+  if (es.fallbacks) atomicAdd(&s->newtonFallbacks, es.fallbacks);
+}
+
+__device__ __forceinline__ long cidx(const DevParams& P, int i, int j, int k) {
+  return ((long)k * P.ny + j) * P.nx + i;
+}
+
+// primitive vector of cell c in the reconstruction type R (a conversion
+// from the FP64 cell data when R is float)
+template <typename R>
+__device__ __forceinline__ void load_prim(const DevParams& P, const gdouble* __restrict__ U, long c,
+                                          R V[8]) {
+  V[P_DENS] = R(U[P.voff[FX_DENS] + c]);
+  V[P_VELX] = R(U[P.voff[FX_VELX] + c]);
+  V[P_VELY] = R(U[P.voff[FX_VELY] + c]);
+  V[P_VELZ] = R(U[P.voff[FX_VELZ] + c]);
+  V[P_PRES] = R(U[P.voff[FX_PRES] + c]);
+  V[P_GAMC] = R(U[P.voff[FX_GAMC] + c]);
+  V[P_GAME] = R(U[P.voff[FX_GAME] + c]);
+  V[P_EINT] = R(U[P.voff[FX_EINT] + c]);
+}
+
+// FlatTilde of one cell along DIR (hy_getRiemannState.F90:365-404)
+template <int DIR, typename R = double>
+__device__ __forceinline__ R flat_tilde(const DevParams& P, const gdouble* __restrict__ U, long c,
+                                        long s) {
+  const gdouble* pr = U + P.voff[FX_PRES];
+  const gdouble* vn = U + P.voff[FX_VELX + DIR];
+  const R dp1 = (R(pr[c + s]) - R(pr[c - s]));
+  const R dp2 = (R(pr[c + 2 * s]) - R(pr[c - 2 * s]));
+  const R dv1 = R(vn[c + s]) - R(vn[c - s]);
+  const R presL = R(pr[c - s]), presR = R(pr[c + s]);
+  const R Sp = (fabs(dp2) > R(1.e-15)) ? dp1 / dp2 - R(0.75) : R(0.);
+  R ft = fmax(R(0.0), fmin(R(1.0), R(10.0) * Sp));
+  if ((fabs(dp1) / fmin(presL, presR) < R(1.) / R(3.)) || dv1 > R(0.)) ft = R(0.);
+  return ft;
+}
+
+// FlatCoeff (hy_getRiemannState.F90:408-453)
+template <int DIR, typename R = double>
+__device__ __forceinline__ R flat_coeff(const DevParams& P, const gdouble* __restrict__ U, long c,
+                                        long s) {
+  const gdouble* pr = U + P.voff[FX_PRES];
+  const R dp1 = (R(pr[c + s]) - R(pr[c - s]));
+  const R f0 = flat_tilde<DIR, R>(P, U, c, s);
+  if (dp1 < R(0.0)) return fmax(f0, flat_tilde<DIR, R>(P, U, c + s, s));
+  if (dp1 == R(0.)) return f0;
+  return fmax(f0, flat_tilde<DIR, R>(P, U, c - s, s));
+}
+
+__device__ __forceinline__ long axis_stride(const DevParams& P, int a) {
+  return a == 0 ? 1L : (a == 1 ? (long)P.nx : (long)P.nx * P.ny);
+}
+
+// sigma of cell c along direction SD (the transverse-only PPM call)
+template <class T> struct NoDeduce { using type = T; };
+
+template <int SD, typename R>
+__device__ __forceinline__ bool sigma_at(const DevParams& P, const gdouble* __restrict__ U, long c,
+                                         R sig[8], typename NoDeduce<EigT<R>>::type* eout) {
+  const long s = axis_stride(P, SD);
+  R Vm[8], Vc[8], Vp[8];
+  load_prim(P, U, c - s, Vm);
+  load_prim(P, U, c, Vc);
+  load_prim(P, U, c + s, Vp);
+  EigT<R> e;
+  if (!eig_make<SD>(Vc, e)) return false;
+  trans_sigma<SD>(P, Vm, Vc, Vp, e, sig);
+  if (eout) *eout = e;
+  return true;
+}
+
+// aux internal-energy fluxes F[6], F[7] from the mass flux f0 (upwind face
+// state)
+template <typename R>
+__device__ __forceinline__ void aux_fluxes(const DevParams& P, R f0, const R* VL, const R* VR,
+                                           double F[8]) {
+  F[6] = 0.;
+  F[7] = 0.;
+  if (P.aux) {
+    if (f0 > R(0.)) {
+      F[7] = double(f0 / VL[P_DENS]);
+      F[6] = double(f0 * VL[P_EINT]);
+    } else {
+      F[7] = double(f0 / VR[P_DENS]);
+      F[6] = double(f0 * VR[P_EINT]);
+    }
+  }
+}
+
+// Whether the face between cells cl and cr takes HLL: RiemannSolver = HLL,
+// or HYBR with a flagged cell on either side (hy_getFaceFlux.F90:353-367).
+__device__ __forceinline__ bool hll_face(const DevParams& P, const gdouble* __restrict__ U, long cl,
+                                         long cr) {
+  if (P.riemann == FX_HLL) return true;
+  if (P.riemann != FX_HYBR) return false;
+  return U[P.shokOff + cl] + U[P.shokOff + cr] > 0.;
+}
+
+// The kernels' solver family: RS = FX_HLLC compiles HLLC alone; RS = FX_HYBR
+// serves RiemannSolver = HLL and HYBR, choosing per face at run time.
+template <int RS>
+__device__ __forceinline__ bool use_hll(const DevParams& P, const gdouble* __restrict__ U, long cl,
+                                        long cr) {
+  if constexpr (RS == FX_HYBR) return hll_face(P, U, cl, cr);
+  else return false;
+}
+
+// Face flux densities F[0..7] (hy_getFaceFlux.F90:324-442): HLL (useHll) or
+// HLLC on the face states VL, VR in the reconstruction type R, then the aux
+// internal-energy fluxes.  The exact solver takes exact_face_flux.
+template <int D, int RS, typename R>
+__device__ __forceinline__ int face_flux(const DevParams& P, const R* VL, const R* VR, bool useHll,
+                                         double F[8], ExactStats& es) {
+  static_assert(RS == FX_HLLC || RS == FX_HYBR, "face_flux serves HLL, HLLC and the hybrid switch");
+  R f0;
+  if constexpr (sizeof(R) == sizeof(double)) {
+    const int ierr = (RS == FX_HYBR && useHll) ? hll<D>(P, VL, VR, F) : hllc<D>(P, VL, VR, F);
+    if (ierr) return ierr;
+    f0 = F[0];
+  } else {
+    R Fr[6];
+    const int ierr = (RS == FX_HYBR && useHll) ? hll<D>(P, VL, VR, Fr) : hllc<D>(P, VL, VR, Fr);
+    if (ierr) return ierr;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) F[q] = double(Fr[q]);
+    f0 = Fr[0];
+  }
+  aux_fluxes(P, f0, VL, VR, F);
+  return 0;
+}
+
+__device__ __forceinline__ long face_index(const DevParams& P, int d, int i, int j, int k) {
+  return ((long)(k - P.lo[2]) * P.fext[d][1] + (j - P.lo[1])) * P.fext[d][0] + (i - P.lo[0]);
+}
+
+}  // namespace fx

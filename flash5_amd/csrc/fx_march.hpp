@@ -1,7 +1,8 @@
 // fx_march.hpp -- fused face-state + flux kernel: one workgroup per
 // (block, TX x TY column tile), marching along z through the block.
 //
-// Included inside fx_hydro.hip's anonymous namespace (uses its helpers).
+// The kernel templates are instantiated in fx_march_inst.hip, one unit per
+// (solver family, real type), behind the launchers of fx_march_launch.hpp.
 //
 // Per z-plane k the workgroup computes, each value once (the work of
 // hy_getRiemannState.F90:356-1327 regrouped by quantity instead of by cell):
@@ -19,6 +20,11 @@
 //            face-averaged pressure of each interior cell to pavg.
 // The conservative update then runs as k_update from those arrays.
 // x/y states exist for interior planes only; z states for planes lo-1..hi+1.
+
+#pragma once
+#include "fx_common.hpp"
+
+namespace fx {
 
 // LDS index helpers, tile-local coordinates a = i - ox, b = j - oy
 template <int TX, int TY> struct TileIdx {
@@ -675,3 +681,5 @@ k_march_mixw(const DevParams* __restrict__ cls, MarchLaunch L,
   }
   flush_exact(L.status, es);
 }
+
+}  // namespace fx
