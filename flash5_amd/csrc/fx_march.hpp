@@ -172,10 +172,14 @@ __device__ __forceinline__ void lds_get6(const R* base, int n, int idx, R v[8]) 
   v[P_GAME] = R(0.);
 }
 
+// First-order reset of one state cell's face states.  A state cell on a
+// column-tile seam is computed by both tiles; only its owner (own = true)
+// counts the reset, so the count is one per (state cell, direction), as the
+// reference's (hy_dataReconstOneStep.F90:1124-1129).
 template <typename R>
 __device__ __forceinline__ void fallback_first_order(const R* Vc, R Wp[8], R Wn[8],
-                                                     DevStatus* st) {
-  atomicAdd(&st->fallbacks, 1ULL);
+                                                     DevStatus* st, bool own) {
+  if (own) atomicAdd(&st->fallbacks, 1ULL);
 #pragma unroll
   for (int v = 0; v < 8; ++v) {
     Wn[v] = Vc[v];
@@ -400,7 +404,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
             lds_get6(SZ, I::NC, I::cc(a, b), sg);
             sub_sigma(Wp, Wn, sg, R(0.5 * dt / dz));
           }
-          if (negative(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st);
+          if (negative(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st, a >= 0 || ox == P.lo[0]);
           put_state(WX, I::NRY, I::ry(a, b), Wp, Wn);
         } else if (w < nX + nY) {
           const int w2 = w - nX;
@@ -424,7 +428,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
             lds_get6(SZ, I::NC, I::cc(a, b), sg);
             sub_sigma(Wp, Wn, sg, R(0.5 * dt / dz));
           }
-          if (negative(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st);
+          if (negative(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st, b >= 0 || oy == P.lo[1]);
           put_state(WY, I::NRX, I::rx(a, b), Wp, Wn);
         } else {
           const int w2 = w - nX - nY;
@@ -452,7 +456,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           sub_sigma(Wp, Wn, sg, R(0.5 * dt / dx));
           lds_get6(syk, I::NRY, I::ry(a, b), sg);
           sub_sigma(Wp, Wn, sg, R(0.5 * dt / dy));
-          if (negative(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st);
+          if (negative(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st, true);
           put_state(WZ, NCOL, w2, Wp, Wn);
         }
       }
@@ -469,9 +473,11 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
         R fac;
         R* base;
         int n, idx;
+        bool own = true;  // the tile owns this state cell (counts its reset)
         if (w < nX) {
           // X: sigma_z over j with the y eigen system, sigma_y over k with z's
           const int a = w % (TX + 1 + ex) - 1, b = w / (TX + 1 + ex);
+          own = a >= 0 || ox == P.lo[0];
           load_prim(P, U, cidx(P, ox + a, oy + b, k), Vc);
           if (!(eig_make<1>(Vc, e1) && eig_make<2>(Vc, e2))) { bad = true; continue; }
           lds_get6(SZ, I::NC, I::cc(a, b - 1), sm);
@@ -488,6 +494,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           // Y: sigma_x over k with z's eigen system, sigma_z over i with x's
           const int w2 = w - nX;
           const int a = w2 % TX, b = w2 / TX - 1;
+          own = b >= 0 || oy == P.lo[1];
           load_prim(P, U, cidx(P, ox + a, oy + b, k), Vc);
           if (!(eig_make<2>(Vc, e1) && eig_make<0>(Vc, e2))) { bad = true; continue; }
           lds_get6(SX + ring(k - 1) * 6 * I::NRX, I::NRX, I::rx(a, b), sm);
@@ -519,7 +526,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
         }
         get_state(base, n, idx, Wp, Wn);
         add_ctu(Wp, Wn, tA, tB, fac);
-        if (nonpositive(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st);
+        if (nonpositive(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st, own);
         put_state(base, n, idx, Wp, Wn);
       }
       __syncthreads();
