@@ -821,6 +821,10 @@ struct Lib {
   bool batchValid = false;
   // per-stage timing (fx_hydro_set_timing)
   int timing = 0;
+  // true while a graph run captures its steps: the captured steps keep the
+  // status of the whole run (first error, summed counters) instead of
+  // resetting it per step; only the per-step shock-tile count is cleared
+  bool capturing = false;
   int legacy = 0;  // 1: force the three-kernel path (fx_hydro_set_kernel_path)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   float stageMs[3] = {0.f, 0.f, 0.f};
@@ -1355,7 +1359,7 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
   if (march) {
     if (g.multi && build_march_items(nTiles, TX, st)) return 4;
     if (ensure_step_scratch(nTiles, st)) return 4;
-    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
+    if (!g.capturing) hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
     if (record(0)) return 4;
     if (launch_flux_stages(nTiles, dt, TX, TY)) return 4;
     launch_update(nTiles, dt, dIn);
@@ -1363,7 +1367,7 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
   } else {
   if (grow(&g.states, &g.statesCap, (size_t)P.scellsPerTile * kStateComps * nTiles, st)) return 4;
   Launch L{g.dtiles, nTiles, g.states, g.status, dt, g.dtDev};
-  hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
+  if (!g.capturing) hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
   if (record(0)) return 4;
   {
     const dim3 gs(grid_for(P.scellsPerTile * nTiles));
@@ -1787,9 +1791,11 @@ int amr_launch_dt(double cfl, fx_status* st) {
 // restriction (launches only; dt from the argument or, when dtDev is set,
 // from the device)
 int amr_launch_step(double dt) {
-  hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
+  if (!g.capturing) hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
   amr_launch_fill();
   if (g.dp.shokOff >= 0) {  // Hydro.F90:94-123: flag, then fill SHOK_VAR's guard cells
+    if (g.capturing)
+      hipLaunchKernelGGL(k_zero_shock_tiles, dim3(1), dim3(1), 0, g.stream, g.status);
     launch_shock_detect(g.amr.nBlocks);
     launch_shock_cfl();
     amr_launch_fill(g.prm.shokVar);
@@ -2072,6 +2078,7 @@ int fx_grid_run_graph(double* U, long nBlocks, long blockStride, const int* nbr,
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int err = 0;
+  g.capturing = true;
   if (hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) err = 4;
   for (int s = 0; !err && s < stepsPerGraph; ++s) {
     // dt from the interior cells (the guard fill does not change them), with
@@ -2090,7 +2097,11 @@ int fx_grid_run_graph(double* U, long nBlocks, long blockStride, const int* nbr,
     if (run_compute(nullptr, (int)nBlocks, 0.0, nullptr, nullptr, true)) err = 4;
   }
   if (hipStreamEndCapture(g.stream, &graph) != hipSuccess) err = 4;
+  g.capturing = false;
   if (!err && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) err = 4;
+  // one status for the whole run: the replays keep the first error and sum
+  // the counters of every step
+  if (!err) hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
   for (int r = 0; !err && r < nReplays; ++r)
     if (hipGraphLaunch(exec, g.stream) != hipSuccess) err = 4;
   if (!err && hipStreamSynchronize(g.stream) != hipSuccess) err = 4;
@@ -2578,13 +2589,18 @@ int fx_amr_run_graph(double cfl, int stepsPerGraph, int nReplays, double* tOut, 
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int err = 0;
+  g.capturing = true;
   if (hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) err = 4;
   for (int s = 0; !err && s < stepsPerGraph; ++s) {
     if (amr_launch_dt(cfl, nullptr)) err = 4;
     if (!err && amr_launch_step(0.0)) err = 4;
   }
   if (hipStreamEndCapture(g.stream, &graph) != hipSuccess) err = 4;
+  g.capturing = false;
   if (!err && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) err = 4;
+  // one status for the whole run: the replays keep the first error and sum
+  // the counters of every step
+  if (!err) hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
   for (int r = 0; !err && r < nReplays; ++r)
     if (hipGraphLaunch(exec, g.stream) != hipSuccess) err = 4;
   if (!err && hipStreamSynchronize(g.stream) != hipSuccess) err = 4;

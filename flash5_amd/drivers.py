@@ -202,6 +202,7 @@ class UniformGrid:
             self.U[0].numel(), self.nbr.ctypes.data_as(C.POINTER(C.c_int)),
             (C.c_double * 3)(*self.deltas), self.cfl, steps_per_graph, replays, C.byref(t),
             C.byref(st))
+        self.last_status = st
         if rc:
             raise FxError("fx_grid_run_graph", st, rc)
         self.t += t.value

@@ -108,6 +108,11 @@ typedef struct {
                                ignored (0) while no classes are set */
 } fx_tile;
 
+/* Status of one call.  The graph-captured runs (fx_grid_run_graph,
+ * fx_amr_run_graph) report one status for the whole run: the first error of
+ * any step with its location (the captured steps still run to the end of the
+ * last replay; the host then aborts, as the reference would), and the
+ * fallback and Newton counters summed over every step of every replay. */
 typedef struct {
   int code;                 /* 0 ok, 1 eigen (imaginary c), 2 Riemann error,
                                3 bad argument, 4 HIP error */
