@@ -20,6 +20,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, AMD published
+FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector, AMD published
 N_IN, N_OUT = 7, 8             # FP64 variables read (with halo) / written per zone
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
@@ -50,6 +51,38 @@ def measured_traffic(kernel: str, workload: str):
         return None
     return (rec["fetch_kib"] * 1024.0 / rec["copy_fetch_factor"] +
             rec["write_kib"] * 1024.0 / rec["copy_write_factor"])
+
+
+def measured_flops_per_zone(kernel: str, workload: str):
+    """FP64 flops per zone of a kernel from the committed VALU counter summary
+    (64 lanes x (ADD + MUL + 2 FMA) per dispatch / zones), or None when the
+    summary was taken on another workload."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)[kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    if rec.get("workload") != workload or "fp64_flops_per_zone" not in rec:
+        return None
+    return float(rec["fp64_flops_per_zone"])
+
+
+def host_cpu():
+    """Host CPU model, logical CPUs and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, usable
 
 
 def copy_probe(nbytes: int = 1 << 30, reps: int = 3) -> None:
@@ -154,7 +187,9 @@ def cpu_baseline(U_gpu, dt, dx, riemann: str = "hllc", target_s: float = 12.0):
     oracle's shock detection."""
     import numpy as np
     import oracle as O
-    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    model, nproc, usable = host_cpu()
+    # every CPU the process may use, unless OMP_NUM_THREADS sets the host's share
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or usable
     if riemann in ("hll", "hybrid"):
         p = O.params(3, 16, 5.0 / 3.0, nunk=11, shokVar=O.SHOK,
                      riemann={"hll": O.HLL, "hybrid": O.HYBR}[riemann])
@@ -178,9 +213,11 @@ def cpu_baseline(U_gpu, dt, dx, riemann: str = "hllc", target_s: float = 12.0):
         if len(rates) >= 20:
             break
     return {"value": float(np.median(rates)), "unit": "zone-updates/s", "cores": nthreads,
-            "kind": "port",
+            "kind": "port", "cpu_model": model, "nproc": nproc, "affinity_cpus": usable,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
             "sample": f"{nb} of the 4096 blocks (same seeded fields), one step each, "
-                      f"median of {len(rates)} rounds, FP64 C oracle, OpenMP over blocks, "
+                      f"median of {len(rates)} rounds, FP64 C oracle, OpenMP over blocks on "
+                      f"{nthreads} threads ({usable} CPUs usable of {nproc}), "
                       f"{spent:.1f} s CPU wall"}
 
 
@@ -294,6 +331,7 @@ def main() -> None:
     fallbacks = int(hy.last_status.fallbacks)
     workload = (f"{args.blocks}x16^3+4g {args.precision} {args.riemann}"
                 + (" learned" if args.learned else ""))
+    flops_zone = measured_flops_per_zone("k_march", workload)
     flagged = float((U[:, 10, g:g + n, g:g + n, g:g + n] > 0).double().mean()) if hybrid else None
     if args.copy_probe:
         copy_probe()
@@ -337,8 +375,16 @@ def main() -> None:
                    "newton_fallbacks": int(hy.last_status.newton_fallbacks),
                    "flagged_cell_fraction": flagged,
                    "learned_seed": train_info, "autotune": tune},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"bound": "fp64-valu" if flops_zone else "hbm",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
+                     "compute": None if not flops_zone else {
+                         "achieved": flops_zone * zones / (march_ms * 1e-3) / 1e12,
+                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": flops_zone * zones / (march_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                         "fp64_flops_per_zone": flops_zone,
+                         "source": "SQ_INSTS_VALU_{ADD,MUL,FMA}_F64 per dispatch, "
+                                   "profiles/traffic.json"},
                      "traffic": measured_traffic("k_march", workload),
                      "kernel": "k_march (face states + fluxes, fused column march)",
                      "kernel_ms": march_ms, "bytes_per_zone": bmz, "zones_per_launch": zones,

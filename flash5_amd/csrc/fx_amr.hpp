@@ -12,9 +12,15 @@
 //   restriction      gr_restrictAllLevels.F90:86-130: PER_MASS variables
 //                    times density on the leaves, amrex_average_down finest to
 //                    coarsest, then back to per-mass form on every block.
-// AMReX itself is not part of the reference tree; the interpolation below is
-// the published form of its cell-conservative linear interpolator: per
-// component, MC-limited central slopes of the coarse data, evaluated at the
+// AMReX itself is not part of the reference tree.  FLASH's
+// CELL_CONSERVATIVE_LINEAR is amrex_interp_cell_cons
+// (RuntimeParameters_mapStrToInt.F90:482-485): AMReX's cell-conservative
+// linear interpolator without its cross-component limiting.  Its published
+// algorithm (mf_cell_cons_lin_interp_mcslope / mf_cell_cons_lin_interp):
+// per component and axis the MC slope sign(dc) min(|dc|, |df|, |db|) of the
+// central and doubled one-sided differences (0 unless df db >= 0), all
+// scaled by one factor alpha <= 1 that keeps every fine value inside the
+// min / max of the coarse cell's 3^ndim neighbourhood, evaluated at the
 // fine-cell offsets +-1/4 of a coarse cell.  PER_MASS variables (velx, vely,
 // velz, ener, eint; hydro unsplit Config:44-53) are interpolated and averaged
 // as density-weighted quantities.
@@ -88,12 +94,13 @@ __device__ __forceinline__ double amr_q(const AmrLaunch& A, const DevParams& P, 
   return (A.perMass >> v) & 1ULL ? G[P.voff[FX_DENS] + c] * x : x;
 }
 
-// MC-limited slope from the left and right differences
-__device__ __forceinline__ double amr_mc(double dl, double dr) {
-  if (!(dl * dr > 0.)) return 0.;
-  const double dc = 0.5 * (dl + dr);
-  const double m = fmin(fabs(dc), fmin(2. * fabs(dl), 2. * fabs(dr)));
-  return copysign(m, dc);
+// AMReX's MC slope of one axis from the coarse values u-, u0, u+
+__device__ __forceinline__ double amr_mcslope(double u0, double um, double up) {
+  const double dc = 0.5 * (up - um);
+  const double df = 2.0 * (up - u0);
+  const double db = 2.0 * (u0 - um);
+  const double s = (df * db >= 0.0) ? fmin(fabs(df), fabs(db)) : 0.0;
+  return copysign(1.0, dc) * fmin(s, fabs(dc));
 }
 
 // Guard cells of every block: same-level copy where the level has a block,
@@ -147,52 +154,63 @@ __global__ void __launch_bounds__(kThreads) k_amr_fill(AmrLaunch A,
       return;
     }
     const AmrBlock& Cb = A.blocks[cb];
-    const DevParams& PC = A.cls[Cb.cls];
-    int nbm[3], nbp[3];
-    long ncm[3], ncp[3];
+    const DevParams& PC = A.cls[Cb.cls];  // every block of level lev - 1 has this class
+    // the coarse cell's 3^ndim neighbourhood (centre included), as addresses
+    // of each cell's variable-0 slot; hood[(dz+1)*9 + (dy+1)*3 + (dx+1)]
+    const gdouble* hood[27];
     bool ok = true;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      if (a >= P.ndim) break;
-      int gm[3] = {gc[0], gc[1], gc[2]}, gp[3] = {gc[0], gc[1], gc[2]};
-      gm[a] -= 1;
-      gp[a] += 1;
-      amr_domain(A, lev - 1, P.ndim, gm);
-      amr_domain(A, lev - 1, P.ndim, gp);
-      ok = ok && amr_find(A, lev - 1, gm, nbm[a], ncm[a]) && amr_find(A, lev - 1, gp, nbp[a], ncp[a]);
+    for (int o = 0; o < 27; ++o) {
+      const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
+      hood[o] = nullptr;
+      if ((P.ndim < 2 && dy != 0) || (P.ndim < 3 && dz != 0)) continue;
+      int gn[3] = {gc[0] + dx, gc[1] + dy, gc[2] + dz};
+      amr_domain(A, lev - 1, P.ndim, gn);
+      int hb;
+      long hc;
+      if (amr_find(A, lev - 1, gn, hb, hc)) hood[o] = as_global(A.blocks[hb].U) + hc;
+      else ok = false;
     }
     if (!ok) {
       report_error(A.status, 3, b, i, j, k, -1);
       return;
     }
-    const double dens = [&] {
-      const int v = (int)(P.voff[FX_DENS] / P.ncell);
-      const double q0 = amr_q(A, PC, Cb.U, cc, v);
-      double q = q0;
+    const long dOff = PC.voff[FX_DENS];
+    // conserved-form value of variable v at neighbourhood cell o
+    auto qv = [&](int o, int v) {
+      const double x = hood[o][(long)v * PC.ncell];
+      return (A.perMass >> v) & 1ULL ? hood[o][dOff] * x : x;
+    };
+    // the interpolated conserved value of variable v at this fine cell
+    auto interp = [&](int v) {
+      const double q0 = qv(13, v);
+      double s[3] = {0.0, 0.0, 0.0};
+      s[0] = amr_mcslope(q0, qv(12, v), qv(14, v));
+      if (P.ndim > 1) s[1] = amr_mcslope(q0, qv(10, v), qv(16, v));
+      if (P.ndim > 2) s[2] = amr_mcslope(q0, qv(4, v), qv(22, v));
+      double alpha = 1.0;
+      if (s[0] != 0.0 || s[1] != 0.0 || s[2] != 0.0) {
+        double dumax = 0.0;
+        for (int a = 0; a < P.ndim; ++a) dumax = dumax + fabs(s[a]) * 0.25;
+        double umin = q0, umax = q0;
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        if (a >= P.ndim) break;
-        const AmrBlock& M = A.blocks[nbm[a]];
-        const AmrBlock& Q = A.blocks[nbp[a]];
-        const double s = amr_mc(q0 - amr_q(A, A.cls[M.cls], M.U, ncm[a], v),
-                                amr_q(A, A.cls[Q.cls], Q.U, ncp[a], v) - q0);
-        q = q + s * (off[a] ? 0.25 : -0.25);
+        for (int o = 0; o < 27; ++o) {
+          if (!hood[o]) continue;
+          const double x = qv(o, v);
+          umin = fmin(umin, x);
+          umax = fmax(umax, x);
+        }
+        if (dumax * alpha > (umax - q0)) alpha = (umax - q0) / dumax;
+        if (dumax * alpha > (q0 - umin)) alpha = (q0 - umin) / dumax;
       }
+      double q = q0;
+      for (int a = 0; a < P.ndim; ++a) q = q + (off[a] ? 0.25 : -0.25) * (s[a] * alpha);
       return q;
-    }();
+    };
+    const double dens = interp((int)(P.voff[FX_DENS] / P.ncell));
     const int v0 = A.onlyVar >= 0 ? A.onlyVar : 0, v1 = A.onlyVar >= 0 ? A.onlyVar + 1 : P.nunk;
     for (int v = v0; v < v1; ++v) {
-      const double q0 = amr_q(A, PC, Cb.U, cc, v);
-      double q = q0;
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        if (a >= P.ndim) break;
-        const AmrBlock& M = A.blocks[nbm[a]];
-        const AmrBlock& Q = A.blocks[nbp[a]];
-        const double s = amr_mc(q0 - amr_q(A, A.cls[M.cls], M.U, ncm[a], v),
-                                amr_q(A, A.cls[Q.cls], Q.U, ncp[a], v) - q0);
-        q = q + s * (off[a] ? 0.25 : -0.25);
-      }
+      const double q = interp(v);
       Ud[(long)v * P.ncell + cd] = (A.perMass >> v) & 1ULL ? q / dens : q;
     }
     if (A.onlyVar >= 0) return;

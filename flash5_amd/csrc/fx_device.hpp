@@ -246,7 +246,10 @@ __device__ __forceinline__ void trans_sigma(const DevParams& P, const R* vm1, co
 // midpoint, so both round alike; copysign keeps the sign of a zero.  This
 // holds for every finite x whose quotient is normal or zero (checked against
 // IEEE division by tests/cdiv/const_div_check.c); for C = 6 and 12 a
-// quotient below 2^-1022 may differ in its last place.  FP32 divides.
+// quotient below 2^-1022 may differ in its last place, and an infinite x
+// gives NaN (fma(-inf, C, inf)) where IEEE division gives inf.  Those are the
+// limits of the bitwise-parity claim: states whose PPM differences are
+// subnormal or infinite.  FP32 divides.
 template <int C>
 __device__ __forceinline__ double div_const(double x) {
   static_assert(C == 3 || C == 6 || C == 12, "constants of the form 3 * 2^k");

@@ -2265,6 +2265,38 @@ int fx_amr_setup(const fx_tile* tiles, const fx_amr_block* blocks, int nBlocks, 
     if (slot >= 0) return bad();  // two blocks at one position
     slot = b;
   }
+  // Each block of a level is either covered by finer blocks over its whole
+  // area or not at all, as FLASH's AMReX grids are (blocking factor 2 NXB,
+  // gr_amrexInit.F90:139-145).  The leaf-cell dt (a block with any finer
+  // block over it leaves the LEAF iteration, block_1lev_iterator.F90:293-330)
+  // then equals the device's covered-cell rule.
+  for (int L = 0; L + 1 < nLevels; ++L) {
+    const AmrLevelDev& C = lev[L];
+    const AmrLevelDev& F = lev[L + 1];
+    for (int pz = 0; pz < C.ext[2]; ++pz)
+      for (int py = 0; py < C.ext[1]; ++py)
+        for (int px = 0; px < C.ext[0]; ++px) {
+          if (index[L][((size_t)pz * C.ext[1] + py) * C.ext[0] + px] < 0) continue;
+          const int p[3] = {px, py, pz};
+          int lo[3], hi[3];
+          for (int a = 0; a < 3; ++a) {
+            if (a >= ndim) {
+              lo[a] = hi[a] = 0;
+              continue;
+            }
+            lo[a] = 2 * p[a] * C.nb[a] / F.nb[a];
+            hi[a] = (2 * (p[a] + 1) * C.nb[a] - 1) / F.nb[a];
+          }
+          long have = 0, all = 0;
+          for (int z = lo[2]; z <= hi[2]; ++z)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+              for (int x = lo[0]; x <= hi[0]; ++x) {
+                ++all;
+                if (index[L + 1][((size_t)z * F.ext[1] + y) * F.ext[0] + x] >= 0) ++have;
+              }
+          if (have > 0 && have < all) return bad();
+        }
+  }
   A.nBlocks = nBlocks;
   A.nLevels = nLevels;
   for (int a = 0; a < 3; ++a) A.periodic[a] = periodic ? periodic[a] != 0 : 0;

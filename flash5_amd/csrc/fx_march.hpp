@@ -38,8 +38,10 @@ template <int TX, int TY> struct TileIdx {
   static __device__ __forceinline__ int lx(int a, int b) { return b * (TX + 4) + (a + 2); }
   // slopes along y: a in [0,TX), b in [-2,TY+1]
   static __device__ __forceinline__ int ly(int a, int b) { return (b + 2) * TX + a; }
+  // eigen scalars of the current plane: a in [-2,TX+1], b in [-2,TY+1]
+  static __device__ __forceinline__ int ec(int a, int b) { return (b + 2) * (TX + 4) + (a + 2); }
   static constexpr int NRX = TX * (TY + 2), NRY = (TX + 2) * TY, NC = (TX + 2) * (TY + 2);
-  static constexpr int NLX = (TX + 4) * TY, NLY = TX * (TY + 4);
+  static constexpr int NLX = (TX + 4) * TY, NLY = TX * (TY + 4), NEC = (TX + 4) * (TY + 4);
 };
 
 // LDS of one column tile (march_tile), as offsets into one array of R
@@ -57,7 +59,8 @@ template <int NDIM, int TX, int TY> struct MarchLds {
   static constexpr int WY = WX + 12 * I::NRY;                 // y-face states
   static constexpr int WZ = WY + 12 * I::NRX;                 // z-face states of plane k
   static constexpr int WZP = WZ + (Z ? 12 * NCOL : 1);        // z-face Wp of plane k-1
-  static constexpr int total = WZP + (Z ? 6 * NCOL : 1);
+  static constexpr int EC = WZP + (Z ? 6 * NCOL : 1);         // eigen scalars of plane k
+  static constexpr int total = EC + 5 * I::NEC;
 };
 
 // components stored for sigma and slopes: dens, vx, vy, vz, pres, eint
@@ -252,10 +255,183 @@ __device__ __forceinline__ R flat_tilde_vals(R pm2, R pm1, R pp1, R pp2, R vm1, 
   return ft;
 }
 
+// ---- eigen scalars of the current plane, computed once per cell ----
+// The direction-independent scalars of eig_make (c, 1/(2c), 1/(2 rho c^2),
+// 1/c^2, rho c^2) of one cell, kept in LDS for every item of the plane that
+// needs the cell's eigen system in any direction; c < 0 marks a cell whose
+// c^2 is not positive.  The values are eig_make's own, so the bits are.
+template <typename R>
+__device__ __forceinline__ void ec_put(R* E, int n, int idx, const EigT<R>& e, bool ok) {
+  E[idx] = ok ? e.cf : R(-1.);
+  E[n + idx] = e.ahinv;
+  E[2 * n + idx] = e.lfp;
+  E[3 * n + idx] = e.a2inv;
+  E[4 * n + idx] = e.rp;
+}
+template <int DIR, typename R>
+__device__ __forceinline__ bool ec_get(const R* E, int n, int idx, const R* V, EigT<R>& e) {
+  e.cf = E[idx];
+  if (!(e.cf > R(0.))) return false;
+  e.ahinv = E[n + idx];
+  e.lfp = E[2 * n + idx];
+  e.a2inv = E[3 * n + idx];
+  e.rp = E[4 * n + idx];
+  e.rho = V[P_DENS];
+  e.un = V[P_VELX + DIR];
+  return true;
+}
+
+// ---- a tile's cell data as the march reads it ----
+// One uniform base address per primitive variable and 32-bit byte offsets of
+// cells (a tile holds far fewer than 2^29 cells): a load is then a scalar
+// base plus one vector offset, and a cell's offset serves every variable.
+typedef __attribute__((address_space(1))) char gchar;
+template <typename T>
+__device__ __forceinline__ T ld_at(const __attribute__((address_space(1))) T* base, unsigned boff) {
+  return *reinterpret_cast<const __attribute__((address_space(1))) T*>(
+      reinterpret_cast<const gchar*>(base) + boff);
+}
+template <typename T>
+__device__ __forceinline__ void st_at(__attribute__((address_space(1))) T* base, unsigned boff,
+                                      T v) {
+  *reinterpret_cast<__attribute__((address_space(1))) T*>(reinterpret_cast<gchar*>(base) + boff) = v;
+}
+
+struct TileView {
+  const gdouble* v[8];   // primitive slots P_DENS .. P_EINT
+  const gdouble* shok;   // SHOK_VAR (hybrid solver), else null
+};
+
+__device__ __forceinline__ TileView tile_view(const DevParams& P, const gdouble* U) {
+  TileView tv;
+  tv.v[P_DENS] = U + P.voff[FX_DENS];
+  tv.v[P_VELX] = U + P.voff[FX_VELX];
+  tv.v[P_VELY] = U + P.voff[FX_VELY];
+  tv.v[P_VELZ] = U + P.voff[FX_VELZ];
+  tv.v[P_PRES] = U + P.voff[FX_PRES];
+  tv.v[P_GAMC] = U + P.voff[FX_GAMC];
+  tv.v[P_GAME] = U + P.voff[FX_GAME];
+  tv.v[P_EINT] = U + P.voff[FX_EINT];
+  tv.shok = P.shokOff >= 0 ? U + P.shokOff : nullptr;
+  return tv;
+}
+
+// byte offset of cell (i, j, k) in its tile
+__device__ __forceinline__ unsigned cell_b(const DevParams& P, int i, int j, int k) {
+  return (unsigned)((k * P.ny + j) * P.nx + i) * 8u;
+}
+
+template <typename R>
+__device__ __forceinline__ void tv_prim(const TileView& tv, unsigned cb, R V[8]) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) V[q] = R(ld_at(tv.v[q], cb));
+}
+
+// sigma of the cell at cb along SD (the transverse-only PPM call; sb: the
+// axis stride in bytes)
+template <int SD, typename R>
+__device__ __forceinline__ bool tv_sigma(const DevParams& P, const TileView& tv, unsigned cb,
+                                         unsigned sb, R sig[8]) {
+  R Vm[8], Vc[8], Vp[8];
+  tv_prim(tv, cb - sb, Vm);
+  tv_prim(tv, cb, Vc);
+  tv_prim(tv, cb + sb, Vp);
+  EigT<R> e;
+  if (!eig_make<SD>(Vc, e)) return false;
+  trans_sigma<SD>(P, Vm, Vc, Vp, e, sig);
+  return true;
+}
+
+template <int DIR, typename R>
+__device__ __forceinline__ bool tv_slope(const DevParams& P, const TileView& tv, unsigned cb,
+                                         unsigned sb, R db[8]) {
+  R Vm[8], Vc[8], Vp[8];
+  tv_prim(tv, cb - sb, Vm);
+  tv_prim(tv, cb, Vc);
+  tv_prim(tv, cb + sb, Vp);
+  EigT<R> e;
+  if (!eig_make<DIR>(Vc, e)) return false;
+  tvd_slope<DIR>(P, Vm, Vc, Vp, e, db);
+  return true;
+}
+
+// the same with the cell's eigen scalars from the plane's cache (E, entry idx)
+template <int SD, typename R>
+__device__ __forceinline__ bool tv_sigma_c(const DevParams& P, const TileView& tv, unsigned cb,
+                                           unsigned sb, const R* E, int n, int idx, R sig[8]) {
+  R Vm[8], Vc[8], Vp[8];
+  tv_prim(tv, cb - sb, Vm);
+  tv_prim(tv, cb, Vc);
+  tv_prim(tv, cb + sb, Vp);
+  EigT<R> e;
+  if (!ec_get<SD>(E, n, idx, Vc, e)) return false;
+  trans_sigma<SD>(P, Vm, Vc, Vp, e, sig);
+  return true;
+}
+template <int DIR, typename R>
+__device__ __forceinline__ bool tv_slope_c(const DevParams& P, const TileView& tv, unsigned cb,
+                                           unsigned sb, const R* E, int n, int idx, R db[8]) {
+  R Vm[8], Vc[8], Vp[8];
+  tv_prim(tv, cb - sb, Vm);
+  tv_prim(tv, cb, Vc);
+  tv_prim(tv, cb + sb, Vp);
+  EigT<R> e;
+  if (!ec_get<DIR>(E, n, idx, Vc, e)) return false;
+  tvd_slope<DIR>(P, Vm, Vc, Vp, e, db);
+  return true;
+}
+
+// FlatTilde and FlatCoeff along DIR (hy_getRiemannState.F90:365-453)
+template <int DIR, typename R>
+__device__ __forceinline__ R tv_flat_tilde(const TileView& tv, unsigned cb, unsigned sb) {
+  const gdouble* pr = tv.v[P_PRES];
+  const gdouble* vn = tv.v[P_VELX + DIR];
+  return flat_tilde_vals(R(ld_at(pr, cb - 2 * sb)), R(ld_at(pr, cb - sb)), R(ld_at(pr, cb + sb)),
+                         R(ld_at(pr, cb + 2 * sb)), R(ld_at(vn, cb - sb)), R(ld_at(vn, cb + sb)));
+}
+template <int DIR, typename R>
+__device__ __forceinline__ R tv_flat_coeff(const TileView& tv, unsigned cb, unsigned sb) {
+  const gdouble* pr = tv.v[P_PRES];
+  const R dp1 = (R(ld_at(pr, cb + sb)) - R(ld_at(pr, cb - sb)));
+  const R f0 = tv_flat_tilde<DIR, R>(tv, cb, sb);
+  if (dp1 < R(0.0)) return fmax(f0, tv_flat_tilde<DIR, R>(tv, cb + sb, sb));
+  if (dp1 == R(0.)) return f0;
+  return fmax(f0, tv_flat_tilde<DIR, R>(tv, cb - sb, sb));
+}
+
+// Whether the face between the cells at cbl and cbr takes HLL (HLL, or the
+// hybrid switch with a flagged cell on either side, hy_getFaceFlux.F90:353-367)
+template <int RS>
+__device__ __forceinline__ bool tv_use_hll(const DevParams& P, const TileView& tv, unsigned cbl,
+                                           unsigned cbr) {
+  if constexpr (RS == FX_HYBR) {
+    if (P.riemann == FX_HLL) return true;
+    if (P.riemann != FX_HYBR) return false;
+    return ld_at(tv.shok, cbl) + ld_at(tv.shok, cbr) > 0.;
+  } else {
+    return false;
+  }
+}
+
+// face flux densities F[0..7] to the tile's flux array of direction D at the
+// face's byte offset rb (q-major, nface[D] faces per flux)
+template <int D>
+__device__ __forceinline__ void write_flux_b(const DevParams& P, gdouble* Fo, unsigned rb,
+                                             const double F[8]) {
+  const long nf = P.nface[D];
+#pragma unroll
+  for (int q = 0; q < FX_NFLUX; ++q) st_at(Fo + q * nf, rb, F[q]);
+}
+
 // One column tile (TX x TY columns at ox, oy, all planes) of tile t.  Every
 // thread of the workgroup calls it with the same arguments; it ends on a
 // barrier, so a workgroup may call it again for another column tile.  fsb:
 // tile t's face-state scratch (RS_STATES).
+//
+// Work items are laid out over fixed extents so that item numbers split by
+// compile-time divisors: x-face states over a in [-1, TX], x faces over
+// a in [0, TX] (y likewise); the high row (a = TX) is skipped unless the
+// tile sits at the block's high side, where it owns that row.
 template <int NDIM, int TX, int TY, int NT, int RS, typename R>
 __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch& L, int t, int ox,
                                            int oy, R* fsb, long faceBase, ExactStats& es,
@@ -276,18 +452,19 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
   R* const WY = smem + M::WY;
   R* const WZ = smem + M::WZ;
   R* const WZP = smem + M::WZP;
+  R* const EC = smem + M::EC;
 
   const int tid = threadIdx.x;
   // A tile owns its high x (y) face row only at the block's high side;
   // otherwise the neighbouring tile computes that row (and the states
   // behind it) as its low row.
-  const int ex = (ox + TX > P.hi[0]) ? 1 : 0;
-  const int ey = (oy + TY > P.hi[1]) ? 1 : 0;
+  const bool ex = (ox + TX > P.hi[0]);
+  const bool ey = (oy + TY > P.hi[1]);
   const fx_tile& T = L.tiles[t];
-  const gdouble* __restrict__ U = as_global(T.U);
+  const TileView tv = tile_view(P, as_global(T.U));
   const double dt = step_dt(L.dt, L.dtp);
   const double dx = T.del[0], dy = T.del[1], dz = T.del[2];
-  const long sx = 1, sy = P.nx, sz = (long)P.nx * P.ny;
+  const unsigned sxb = 8u, syb = (unsigned)P.nx * 8u, szb = (unsigned)(P.nx * P.ny) * 8u;
   const int klo = P.lo[2], khi = P.hi[2];
   DevStatus* st = L.status;
   bool bad = false;  // eigen failure seen by this thread
@@ -299,26 +476,23 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
     R sg[8];
     if (w < I::NRX) {
       const int a = w % TX, b = w / TX - 1;
-      if (!sigma_at<0>(P, U, cidx(P, ox + a, oy + b, kk), sg, nullptr)) { bad = true; return; }
+      if (!tv_sigma<0>(P, tv, cell_b(P, ox + a, oy + b, kk), sxb, sg)) { bad = true; return; }
       lds_put6(SX + ring(kk) * 6 * I::NRX, I::NRX, w, sg);
     } else {
       const int w2 = w - I::NRX;
       const int a = w2 % (TX + 2) - 1, b = w2 / (TX + 2);
-      if (!sigma_at<1>(P, U, cidx(P, ox + a, oy + b, kk), sg, nullptr)) { bad = true; return; }
+      if (!tv_sigma<1>(P, tv, cell_b(P, ox + a, oy + b, kk), syb, sg)) { bad = true; return; }
       lds_put6(SY + ring(kk) * 6 * I::NRY, I::NRY, w2, sg);
     }
   };
   // stage-A item: z-slope and z flattening tilde of column w at plane m
   auto zslope_item = [&](int w, int m) {
     const int a = w % TX, b = w / TX;
-    const long c = cidx(P, ox + a, oy + b, m);
+    const unsigned cb = cell_b(P, ox + a, oy + b, m);
     R db[8];
-    if (!slope_cell<2>(P, U, c, db)) { bad = true; return; }
+    if (!tv_slope<2>(P, tv, cb, szb, db)) { bad = true; return; }
     lds_put6(LZ + ring(m) * 6 * NCOL, NCOL, w, db);
-    const gdouble* pr = U + P.voff[FX_PRES];
-    const gdouble* vz = U + P.voff[FX_VELZ];
-    FZ[ring(m) * NCOL + w] = flat_tilde_vals(R(pr[c - 2 * sz]), R(pr[c - sz]), R(pr[c + sz]),
-                                             R(pr[c + 2 * sz]), R(vz[c - sz]), R(vz[c + sz]));
+    FZ[ring(m) * NCOL + w] = tv_flat_tilde<2, R>(tv, cb, szb);
   };
 
   const int kBeg = Z ? klo - 1 : 0;
@@ -332,8 +506,31 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
     }
   }
 
+  // fixed item extents of stages B and C (see above)
+  constexpr int NBX = (TX + 2) * TY, NBY = TX * (TY + 2);
+  constexpr int NFX = (TX + 1) * TY, NFY = TX * (TY + 1);
+
   for (int k = kBeg; k <= kEnd; ++k) {
     const bool interior = !Z || (k >= klo && k <= khi);
+    // ----------------- stage A0: eigen scalars of plane k ----------------
+    // every cell of plane k whose eigen system an item of this plane needs:
+    // a in [-1,TX] x b in [-1,TY], plus the x-slope ends (a = -2, TX+1,
+    // b in [0,TY)) and the y-slope ends (b = -2, TY+1, a in [0,TX))
+    for (int w = tid; w < I::NEC; w += NT) {
+      const int a = w % (TX + 4) - 2, b = w / (TX + 4) - 2;
+      const bool xend = a < -1 || a > TX, yend = b < -1 || b > TY;
+      if ((xend && (yend || b < 0 || b >= TY)) || (yend && (a < 0 || a >= TX))) continue;
+      const unsigned cb = cell_b(P, ox + a, oy + b, k);
+      R V[8];
+      V[P_DENS] = R(ld_at(tv.v[P_DENS], cb));
+      V[P_PRES] = R(ld_at(tv.v[P_PRES], cb));
+      V[P_GAMC] = R(ld_at(tv.v[P_GAMC], cb));
+      V[P_VELX] = R(0.);
+      EigT<R> e;
+      const bool ok = eig_make<0>(V, e);
+      ec_put(EC, I::NEC, w, e, ok);
+    }
+    __syncthreads();
     // ------------------------------ stage A ------------------------------
     {
       const int nSig = (!Z || k + 1 <= kEnd) ? I::NRX + I::NRY : 0;
@@ -348,7 +545,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
         if (r < nSz) {
           R v8[8];
           const int a = r % (TX + 2) - 1, b = r / (TX + 2) - 1;
-          if (!sigma_at<2>(P, U, cidx(P, ox + a, oy + b, k), v8, nullptr)) { bad = true; continue; }
+          if (!tv_sigma_c<2>(P, tv, cell_b(P, ox + a, oy + b, k), szb, EC, I::NEC, I::ec(a, b), v8)) { bad = true; continue; }
           lds_put6(SZ, I::NC, r, v8);
           continue;
         }
@@ -357,12 +554,12 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           R v8[8];
           if (r < I::NLX) {
             const int a = r % (TX + 4) - 2, b = r / (TX + 4);
-            if (!slope_cell<0>(P, U, cidx(P, ox + a, oy + b, k), v8)) { bad = true; continue; }
+            if (!tv_slope_c<0>(P, tv, cell_b(P, ox + a, oy + b, k), sxb, EC, I::NEC, I::ec(a, b), v8)) { bad = true; continue; }
             lds_put6(LX, I::NLX, r, v8);
           } else {
             r -= I::NLX;
             const int a = r % TX, b = r / TX - 2;
-            if (!slope_cell<1>(P, U, cidx(P, ox + a, oy + b, k), v8)) { bad = true; continue; }
+            if (!tv_slope_c<1>(P, tv, cell_b(P, ox + a, oy + b, k), syb, EC, I::NEC, I::ec(a, b), v8)) { bad = true; continue; }
             lds_put6(LY, I::NLY, r, v8);
           }
           continue;
@@ -377,25 +574,26 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
     const R* syk = SY + ring(k) * 6 * I::NRY;
     // ------------------------------ stage B1 -----------------------------
     {
-      const int nX = interior ? (TX + 1 + ex) * TY : 0;
-      const int nY = interior ? TX * (TY + 1 + ey) : 0;
+      const int nX = interior ? NBX : 0;
+      const int nY = interior ? NBY : 0;
       const int nZ = Z ? NCOL : 0;
       for (int w = tid; w < nX + nY + nZ; w += NT) {
         R Wp[8], Wn[8], Vm[8], Vc[8], Vp[8], dbN[8], db0[8], dbP[8];
         EigT<R> e0;
         if (w < nX) {
-          const int a = w % (TX + 1 + ex) - 1, b = w / (TX + 1 + ex);
-          const long c = cidx(P, ox + a, oy + b, k);
-          load_prim(P, U, c - sx, Vm);
-          load_prim(P, U, c, Vc);
-          load_prim(P, U, c + sx, Vp);
-          const R flat = P.flat ? flat_coeff<0, R>(P, U, c, sx) : R(0.);
+          const int a = w % (TX + 2) - 1, b = w / (TX + 2);
+          if (a == TX && !ex) continue;
+          const unsigned cb = cell_b(P, ox + a, oy + b, k);
+          tv_prim(tv, cb - sxb, Vm);
+          tv_prim(tv, cb, Vc);
+          tv_prim(tv, cb + sxb, Vp);
+          const R flat = P.flat ? tv_flat_coeff<0, R>(tv, cb, sxb) : R(0.);
           lds_get6(LX, I::NLX, I::lx(a - 1, b), dbN);
           lds_get6(LX, I::NLX, I::lx(a, b), db0);
           lds_get6(LX, I::NLX, I::lx(a + 1, b), dbP);
-          if (!eig_make<0>(Vc, e0)) { bad = true; continue; }
-          const R dmm = P.steep ? R(U[P.voff[FX_DENS] + c - 2 * sx]) : R(0.);
-          const R dpp = P.steep ? R(U[P.voff[FX_DENS] + c + 2 * sx]) : R(0.);
+          if (!ec_get<0>(EC, I::NEC, I::ec(a, b), Vc, e0)) { bad = true; continue; }
+          const R dmm = P.steep ? R(ld_at(tv.v[P_DENS], cb - 2 * sxb)) : R(0.);
+          const R dpp = P.steep ? R(ld_at(tv.v[P_DENS], cb + 2 * sxb)) : R(0.);
           ppm_trace<0>(P, R(dt / dx), R(dx), Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
           R sg[8];
           lds_get6(syk, I::NRY, I::ry(a, b), sg);
@@ -409,17 +607,18 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
         } else if (w < nX + nY) {
           const int w2 = w - nX;
           const int a = w2 % TX, b = w2 / TX - 1;
-          const long c = cidx(P, ox + a, oy + b, k);
-          load_prim(P, U, c - sy, Vm);
-          load_prim(P, U, c, Vc);
-          load_prim(P, U, c + sy, Vp);
-          const R flat = P.flat ? flat_coeff<1, R>(P, U, c, sy) : R(0.);
+          if (b == TY && !ey) continue;
+          const unsigned cb = cell_b(P, ox + a, oy + b, k);
+          tv_prim(tv, cb - syb, Vm);
+          tv_prim(tv, cb, Vc);
+          tv_prim(tv, cb + syb, Vp);
+          const R flat = P.flat ? tv_flat_coeff<1, R>(tv, cb, syb) : R(0.);
           lds_get6(LY, I::NLY, I::ly(a, b - 1), dbN);
           lds_get6(LY, I::NLY, I::ly(a, b), db0);
           lds_get6(LY, I::NLY, I::ly(a, b + 1), dbP);
-          if (!eig_make<1>(Vc, e0)) { bad = true; continue; }
-          const R dmm = P.steep ? R(U[P.voff[FX_DENS] + c - 2 * sy]) : R(0.);
-          const R dpp = P.steep ? R(U[P.voff[FX_DENS] + c + 2 * sy]) : R(0.);
+          if (!ec_get<1>(EC, I::NEC, I::ec(a, b), Vc, e0)) { bad = true; continue; }
+          const R dmm = P.steep ? R(ld_at(tv.v[P_DENS], cb - 2 * syb)) : R(0.);
+          const R dpp = P.steep ? R(ld_at(tv.v[P_DENS], cb + 2 * syb)) : R(0.);
           ppm_trace<1>(P, R(dt / dy), R(dy), Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
           R sg[8];
           lds_get6(sxk, I::NRX, I::rx(a, b), sg);
@@ -433,10 +632,10 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
         } else {
           const int w2 = w - nX - nY;
           const int a = w2 % TX, b = w2 / TX;
-          const long c = cidx(P, ox + a, oy + b, k);
-          load_prim(P, U, c - sz, Vm);
-          load_prim(P, U, c, Vc);
-          load_prim(P, U, c + sz, Vp);
+          const unsigned cb = cell_b(P, ox + a, oy + b, k);
+          tv_prim(tv, cb - szb, Vm);
+          tv_prim(tv, cb, Vc);
+          tv_prim(tv, cb + szb, Vp);
           R flat = R(0.);
           if (P.flat) {
             const R dp1 = Vp[P_PRES] - Vm[P_PRES];
@@ -447,9 +646,9 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           lds_get6(LZ + ring(k - 1) * 6 * NCOL, NCOL, w2, dbN);
           lds_get6(LZ + ring(k) * 6 * NCOL, NCOL, w2, db0);
           lds_get6(LZ + ring(k + 1) * 6 * NCOL, NCOL, w2, dbP);
-          if (!eig_make<2>(Vc, e0)) { bad = true; continue; }
-          const R dmm = P.steep ? R(U[P.voff[FX_DENS] + c - 2 * sz]) : R(0.);
-          const R dpp = P.steep ? R(U[P.voff[FX_DENS] + c + 2 * sz]) : R(0.);
+          if (!ec_get<2>(EC, I::NEC, I::ec(a, b), Vc, e0)) { bad = true; continue; }
+          const R dmm = P.steep ? R(ld_at(tv.v[P_DENS], cb - 2 * szb)) : R(0.);
+          const R dpp = P.steep ? R(ld_at(tv.v[P_DENS], cb + 2 * szb)) : R(0.);
           ppm_trace<2>(P, R(dt / dz), R(dz), Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
           R sg[8];
           lds_get6(sxk, I::NRX, I::rx(a, b), sg);
@@ -465,8 +664,8 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
 
     // ------------------------------ stage B2 -----------------------------
     if (Z && P.ctu3d) {
-      const int nX = interior ? (TX + 1 + ex) * TY : 0;
-      const int nY = interior ? TX * (TY + 1 + ey) : 0;
+      const int nX = interior ? NBX : 0;
+      const int nY = interior ? NBY : 0;
       for (int w = tid; w < nX + nY + NCOL; w += NT) {
         R Wp[8], Wn[8], Vc[8], sm[8], sc[8], sp[8], tA[8], tB[8];
         EigT<R> e1, e2;
@@ -476,10 +675,12 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
         bool own = true;  // the tile owns this state cell (counts its reset)
         if (w < nX) {
           // X: sigma_z over j with the y eigen system, sigma_y over k with z's
-          const int a = w % (TX + 1 + ex) - 1, b = w / (TX + 1 + ex);
+          const int a = w % (TX + 2) - 1, b = w / (TX + 2);
+          if (a == TX && !ex) continue;
           own = a >= 0 || ox == P.lo[0];
-          load_prim(P, U, cidx(P, ox + a, oy + b, k), Vc);
-          if (!(eig_make<1>(Vc, e1) && eig_make<2>(Vc, e2))) { bad = true; continue; }
+          tv_prim(tv, cell_b(P, ox + a, oy + b, k), Vc);
+          if (!(ec_get<1>(EC, I::NEC, I::ec(a, b), Vc, e1) &&
+                ec_get<2>(EC, I::NEC, I::ec(a, b), Vc, e2))) { bad = true; continue; }
           lds_get6(SZ, I::NC, I::cc(a, b - 1), sm);
           lds_get6(SZ, I::NC, I::cc(a, b), sc);
           lds_get6(SZ, I::NC, I::cc(a, b + 1), sp);
@@ -494,9 +695,11 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           // Y: sigma_x over k with z's eigen system, sigma_z over i with x's
           const int w2 = w - nX;
           const int a = w2 % TX, b = w2 / TX - 1;
+          if (b == TY && !ey) continue;
           own = b >= 0 || oy == P.lo[1];
-          load_prim(P, U, cidx(P, ox + a, oy + b, k), Vc);
-          if (!(eig_make<2>(Vc, e1) && eig_make<0>(Vc, e2))) { bad = true; continue; }
+          tv_prim(tv, cell_b(P, ox + a, oy + b, k), Vc);
+          if (!(ec_get<2>(EC, I::NEC, I::ec(a, b), Vc, e1) &&
+                ec_get<0>(EC, I::NEC, I::ec(a, b), Vc, e2))) { bad = true; continue; }
           lds_get6(SX + ring(k - 1) * 6 * I::NRX, I::NRX, I::rx(a, b), sm);
           lds_get6(sxk, I::NRX, I::rx(a, b), sc);
           lds_get6(SX + ring(k + 1) * 6 * I::NRX, I::NRX, I::rx(a, b), sp);
@@ -511,8 +714,9 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           // Z: sigma_y over i with x's eigen system, sigma_x over j with y's
           const int w2 = w - nX - nY;
           const int a = w2 % TX, b = w2 / TX;
-          load_prim(P, U, cidx(P, ox + a, oy + b, k), Vc);
-          if (!(eig_make<0>(Vc, e1) && eig_make<1>(Vc, e2))) { bad = true; continue; }
+          tv_prim(tv, cell_b(P, ox + a, oy + b, k), Vc);
+          if (!(ec_get<0>(EC, I::NEC, I::ec(a, b), Vc, e1) &&
+                ec_get<1>(EC, I::NEC, I::ec(a, b), Vc, e2))) { bad = true; continue; }
           lds_get6(syk, I::NRY, I::ry(a - 1, b), sm);
           lds_get6(syk, I::NRY, I::ry(a, b), sc);
           lds_get6(syk, I::NRY, I::ry(a + 1, b), sp);
@@ -534,71 +738,73 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
 
     // ------------------------------ stage C ------------------------------
     {
-      const int nFX = interior ? (TX + ex) * TY : 0;
-      const int nFY = interior ? TX * (TY + ey) : 0;
+      const int nFX = interior ? NFX : 0;
+      const int nFY = interior ? NFY : 0;
       const int nC = Z ? NCOL : (interior ? NCOL : 0);
       for (int w = tid; w < nFX + nFY + nC; w += NT) {
         R VL[8], VR[8];
         double F[8];
         if (w < nFX) {
-          const int a = w % (TX + ex), b = w / (TX + ex);
+          const int a = w % (TX + 1), b = w / (TX + 1);
+          if (a == TX && !ex) continue;
           lds_get6(WX, I::NRY, I::ry(a - 1, b), VL);
           lds_get6(WX + 6 * I::NRY, I::NRY, I::ry(a, b), VR);
-          const long c = cidx(P, ox + a, oy + b, k);
-          VL[P_GAMC] = R(U[P.voff[FX_GAMC] + c - sx]);
-          VL[P_GAME] = R(U[P.voff[FX_GAME] + c - sx]);
-          VR[P_GAMC] = R(U[P.voff[FX_GAMC] + c]);
-          VR[P_GAME] = R(U[P.voff[FX_GAME] + c]);
+          const unsigned cb = cell_b(P, ox + a, oy + b, k);
+          VL[P_GAMC] = R(ld_at(tv.v[P_GAMC], cb - sxb));
+          VL[P_GAME] = R(ld_at(tv.v[P_GAME], cb - sxb));
+          VR[P_GAMC] = R(ld_at(tv.v[P_GAMC], cb));
+          VR[P_GAME] = R(ld_at(tv.v[P_GAME], cb));
+          const unsigned fr = (unsigned)face_index(P, 0, ox + a, oy + b, k);
           if constexpr (is_states(RS)) {
-            const long fr = face_index(P, 0, ox + a, oy + b, k);
             write_face_states<0>(P, fsb + fstate_dir_offset(P, 0), fr, VL, VR);
             if constexpr (RS == RS_STATES_GATED)
               if (L.seedY) seed_gate<0>(P, L, faceBase, fr, VL, VR);
             continue;
           } else {
-            if (face_flux<0, RS>(P, VL, VR, use_hll<RS>(P, U, c - sx, c), F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 0); continue; }
-            write_flux<0>(P, as_global(T.flux[0]), face_index(P, 0, ox + a, oy + b, k), F);
+            if (face_flux<0, RS>(P, VL, VR, tv_use_hll<RS>(P, tv, cb - sxb, cb), F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 0); continue; }
+            write_flux_b<0>(P, as_global(T.flux[0]), fr * 8u, F);
           }
         } else if (w < nFX + nFY) {
           const int w2 = w - nFX;
           const int a = w2 % TX, b = w2 / TX;
+          if (b == TY && !ey) continue;
           lds_get6(WY, I::NRX, I::rx(a, b - 1), VL);
           lds_get6(WY + 6 * I::NRX, I::NRX, I::rx(a, b), VR);
-          const long c = cidx(P, ox + a, oy + b, k);
-          VL[P_GAMC] = R(U[P.voff[FX_GAMC] + c - sy]);
-          VL[P_GAME] = R(U[P.voff[FX_GAME] + c - sy]);
-          VR[P_GAMC] = R(U[P.voff[FX_GAMC] + c]);
-          VR[P_GAME] = R(U[P.voff[FX_GAME] + c]);
+          const unsigned cb = cell_b(P, ox + a, oy + b, k);
+          VL[P_GAMC] = R(ld_at(tv.v[P_GAMC], cb - syb));
+          VL[P_GAME] = R(ld_at(tv.v[P_GAME], cb - syb));
+          VR[P_GAMC] = R(ld_at(tv.v[P_GAMC], cb));
+          VR[P_GAME] = R(ld_at(tv.v[P_GAME], cb));
+          const unsigned fr = (unsigned)face_index(P, 1, ox + a, oy + b, k);
           if constexpr (is_states(RS)) {
-            const long fr = face_index(P, 1, ox + a, oy + b, k);
             write_face_states<1>(P, fsb + fstate_dir_offset(P, 1), fr, VL, VR);
             if constexpr (RS == RS_STATES_GATED)
               if (L.seedY) seed_gate<1>(P, L, faceBase, fr, VL, VR);
             continue;
           } else {
-            if (face_flux<1, RS>(P, VL, VR, use_hll<RS>(P, U, c - sy, c), F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 1); continue; }
-            write_flux<1>(P, as_global(T.flux[1]), face_index(P, 1, ox + a, oy + b, k), F);
+            if (face_flux<1, RS>(P, VL, VR, tv_use_hll<RS>(P, tv, cb - syb, cb), F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 1); continue; }
+            write_flux_b<1>(P, as_global(T.flux[1]), fr * 8u, F);
           }
         } else {
           // column item: z-face flux at face k, pavg of the cell, carry Wp
           const int w2 = w - nFX - nFY;
           const int a = w2 % TX, b = w2 / TX;
-          const long c = cidx(P, ox + a, oy + b, k);
+          const unsigned cb = cell_b(P, ox + a, oy + b, k);
           if (Z && k >= klo) {
             lds_get6(WZP, NCOL, w2, VL);
             lds_get6(WZ + 6 * NCOL, NCOL, w2, VR);
-            VL[P_GAMC] = R(U[P.voff[FX_GAMC] + c - sz]);
-            VL[P_GAME] = R(U[P.voff[FX_GAME] + c - sz]);
-            VR[P_GAMC] = R(U[P.voff[FX_GAMC] + c]);
-            VR[P_GAME] = R(U[P.voff[FX_GAME] + c]);
+            VL[P_GAMC] = R(ld_at(tv.v[P_GAMC], cb - szb));
+            VL[P_GAME] = R(ld_at(tv.v[P_GAME], cb - szb));
+            VR[P_GAMC] = R(ld_at(tv.v[P_GAMC], cb));
+            VR[P_GAME] = R(ld_at(tv.v[P_GAME], cb));
+            const unsigned fr = (unsigned)face_index(P, 2, ox + a, oy + b, k);
             if constexpr (is_states(RS)) {
-              const long fr = face_index(P, 2, ox + a, oy + b, k);
               write_face_states<2>(P, fsb + fstate_dir_offset(P, 2), fr, VL, VR);
               if constexpr (RS == RS_STATES_GATED)
                 if (L.seedY) seed_gate<2>(P, L, faceBase, fr, VL, VR);
             } else {
-              if (face_flux<2, RS>(P, VL, VR, use_hll<RS>(P, U, c - sz, c), F, es)) report_error(st, 2, t, ox + a, oy + b, k, 2);
-              else write_flux<2>(P, as_global(T.flux[2]), face_index(P, 2, ox + a, oy + b, k), F);
+              if (face_flux<2, RS>(P, VL, VR, tv_use_hll<RS>(P, tv, cb - szb, cb), F, es)) report_error(st, 2, t, ox + a, oy + b, k, 2);
+              else write_flux_b<2>(P, as_global(T.flux[2]), fr * 8u, F);
             }
           }
           if (interior && P.aux) {
@@ -607,8 +813,8 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
             R pres = R(0.5) * (WX[10 * I::NRY + I::ry(a, b)] + WX[4 * I::NRY + I::ry(a, b)]);
             pres = pres * R(0.5) + (WY[10 * I::NRX + I::rx(a, b)] + WY[4 * I::NRX + I::rx(a, b)]) * R(0.25);
             if (Z) pres = div_const<6>(pres * R(4.0) + (WZ[10 * NCOL + w2] + WZ[4 * NCOL + w2]));
-            const long r = ((long)(k - P.lo[2]) * P.nyb + (oy + b - P.lo[1])) * P.nxb + (ox + a - P.lo[0]);
-            as_global(T.pavg)[r] = double(pres);
+            const unsigned r = (unsigned)(((k - P.lo[2]) * P.nyb + (oy + b - P.lo[1])) * P.nxb + (ox + a - P.lo[0]));
+            st_at(as_global(T.pavg), r * 8u, double(pres));
           }
           if (Z) {
 #pragma unroll

@@ -17,14 +17,27 @@ Reference semantics (paths relative to source/Grid/GridMain/AMR/Amrex):
                  for each level pair, finest pair first; the face list covers
                  every coarse copy of a face on the boundary of the finer level
                  (TestFluxCorrection's Driver_evolveFlash.F90 states the cases).
-AMReX is not part of the reference tree.  Its cell-conservative linear
-interpolator is restated from its published description: per component,
-central slopes limited by the MC limiter, evaluated at the fine-cell offsets
-+-1/4 of a coarse cell.  Pinned here by invariants (tests/test_oracle_amr.py):
-the children of a coarse cell average back to it, linear data is reproduced
-exactly, constants are preserved, and restriction after prolongation is the
-identity.  The exact AMReX limiter variant (its optional extra bounds pass)
-is parity unpinned.
+AMReX is not part of the reference tree.  FLASH maps CELL_CONSERVATIVE_LINEAR
+to amrex_interp_cell_cons (RuntimeParameters_mapStrToInt.F90:482-485), the
+cell-conservative linear interpolator without AMReX's cross-component
+("lincc") limiting.  Its published algorithm (AMReX_Interp_{2,3}D_C.H,
+mf_cell_cons_lin_interp_mcslope and mf_cell_cons_lin_interp) is restated in
+prolong: per component and axis the MC slope
+    s = sign(dc) min(|dc|, min(|df|, |db|))  (0 unless df db >= 0),
+    dc = (u+ - u-)/2, df = 2 (u+ - u0), db = 2 (u0 - u-),
+then one factor alpha per component and coarse cell that keeps every fine
+value inside the min / max of the coarse cell's 3^ndim neighbourhood:
+    dumax = sum_a |s_a| (r-1)/(2r)  (1/4 at refinement ratio r = 2),
+    alpha = 1, lowered to (umax - u0)/dumax, then to (u0 - umin)/dumax,
+    where dumax alpha exceeds the room above, then below,
+and the fine value u0 + sum_a off_a alpha s_a at the offsets off = +-1/4.
+Pinned by (tests/test_oracle_amr.py): children averaging back to their
+parent, linear data reproduced exactly, constants kept, restriction after
+prolongation being the identity, every fine value inside its neighbourhood
+bounds with equality at the limited extreme, and a hand-computed 3-D case
+where the bound lowers the slopes (plain MC would overshoot).  In 1-D and
+2-D the factor never binds: an MC slope is at most twice either one-sided
+difference, so the neighbourhood always has room for sum_a |s_a|/4.
 
 A grid is a dict: ndim, ng, nb[L] (cells per block along x, y, z on level
 L, 0-based), ext[L] (blocks per axis covering the domain on level L),
@@ -84,10 +97,29 @@ def _gather(U, ids, loc, v, cons):
     return out
 
 
-def _mc(dl, dr):
-    dc = 0.5 * (dl + dr)
-    m = np.minimum(np.abs(dc), np.minimum(2.0 * np.abs(dl), 2.0 * np.abs(dr)))
-    return np.where(dl * dr > 0.0, np.copysign(m, dc), 0.0)
+def _mcslope(u0, um, up):
+    """AMReX's MC slope of one axis (mf_cell_cons_lin_interp_mcslope)."""
+    dc = 0.5 * (up - um)
+    df = 2.0 * (up - u0)
+    db = 2.0 * (u0 - um)
+    s = np.where(df * db >= 0.0, np.minimum(np.abs(df), np.abs(db)), 0.0)
+    return np.copysign(1.0, dc) * np.minimum(s, np.abs(dc))
+
+
+def _bound_factor(u0, slopes, umin, umax):
+    """AMReX's alpha: the largest factor <= 1 keeping u0 +- sum |s_a|/4 inside
+    [umin, umax], in the published operation order."""
+    dumax = np.zeros_like(u0)
+    for s in slopes:
+        dumax = dumax + np.abs(s) * 0.25
+    alpha = np.ones_like(u0)
+    live = dumax > 0.0
+    safe = np.where(live, dumax, 1.0)
+    hi = live & (dumax * alpha > (umax - u0))
+    alpha = np.where(hi, (umax - u0) / safe, alpha)
+    lo = live & (dumax * alpha > (u0 - umin))
+    alpha = np.where(lo, (u0 - umin) / safe, alpha)
+    return alpha
 
 
 def eos_consts(gamma, gas_constant=8.3144598e7):
@@ -111,8 +143,9 @@ def eos_dens_ei(c, dens, vx, vy, vz, eint_in, ener_in, small_e=1e-10, eint_switc
 
 def prolong(grid, U, idx, L, g, gamma, nunk=O.NVAR_USED, **eos_kw):
     """Values (nunk, n) at global cells g (3, n) of level L from level L - 1:
-    conservative linear interpolation in conserved form, per-mass form back,
-    then MODE_DENS_EI EOS."""
+    AMReX's cell-conservative linear interpolation (MC slopes bounded by the
+    3^ndim neighbourhood) in conserved form, per-mass form back, then
+    MODE_DENS_EI EOS."""
     ndim = grid["ndim"]
     c = eos_consts(gamma)
     gc = g // 2
@@ -126,14 +159,27 @@ def prolong(grid, U, idx, L, g, gamma, nunk=O.NVAR_USED, **eos_kw):
         hi[a] += 1
         lo, hi = _domain(grid, L - 1, lo), _domain(grid, L - 1, hi)
         nbr.append((_lookup(grid, idx, L - 1, lo), _lookup(grid, idx, L - 1, hi)))
+    hood = []                         # the 3^ndim neighbourhood, centre included
+    for d in np.ndindex(*([3] * ndim)):
+        gn = gc.copy()
+        for a in range(ndim):
+            gn[a] += d[a] - 1
+        hood.append(_lookup(grid, idx, L - 1, _domain(grid, L - 1, gn)))
 
     def interp(v):
         q0 = _gather(U, cid, cloc, v, True)
-        q = q0
+        slopes = []
         for a in range(ndim):
             (mi, ml), (pi, pl) = nbr[a]
-            s = _mc(q0 - _gather(U, mi, ml, v, True), _gather(U, pi, pl, v, True) - q0)
-            q = q + s * np.where(off[a] == 1, 0.25, -0.25)
+            slopes.append(_mcslope(q0, _gather(U, mi, ml, v, True), _gather(U, pi, pl, v, True)))
+        umin, umax = q0.copy(), q0.copy()
+        for hi_, hl in hood:
+            x = _gather(U, hi_, hl, v, True)
+            umin, umax = np.minimum(umin, x), np.maximum(umax, x)
+        alpha = _bound_factor(q0, slopes, umin, umax)
+        q = q0
+        for a in range(ndim):
+            q = q + np.where(off[a] == 1, 0.25, -0.25) * (slopes[a] * alpha)
         return q
     vals = np.empty((nunk, g.shape[1]))
     dens = interp(O.DENS)

@@ -87,9 +87,37 @@ def _leaf_mass(grid, U):
     return total
 
 
-@pytest.mark.parametrize("ndim", [2, 3])
-def test_amr_step_matches_oracle(ndim):
-    grid = _grid2d() if ndim == 2 else _grid3d()
+def _grid3d_config5():
+    """Config 5's block sizes on a small grid: level 0 is 2^3 blocks of 32^3,
+    level 1 (16^3) the 4^3 blocks under block (0,0,0), level 2 (8^3) the 4^3
+    blocks under level-1 block (1,1,1)."""
+    return LevelGrid.nested(3, [(32, 32, 32), (16, 16, 16), (8, 8, 8)], (2, 2, 2),
+                            [lambda p: p == (0, 0, 0), lambda p: p == (1, 1, 1)])
+
+
+def _rough(seed):
+    """Rough random primitive fields (every cell independent), so that the
+    prolongation's neighbourhood bound binds in 3-D."""
+    rng = np.random.default_rng(seed)
+
+    def fn(X, Y, Z):
+        return (1.0 + rng.uniform(0, 1, X.shape), rng.uniform(-1, 1, X.shape),
+                rng.uniform(-1, 1, X.shape), rng.uniform(-1, 1, X.shape),
+                1.0 + rng.uniform(0, 1, X.shape))
+    return fn
+
+
+def _interior(grid, L, u):
+    nb = grid.nb[L]
+    return u[(slice(None),) + tuple(slice(4, 4 + nb[a]) if a < grid.ndim else slice(0, 1)
+                                    for a in (2, 1, 0))]
+
+
+@pytest.mark.parametrize("shape", ["2d", "3d", "3d_32_16_8"])
+def test_amr_step_matches_oracle(shape):
+    grid = {"2d": _grid2d, "3d": _grid3d, "3d_32_16_8": _grid3d_config5}[shape]()
+    ndim = grid.ndim
+    print(f"amr step vs oracle: {shape}, {len(grid.blocks)} blocks", flush=True)
     U = _state(grid, 21)
     hp = HydroParams(ndim=ndim, nxb=grid.nb[0][0], nyb=grid.nb[0][1], nzb=grid.nb[0][2],
                      gamma=GAMMA)
@@ -106,9 +134,34 @@ def test_amr_step_matches_oracle(ndim):
                                 f"differ, first {diff[:3].tolist()}")
 
 
-def test_amr_fill_and_restrict_match_oracle():
+def test_amr_exact_step_matches_oracle():
+    """One FP64 exact-solver step of the three-level 3-D grid against the
+    oracle: the device takes powers as exp2(e log2 x) where the oracle calls
+    pow, so the bar is 1e-12 relative L1 per variable and block."""
     grid = _grid3d()
-    U = _state(grid, 23)
+    U = _state(grid, 37)
+    hp = HydroParams(ndim=3, nxb=16, nyb=16, nzb=16, gamma=GAMMA, RiemannSolver="exact")
+    dt = _dt(grid, U)
+    hy, tiles = _setup(grid, U, hp)
+    st = hy.amr_step(dt)
+    assert st.code == 0 and st.newton_iters > 0
+    Uo = [u.copy() for u in U]
+    OA.step(grid.as_dict(), Uo, dt, GAMMA, **_oracle_kw(hp))
+    worst = 0.0
+    for b, t in enumerate(tiles):
+        L = grid.blocks[b][0]
+        got, ref = _interior(grid, L, t.U.cpu().numpy()), _interior(grid, L, Uo[b])
+        for v in (O.DENS, O.VELX, O.VELY, O.VELZ, O.PRES, O.ENER, O.EINT, O.TEMP):
+            l1 = np.abs(got[v] - ref[v]).sum() / max(np.abs(ref[v]).sum(), 1e-99)
+            worst = max(worst, l1)
+            assert l1 <= 1e-12, f"block {b} var {v}: L1 {l1:.2e}"
+    print(f"worst per-variable L1 vs oracle: {worst:.2e}")
+
+
+@pytest.mark.parametrize("rough", [False, True])
+def test_amr_fill_and_restrict_match_oracle(rough):
+    grid = _grid3d()
+    U = _state(grid, 23, fn=_rough(23) if rough else None)
     hp = HydroParams(ndim=3, nxb=16, nyb=16, nzb=16, gamma=GAMMA)
     hy, tiles = _setup(grid, U, hp)
     hy.amr_fill_guards()
@@ -297,6 +350,14 @@ def test_amr_rejects_inconsistent_grids():
     Ul = _state(lone, 33)
     with pytest.raises(FxError) as e:
         _setup(lone, Ul, hp)
+    assert e.value.status.code == 3
+    # a level-0 block only partly covered by level 1 (one of its sixteen
+    # children missing): FLASH's grids refine whole blocks, and the leaf-cell
+    # dt is the reference's LEAF-block rule only then
+    partial = LevelGrid(2, grid.nb, grid.ext, [b for b in grid.blocks if b != (1, (0, 0, 0))])
+    Up = _state(partial, 39)
+    with pytest.raises(FxError) as e:
+        _setup(partial, Up, hp)
     assert e.value.status.code == 3
     # the library is usable after the refusals
     hy3, tiles3 = _setup(grid, U, hp)

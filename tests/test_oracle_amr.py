@@ -187,3 +187,123 @@ def test_eos_restatement_matches_oracle_eos():
                                         V[O.ENER])
     for v, x in ((O.PRES, pr), (O.TEMP, tp), (O.ENER, en), (O.EINT, ei), (O.GAME, ge)):
         np.testing.assert_array_equal(U[v], x)
+
+
+def _grid3d_small():
+    """Level 0: 2^3 blocks of 8^3 (cells 1/16); level 1 (8^3) under block (0,0,0)."""
+    return LevelGrid.nested(3, [(8, 8, 8), (8, 8, 8)], (2, 2, 2), [lambda p: p == (0, 0, 0)])
+
+
+def _coarse_hood(d, idx, U, gc, v):
+    """min and max of variable v (conserved form) over the 3^ndim
+    neighbourhood of coarse cells gc, brute force."""
+    ndim = d["ndim"]
+    lo = hi = None
+    for off in itertools.product(*[(-1, 0, 1)] * ndim):
+        gn = gc.copy()
+        for a in range(ndim):
+            gn[a] += off[a]
+        ids, loc = OA._lookup(d, idx, 0, OA._domain(d, 0, gn))
+        x = OA._gather(U, ids, loc, v, True)
+        lo = x if lo is None else np.minimum(lo, x)
+        hi = x if hi is None else np.maximum(hi, x)
+    return lo, hi
+
+
+def test_prolongation_bound_lowers_slopes_in_3d():
+    """Hand-computed AMReX case: coarse cell (5,5,5) holds 1, its -x/-y/-z
+    neighbours 0.5, its +x/+y/+z neighbours 1.25, every other cell 1.  Each
+    MC slope is min(2*0.25, 2*0.5, 0.375) = 0.375; the corner child would
+    reach 1 + 3*0.375/4 = 1.28125 > 1.25, so alpha = 0.25/0.28125 = 8/9 and
+    the corner child lands on the neighbourhood maximum 1.25."""
+    grid = _grid3d_small()
+    n0 = grid.ext[0][0] * grid.nb[0][0]
+
+    def fn(X, Y, Z):
+        ic = [np.floor(c * n0).astype(int) for c in (X, Y, Z)]
+        dens = np.ones(X.shape)
+        for a in range(3):
+            other = [ic[b] == 5 for b in range(3) if b != a]
+            dens = np.where(other[0] & other[1] & (ic[a] == 4), 0.5, dens)
+            dens = np.where(other[0] & other[1] & (ic[a] == 6), 1.25, dens)
+        return dens, 0 * X, 0 * X, 0 * X, 1.0 + 0 * X
+
+    U = _state(grid, 1, fn=fn)
+    d = grid.as_dict()
+    idx = OA._index(d)
+    kids = np.array([[10 + ox, 10 + oy, 10 + oz] for oz in (0, 1) for oy in (0, 1)
+                     for ox in (0, 1)], dtype=np.int64).T
+    vals = OA.prolong(d, U, idx, 1, kids, GAMMA)[O.DENS]
+    alpha = 0.25 / 0.28125
+    s = 0.375 * alpha
+    assert vals[-1] == pytest.approx(1.25, rel=0, abs=5e-16)      # (+,+,+) child at the bound
+    assert vals[-1] <= 1.25
+    assert vals[0] == pytest.approx(1.0 - 0.75 * s, abs=5e-16)    # (-,-,-) child
+    assert vals.mean() == pytest.approx(1.0, abs=1e-15)
+    plain = 1.0 + 3 * 0.25 * 0.375                                 # unbounded MC corner child
+    assert plain > 1.25
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_prolongation_stays_inside_the_neighbourhood(ndim):
+    """Rough random data: every child (conserved form) lies inside the min and
+    max of its parent's 3^ndim neighbourhood.  In 3-D the bound binds for
+    some parents (alpha < 1), and there the extreme child sits on the bound;
+    in 2-D it never binds (MC slopes alone keep the children inside)."""
+    grid = _grid2d() if ndim == 2 else _grid3d_small()
+    rng = np.random.default_rng(11 + ndim)
+
+    def fn(X, Y, Z):
+        return (1.0 + rng.uniform(0, 1, X.shape), rng.uniform(-1, 1, X.shape),
+                rng.uniform(-1, 1, X.shape), rng.uniform(-1, 1, X.shape),
+                1.0 + rng.uniform(0, 1, X.shape))
+
+    U = _state(grid, 0, fn=fn)
+    g, idx, d = _children(grid, 1)
+    vals = OA.prolong(d, U, idx, 1, g, GAMMA)
+    gc = g // 2
+    nchild = 2 ** ndim
+    bound = 0
+    for v in (O.DENS, O.VELX, O.VELY, O.VELZ):
+        q = vals[v] * vals[O.DENS] if v in OA.PER_MASS else vals[v]
+        lo, hi = _coarse_hood(d, idx, U, gc, v)
+        tol = 4e-16 * np.maximum(np.abs(lo), np.abs(hi))
+        assert np.all(q >= lo - tol) and np.all(q <= hi + tol), f"var {v}"
+        # the factor per parent, from the slopes of its axis neighbours
+        par = gc[:, ::nchild]
+        cid, cloc = OA._lookup(d, idx, 0, par)
+        q0 = OA._gather(U, cid, cloc, v, True)
+        slopes = []
+        for a in range(ndim):
+            m, p_ = par.copy(), par.copy()
+            m[a] -= 1
+            p_[a] += 1
+            um = OA._gather(U, *OA._lookup(d, idx, 0, OA._domain(d, 0, m)), v, True)
+            up = OA._gather(U, *OA._lookup(d, idx, 0, OA._domain(d, 0, p_)), v, True)
+            slopes.append(OA._mcslope(q0, um, up))
+        plo, phi = lo[::nchild], hi[::nchild]
+        alpha = OA._bound_factor(q0, slopes, plo, phi)
+        lim = alpha < 1.0
+        bound += int(lim.sum())
+        qc = q.reshape(-1, nchild)
+        ext = np.where(np.abs(qc.max(axis=1) - phi) < np.abs(qc.min(axis=1) - plo),
+                       qc.max(axis=1) - phi, qc.min(axis=1) - plo)
+        assert np.all(np.abs(ext[lim]) <= 8e-16 * np.abs(phi[lim]) + 1e-15), f"var {v}"
+        # children average back to the parent
+        np.testing.assert_allclose(qc.mean(axis=1), q0, rtol=1e-14, atol=1e-15)
+    if ndim == 3:
+        assert bound > 0
+    else:
+        assert bound == 0
+
+
+def test_mcslope_matches_the_mc_limiter_on_a_table():
+    """AMReX's slope equals the textbook MC limiter of the one-sided
+    differences, (sgn a + sgn b)/2 min(2|a|, 2|b|, |a+b|/2), on a table
+    (up to the rounding of the central difference)."""
+    vals = np.array([-2.0, -1.0, -0.25, 0.0, 0.3, 1.0, 3.0])
+    for um, u0, up in itertools.product(vals, vals, vals):
+        a, b = u0 - um, up - u0
+        want = 0.5 * (np.sign(a) + np.sign(b)) * min(2 * abs(a), 2 * abs(b), abs(a + b) / 2)
+        got = OA._mcslope(np.array([u0]), np.array([um]), np.array([up]))[0]
+        assert got == pytest.approx(want, abs=1e-15), (um, u0, up)
