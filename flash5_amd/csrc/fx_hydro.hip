@@ -34,6 +34,15 @@
 
 using namespace fx;
 
+// Scratch of fx_hydro_advance_batch_noalloc: the status record, the tile
+// table, and the face fluxes and face-averaged pressure of every block of a
+// call, carved from one fixed device array.  Each GPU the library runs on
+// holds its own copy until the process ends.  Host code never uses the
+// array's name as an address: it takes the current device's address from
+// hipGetSymbolAddress (noalloc_scratch below).
+constexpr size_t kNoAllocBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 namespace {
 
 
@@ -576,6 +585,33 @@ __global__ void k_reset_status(DevStatus* s) {
   s->shockTiles = 0;
 }
 
+// Tile table of a strided batch built on the device (the no-allocation
+// batch step): block b's U at U + b*stride, its flux arrays (directions
+// below ndim) and pavg carved from flux / pavg at fixed per-block strides.
+__global__ void __launch_bounds__(kThreads) k_batch_tiles(fx_tile* __restrict__ tiles, double* U,
+                                                          long n, long stride, double dx, double dy,
+                                                          double dz, double* flux, long nf0,
+                                                          long nf1, long nf2, int ndim,
+                                                          double* pavg, long icells) {
+  const long perBlock = (nf0 + (ndim > 1 ? nf1 : 0) + (ndim > 2 ? nf2 : 0)) * FX_NFLUX;
+  for (long b = (long)blockIdx.x * blockDim.x + threadIdx.x; b < n;
+       b += (long)gridDim.x * blockDim.x) {
+    fx_tile t;
+    t.U = U + b * stride;
+    double* f = flux + b * perBlock;
+    t.flux[0] = f;
+    t.flux[1] = ndim > 1 ? f + nf0 * FX_NFLUX : nullptr;
+    t.flux[2] = ndim > 2 ? f + (nf0 + nf1) * FX_NFLUX : nullptr;
+    t.pavg = pavg + b * icells;
+    t.del[0] = dx;
+    t.del[1] = dy;
+    t.del[2] = dz;
+    t.level = 1;
+    t.block_class = 0;
+    tiles[b] = t;
+  }
+}
+
 // ---------------------------------------------------------------------
 // host state
 // ---------------------------------------------------------------------
@@ -977,11 +1013,13 @@ int validate(const fx_params* p) {
 // hy_shockDetectBlk.F90:228-235: the CFL cap of a step with a flagged cell
 double shock_cfl_max(int ndim) { return ndim == 1 ? 0.60 : (ndim == 2 ? 0.45 : 0.25); }
 
-int finish(fx_status* st) {
+// Synchronise and copy the device status record ds into st (nothing in
+// async mode, where the call returns before its kernels end).
+int finish_from(const DevStatus* ds, fx_status* st) {
   if (g.async) return 0;
   HIPCALL(hipStreamSynchronize(g.stream), st);
   DevStatus h{};
-  HIPCALL(hipMemcpy(&h, g.status, sizeof h, hipMemcpyDeviceToHost), st);
+  HIPCALL(hipMemcpy(&h, ds, sizeof h, hipMemcpyDeviceToHost), st);
   if (st) {
     st->code = h.code;
     st->first_tile = h.tile;
@@ -996,6 +1034,48 @@ int finish(fx_status* st) {
     if (g.prm.shockLowerCFL && h.shockTiles) st->min_cell_cfl = shock_cfl_max(g.prm.ndim);
   }
   return h.code;
+}
+
+int finish(fx_status* st) { return finish_from(g.status, st); }
+
+// ---- the batch step without allocation ----
+// Device address of the scratch array on each device, taken once per device
+// (with that device current) by hipGetSymbolAddress.
+constexpr int kMaxDevices = 64;
+void* g_scratchAddr[kMaxDevices] = {};
+
+int noalloc_scratch(void** out, fx_status* st) {
+  int dev = 0;
+  HIPCALL(hipGetDevice(&dev), st);
+  if (dev < 0 || dev >= kMaxDevices) {
+    fprintf(stderr, "fx_hydro: device %d beyond the %d devices the no-allocation scratch serves\n",
+            dev, kMaxDevices);
+    if (st) st->code = 3;
+    return 3;
+  }
+  if (!g_scratchAddr[dev]) {
+    void* p = nullptr;
+    HIPCALL(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), st);
+    g_scratchAddr[dev] = p;
+  }
+  *out = g_scratchAddr[dev];
+  return 0;
+}
+
+// Byte offsets of the no-allocation step's parts inside the scratch array.
+struct NoAllocLayout {
+  size_t tiles, flux, pavg, total;  // the status record sits at offset 0
+};
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+NoAllocLayout noalloc_layout(const DevParams& P, long nBlocks) {
+  NoAllocLayout L;
+  long facesPerBlock = 0;
+  for (int d = 0; d < P.ndim; ++d) facesPerBlock += P.nface[d];
+  L.tiles = align256(sizeof(DevStatus));
+  L.flux = align256(L.tiles + sizeof(fx_tile) * (size_t)nBlocks);
+  L.pavg = align256(L.flux + sizeof(double) * FX_NFLUX * (size_t)facesPerBlock * (size_t)nBlocks);
+  L.total = L.pavg + sizeof(double) * (size_t)P.icells * (size_t)nBlocks;
+  return L;
 }
 
 // Inside a graph-captured run under shockLowerCFL: the device slot holding
@@ -1826,6 +1906,9 @@ int fx_hydro_init(const fx_params* p) {
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
       g.numCU = prop.multiProcessorCount;
   }
+  void* scratchAddr = nullptr;  // the no-allocation step's scratch on this device
+  rc = noalloc_scratch(&scratchAddr, nullptr);
+  if (rc) return rc;
   g.init = true;
   return 0;
 }
@@ -1885,6 +1968,53 @@ int fx_hydro_advance_batch(double* U, long nBlocks, long blockStride, const doub
   const int rc = prepare_batch(U, nBlocks, blockStride, del, st);
   if (rc) return rc;
   return run_compute(nullptr, (int)nBlocks, dt, nullptr, st, true);
+}
+
+int fx_hydro_advance_batch_noalloc(double* U, long nBlocks, long blockStride,
+                                   const double del[3], double dt, fx_status* st) {
+  if (st) memset(st, 0, sizeof *st);
+  if (!g.init || !U || !del || nBlocks < 0 || nBlocks > 0x7fffffffL ||
+      blockStride < g.dp.ncell * g.dp.nunk) {
+    if (st) st->code = 3;
+    return 3;
+  }
+  if (nBlocks == 0) return 0;
+  const DevParams& P = g.dp;
+  int TX = 0, TY = 0, NT = 0;
+  if (P.riemann == FX_EXACT || !march_shape(P, &TX, &TY, &NT)) {
+    fprintf(stderr, "fx_hydro_advance_batch_noalloc: takes the HLLC, HLL and hybrid solvers on "
+                    "2-D/3-D blocks with x and y extents multiples of 8\n");
+    if (st) st->code = 3;
+    return 3;
+  }
+  const NoAllocLayout Lay = noalloc_layout(P, nBlocks);
+  if (Lay.total > kNoAllocBytes) {
+    fprintf(stderr, "fx_hydro_advance_batch_noalloc: %ld blocks need %zu bytes of scratch, "
+                    "more than its fixed %zu\n", nBlocks, Lay.total, kNoAllocBytes);
+    if (st) st->code = 3;
+    return 3;
+  }
+  void* base = nullptr;
+  const int rc = noalloc_scratch(&base, st);
+  if (rc) return rc;
+  char* s = static_cast<char*>(base);
+  DevStatus* status = reinterpret_cast<DevStatus*>(s);
+  fx_tile* tiles = reinterpret_cast<fx_tile*>(s + Lay.tiles);
+  hipLaunchKernelGGL(k_batch_tiles, dim3(grid_for(nBlocks)), dim3(kThreads), 0, g.stream, tiles,
+                     U, nBlocks, blockStride, del[0], del[1], del[2],
+                     reinterpret_cast<double*>(s + Lay.flux), P.nface[0], P.nface[1], P.nface[2],
+                     P.ndim, reinterpret_cast<double*>(s + Lay.pavg), P.icells);
+  hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, status);
+  MarchLaunch M{tiles, (int)nBlocks, P.nxb / TX, (P.ndim > 1 ? P.nyb : 1) / TY, status, dt,
+                nullptr, nullptr, nullptr, nullptr, nullptr, make_seed_gate(g.seedGate), 0u};
+  const dim3 grid((unsigned)(nBlocks * M.tilesX * M.tilesY));
+  if (P.riemann == FX_HLLC) launch_march<FX_HLLC>(P, M, TX, grid);
+  else launch_march<FX_HYBR>(P, M, TX, grid);
+  Launch L{tiles, (int)nBlocks, nullptr, status, dt, nullptr};
+  hipLaunchKernelGGL((k_update<false, false>), dim3(grid_for(P.icells * nBlocks)), dim3(kThreads),
+                     0, g.stream, P, L, nullptr);
+  HIPCALL(hipGetLastError(), st);
+  return finish_from(status, st);
 }
 
 int fx_hydro_shock_detect_batch(double* U, long nBlocks, long blockStride, const double del[3],

@@ -95,6 +95,8 @@ EXPORTS = {
                                   C.POINTER(FxStatus)], C.c_int),
     "fx_hydro_advance_batch": ([_DP, C.c_long, C.c_long, C.c_double * 3, C.c_double,
                                 C.POINTER(FxStatus)], C.c_int),
+    "fx_hydro_advance_batch_noalloc": ([_DP, C.c_long, C.c_long, C.c_double * 3, C.c_double,
+                                        C.POINTER(FxStatus)], C.c_int),
     "fx_hydro_shock_detect": ([C.POINTER(FxTile), C.c_int, C.POINTER(FxStatus)], C.c_int),
     "fx_hydro_shock_detect_batch": ([_DP, C.c_long, C.c_long, C.c_double * 3,
                                      C.POINTER(FxStatus)], C.c_int),
@@ -556,6 +558,21 @@ class Hydro:
         self.last_status = st
         if rc:
             raise FxError("fx_hydro_advance_batch", st, rc)
+        return st
+
+    def advance_batch_noalloc(self, U, dt: float, deltas: Sequence[float]) -> FxStatus:
+        """advance_batch through the entry point that allocates nothing (its
+        scratch is a fixed device array: at most 77 blocks of 16^3 a call)."""
+        self._reinit()
+        nb = U.shape[0]
+        stride = U[0].numel() if nb else 0
+        st = FxStatus()
+        rc = self.lib.fx_hydro_advance_batch_noalloc(_ptr(U), nb, stride,
+                                                     (C.c_double * 3)(*[float(x) for x in deltas]),
+                                                     dt, C.byref(st))
+        self.last_status = st
+        if rc:
+            raise FxError("fx_hydro_advance_batch_noalloc", st, rc)
         return st
 
     def compute_dt(self, U, deltas: Sequence[float], cfl: float) -> float:

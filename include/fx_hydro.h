@@ -173,6 +173,22 @@ int fx_hydro_shock_detect_batch(double* U, long nBlocks, long blockStride,
 int fx_hydro_advance_batch(double* U, long nBlocks, long blockStride,
                            const double del[3], double dt, fx_status* st);
 
+/* fx_hydro_advance_batch without any allocation.  No counterpart in the
+ * reference: it serves callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and bits as fx_hydro_advance_batch,
+ * for the HLLC, HLL and hybrid solvers on 2-D and 3-D blocks whose interior
+ * x and y extents are multiples of 8; other configurations return 3.  Its
+ * scratch (status record, tile table, face fluxes and face-averaged
+ * pressure: 848 KiB per 16^3 block, 112 KiB per 8^3 block) lives in a fixed
+ * 64 MiB device array that each GPU holds until the process ends, so one
+ * call takes at most 77 blocks of 16^3 or 584 of 8^3; a larger call
+ * returns 3 with a message on stderr.  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (other host threads or
+ * streams) race.  fx_hydro_init must have run. */
+int fx_hydro_advance_batch_noalloc(double* U, long nBlocks, long blockStride,
+                                   const double del[3], double dt, fx_status* st);
+
 /* Hydro_computeDt over a strided batch (device-side min reduction):
  * dt = cfl / max over interior cells of max_d (|v_d| + c) / del_d. */
 int fx_hydro_compute_dt(const double* U, long nBlocks, long blockStride,
