@@ -1661,10 +1661,14 @@ int tune_flux_stages(int nTiles, double dt, int reps, fx_tune_result* out, int c
     can16 = P.nxb % 16 == 0 && P.nyb % 8 == 0;
   }
   std::vector<Cand> cands;
-  const bool occHint = P.precision == FX_PREC_MIXED && P.ndim == 3;
-  for (int occ : {0, 3}) {
-    if (occ && !occHint) continue;
+  // waves-per-SIMD hints: 3 for the FP32 3-D march (fewer registers, maybe
+  // spills), 2 for the FP64 3-D 8-wide march (two workgroups per CU)
+  const bool occ3 = P.precision == FX_PREC_MIXED && P.ndim == 3;
+  const bool occ2 = P.precision == FX_PREC_FP64 && P.ndim == 3;
+  for (int occ : {0, 3, 2}) {
+    if ((occ == 3 && !occ3) || (occ == 2 && !occ2)) continue;
     for (int tx : {16, kPerClassTX, 8}) {
+      if (occ == 2 && tx != 8) continue;
       if (tx == 16 && !can16) continue;
       if (tx == kPerClassTX && (!g.multi || can16 || !any16)) continue;
       if (g.multi) {

@@ -22,6 +22,11 @@ void launch_march_r(const DevParams& P, const MarchLaunch& M, int TX, dim3 grid,
         hipLaunchKernelGGL((k_march<3, 8, 8, 256, RS, R, 3>), grid, dim3(256), 0, H.stream, P, M);
       return;
     }
+  } else {  // FP64 3-D, 8-wide tiles: 2 waves per SIMD, so two workgroups share a CU
+    if (P.ndim == 3 && H.tuneOcc == 2 && TX == 8) {
+      hipLaunchKernelGGL((k_march<3, 8, 8, 256, RS, R, 2>), grid, dim3(256), 0, H.stream, P, M);
+      return;
+    }
   }
   if (P.ndim == 3 && TX == 16) hipLaunchKernelGGL((k_march<3, 16, 8, 512, RS, R>), grid, dim3(512), 0, H.stream, P, M);
   else if (P.ndim == 3) hipLaunchKernelGGL((k_march<3, 8, 8, 256, RS, R>), grid, dim3(256), 0, H.stream, P, M);
@@ -86,6 +91,11 @@ void launch_march_multi_r(int ndim, int TX, const MarchLaunch& M, const MarchHos
     if (ndim == 3 && H.tuneOcc == 3) {
       if (TX == 16) launch_march_multi_k<3, 16, 8, 512, RS, R, 3>(M, H);
       else launch_march_multi_k<3, 8, 8, 256, RS, R, 3>(M, H);
+      return;
+    }
+  } else {  // FP64 3-D, 8-wide tiles: 2 waves per SIMD
+    if (ndim == 3 && H.tuneOcc == 2 && TX == 8) {
+      launch_march_multi_k<3, 8, 8, 256, RS, R, 2>(M, H);
       return;
     }
   }

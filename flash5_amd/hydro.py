@@ -31,7 +31,7 @@ NFLUX = 8
 DENS, VELX, VELY, VELZ, PRES, ENER, GAMC, GAME, TEMP, EINT = range(10)
 MINMOD, MC, HYBRID, VANLEER, LIMITED = 1, 2, 3, 4, 5
 ROE, HLL, HLLC, HYBR, EXACT = 1, 3, 4, 8, 20
-PREC_FP64, PREC_FP64_FAST, PREC_MIXED = 0, 1, 2
+PREC_FP64, PREC_MIXED = 0, 2
 
 _LIMITERS = {"minmod": MINMOD, "mc": MC, "hybrid": HYBRID, "vanleer": VANLEER,
              "limited": LIMITED}

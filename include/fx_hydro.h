@@ -51,9 +51,8 @@ enum { FX_MINMOD = 1, FX_MC = 2, FX_HYBRID = 3, FX_VANLEER = 4, FX_LIMITED = 5 }
  * at faces next to a flagged shock cell, HLLC elsewhere,
  * hy_getFaceFlux.F90:353-367); FX_EXACT = exact iterative solver */
 enum { FX_ROE = 1, FX_HLL = 3, FX_HLLC = 4, FX_HYBR = 8, FX_EXACT = 20 };
-/* precision modes */
+/* precision modes (value 1 is unused; fx_hydro_init rejects it) */
 enum { FX_PREC_FP64 = 0,      /* plain FP64, IEEE ops without contraction */
-       FX_PREC_FP64_FAST = 1, /* reserved (rejected by fx_hydro_init) */
        FX_PREC_MIXED = 2 };   /* FP32 reconstruction and fluxes (HLLC, HLL
                                  and the exact solver, whose face states are
                                  stored in FP32); FP64 update, e_int
@@ -95,7 +94,11 @@ typedef struct {
                                shock detection is on */
 } fx_params;
 
-/* One tile as the reference's tile iterator yields it (Grid_tile.F90). */
+/* One tile as the reference's tile iterator yields it with tiling off
+ * (Grid_tile.F90; gr_useTiling defaults to .FALSE., Grid/GridMain/Config:183):
+ * a whole block.  There are no tile limits: the library computes every
+ * interior cell of the block, so a caller iterating with tiling on must still
+ * pass each block once. */
 typedef struct {
   double* U;                /* U[var][k][j][i], guard-inclusive */
   double* flux[3];          /* flux densities per direction, or NULL */
@@ -323,7 +326,9 @@ int fx_hydro_set_block_classes(const int* nb, int nClasses);
  * fastest for later calls.  Candidates: the column-tile width (16 or 8
  * columns across x, where every tile shape divides it, or per class for
  * tiles of several classes); in the mixed mode in
- * 3-D, a register budget for 3 waves per SIMD (occupancy against spills);
+ * 3-D, a register budget for 3 waves per SIMD (occupancy against spills); in
+ * FP64 3-D, 8-wide column tiles held to 2 waves per SIMD, so that two
+ * workgroups share a CU;
  * and for tiles of several classes the persistent workgroups per CU (1, 2
  * or 4 times what stays resident) and the order of the column work items
  * (longest columns first, or tile by tile).  Each candidate runs reps times after one
@@ -336,7 +341,8 @@ typedef struct {
                                class: 16 where the class allows it, else 8) */
   int persist;              /* workgroups per CU multiple (multi-class only; 0 = n/a) */
   int order;                /* multi-class items: 0 longest first, 1 tile by tile */
-  int occupancy;            /* waves-per-SIMD hint of the FP32 3-D march (0 = none) */
+  int occupancy;            /* waves-per-SIMD hint of the 3-D march: 3 (FP32),
+                               2 (FP64, 8-wide tiles), 0 = none */
   int chosen;
   float ms;                 /* fastest of reps */
 } fx_tune_result;
