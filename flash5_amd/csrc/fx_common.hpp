@@ -213,4 +213,90 @@ __device__ __forceinline__ long face_index(const DevParams& P, int d, int i, int
   return ((long)(k - P.lo[2]) * P.fext[d][1] + (j - P.lo[1])) * P.fext[d][0] + (i - P.lo[0]);
 }
 
+// ---- the conservative update of one 3-D cell in two parts ----
+// hy_unsplitUpdate.F90:440-925, hy_energyFix.F90:188-251 and the EOS, as
+// update_cell does them, split where a plane-by-plane sweep knows a cell's
+// faces: the x and y terms once the cell's plane has its fluxes
+// (upd_start), the z term (which needs the face above), the energy fix and
+// the EOS one plane later (upd_finish).  Each quantity still takes its x, y
+// and z terms in that order, so the bits are update_cell's.
+struct UpdPart {
+  double U0[5];     // dens, momenta, total energy after the x and y terms
+  double IntEner;   // aux internal energy after the x and y terms
+  double S1, S2;    // pressure sources of x and y
+  double pres;      // the cell's face-averaged pressure
+};
+
+// one direction's terms (update_cell's apply)
+__device__ __forceinline__ void upd_dir(const DevParams& P, double dt, double del, double pres,
+                                        const double FL[8], const double FR[8], double U0[5],
+                                        double& IntEner, double& S) {
+  S = (FL[5] - FR[5]) * (1.0 - P.fP) / del;
+  if (P.aux) IntEner = IntEner + dt / del * (FL[6] - FR[6] + pres * (FL[7] - FR[7]));
+#pragma unroll
+  for (int q = 0; q < 5; ++q) U0[q] = U0[q] - dt / del * (FR[q] - FL[q]);
+}
+
+// x and y terms from the cell's primitive state and its four side faces
+__device__ __forceinline__ UpdPart upd_start(const DevParams& P, double dt, double dx, double dy,
+                                             double dens0, double vx, double vy, double vz,
+                                             double ener, double eint, double pres,
+                                             const double FLx[8], const double FRx[8],
+                                             const double FLy[8], const double FRy[8]) {
+  UpdPart u;
+  u.U0[0] = dens0;
+  u.U0[1] = vx * dens0;
+  u.U0[2] = vy * dens0;
+  u.U0[3] = vz * dens0;
+  u.U0[4] = dens0 * ener;
+  u.IntEner = 0.;
+  if (P.aux) u.IntEner = dens0 * eint;
+  u.pres = pres;
+  upd_dir(P, dt, dx, pres, FLx, FRx, u.U0, u.IntEner, u.S1);
+  upd_dir(P, dt, dy, pres, FLy, FRy, u.U0, u.IntEner, u.S2);
+  return u;
+}
+
+// z term, sources, energy fix and EOS; out = dens, velx, vely, velz, ener,
+// eint, pres, temp, gamc, game (eintOld: the cell's e_int before the step,
+// used when the aux equation is off)
+__device__ __forceinline__ void upd_finish(const DevParams& P, double dt, double dz, UpdPart u,
+                                           const double FLz[8], const double FRz[8],
+                                           double eintOld, double out[10]) {
+  double Sz;
+  upd_dir(P, dt, dz, u.pres, FLz, FRz, u.U0, u.IntEner, Sz);
+  if (P.aux) u.IntEner = u.IntEner + dt * 0.0;
+  const double Sphys[5] = {0., u.S1, u.S2, Sz, 0.};
+#pragma unroll
+  for (int q = 0; q < 5; ++q) u.U0[q] = u.U0[q] + dt * (0. + Sphys[q]);
+  const double densNew = fmax(u.U0[0], P.smlrho);
+  const double vx = u.U0[1] / densNew, vy = u.U0[2] / densNew, vz = u.U0[3] / densNew;
+  const double ener = u.U0[4] / densNew;
+  const double eintAux = P.aux ? u.IntEner / densNew : eintOld;
+  const double ekin = 0.5 * (vx * vx + vy * vy + vz * vz);
+  const double eint = ener - ekin - 0.0;
+  double newEint;
+  if (!P.aux || eint > P.eintSwitch * (ekin + 0.0)) newEint = fmax(P.smallE, eint);
+  else newEint = fmax(P.smallE, eintAux);
+  const double enerFix = newEint + ekin;
+  const EosOut eo = eos_dens_ei(P, densNew, vx, vy, vz, newEint, enerFix);
+  out[0] = densNew;
+  out[1] = vx;
+  out[2] = vy;
+  out[3] = vz;
+  out[4] = eo.ener;
+  out[5] = eo.eint;
+  out[6] = eo.pres;
+  out[7] = eo.temp;
+  out[8] = P.gamma;
+  out[9] = eo.game;
+}
+
+// U slots of upd_finish's outputs
+__device__ __forceinline__ int upd_var(int v) {
+  constexpr int slots[10] = {FX_DENS, FX_VELX, FX_VELY, FX_VELZ, FX_ENER, FX_EINT, FX_PRES,
+                             FX_TEMP, FX_GAMC, FX_GAME};
+  return slots[v];
+}
+
 }  // namespace fx

@@ -771,6 +771,7 @@ struct Lib {
   unsigned* strongCount = nullptr;
   double* fstates = nullptr;           // face states for the separate Riemann stage
   size_t fstatesCap = 0;
+  int solveUpdOff = 0;                 // 1: exact batch steps keep the separate update kernel
   const double* dtDev = nullptr;       // set while capturing steps whose dt lives on the device
   double* dtBuf = nullptr;             // [dt, t, next dt's CFL] of graph-captured runs
   double runCfl = 0.;                  // the CFL argument of the graph-captured run
@@ -1296,7 +1297,8 @@ int record(int n) {
 // the network over the listed strong faces first.  R is the type of the
 // stored face states and of the solve: float in the mixed-precision mode.
 template <typename R>
-void launch_face_solve(const DevParams& P, const FaceLaunch& F, long faces, bool multi) {
+void launch_face_solve(const DevParams& P, const FaceLaunch& F, long faces, bool multi,
+                       bool withUpdate, double dt) {
   const dim3 gf(grid_for(faces));
   if (multi && P.learned)
     hipLaunchKernelGGL((k_face_seed<true, R>), wave_grid(faces), dim3(kLearnThreads), 0, g.stream,
@@ -1304,6 +1306,15 @@ void launch_face_solve(const DevParams& P, const FaceLaunch& F, long faces, bool
   else if (P.learned)
     hipLaunchKernelGGL((k_face_seed<false, R>), wave_grid(faces), dim3(kLearnThreads), 0,
                        g.stream, P, F, g.seedPacked, g.seedY, g.strongList, g.strongCount, faces);
+  if (withUpdate) {  // the face stage and the update in one kernel (batch steps)
+    if (P.learned)
+      hipLaunchKernelGGL((k_solve_update<true, R>), dim3((unsigned)F.nTiles), dim3(kSolveThreads),
+                         0, g.stream, P, F, dt, g.dtDev);
+    else
+      hipLaunchKernelGGL((k_solve_update<false, R>), dim3((unsigned)F.nTiles), dim3(kSolveThreads),
+                         0, g.stream, P, F, dt, g.dtDev);
+    return;
+  }
   if (multi && P.learned)
     hipLaunchKernelGGL((k_face_solve<true, true, R>), gf, dim3(kThreads), 0, g.stream, P, F);
   else if (multi)
@@ -1317,7 +1328,7 @@ void launch_face_solve(const DevParams& P, const FaceLaunch& F, long faces, bool
 // Flux stages of a step on the uploaded tiles (march path): the column march,
 // then the exact solver's face stage (records events 1 and 2).  Launches
 // only: scratch and work items are in place.
-int launch_flux_stages(int nTiles, double dt, int TX, int TY) {
+int launch_flux_stages(int nTiles, double dt, int TX, int TY, bool solveUpdate = false) {
   const DevParams& P = g.cur;
   MarchLaunch M{g.dtiles, nTiles, g.multi ? 0 : P.nxb / TX,
                 g.multi ? 0 : (P.ndim > 1 ? P.nyb : 1) / TY, g.status, dt, nullptr, g.dtDev,
@@ -1350,8 +1361,8 @@ int launch_flux_stages(int nTiles, double dt, int TX, int TY) {
     g.lastTiles = nTiles;
     g.lastMulti = g.multi;
     g.lastPrec = P.precision;
-    if (P.precision == FX_PREC_MIXED) launch_face_solve<float>(P, F, faces, g.multi);
-    else launch_face_solve<double>(P, F, faces, g.multi);
+    if (P.precision == FX_PREC_MIXED) launch_face_solve<float>(P, F, faces, g.multi, solveUpdate, dt);
+    else launch_face_solve<double>(P, F, faces, g.multi, solveUpdate, dt);
     if (record(2)) return 4;
   } else if (P.riemann == FX_HLLC) {
     if (g.multi) launch_march_multi<FX_HLLC>(P.ndim, P.precision, TX, M);
@@ -1437,12 +1448,19 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
     return 3;
   }
   if (march) {
+    // batch steps with the exact solver: its face stage and the update run
+    // as one kernel (k_solve_update) for 3-D blocks up to 16 x 16 across
+    const bool solveUpdate = tilesUploaded && !tiles && !inPlace && !g.multi && !g.solveUpdOff &&
+                             P.riemann == FX_EXACT && P.ndim == 3 &&
+                             P.nxb * P.nyb <= kSolveMaxCols &&
+                             2 * (P.nxb + 1) * P.nyb + P.nxb * P.nyb <= kSolveMaxFaces &&
+                             P.nxb + 1 <= 17 && P.nyb + 1 <= 17;
     if (g.multi && build_march_items(nTiles, TX, st)) return 4;
     if (ensure_step_scratch(nTiles, st)) return 4;
     if (!g.capturing) hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
     if (record(0)) return 4;
-    if (launch_flux_stages(nTiles, dt, TX, TY)) return 4;
-    launch_update(nTiles, dt, dIn);
+    if (launch_flux_stages(nTiles, dt, TX, TY, solveUpdate)) return 4;
+    if (!solveUpdate) launch_update(nTiles, dt, dIn);
     if (record(3)) return 4;
   } else {
   if (grow(&g.states, &g.statesCap, (size_t)P.scellsPerTile * kStateComps * nTiles, st)) return 4;
@@ -2301,6 +2319,7 @@ void fx_hydro_set_stream(void* stream) { g.stream = (hipStream_t)stream; }
 void fx_hydro_set_timing(int on) { g.timing = on; }
 void fx_hydro_set_kernel_path(int path) {
   g.legacy = (path == 1);
+  g.solveUpdOff = (path == 2);
   g.batchValid = false;
 }
 int fx_hydro_stage_times(float* ms, int n) {

@@ -317,6 +317,114 @@ k_face_solve(DevParams P0, FaceLaunch F) {
   flush_exact(F.status, es);
 }
 
+// The exact solver's face stage and the conservative update in one kernel,
+// for batches of 3-D blocks (no caller tile list, every block in place):
+// one workgroup per block sweeps its planes upward.  Per plane k it solves
+// the x and y faces of plane k and the z face below it (face k), keeping
+// the fluxes in LDS; then each column finishes its cell of plane k-1 (the z
+// term with faces k-1 and k, the energy fix, the EOS) and starts its cell of
+// plane k (the x and y terms), as upd_start / upd_finish, so the bits are
+// those of k_face_solve followed by k_update.  Every face reads a cell's
+// gamc and game before that cell is updated: the faces of plane k are
+// solved before plane k-1 is written.  Blocks of at most 16 x 16 cells
+// across (the LDS holds one plane of fluxes).
+constexpr int kSolveThreads = 256;
+constexpr int kSolveMaxCols = kSolveThreads;   // nxb * nyb: one column per thread
+constexpr int kSolveMaxFaces = 17 * 16 * 2 + 16 * 16;
+
+template <bool SEEDED, typename R>
+__global__ void __launch_bounds__(kSolveThreads) __attribute__((amdgpu_waves_per_eu(2)))
+k_solve_update(DevParams P, FaceLaunch F, double dt0, const double* dtp) {
+  // this plane's face fluxes (x, y, then z), in the solver's type R, which
+  // holds them exactly (the solver forms them in R)
+  __shared__ R flx[8 * kSolveMaxFaces];
+  const int t = blockIdx.x;
+  const int c = threadIdx.x;                     // this thread's column
+  const double dt = step_dt(dt0, dtp);
+  const fx_tile& T = F.tiles[t];
+  gdouble* const U = as_global(T.U);
+  const int nx = P.nxb, ny = P.nyb, ncol = nx * ny;
+  const int nfx = (nx + 1) * ny, nfy = nx * (ny + 1);   // faces of one plane
+  const long perTile = P.nface[0] + P.nface[1] + P.nface[2];
+  const double dx = T.del[0], dy = T.del[1], dz = T.del[2];
+  const int a = c % nx, bb = c / nx;
+  UpdPart part{};                                // the column's cell in flight
+  double zprev[8];                               // the z flux below it
+  ExactStats es{0ULL, 0ULL};
+  for (int k = P.lo[2]; k <= P.hi[2] + 1; ++k) {
+    const bool inPlane = k <= P.hi[2];
+    const int zBase = inPlane ? nfx + nfy : 0;
+    const int nItems = zBase + ncol;
+    for (int w = c; w < nItems; w += kSolveThreads) {
+      int d, a0, a1;
+      if (w < zBase && w < nfx) {
+        d = 0; a0 = w % (nx + 1); a1 = w / (nx + 1);
+      } else if (w < zBase) {
+        d = 1; const int r = w - nfx; a0 = r % nx; a1 = r / nx;
+      } else {
+        d = 2; const int r = w - zBase; a0 = r % nx; a1 = r / nx;
+      }
+      const int i = a0 + P.lo[0], j = a1 + P.lo[1];
+      const long r = face_index(P, d, i, j, k);
+      const long cR = cidx(P, i, j, k), cL = cR - axis_stride(P, d);
+      const R* b = face_states_at<R, false>(P, F, t, d);
+      const long nf = P.nface[d];
+      const R* Lb = b + r;
+      const R* Rb = b + 6 * nf + r;
+      double guess = -1.0;
+      if (SEEDED) {
+        long gid = (long)t * perTile + r;
+        for (int e = 0; e < d; ++e) gid += P.nface[e];
+        const float y = F.y[gid];
+        if (y != kNoSeed) guess = seed_from_y(y, Lb[4 * nf], Rb[4 * nf]);
+      }
+      double Fo[8];
+      const int ierr = exact_face_flux<SEEDED, R>(P, d, Lb, Rb, nf, U[P.voff[FX_GAMC] + cL],
+                                                  U[P.voff[FX_GAME] + cL], U[P.voff[FX_GAME] + cR],
+                                                  guess, Fo, es);
+      if (ierr) report_error(F.status, 2, t, i, j, k, d);
+      const int slot = d == 2 ? nfx + nfy + (w - zBase) : w;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) flx[q * kSolveMaxFaces + slot] = ierr ? R(0.) : R(Fo[q]);
+    }
+    __syncthreads();
+    if (c < ncol) {
+      const int i = a + P.lo[0], j = bb + P.lo[1];
+      double FR[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) FR[q] = double(flx[q * kSolveMaxFaces + nfx + nfy + c]);
+      if (k - 1 >= P.lo[2]) {
+        double out[10];
+        const long cb = cidx(P, i, j, k - 1);
+        upd_finish(P, dt, dz, part, zprev, FR, P.aux ? 0. : U[P.voff[FX_EINT] + cb], out);
+#pragma unroll
+        for (int v = 0; v < 10; ++v) U[P.voff[upd_var(v)] + cb] = out[v];
+      }
+      if (inPlane) {
+        const long cb = cidx(P, i, j, k);
+        double FLx[8], FRx[8], FLy[8], FRy[8];
+        const int fx0 = bb * (nx + 1) + a, fy0 = nfx + bb * nx + a;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          FLx[q] = double(flx[q * kSolveMaxFaces + fx0]);
+          FRx[q] = double(flx[q * kSolveMaxFaces + fx0 + 1]);
+          FLy[q] = double(flx[q * kSolveMaxFaces + fy0]);
+          FRy[q] = double(flx[q * kSolveMaxFaces + fy0 + nx]);
+        }
+        const long rp = ((long)(k - P.lo[2]) * ny + bb) * nx + a;
+        part = upd_start(P, dt, dx, dy, U[P.voff[FX_DENS] + cb], U[P.voff[FX_VELX] + cb],
+                         U[P.voff[FX_VELY] + cb], U[P.voff[FX_VELZ] + cb],
+                         U[P.voff[FX_ENER] + cb], U[P.voff[FX_EINT] + cb],
+                         P.aux ? as_global(T.pavg)[rp] : 0., FLx, FRx, FLy, FRy);
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) zprev[q] = FR[q];
+    }
+    __syncthreads();
+  }
+  flush_exact(F.status, es);
+}
+
 // Network seed ln(p*/P0) of the faces the march listed as strong (at or
 // above the gate); every other face already holds "no seed".  The list
 // length is read on the device, so the launch needs no host sync.

@@ -38,10 +38,8 @@ template <int TX, int TY> struct TileIdx {
   static __device__ __forceinline__ int lx(int a, int b) { return b * (TX + 4) + (a + 2); }
   // slopes along y: a in [0,TX), b in [-2,TY+1]
   static __device__ __forceinline__ int ly(int a, int b) { return (b + 2) * TX + a; }
-  // eigen scalars of the current plane: a in [-2,TX+1], b in [-2,TY+1]
-  static __device__ __forceinline__ int ec(int a, int b) { return (b + 2) * (TX + 4) + (a + 2); }
   static constexpr int NRX = TX * (TY + 2), NRY = (TX + 2) * TY, NC = (TX + 2) * (TY + 2);
-  static constexpr int NLX = (TX + 4) * TY, NLY = TX * (TY + 4), NEC = (TX + 4) * (TY + 4);
+  static constexpr int NLX = (TX + 4) * TY, NLY = TX * (TY + 4);
 };
 
 // LDS of one column tile (march_tile), as offsets into one array of R
@@ -59,8 +57,7 @@ template <int NDIM, int TX, int TY> struct MarchLds {
   static constexpr int WY = WX + 12 * I::NRY;                 // y-face states
   static constexpr int WZ = WY + 12 * I::NRX;                 // z-face states of plane k
   static constexpr int WZP = WZ + (Z ? 12 * NCOL : 1);        // z-face Wp of plane k-1
-  static constexpr int EC = WZP + (Z ? 6 * NCOL : 1);         // eigen scalars of plane k
-  static constexpr int total = EC + 5 * I::NEC;
+  static constexpr int total = WZP + (Z ? 6 * NCOL : 1);
 };
 
 // components stored for sigma and slopes: dens, vx, vy, vz, pres, eint
@@ -255,31 +252,10 @@ __device__ __forceinline__ R flat_tilde_vals(R pm2, R pm1, R pp1, R pp2, R vm1, 
   return ft;
 }
 
-// ---- eigen scalars of the current plane, computed once per cell ----
-// The direction-independent scalars of eig_make (c, 1/(2c), 1/(2 rho c^2),
-// 1/c^2, rho c^2) of one cell, kept in LDS for every item of the plane that
-// needs the cell's eigen system in any direction; c < 0 marks a cell whose
-// c^2 is not positive.  The values are eig_make's own, so the bits are.
-template <typename R>
-__device__ __forceinline__ void ec_put(R* E, int n, int idx, const EigT<R>& e, bool ok) {
-  E[idx] = ok ? e.cf : R(-1.);
-  E[n + idx] = e.ahinv;
-  E[2 * n + idx] = e.lfp;
-  E[3 * n + idx] = e.a2inv;
-  E[4 * n + idx] = e.rp;
-}
-template <int DIR, typename R>
-__device__ __forceinline__ bool ec_get(const R* E, int n, int idx, const R* V, EigT<R>& e) {
-  e.cf = E[idx];
-  if (!(e.cf > R(0.))) return false;
-  e.ahinv = E[n + idx];
-  e.lfp = E[2 * n + idx];
-  e.a2inv = E[3 * n + idx];
-  e.rp = E[4 * n + idx];
-  e.rho = V[P_DENS];
-  e.un = V[P_VELX + DIR];
-  return true;
-}
+// Item counts rounded up to whole wavefronts: each stage lays its item kinds
+// out in wave-aligned segments, so that no wavefront runs two kinds of item
+// (two code paths) one after the other.
+constexpr int wave_up(int n) { return (n + 63) / 64 * 64; }
 
 // ---- a tile's cell data as the march reads it ----
 // One uniform base address per primitive variable and 32-bit byte offsets of
@@ -355,32 +331,6 @@ __device__ __forceinline__ bool tv_slope(const DevParams& P, const TileView& tv,
   return true;
 }
 
-// the same with the cell's eigen scalars from the plane's cache (E, entry idx)
-template <int SD, typename R>
-__device__ __forceinline__ bool tv_sigma_c(const DevParams& P, const TileView& tv, unsigned cb,
-                                           unsigned sb, const R* E, int n, int idx, R sig[8]) {
-  R Vm[8], Vc[8], Vp[8];
-  tv_prim(tv, cb - sb, Vm);
-  tv_prim(tv, cb, Vc);
-  tv_prim(tv, cb + sb, Vp);
-  EigT<R> e;
-  if (!ec_get<SD>(E, n, idx, Vc, e)) return false;
-  trans_sigma<SD>(P, Vm, Vc, Vp, e, sig);
-  return true;
-}
-template <int DIR, typename R>
-__device__ __forceinline__ bool tv_slope_c(const DevParams& P, const TileView& tv, unsigned cb,
-                                           unsigned sb, const R* E, int n, int idx, R db[8]) {
-  R Vm[8], Vc[8], Vp[8];
-  tv_prim(tv, cb - sb, Vm);
-  tv_prim(tv, cb, Vc);
-  tv_prim(tv, cb + sb, Vp);
-  EigT<R> e;
-  if (!ec_get<DIR>(E, n, idx, Vc, e)) return false;
-  tvd_slope<DIR>(P, Vm, Vc, Vp, e, db);
-  return true;
-}
-
 // FlatTilde and FlatCoeff along DIR (hy_getRiemannState.F90:365-453)
 template <int DIR, typename R>
 __device__ __forceinline__ R tv_flat_tilde(const TileView& tv, unsigned cb, unsigned sb) {
@@ -452,7 +402,6 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
   R* const WY = smem + M::WY;
   R* const WZ = smem + M::WZ;
   R* const WZP = smem + M::WZP;
-  R* const EC = smem + M::EC;
 
   const int tid = threadIdx.x;
   // A tile owns its high x (y) face row only at the block's high side;
@@ -471,19 +420,18 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
 
   auto ring = [](int k) { return NR == 3 ? (k % 3) : 0; };
 
-  // stage-A item: sigma_x / sigma_y of plane kk (w < NRX + NRY)
-  auto sig_xy_item = [&](int w, int kk) {
+  // stage-A items: sigma_x (w < NRX) and sigma_y (w < NRY) of plane kk
+  auto sig_x_item = [&](int w, int kk) {
     R sg[8];
-    if (w < I::NRX) {
-      const int a = w % TX, b = w / TX - 1;
-      if (!tv_sigma<0>(P, tv, cell_b(P, ox + a, oy + b, kk), sxb, sg)) { bad = true; return; }
-      lds_put6(SX + ring(kk) * 6 * I::NRX, I::NRX, w, sg);
-    } else {
-      const int w2 = w - I::NRX;
-      const int a = w2 % (TX + 2) - 1, b = w2 / (TX + 2);
-      if (!tv_sigma<1>(P, tv, cell_b(P, ox + a, oy + b, kk), syb, sg)) { bad = true; return; }
-      lds_put6(SY + ring(kk) * 6 * I::NRY, I::NRY, w2, sg);
-    }
+    const int a = w % TX, b = w / TX - 1;
+    if (!tv_sigma<0>(P, tv, cell_b(P, ox + a, oy + b, kk), sxb, sg)) { bad = true; return; }
+    lds_put6(SX + ring(kk) * 6 * I::NRX, I::NRX, w, sg);
+  };
+  auto sig_y_item = [&](int w, int kk) {
+    R sg[8];
+    const int a = w % (TX + 2) - 1, b = w / (TX + 2);
+    if (!tv_sigma<1>(P, tv, cell_b(P, ox + a, oy + b, kk), syb, sg)) { bad = true; return; }
+    lds_put6(SY + ring(kk) * 6 * I::NRY, I::NRY, w, sg);
   };
   // stage-A item: z-slope and z flattening tilde of column w at plane m
   auto zslope_item = [&](int w, int m) {
@@ -499,10 +447,12 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
   const int kEnd = Z ? khi + 1 : 0;
   if (Z) {
     // prologue: sigma_x/y of plane kBeg, z slopes of kBeg-1 and kBeg
-    for (int w = tid; w < I::NRX + I::NRY + 2 * NCOL; w += NT) {
-      if (w < I::NRX + I::NRY) sig_xy_item(w, kBeg);
-      else if (w < I::NRX + I::NRY + NCOL) zslope_item(w - I::NRX - I::NRY, kBeg - 1);
-      else zslope_item(w - I::NRX - I::NRY - NCOL, kBeg);
+    constexpr int Q1 = wave_up(I::NRX), Q2 = Q1 + wave_up(I::NRY), Q3 = Q2 + wave_up(NCOL);
+    for (int w = tid; w < Q3 + NCOL; w += NT) {
+      if (w < Q1) { if (w < I::NRX) sig_x_item(w, kBeg); }
+      else if (w < Q2) { if (w - Q1 < I::NRY) sig_y_item(w - Q1, kBeg); }
+      else if (w < Q3) { if (w - Q2 < NCOL) zslope_item(w - Q2, kBeg - 1); }
+      else zslope_item(w - Q3, kBeg);
     }
   }
 
@@ -512,25 +462,6 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
 
   for (int k = kBeg; k <= kEnd; ++k) {
     const bool interior = !Z || (k >= klo && k <= khi);
-    // ----------------- stage A0: eigen scalars of plane k ----------------
-    // every cell of plane k whose eigen system an item of this plane needs:
-    // a in [-1,TX] x b in [-1,TY], plus the x-slope ends (a = -2, TX+1,
-    // b in [0,TY)) and the y-slope ends (b = -2, TY+1, a in [0,TX))
-    for (int w = tid; w < I::NEC; w += NT) {
-      const int a = w % (TX + 4) - 2, b = w / (TX + 4) - 2;
-      const bool xend = a < -1 || a > TX, yend = b < -1 || b > TY;
-      if ((xend && (yend || b < 0 || b >= TY)) || (yend && (a < 0 || a >= TX))) continue;
-      const unsigned cb = cell_b(P, ox + a, oy + b, k);
-      R V[8];
-      V[P_DENS] = R(ld_at(tv.v[P_DENS], cb));
-      V[P_PRES] = R(ld_at(tv.v[P_PRES], cb));
-      V[P_GAMC] = R(ld_at(tv.v[P_GAMC], cb));
-      V[P_VELX] = R(0.);
-      EigT<R> e;
-      const bool ok = eig_make<0>(V, e);
-      ec_put(EC, I::NEC, w, e, ok);
-    }
-    __syncthreads();
     // ------------------------------ stage A ------------------------------
     {
       const int nSig = (!Z || k + 1 <= kEnd) ? I::NRX + I::NRY : 0;
@@ -540,12 +471,12 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
       const int nA = nSig + nSz + nL + nZs;
       for (int w = tid; w < nA; w += NT) {
         int r = w;
-        if (r < nSig) { sig_xy_item(r, Z ? k + 1 : 0); continue; }
+        if (r < nSig) { if (r < I::NRX) sig_x_item(r, Z ? k + 1 : 0); else sig_y_item(r - I::NRX, Z ? k + 1 : 0); continue; }
         r -= nSig;
         if (r < nSz) {
           R v8[8];
           const int a = r % (TX + 2) - 1, b = r / (TX + 2) - 1;
-          if (!tv_sigma_c<2>(P, tv, cell_b(P, ox + a, oy + b, k), szb, EC, I::NEC, I::ec(a, b), v8)) { bad = true; continue; }
+          if (!tv_sigma<2>(P, tv, cell_b(P, ox + a, oy + b, k), szb, v8)) { bad = true; continue; }
           lds_put6(SZ, I::NC, r, v8);
           continue;
         }
@@ -554,12 +485,12 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           R v8[8];
           if (r < I::NLX) {
             const int a = r % (TX + 4) - 2, b = r / (TX + 4);
-            if (!tv_slope_c<0>(P, tv, cell_b(P, ox + a, oy + b, k), sxb, EC, I::NEC, I::ec(a, b), v8)) { bad = true; continue; }
+            if (!tv_slope<0>(P, tv, cell_b(P, ox + a, oy + b, k), sxb, v8)) { bad = true; continue; }
             lds_put6(LX, I::NLX, r, v8);
           } else {
             r -= I::NLX;
             const int a = r % TX, b = r / TX - 2;
-            if (!tv_slope_c<1>(P, tv, cell_b(P, ox + a, oy + b, k), syb, EC, I::NEC, I::ec(a, b), v8)) { bad = true; continue; }
+            if (!tv_slope<1>(P, tv, cell_b(P, ox + a, oy + b, k), syb, v8)) { bad = true; continue; }
             lds_put6(LY, I::NLY, r, v8);
           }
           continue;
@@ -574,13 +505,13 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
     const R* syk = SY + ring(k) * 6 * I::NRY;
     // ------------------------------ stage B1 -----------------------------
     {
-      const int nX = interior ? NBX : 0;
-      const int nY = interior ? NBY : 0;
-      const int nZ = Z ? NCOL : 0;
-      for (int w = tid; w < nX + nY + nZ; w += NT) {
+      // x-, y- and z-face states in wave-aligned segments
+      constexpr int B1 = wave_up(NBX), B2 = B1 + wave_up(NBY), B3 = B2 + (Z ? NCOL : 0);
+      for (int w = tid; w < B3; w += NT) {
         R Wp[8], Wn[8], Vm[8], Vc[8], Vp[8], dbN[8], db0[8], dbP[8];
         EigT<R> e0;
-        if (w < nX) {
+        if (w < B1) {
+          if (!interior || w >= NBX) continue;
           const int a = w % (TX + 2) - 1, b = w / (TX + 2);
           if (a == TX && !ex) continue;
           const unsigned cb = cell_b(P, ox + a, oy + b, k);
@@ -591,7 +522,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           lds_get6(LX, I::NLX, I::lx(a - 1, b), dbN);
           lds_get6(LX, I::NLX, I::lx(a, b), db0);
           lds_get6(LX, I::NLX, I::lx(a + 1, b), dbP);
-          if (!ec_get<0>(EC, I::NEC, I::ec(a, b), Vc, e0)) { bad = true; continue; }
+          if (!eig_make<0>(Vc, e0)) { bad = true; continue; }
           const R dmm = P.steep ? R(ld_at(tv.v[P_DENS], cb - 2 * sxb)) : R(0.);
           const R dpp = P.steep ? R(ld_at(tv.v[P_DENS], cb + 2 * sxb)) : R(0.);
           ppm_trace<0>(P, R(dt / dx), R(dx), Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
@@ -604,8 +535,9 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           }
           if (negative(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st, a >= 0 || ox == P.lo[0]);
           put_state(WX, I::NRY, I::ry(a, b), Wp, Wn);
-        } else if (w < nX + nY) {
-          const int w2 = w - nX;
+        } else if (w < B2) {
+          const int w2 = w - B1;
+          if (!interior || w2 >= NBY) continue;
           const int a = w2 % TX, b = w2 / TX - 1;
           if (b == TY && !ey) continue;
           const unsigned cb = cell_b(P, ox + a, oy + b, k);
@@ -616,7 +548,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           lds_get6(LY, I::NLY, I::ly(a, b - 1), dbN);
           lds_get6(LY, I::NLY, I::ly(a, b), db0);
           lds_get6(LY, I::NLY, I::ly(a, b + 1), dbP);
-          if (!ec_get<1>(EC, I::NEC, I::ec(a, b), Vc, e0)) { bad = true; continue; }
+          if (!eig_make<1>(Vc, e0)) { bad = true; continue; }
           const R dmm = P.steep ? R(ld_at(tv.v[P_DENS], cb - 2 * syb)) : R(0.);
           const R dpp = P.steep ? R(ld_at(tv.v[P_DENS], cb + 2 * syb)) : R(0.);
           ppm_trace<1>(P, R(dt / dy), R(dy), Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
@@ -630,7 +562,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           if (negative(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st, b >= 0 || oy == P.lo[1]);
           put_state(WY, I::NRX, I::rx(a, b), Wp, Wn);
         } else {
-          const int w2 = w - nX - nY;
+          const int w2 = w - B2;
           const int a = w2 % TX, b = w2 / TX;
           const unsigned cb = cell_b(P, ox + a, oy + b, k);
           tv_prim(tv, cb - szb, Vm);
@@ -646,7 +578,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           lds_get6(LZ + ring(k - 1) * 6 * NCOL, NCOL, w2, dbN);
           lds_get6(LZ + ring(k) * 6 * NCOL, NCOL, w2, db0);
           lds_get6(LZ + ring(k + 1) * 6 * NCOL, NCOL, w2, dbP);
-          if (!ec_get<2>(EC, I::NEC, I::ec(a, b), Vc, e0)) { bad = true; continue; }
+          if (!eig_make<2>(Vc, e0)) { bad = true; continue; }
           const R dmm = P.steep ? R(ld_at(tv.v[P_DENS], cb - 2 * szb)) : R(0.);
           const R dpp = P.steep ? R(ld_at(tv.v[P_DENS], cb + 2 * szb)) : R(0.);
           ppm_trace<2>(P, R(dt / dz), R(dz), Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
@@ -664,23 +596,23 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
 
     // ------------------------------ stage B2 -----------------------------
     if (Z && P.ctu3d) {
-      const int nX = interior ? NBX : 0;
-      const int nY = interior ? NBY : 0;
-      for (int w = tid; w < nX + nY + NCOL; w += NT) {
+      constexpr int C1 = wave_up(NBX), C2 = C1 + wave_up(NBY), C3 = C2 + NCOL;
+      for (int w = tid; w < C3; w += NT) {
         R Wp[8], Wn[8], Vc[8], sm[8], sc[8], sp[8], tA[8], tB[8];
         EigT<R> e1, e2;
         R fac;
         R* base;
         int n, idx;
         bool own = true;  // the tile owns this state cell (counts its reset)
-        if (w < nX) {
+        if (w < C1) {
           // X: sigma_z over j with the y eigen system, sigma_y over k with z's
+          if (!interior || w >= NBX) continue;
           const int a = w % (TX + 2) - 1, b = w / (TX + 2);
           if (a == TX && !ex) continue;
           own = a >= 0 || ox == P.lo[0];
           tv_prim(tv, cell_b(P, ox + a, oy + b, k), Vc);
-          if (!(ec_get<1>(EC, I::NEC, I::ec(a, b), Vc, e1) &&
-                ec_get<2>(EC, I::NEC, I::ec(a, b), Vc, e2))) { bad = true; continue; }
+          if (!(eig_make<1>(Vc, e1) &&
+                eig_make<2>(Vc, e2))) { bad = true; continue; }
           lds_get6(SZ, I::NC, I::cc(a, b - 1), sm);
           lds_get6(SZ, I::NC, I::cc(a, b), sc);
           lds_get6(SZ, I::NC, I::cc(a, b + 1), sp);
@@ -691,15 +623,16 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           trans_sigma<2>(P, sm, sc, sp, e2, tB);
           fac = R(dt * dt / (6. * dy * dz));
           base = WX; n = I::NRY; idx = I::ry(a, b);
-        } else if (w < nX + nY) {
+        } else if (w < C2) {
           // Y: sigma_x over k with z's eigen system, sigma_z over i with x's
-          const int w2 = w - nX;
+          const int w2 = w - C1;
+          if (!interior || w2 >= NBY) continue;
           const int a = w2 % TX, b = w2 / TX - 1;
           if (b == TY && !ey) continue;
           own = b >= 0 || oy == P.lo[1];
           tv_prim(tv, cell_b(P, ox + a, oy + b, k), Vc);
-          if (!(ec_get<2>(EC, I::NEC, I::ec(a, b), Vc, e1) &&
-                ec_get<0>(EC, I::NEC, I::ec(a, b), Vc, e2))) { bad = true; continue; }
+          if (!(eig_make<2>(Vc, e1) &&
+                eig_make<0>(Vc, e2))) { bad = true; continue; }
           lds_get6(SX + ring(k - 1) * 6 * I::NRX, I::NRX, I::rx(a, b), sm);
           lds_get6(sxk, I::NRX, I::rx(a, b), sc);
           lds_get6(SX + ring(k + 1) * 6 * I::NRX, I::NRX, I::rx(a, b), sp);
@@ -712,11 +645,11 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           base = WY; n = I::NRX; idx = I::rx(a, b);
         } else {
           // Z: sigma_y over i with x's eigen system, sigma_x over j with y's
-          const int w2 = w - nX - nY;
+          const int w2 = w - C2;
           const int a = w2 % TX, b = w2 / TX;
           tv_prim(tv, cell_b(P, ox + a, oy + b, k), Vc);
-          if (!(ec_get<0>(EC, I::NEC, I::ec(a, b), Vc, e1) &&
-                ec_get<1>(EC, I::NEC, I::ec(a, b), Vc, e2))) { bad = true; continue; }
+          if (!(eig_make<0>(Vc, e1) &&
+                eig_make<1>(Vc, e2))) { bad = true; continue; }
           lds_get6(syk, I::NRY, I::ry(a - 1, b), sm);
           lds_get6(syk, I::NRY, I::ry(a, b), sc);
           lds_get6(syk, I::NRY, I::ry(a + 1, b), sp);
@@ -738,13 +671,14 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
 
     // ------------------------------ stage C ------------------------------
     {
-      const int nFX = interior ? NFX : 0;
-      const int nFY = interior ? NFY : 0;
-      const int nC = Z ? NCOL : (interior ? NCOL : 0);
-      for (int w = tid; w < nFX + nFY + nC; w += NT) {
+      // x faces, y faces, column items (z face, pavg) in wave-aligned segments
+      constexpr int D1 = wave_up(NFX), D2 = D1 + wave_up(NFY), D3 = D2 + NCOL;
+      const bool cols = Z || interior;
+      for (int w = tid; w < D3; w += NT) {
         R VL[8], VR[8];
         double F[8];
-        if (w < nFX) {
+        if (w < D1) {
+          if (!interior || w >= NFX) continue;
           const int a = w % (TX + 1), b = w / (TX + 1);
           if (a == TX && !ex) continue;
           lds_get6(WX, I::NRY, I::ry(a - 1, b), VL);
@@ -764,8 +698,9 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
             if (face_flux<0, RS>(P, VL, VR, tv_use_hll<RS>(P, tv, cb - sxb, cb), F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 0); continue; }
             write_flux_b<0>(P, as_global(T.flux[0]), fr * 8u, F);
           }
-        } else if (w < nFX + nFY) {
-          const int w2 = w - nFX;
+        } else if (w < D2) {
+          const int w2 = w - D1;
+          if (!interior || w2 >= NFY) continue;
           const int a = w2 % TX, b = w2 / TX;
           if (b == TY && !ey) continue;
           lds_get6(WY, I::NRX, I::rx(a, b - 1), VL);
@@ -787,7 +722,8 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           }
         } else {
           // column item: z-face flux at face k, pavg of the cell, carry Wp
-          const int w2 = w - nFX - nFY;
+          if (!cols) continue;
+          const int w2 = w - D2;
           const int a = w2 % TX, b = w2 / TX;
           const unsigned cb = cell_b(P, ox + a, oy + b, k);
           if (Z && k >= klo) {

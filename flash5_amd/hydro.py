@@ -412,7 +412,9 @@ class Hydro:
         return t.value
 
     def set_kernel_path(self, path: int) -> None:
-        """0 = fused column-march kernel (default), 1 = three-kernel path."""
+        """0 = fused column-march kernel (default; exact-solver batch steps of
+        3-D blocks up to 16 x 16 across solve and update in one kernel),
+        1 = three-kernel path, 2 = as 0 with the update always separate."""
         self.lib.fx_hydro_set_kernel_path(int(path))
 
     def set_timing(self, on: bool) -> None:

@@ -300,7 +300,10 @@ void fx_hydro_set_async(int async_);
 void fx_hydro_set_timing(int on);
 int fx_hydro_stage_times(float* ms, int n);
 /* Kernel path: 0 = fused column-march kernel where the block shape allows
- * (default), 1 = three-kernel path (states, fluxes, update). */
+ * (default; batch steps of 3-D blocks up to 16 x 16 across with the exact
+ * solver also run its face solve and the conservative update as one
+ * kernel), 1 = three-kernel path (states, fluxes, update), 2 = as 0 but the
+ * exact solver's batch steps keep the update as a kernel of its own. */
 void fx_hydro_set_kernel_path(int path);
 
 /* Block classes: tiles of several interior shapes in one call, e.g. levels

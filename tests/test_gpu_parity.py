@@ -149,8 +149,8 @@ def test_fluxes_then_update_solution_matches_oracle():
     _check(hp, Uo, Ug, bitwise=True)
 
 
-@pytest.mark.parametrize("path", [0, 1], ids=["fused", "three-kernel"])
-@pytest.mark.parametrize("ndim,n", [(3, 8), (2, 16)])
+@pytest.mark.parametrize("path", [0, 1, 2], ids=["fused", "three-kernel", "separate-update"])
+@pytest.mark.parametrize("ndim,n", [(3, 8), (3, 16), (2, 16)])
 def test_fp64_exact_riemann_solver_matches_oracle(ndim, n, path):
     """Exact solver in place of HLLC.  The device takes powers as
     exp2(e log2 x) and the oracle calls libm pow; the two round differently
@@ -167,6 +167,40 @@ def test_fp64_exact_riemann_solver_matches_oracle(ndim, n, path):
     g = stats[-1]
     assert g.newton_iters > 0 and g.newton_fallbacks == 0
     assert abs(g.newton_iters - o_iters) <= 1e-3 * o_iters, (g.newton_iters, o_iters)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("shape", [(16, 16, 16), (8, 16, 8), (16, 8, 32)])
+def test_exact_solve_with_update_matches_separate_update(prec, shape):
+    """Exact-solver batch steps of 3-D blocks up to 16 x 16 across run the
+    face solve and the update as one kernel (path 0); path 2 keeps the face
+    solve and the update apart.  Same fluxes, same update order: bitwise."""
+    from flash5_amd.hydro import PREC_MIXED
+    nx, ny, nz = shape
+    hp = HydroParams(ndim=3, nxb=nx, nyb=ny, nzb=nz, gamma=1.4, RiemannSolver="exact",
+                     precision=PREC_MIXED if prec == "mixed" else 0)
+    # non-cubic blocks: a corner of cubic ones, ghost cells included
+    g = hp.ng
+    U = W.fourier_blocks(5, n=max(shape), gamma=hp.gamma, seed=W.SEED_BASE + 17).numpy()
+    U = np.ascontiguousarray(U[:, :, :nz + 2 * g, :ny + 2 * g, :nx + 2 * g])
+    dx = 1.0 / 16
+    dt = W.cfl_dt(torch.from_numpy(U), 3, 4, (dx,) * 3, 0.8)
+    out, iters = [], []
+    hy = Hydro(hp)
+    try:
+        for path in (0, 2):
+            hy.set_kernel_path(path)
+            Ug = torch.from_numpy(U.copy()).cuda()
+            st = hy.advance_batch(Ug, dt, (dx,) * 3)
+            assert st.code == 0
+            out.append(Ug.cpu().numpy())
+            iters.append(st.newton_iters)
+    finally:
+        hy.set_kernel_path(0)
+    assert iters[0] == iters[1] and iters[0] > 0
+    diff = np.argwhere(out[0] != out[1])
+    assert diff.size == 0, f"{len(diff)} values differ, first {diff[:4]}"
+    assert not np.array_equal(out[0], U)
 
 
 @pytest.mark.parametrize("riemann", ["hllc", "exact"])
