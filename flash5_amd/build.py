@@ -35,7 +35,7 @@ HIP_FLAGS = [
 # The column-march kernels of each (solver family, real type) pair form a
 # translation unit of their own (fx_march_inst.hip), so the units build in
 # parallel; the main unit holds every other kernel and the C ABI.
-MARCH_UNITS = [(rs, r) for rs in ("FX_HLLC", "FX_HYBR", "RS_STATES", "RS_STATES_GATED")
+MARCH_UNITS = [(rs, r) for rs in ("FX_HLLC", "FX_HYBR", "RS_STATES")
                for r in ("double", "float")]
 OBJ_DIR = os.path.join(HERE, "_build")
 

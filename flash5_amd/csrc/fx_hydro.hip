@@ -766,12 +766,16 @@ struct Lib {
   double seedGate = 0.2;               // fx_mlp_set_gate
   float* seedY = nullptr;              // network outputs per interface / face
   size_t seedYCap = 0;
-  unsigned* strongList = nullptr;      // faces at or above the gate, listed by the march
+  unsigned* strongList = nullptr;      // faces at or above the gate (k_face_gate)
   size_t strongListCap = 0;
   unsigned* strongCount = nullptr;
   double* fstates = nullptr;           // face states for the separate Riemann stage
   size_t fstatesCap = 0;
   int solveUpdOff = 0;                 // 1: exact batch steps keep the separate update kernel
+  double* seedFlux = nullptr;          // fluxes of the seeded strong faces (batch steps), in R
+  size_t seedFluxCap = 0;
+  unsigned* seedBits = nullptr;        // one bit per face: flux in seedFlux (kept all zero between steps)
+  size_t seedBitsCap = 0;
   const double* dtDev = nullptr;       // set while capturing steps whose dt lives on the device
   double* dtBuf = nullptr;             // [dt, t, next dt's CFL] of graph-captured runs
   double runCfl = 0.;                  // the CFL argument of the graph-captured run
@@ -1300,6 +1304,18 @@ template <typename R>
 void launch_face_solve(const DevParams& P, const FaceLaunch& F, long faces, bool multi,
                        bool withUpdate, double dt) {
   const dim3 gf(grid_for(faces));
+  if (P.learned) {  // the gate lists the strong faces for the seed kernel
+    float* y = withUpdate ? nullptr : g.seedY;
+    hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, g.stream, g.strongCount);
+    if (multi)
+      hipLaunchKernelGGL((k_face_gate<true, R>), gf, dim3(kThreads), 0, g.stream, P, F,
+                         make_seed_gate(g.seedGate), y, g.strongList, g.strongCount,
+                         (unsigned)faces);
+    else
+      hipLaunchKernelGGL((k_face_gate<false, R>), gf, dim3(kThreads), 0, g.stream, P, F,
+                         make_seed_gate(g.seedGate), y, g.strongList, g.strongCount,
+                         (unsigned)faces);
+  }
   if (multi && P.learned)
     hipLaunchKernelGGL((k_face_seed<true, R>), wave_grid(faces), dim3(kLearnThreads), 0, g.stream,
                        P, F, g.seedPacked, g.seedY, g.strongList, g.strongCount, faces);
@@ -1307,12 +1323,17 @@ void launch_face_solve(const DevParams& P, const FaceLaunch& F, long faces, bool
     hipLaunchKernelGGL((k_face_seed<false, R>), wave_grid(faces), dim3(kLearnThreads), 0,
                        g.stream, P, F, g.seedPacked, g.seedY, g.strongList, g.strongCount, faces);
   if (withUpdate) {  // the face stage and the update in one kernel (batch steps)
-    if (P.learned)
+    if (P.learned) {   // the listed strong faces first, seeded; the rest unseeded
+      R* over = reinterpret_cast<R*>(g.seedFlux);
+      hipLaunchKernelGGL((k_seeded_solve<R>), dim3((unsigned)std::min<long>(grid_for(faces), 4L * g.numCU)),
+                         dim3(kThreads), 0, g.stream, P, F, g.strongList, g.strongCount, faces,
+                         faces, over, g.seedBits);
       hipLaunchKernelGGL((k_solve_update<true, R>), dim3((unsigned)F.nTiles), dim3(kSolveThreads),
-                         0, g.stream, P, F, dt, g.dtDev);
-    else
+                         0, g.stream, P, F, dt, g.dtDev, over, faces, g.seedBits);
+    } else {
       hipLaunchKernelGGL((k_solve_update<false, R>), dim3((unsigned)F.nTiles), dim3(kSolveThreads),
-                         0, g.stream, P, F, dt, g.dtDev);
+                         0, g.stream, P, F, dt, g.dtDev, nullptr, 0L, nullptr);
+    }
     return;
   }
   if (multi && P.learned)
@@ -1331,27 +1352,14 @@ void launch_face_solve(const DevParams& P, const FaceLaunch& F, long faces, bool
 int launch_flux_stages(int nTiles, double dt, int TX, int TY, bool solveUpdate = false) {
   const DevParams& P = g.cur;
   MarchLaunch M{g.dtiles, nTiles, g.multi ? 0 : P.nxb / TX,
-                g.multi ? 0 : (P.ndim > 1 ? P.nyb : 1) / TY, g.status, dt, nullptr, g.dtDev,
-                nullptr, nullptr, nullptr, make_seed_gate(g.seedGate), 0u};
+                g.multi ? 0 : (P.ndim > 1 ? P.nyb : 1) / TY, g.status, dt, nullptr, g.dtDev};
   const dim3 grid((unsigned)(g.multi ? 1 : nTiles * M.tilesX * M.tilesY));
   if (P.riemann == FX_EXACT) {
-    // exact solver: the march stores face states (and, with the learned
-    // seed, lists the strong faces), a separate stage solves
+    // exact solver: the march stores face states, a separate stage solves
+    // (with the learned seed, after the gate and the seed kernel)
     M.fstates = g.fstates;
-    if (P.learned) {
-      M.seedY = g.seedY;
-      M.strongCount = g.strongCount;
-      M.strongList = g.strongList;
-      M.strongCap = (unsigned)g.hfaceStart[nTiles];
-      hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, g.stream, g.strongCount);
-    }
-    if (P.learned) {
-      if (g.multi) launch_march_multi<RS_STATES_GATED>(P.ndim, P.precision, TX, M);
-      else launch_march<RS_STATES_GATED>(P, M, TX, grid);
-    } else {
-      if (g.multi) launch_march_multi<RS_STATES>(P.ndim, P.precision, TX, M);
-      else launch_march<RS_STATES>(P, M, TX, grid);
-    }
+    if (g.multi) launch_march_multi<RS_STATES>(P.ndim, P.precision, TX, M);
+    else launch_march<RS_STATES>(P, M, TX, grid);
     if (record(1)) return 4;
     const long faces = g.hfaceStart[nTiles];
     FaceLaunch F{g.dtiles, nTiles, g.fstates, P.learned ? g.seedY : nullptr, g.status,
@@ -1409,6 +1417,25 @@ int ensure_step_scratch(int nTiles, fx_status* st) {
   return 0;
 }
 
+// the seeded strong faces' fluxes and bits of a learned batch step; the
+// bits start at zero and every step clears those it sets
+__global__ void k_zero_words(unsigned* p, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+int ensure_seed_patch(long faces, int nTiles, fx_status* st) {
+  if (grow(&g.seedFlux, &g.seedFluxCap, (size_t)8 * faces, st)) return 4;
+  const size_t words = (size_t)seed_bit_words(g.cur) * (size_t)nTiles;
+  if (g.seedBitsCap < words) {
+    if (grow(&g.seedBits, &g.seedBitsCap, words, st)) return 4;
+    hipLaunchKernelGGL(k_zero_words, dim3(grid_for((long)words)), dim3(kThreads), 0, g.stream,
+                       g.seedBits, (long)words);
+  }
+  return 0;
+}
+
 int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char* inPlace,
                 fx_status* st, bool tilesUploaded = false) {
   if (!g.init) {
@@ -1454,9 +1481,11 @@ int run_compute(const fx_tile* tiles, int nTiles, double dt, const unsigned char
                              P.riemann == FX_EXACT && P.ndim == 3 &&
                              P.nxb * P.nyb <= kSolveMaxCols &&
                              2 * (P.nxb + 1) * P.nyb + P.nxb * P.nyb <= kSolveMaxFaces &&
-                             P.nxb + 1 <= 17 && P.nyb + 1 <= 17;
+                             P.nxb + 1 <= 17 && P.nyb + 1 <= 17 &&
+                             (!P.learned || seed_bit_words(P) <= kSolveMaxBitWords);
     if (g.multi && build_march_items(nTiles, TX, st)) return 4;
     if (ensure_step_scratch(nTiles, st)) return 4;
+    if (solveUpdate && P.learned && ensure_seed_patch(g.hfaceStart[nTiles], nTiles, st)) return 4;
     if (!g.capturing) hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, g.status);
     if (record(0)) return 4;
     if (launch_flux_stages(nTiles, dt, TX, TY, solveUpdate)) return 4;
@@ -1939,7 +1968,7 @@ void fx_hydro_finalize(void) {
   void* bufs[] = {g.strongList, g.strongCount, g.dwCell, g.dwFace, g.states, g.flux, g.pavg, g.dtiles, g.dinplace, g.status, g.partial,
                   g.dregTiles, g.dfaces, g.mlpPacked, g.seedPacked, g.seedY, g.fstates,
                   g.dnbr, g.dtBuf, g.dcls, g.dtcls, g.dcellStart, g.dfaceStart,
-                  g.ditems, g.dregCls};
+                  g.ditems, g.dregCls, g.seedFlux, g.seedBits};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : g.ev)
@@ -2028,7 +2057,7 @@ int fx_hydro_advance_batch_noalloc(double* U, long nBlocks, long blockStride,
                      P.ndim, reinterpret_cast<double*>(s + Lay.pavg), P.icells);
   hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, g.stream, status);
   MarchLaunch M{tiles, (int)nBlocks, P.nxb / TX, (P.ndim > 1 ? P.nyb : 1) / TY, status, dt,
-                nullptr, nullptr, nullptr, nullptr, nullptr, make_seed_gate(g.seedGate), 0u};
+                nullptr, nullptr};
   const dim3 grid((unsigned)(nBlocks * M.tilesX * M.tilesY));
   if (P.riemann == FX_HLLC) launch_march<FX_HLLC>(P, M, TX, grid);
   else launch_march<FX_HYBR>(P, M, TX, grid);

@@ -295,16 +295,18 @@ k_face_solve(DevParams P0, FaceLaunch F) {
     // face states: comps 0..5 dens, velx, vely, velz, pres, eint; left, then right
     const R* Lb = b + f.r;
     const R* Rb = b + 6 * nf + f.r;
-    double guess = -1.0;
-    if (SEEDED) {
-      const float y = F.y[gid];
-      if (y != kNoSeed) guess = seed_from_y(y, Lb[4 * nf], Rb[4 * nf]);
-    }
+    // a face without a seed takes the unseeded solver, as the batch step's
+    // solve+update kernel gives it (the same FP32 code, the same bits)
+    const float y = SEEDED ? F.y[gid] : kNoSeed;
     const gdouble* U = as_global(F.tiles[f.t].U);
+    const double gamc = U[P.voff[FX_GAMC] + f.cL], gL = U[P.voff[FX_GAME] + f.cL],
+                 gR = U[P.voff[FX_GAME] + f.cR];
     double Fo[8];
-    const int ierr = exact_face_flux<SEEDED, R>(P, d, Lb, Rb, nf, U[P.voff[FX_GAMC] + f.cL],
-                                                U[P.voff[FX_GAME] + f.cL],
-                                                U[P.voff[FX_GAME] + f.cR], guess, Fo, es);
+    const int ierr =
+        y != kNoSeed
+            ? exact_face_flux<true, R>(P, d, Lb, Rb, nf, gamc, gL, gR,
+                                       seed_from_y(y, Lb[4 * nf], Rb[4 * nf]), Fo, es)
+            : exact_face_flux<false, R>(P, d, Lb, Rb, nf, gamc, gL, gR, -1.0, Fo, es);
     if (ierr) {
       report_error(F.status, 2, f.t, (int)(f.cR % P.nx), (int)((f.cR / P.nx) % P.ny),
                    (int)(f.cR / ((long)P.nx * P.ny)), d);
@@ -313,6 +315,56 @@ k_face_solve(DevParams P0, FaceLaunch F) {
     gdouble* out = as_global(F.tiles[f.t].flux[d]);
 #pragma unroll
     for (int q = 0; q < FX_NFLUX; ++q) out[q * nf + f.r] = Fo[q];
+  }
+  flush_exact(F.status, es);
+}
+
+// Bits of the seeded faces of a batch step: tile t owns words
+// [t * W, (t + 1) * W) with W = seed_bit_words(P), bit rt of them for its
+// face rt (directions in order), so that no word spans two tiles.
+__host__ __device__ inline long seed_bit_words(const DevParams& P) {
+  return (P.nface[0] + P.nface[1] + P.nface[2] + 31) / 32;
+}
+
+// The learned seed's strong faces of a batch step, solved ahead of
+// k_solve_update: each listed face (its network seed already in F.y) is
+// solved seeded, its flux goes to over[q * faces + gid] in R and its bit in
+// bits is set; k_solve_update takes that flux and clears the bit, so the
+// unseeded solver there carries no seeded code.  A face the seeded solve
+// fails on gets a zero flux, as k_solve_update gives its own failures.
+template <typename R>
+__global__ void __launch_bounds__(kThreads) k_seeded_solve(DevParams P, FaceLaunch F,
+                                                           const unsigned* __restrict__ list,
+                                                           const unsigned* __restrict__ count,
+                                                           long cap, long faces, R* over,
+                                                           unsigned* bits) {
+  const long total = (long)*count < cap ? (long)*count : cap;
+  ExactStats es{0ULL, 0ULL};
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long)gridDim.x * blockDim.x) {
+    const long gid = (long)list[i];
+    int t;
+    long rt;
+    face_tile<false>(P, F, gid, t, rt);
+    const FaceRef f = face_ref(P, t, rt);
+    const int d = f.d;
+    const R* b = face_states_at<R, false>(P, F, t, d);
+    const long nf = P.nface[d];
+    const R* Lb = b + f.r;
+    const R* Rb = b + 6 * nf + f.r;
+    const float y = F.y[gid];
+    const double guess = y != kNoSeed ? seed_from_y(y, Lb[4 * nf], Rb[4 * nf]) : -1.0;
+    const gdouble* U = as_global(F.tiles[t].U);
+    double Fo[8];
+    const int ierr = exact_face_flux<true, R>(P, d, Lb, Rb, nf, U[P.voff[FX_GAMC] + f.cL],
+                                              U[P.voff[FX_GAME] + f.cL],
+                                              U[P.voff[FX_GAME] + f.cR], guess, Fo, es);
+    if (ierr)
+      report_error(F.status, 2, t, (int)(f.cR % P.nx), (int)((f.cR / P.nx) % P.ny),
+                   (int)(f.cR / ((long)P.nx * P.ny)), d);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) over[q * faces + gid] = ierr ? R(0.) : R(Fo[q]);
+    atomicOr(&bits[t * seed_bit_words(P) + (rt >> 5)], 1u << (rt & 31));
   }
   flush_exact(F.status, es);
 }
@@ -327,14 +379,19 @@ k_face_solve(DevParams P0, FaceLaunch F) {
 // those of k_face_solve followed by k_update.  Every face reads a cell's
 // gamc and game before that cell is updated: the faces of plane k are
 // solved before plane k-1 is written.  Blocks of at most 16 x 16 cells
-// across (the LDS holds one plane of fluxes).
+// across (the LDS holds one plane of fluxes).  SEEDED: faces whose bit is
+// set in bits take k_seeded_solve's flux from over; the block's bit words
+// are copied to LDS at the start and cleared in memory (at most
+// kSolveMaxBitWords of them).
 constexpr int kSolveThreads = 256;
 constexpr int kSolveMaxCols = kSolveThreads;   // nxb * nyb: one column per thread
 constexpr int kSolveMaxFaces = 17 * 16 * 2 + 16 * 16;
+constexpr int kSolveMaxBitWords = 2048;
 
 template <bool SEEDED, typename R>
 __global__ void __launch_bounds__(kSolveThreads) __attribute__((amdgpu_waves_per_eu(2)))
-k_solve_update(DevParams P, FaceLaunch F, double dt0, const double* dtp) {
+k_solve_update(DevParams P, FaceLaunch F, double dt0, const double* dtp, const R* over,
+               long faces, unsigned* bits) {
   // this plane's face fluxes (x, y, then z), in the solver's type R, which
   // holds them exactly (the solver forms them in R)
   __shared__ R flx[8 * kSolveMaxFaces];
@@ -351,6 +408,17 @@ k_solve_update(DevParams P, FaceLaunch F, double dt0, const double* dtp) {
   UpdPart part{};                                // the column's cell in flight
   double zprev[8];                               // the z flux below it
   ExactStats es{0ULL, 0ULL};
+  __shared__ unsigned sbits[SEEDED ? kSolveMaxBitWords : 1];
+  if (SEEDED) {
+    const long nw = seed_bit_words(P);
+    unsigned* const tb = bits + (long)t * nw;
+    for (int i = c; i < nw; i += kSolveThreads) {
+      const unsigned v = tb[i];
+      sbits[i] = v;
+      if (v) tb[i] = 0u;
+    }
+    __syncthreads();
+  }
   for (int k = P.lo[2]; k <= P.hi[2] + 1; ++k) {
     const bool inPlane = k <= P.hi[2];
     const int zBase = inPlane ? nfx + nfy : 0;
@@ -367,23 +435,26 @@ k_solve_update(DevParams P, FaceLaunch F, double dt0, const double* dtp) {
       const int i = a0 + P.lo[0], j = a1 + P.lo[1];
       const long r = face_index(P, d, i, j, k);
       const long cR = cidx(P, i, j, k), cL = cR - axis_stride(P, d);
+      const int slot = d == 2 ? nfx + nfy + (w - zBase) : w;
+      if (SEEDED) {
+        long rt = r;
+        for (int e = 0; e < d; ++e) rt += P.nface[e];
+        if ((sbits[rt >> 5] >> (rt & 31)) & 1u) {
+          const long gid = (long)t * perTile + rt;
+#pragma unroll
+          for (int q = 0; q < 8; ++q) flx[q * kSolveMaxFaces + slot] = over[q * faces + gid];
+          continue;
+        }
+      }
       const R* b = face_states_at<R, false>(P, F, t, d);
       const long nf = P.nface[d];
       const R* Lb = b + r;
       const R* Rb = b + 6 * nf + r;
-      double guess = -1.0;
-      if (SEEDED) {
-        long gid = (long)t * perTile + r;
-        for (int e = 0; e < d; ++e) gid += P.nface[e];
-        const float y = F.y[gid];
-        if (y != kNoSeed) guess = seed_from_y(y, Lb[4 * nf], Rb[4 * nf]);
-      }
       double Fo[8];
-      const int ierr = exact_face_flux<SEEDED, R>(P, d, Lb, Rb, nf, U[P.voff[FX_GAMC] + cL],
-                                                  U[P.voff[FX_GAME] + cL], U[P.voff[FX_GAME] + cR],
-                                                  guess, Fo, es);
+      const int ierr = exact_face_flux<false, R>(P, d, Lb, Rb, nf, U[P.voff[FX_GAMC] + cL],
+                                                 U[P.voff[FX_GAME] + cL], U[P.voff[FX_GAME] + cR],
+                                                 -1.0, Fo, es);
       if (ierr) report_error(F.status, 2, t, i, j, k, d);
-      const int slot = d == 2 ? nfx + nfy + (w - zBase) : w;
 #pragma unroll
       for (int q = 0; q < 8; ++q) flx[q * kSolveMaxFaces + slot] = ierr ? R(0.) : R(Fo[q]);
     }
@@ -475,6 +546,62 @@ __global__ void __launch_bounds__(kLearnThreads) k_face_seed(DevParams P0, FaceL
 }
 
 __global__ void k_zero_count(unsigned* c) { c[threadIdx.x] = 0u; }
+
+// The learned seed's gate over every face of the launch, from the face
+// states the march stored: a face at or above the gate (seed_gate_pass, in
+// FP64 from the stored values) joins the strong list (one atomic per
+// wavefront, ranks by ballot) for k_face_seed.  With y, every face's seed
+// slot also becomes "no seed" first (k_face_seed then fills the listed
+// ones): the face stage that reads a seed per face needs it, the batch
+// step's list-driven solve does not.  One pass over the states, apart from
+// the march, so that neither march carries the gate.
+template <bool MULTI, typename S>
+__global__ void __launch_bounds__(kThreads) k_face_gate(DevParams P0, FaceLaunch F, SeedGate G,
+                                                        float* __restrict__ y,
+                                                        unsigned* __restrict__ list,
+                                                        unsigned* __restrict__ count,
+                                                        unsigned cap) {
+  const long total = face_total<MULTI>(P0, F);
+  const int lane = (int)(threadIdx.x & 63);
+  // the loop bound is the same for every lane of a workgroup, so whole
+  // wavefronts reach the ballot together
+  for (long g0 = (long)blockIdx.x * blockDim.x; g0 < total; g0 += (long)gridDim.x * blockDim.x) {
+    const long gid = g0 + threadIdx.x;
+    bool strong = false;
+    if (gid < total) {
+      int t;
+      long rt;
+      if constexpr (!MULTI) {  // face numbers fit 32 bits (the strong list holds them)
+        const unsigned pt = (unsigned)(P0.nface[0] + P0.nface[1] + P0.nface[2]);
+        const unsigned ut = (unsigned)gid / pt;
+        t = (int)ut;
+        rt = (long)((unsigned)gid - ut * pt);
+      }
+      const DevParams& P = MULTI ? face_tile<MULTI>(P0, F, gid, t, rt) : P0;
+      int r = (int)rt, d = 0;
+      while (d < P.ndim - 1 && r >= (int)P.nface[d]) {
+        r -= (int)P.nface[d];
+        ++d;
+      }
+      const S* b = face_states_at<S, MULTI>(P, F, t, d);
+      const long nf = P.nface[d];
+      const double rl = b[r], ul = b[(1 + d) * nf + r], pl = b[4 * nf + r];
+      const double rr = b[6 * nf + r], ur = b[(7 + d) * nf + r], pr = b[10 * nf + r];
+      strong = rl > 0. && rr > 0. && pl > 0. && pr > 0. &&
+               seed_gate_pass(G, P.gamma, rl, ul, pl, rr, ur, pr);
+      if (y) y[gid] = kNoSeed;
+    }
+    const unsigned long long mask = __ballot(strong);
+    if (mask) {
+      const int leader = __builtin_ctzll(mask);
+      unsigned base = 0;
+      if (lane == leader) base = atomicAdd(count, (unsigned)__popcll(mask));
+      base = __shfl(base, leader);
+      const unsigned slot = base + (unsigned)__popcll(mask & ((1ULL << lane) - 1ULL));
+      if (strong && slot < cap) list[slot] = (unsigned)gid;
+    }
+  }
+}
 
 // face states of the last exact-solver step as Riemann problems, SoA [6][n]
 // (rhoL, uL, pL, rhoR, uR, pR with u the face-normal velocity); S is the

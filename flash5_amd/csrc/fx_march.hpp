@@ -71,23 +71,12 @@ struct MarchLaunch {
   double dt;
   void* fstates;       // RS_STATES: per-face states for the separate Riemann stage, in R
   const double* dtp;   // device-resident dt (graph-captured steps), or null
-  // RS_STATES_GATED (the learned seed): every face's seed slot is set to
-  // "no seed", and faces whose strength reaches the gate are appended to
-  // strongList (their flat face numbers) for the seed kernel
-  float* seedY;
-  unsigned* strongCount;
-  unsigned* strongList;
-  SeedGate gate;
-  unsigned strongCap;  // capacity of strongList (the launch's face count)
 };
 
 // Riemann-stage variant of the march: stage C stores each face's left and
 // right states (dens, vx, vy, vz, pres, eint) instead of solving there.
 constexpr int RS_STATES = 99;
-// RS_STATES with the learned seed's gate at every face (compiled apart, so
-// that the unseeded exact path carries none of the gate's registers)
-constexpr int RS_STATES_GATED = 98;
-constexpr bool is_states(int rs) { return rs == RS_STATES || rs == RS_STATES_GATED; }
+constexpr bool is_states(int rs) { return rs == RS_STATES; }
 constexpr int kFaceComps = 12;
 
 // face-state scratch of one tile: [dir][comp 0..11][face], in the march's
@@ -109,33 +98,6 @@ __device__ __forceinline__ void write_face_states(const DevParams& P, R* base, l
   for (int q = 0; q < 6; ++q) {
     base[q * nf + r] = VL[comp_slot(q)];
     base[(6 + q) * nf + r] = VR[comp_slot(q)];
-  }
-}
-
-// The learned seed's gate on face r (direction D) of the tile whose faces
-// start at flat number faceBase: the face's seed slot becomes "no seed", and
-// a face at or above the gate joins the strong list (one atomic per
-// wavefront, ranks by ballot).
-template <int D, typename R>
-__device__ __forceinline__ void seed_gate(const DevParams& P, const MarchLaunch& L, long faceBase,
-                                          long r, const R VL[8], const R VR[8]) {
-  long gid = faceBase + r;
-#pragma unroll
-  for (int e = 0; e < D; ++e) gid += P.nface[e];
-  const double rl = VL[P_DENS], pl = VL[P_PRES], rr = VR[P_DENS], pr = VR[P_PRES];
-  const bool strong = rl > 0. && rr > 0. && pl > 0. && pr > 0. &&
-                      seed_gate_pass(L.gate, P.gamma, rl, double(VL[P_VELX + D]), pl, rr,
-                                     double(VR[P_VELX + D]), pr);
-  L.seedY[gid] = kNoSeed;
-  const unsigned long long mask = __ballot(strong);
-  if (mask) {
-    const int lane = (int)(threadIdx.x & 63);
-    const int leader = __builtin_ctzll(mask);
-    unsigned base = 0;
-    if (lane == leader) base = atomicAdd(L.strongCount, (unsigned)__popcll(mask));
-    base = __shfl(base, leader);
-    const unsigned slot = base + (unsigned)__popcll(mask & ((1ULL << lane) - 1ULL));
-    if (strong && slot < L.strongCap) L.strongList[slot] = (unsigned)gid;
   }
 }
 
@@ -384,7 +346,7 @@ __device__ __forceinline__ void write_flux_b(const DevParams& P, gdouble* Fo, un
 // tile sits at the block's high side, where it owns that row.
 template <int NDIM, int TX, int TY, int NT, int RS, typename R>
 __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch& L, int t, int ox,
-                                           int oy, R* fsb, long faceBase, ExactStats& es,
+                                           int oy, R* fsb, ExactStats& es,
                                            R* smem) {
   using I = TileIdx<TX, TY>;
   using M = MarchLds<NDIM, TX, TY>;
@@ -691,8 +653,6 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           const unsigned fr = (unsigned)face_index(P, 0, ox + a, oy + b, k);
           if constexpr (is_states(RS)) {
             write_face_states<0>(P, fsb + fstate_dir_offset(P, 0), fr, VL, VR);
-            if constexpr (RS == RS_STATES_GATED)
-              if (L.seedY) seed_gate<0>(P, L, faceBase, fr, VL, VR);
             continue;
           } else {
             if (face_flux<0, RS>(P, VL, VR, tv_use_hll<RS>(P, tv, cb - sxb, cb), F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 0); continue; }
@@ -713,8 +673,6 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           const unsigned fr = (unsigned)face_index(P, 1, ox + a, oy + b, k);
           if constexpr (is_states(RS)) {
             write_face_states<1>(P, fsb + fstate_dir_offset(P, 1), fr, VL, VR);
-            if constexpr (RS == RS_STATES_GATED)
-              if (L.seedY) seed_gate<1>(P, L, faceBase, fr, VL, VR);
             continue;
           } else {
             if (face_flux<1, RS>(P, VL, VR, tv_use_hll<RS>(P, tv, cb - syb, cb), F, es)) { report_error(st, 2, t, ox + a, oy + b, k, 1); continue; }
@@ -736,8 +694,6 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
             const unsigned fr = (unsigned)face_index(P, 2, ox + a, oy + b, k);
             if constexpr (is_states(RS)) {
               write_face_states<2>(P, fsb + fstate_dir_offset(P, 2), fr, VL, VR);
-              if constexpr (RS == RS_STATES_GATED)
-                if (L.seedY) seed_gate<2>(P, L, faceBase, fr, VL, VR);
             } else {
               if (face_flux<2, RS>(P, VL, VR, tv_use_hll<RS>(P, tv, cb - szb, cb), F, es)) report_error(st, 2, t, ox + a, oy + b, k, 2);
               else write_flux_b<2>(P, as_global(T.flux[2]), fr * 8u, F);
@@ -777,9 +733,8 @@ k_march(DevParams P, MarchLaunch L) {
   const int oy = P.lo[1] + (tr / L.tilesX) * TY;
   ExactStats es{0ULL, 0ULL};
   R* fsb = is_states(RS) ? static_cast<R*>(L.fstates) + t * fstate_tile_stride(P) : nullptr;
-  const long faceBase = (long)t * (P.nface[0] + P.nface[1] + P.nface[2]);
   __shared__ R smem[MarchLds<NDIM, TX, TY>::total];
-  march_tile<NDIM, TX, TY, NT, RS, R>(P, L, t, ox, oy, fsb, faceBase, es, smem);
+  march_tile<NDIM, TX, TY, NT, RS, R>(P, L, t, ox, oy, fsb, es, smem);
   flush_exact(L.status, es);
 }
 
@@ -805,7 +760,7 @@ k_march_multi(const DevParams* __restrict__ cls,
     const DevParams& P = cls[w.cls];
     const long faceBase = is_states(RS) ? faceStart[w.t] : 0;
     R* fsb = is_states(RS) ? static_cast<R*>(L.fstates) + kFaceComps * faceBase : nullptr;
-    march_tile<NDIM, TX, TY, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, faceBase, es, smem);
+    march_tile<NDIM, TX, TY, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, es, smem);
   }
   flush_exact(L.status, es);
 }
@@ -825,8 +780,8 @@ k_march_mixw(const DevParams* __restrict__ cls, MarchLaunch L,
     const DevParams& P = cls[w.cls];
     const long faceBase = is_states(RS) ? faceStart[w.t] : 0;
     R* fsb = is_states(RS) ? static_cast<R*>(L.fstates) + kFaceComps * faceBase : nullptr;
-    if (w.tx == 16) march_tile<NDIM, 16, 8, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, faceBase, es, smem);
-    else march_tile<NDIM, 8, 8, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, faceBase, es, smem);
+    if (w.tx == 16) march_tile<NDIM, 16, 8, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, es, smem);
+    else march_tile<NDIM, 8, 8, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, es, smem);
   }
   flush_exact(L.status, es);
 }

@@ -44,8 +44,6 @@ FX_MARCH_DECLARE(FX_HYBR, double)
 FX_MARCH_DECLARE(FX_HYBR, float)
 FX_MARCH_DECLARE(RS_STATES, double)
 FX_MARCH_DECLARE(RS_STATES, float)
-FX_MARCH_DECLARE(RS_STATES_GATED, double)
-FX_MARCH_DECLARE(RS_STATES_GATED, float)
 #undef FX_MARCH_DECLARE
 
 }  // namespace fx

@@ -121,6 +121,46 @@ def test_learned_seed_inside_hydro_step():
     hy2.set_seed_params(None)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("gate", [0.0, 0.05, 0.2])
+def test_learned_batch_solve_update_matches_face_stage(prec, gate):
+    """Learned-seed batch steps of 3-D blocks solve the listed strong faces
+    seeded first, then every other face unseeded inside the solve+update
+    kernel (kernel path 0); path 2 runs the face stage that reads a seed
+    per face, then the update.  Same seeds, same solves: bitwise, with the
+    same Newton iteration and fallback counts, at gate 0 (every face
+    seeded), at a gate some faces pass, and at the default gate."""
+    from flash5_amd import workloads as W
+    from flash5_amd.hydro import PREC_MIXED, Hydro, HydroParams
+    n = 16
+    U = W.fourier_blocks(6, n=n, gamma=G, seed=W.SEED_BASE + 4).numpy()
+    dx = 1.0 / n
+    dt = W.cfl_dt(torch.from_numpy(U), 3, 4, (dx,) * 3, 0.8)
+    kw = {"precision": PREC_MIXED} if prec == "mixed" else {}
+    hy = Hydro(HydroParams(ndim=3, nxb=n, nyb=n, nzb=n, gamma=G, RiemannSolver="exact",
+                           learnedSeed=True, **kw))
+    hy.set_seed_params(_params(7))
+    out, stats = [], []
+    L.set_gate(gate)
+    try:
+        for path in (0, 2, 0):
+            hy.set_kernel_path(path)
+            Ug = torch.from_numpy(U.copy()).cuda()
+            st = hy.advance_batch(Ug, dt, (dx,) * 3)
+            assert st.code == 0
+            out.append(Ug.cpu().numpy())
+            stats.append((int(st.newton_iters), int(st.newton_fallbacks)))
+    finally:
+        L.set_gate(0.2)
+        hy.set_kernel_path(0)
+        hy.set_seed_params(None)
+    for i in (1, 2):
+        diff = np.argwhere(out[i] != out[0])
+        assert diff.size == 0, f"run {i}: {len(diff)} values differ, first {diff[:4]}"
+    assert stats[0] == stats[1] == stats[2], stats
+    assert not np.array_equal(out[0], U)
+
+
 def test_mixed_mode_face_states_are_fp32():
     """The mixed-precision exact step stores its face states in FP32: the
     collected states are FP32 numbers, within FP32 reconstruction error of the
