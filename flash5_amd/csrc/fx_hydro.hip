@@ -821,7 +821,7 @@ struct Lib {
   int itemsOrder = 0;                  // item order they were built in
   int persist = 0;                     // multi-class march: multiple of the resident workgroups (0 = 1)
   int tuneTX = 0;                      // autotuned column-tile width (0: built-in choice)
-  int tuneOcc = 0;                     // autotuned waves-per-SIMD hint of the FP32 3-D march (0: none)
+  int tuneOcc = 0;                     // autotuned waves-per-SIMD hint of the FP32 3-D march (0 or 4)
   int itemOrder = 0;                   // multi-class items: 0 longest first, 1 tile by tile
   unsigned long uploadGen = 0;         // counts tile-list uploads
   // multi-level grid (fx_amr_*)
@@ -1708,14 +1708,12 @@ int tune_flux_stages(int nTiles, double dt, int reps, fx_tune_result* out, int c
     can16 = P.nxb % 16 == 0 && P.nyb % 8 == 0;
   }
   std::vector<Cand> cands;
-  // waves-per-SIMD hints: 3 for the FP32 3-D march (fewer registers, maybe
-  // spills), 2 for the FP64 3-D 8-wide march (two workgroups per CU)
-  const bool occ3 = P.precision == FX_PREC_MIXED && P.ndim == 3;
-  const bool occ2 = P.precision == FX_PREC_FP64 && P.ndim == 3;
-  for (int occ : {0, 3, 2}) {
-    if ((occ == 3 && !occ3) || (occ == 2 && !occ2)) continue;
+  // waves-per-SIMD hint 4 for the FP32 3-D march: at most 128 VGPRs (a few
+  // spills), so that two 512-thread workgroups share a CU
+  const bool occ4 = P.precision == FX_PREC_MIXED && P.ndim == 3;
+  for (int occ : {0, 4}) {
+    if (occ == 4 && !occ4) continue;
     for (int tx : {16, kPerClassTX, 8}) {
-      if (occ == 2 && tx != 8) continue;
       if (tx == 16 && !can16) continue;
       if (tx == kPerClassTX && (!g.multi || can16 || !any16)) continue;
       if (g.multi) {

@@ -14,17 +14,12 @@ namespace fx {
 template <int RS, typename R>
 void launch_march_r(const DevParams& P, const MarchLaunch& M, int TX, dim3 grid,
                     const MarchHost& H) {
-  if constexpr (sizeof(R) == 4) {  // FP32 3-D: the autotuner may ask for 3 waves per SIMD
-    if (P.ndim == 3 && H.tuneOcc == 3) {
+  if constexpr (sizeof(R) == 4) {  // FP32 3-D: the autotuner may ask for 4 waves per SIMD
+    if (P.ndim == 3 && H.tuneOcc == 4) {
       if (TX == 16)
-        hipLaunchKernelGGL((k_march<3, 16, 8, 512, RS, R, 3>), grid, dim3(512), 0, H.stream, P, M);
+        hipLaunchKernelGGL((k_march<3, 16, 8, 512, RS, R, 4>), grid, dim3(512), 0, H.stream, P, M);
       else
-        hipLaunchKernelGGL((k_march<3, 8, 8, 256, RS, R, 3>), grid, dim3(256), 0, H.stream, P, M);
-      return;
-    }
-  } else {  // FP64 3-D, 8-wide tiles: 2 waves per SIMD, so two workgroups share a CU
-    if (P.ndim == 3 && H.tuneOcc == 2 && TX == 8) {
-      hipLaunchKernelGGL((k_march<3, 8, 8, 256, RS, R, 2>), grid, dim3(256), 0, H.stream, P, M);
+        hipLaunchKernelGGL((k_march<3, 8, 8, 256, RS, R, 4>), grid, dim3(256), 0, H.stream, P, M);
       return;
     }
   }
@@ -33,8 +28,6 @@ void launch_march_r(const DevParams& P, const MarchLaunch& M, int TX, dim3 grid,
   else if (TX == 16) hipLaunchKernelGGL((k_march<2, 16, 8, 256, RS, R>), grid, dim3(256), 0, H.stream, P, M);
   else hipLaunchKernelGGL((k_march<2, 8, 8, 256, RS, R>), grid, dim3(256), 0, H.stream, P, M);
 }
-
-// FX_PREC_MIXED reconstructs and solves HLLC in FP32; the update kernel and
 
 // Persistent multi-class march: as many workgroups as stay resident (or
 // H.persist per CU), each taking every G-th column tile.
@@ -78,8 +71,8 @@ template <int RS, typename R>
 void launch_march_multi_r(int ndim, int TX, const MarchLaunch& M, const MarchHost& H) {
   if (TX == kPerClassTX) {  // 16- and 8-wide column tiles in one launch
     if constexpr (sizeof(R) == 4) {
-      if (ndim == 3 && H.tuneOcc == 3) {
-        launch_march_mixw_k<3, 512, RS, R, 3>(M, H);
+      if (ndim == 3 && H.tuneOcc == 4) {
+        launch_march_mixw_k<3, 512, RS, R, 4>(M, H);
         return;
       }
     }
@@ -87,15 +80,10 @@ void launch_march_multi_r(int ndim, int TX, const MarchLaunch& M, const MarchHos
     else launch_march_mixw_k<2, 256, RS, R>(M, H);
     return;
   }
-  if constexpr (sizeof(R) == 4) {  // FP32 3-D: the autotuner may ask for 3 waves per SIMD
-    if (ndim == 3 && H.tuneOcc == 3) {
-      if (TX == 16) launch_march_multi_k<3, 16, 8, 512, RS, R, 3>(M, H);
-      else launch_march_multi_k<3, 8, 8, 256, RS, R, 3>(M, H);
-      return;
-    }
-  } else {  // FP64 3-D, 8-wide tiles: 2 waves per SIMD
-    if (ndim == 3 && H.tuneOcc == 2 && TX == 8) {
-      launch_march_multi_k<3, 8, 8, 256, RS, R, 2>(M, H);
+  if constexpr (sizeof(R) == 4) {  // FP32 3-D: the autotuner may ask for 4 waves per SIMD
+    if (ndim == 3 && H.tuneOcc == 4) {
+      if (TX == 16) launch_march_multi_k<3, 16, 8, 512, RS, R, 4>(M, H);
+      else launch_march_multi_k<3, 8, 8, 256, RS, R, 4>(M, H);
       return;
     }
   }

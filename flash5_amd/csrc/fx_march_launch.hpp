@@ -16,7 +16,7 @@ constexpr int kPerClassTX = 1;
 // host state a launch needs
 struct MarchHost {
   hipStream_t stream;
-  int tuneOcc;               // autotuned waves-per-SIMD hint of the FP32 3-D march (0: none)
+  int tuneOcc;               // autotuned waves-per-SIMD hint of the FP32 3-D march (0 or 4)
   int numCU;                 // compute units of the device
   int persist;               // multi-class march: multiple of the resident workgroups (0 = 1)
   const DevParams* dcls;     // multi-class: parameters per class (device)

@@ -329,9 +329,8 @@ int fx_hydro_set_block_classes(const int* nb, int nClasses);
  * fastest for later calls.  Candidates: the column-tile width (16 or 8
  * columns across x, where every tile shape divides it, or per class for
  * tiles of several classes); in the mixed mode in
- * 3-D, a register budget for 3 waves per SIMD (occupancy against spills); in
- * FP64 3-D, 8-wide column tiles held to 2 waves per SIMD, so that two
- * workgroups share a CU;
+ * 3-D, a register budget for 4 waves per SIMD (occupancy against spills: two
+ * 512-thread workgroups share a CU);
  * and for tiles of several classes the persistent workgroups per CU (1, 2
  * or 4 times what stays resident) and the order of the column work items
  * (longest columns first, or tile by tile).  Each candidate runs reps times after one
@@ -344,8 +343,8 @@ typedef struct {
                                class: 16 where the class allows it, else 8) */
   int persist;              /* workgroups per CU multiple (multi-class only; 0 = n/a) */
   int order;                /* multi-class items: 0 longest first, 1 tile by tile */
-  int occupancy;            /* waves-per-SIMD hint of the 3-D march: 3 (FP32),
-                               2 (FP64, 8-wide tiles), 0 = none */
+  int occupancy;            /* waves-per-SIMD hint of the 3-D march: 4 (FP32),
+                               0 = none */
   int chosen;
   float ms;                 /* fastest of reps */
 } fx_tune_result;
