@@ -103,95 +103,192 @@ __device__ __forceinline__ double amr_mcslope(double u0, double um, double up) {
   return copysign(1.0, dc) * fmin(s, fabs(dc));
 }
 
-// Guard cells of every block: same-level copy where the level has a block,
-// else conservative linear prolongation from the next coarser level followed
-// by the EOS (MODE_DENS_EI).  Workgroup w takes kThreads guard cells of one
-// block, chunks[w] = (block, first guard cell in its class's list), so the
-// block and its parameters are uniform over the workgroup and every lane
-// has a guard cell (the lists hold guard cells only, in box order).
-__global__ void __launch_bounds__(kThreads) k_amr_fill(AmrLaunch A,
-                                                        const int2* __restrict__ chunks) {
-  {
-    const int2 ch = chunks[blockIdx.x];
-    const int b = ch.x;
-    const AmrBlock& B = A.blocks[b];
-    const int o = A.guardOff[B.cls] + ch.y + (int)threadIdx.x;
-    if (o >= A.guardOff[B.cls + 1]) return;
-    const int r = A.guard[o];
-    const DevParams& P = A.cls[B.cls];
-    const int i = r % P.nx, j = (r / P.nx) % P.ny, k = r / (P.nx * P.ny);
-    const int loc[3] = {i, j, k};
-    const int lev = B.level;
-    const AmrLevelDev& Lf = A.levels[lev];
-    int g[3] = {0, 0, 0};
+// Source of one guard-fill work slot (chunk w, lane t: the guard cell
+// guard[guardOff[cls] + chunks[w].y + t] of block chunks[w].x), fixed by
+// the grid's layout: x >= 0: the same-level block x holds the cell, at cell
+// index y of its box; x <= kPlanCoarse: prolongation from the coarse cell y
+// of block kPlanCoarse - x of the next coarser level; kPlanNone: an idle
+// lane; kPlanBad: neither exists (the levels are not properly nested, or a
+// level-0 cell has no block).
+constexpr int kPlanNone = -1, kPlanBad = -2, kPlanCoarse = -3;
+
+// the guard cell of a work slot: its block, cell index in that block's box
+// and global cell at the block's level (wrapped or clamped into the domain)
+struct AmrGuardCell {
+  int b, r, loc[3], g[3];
+};
+__device__ __forceinline__ bool amr_guard_cell(const AmrLaunch& A, const int2* chunks, int w,
+                                               int t, AmrGuardCell& gc) {
+  const int2 ch = chunks[w];
+  gc.b = ch.x;
+  const AmrBlock& B = A.blocks[gc.b];
+  const int o = A.guardOff[B.cls] + ch.y + t;
+  if (o >= A.guardOff[B.cls + 1]) return false;
+  gc.r = A.guard[o];
+  const DevParams& P = A.cls[B.cls];
+  gc.loc[0] = gc.r % P.nx;
+  gc.loc[1] = (gc.r / P.nx) % P.ny;
+  gc.loc[2] = gc.r / (P.nx * P.ny);
+  const AmrLevelDev& Lf = A.levels[B.level];
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
-      if (a < P.ndim) g[a] = B.pos[a] * Lf.nb[a] + loc[a] - P.lo[a];
-    amr_domain(A, lev, P.ndim, g);
-    gdouble* Ud = as_global(B.U);
-    const long cd = (long)r;
+  for (int a = 0; a < 3; ++a) gc.g[a] = a < P.ndim ? B.pos[a] * Lf.nb[a] + gc.loc[a] - P.lo[a] : 0;
+  amr_domain(A, B.level, P.ndim, gc.g);
+  return true;
+}
+
+// The guard-fill plan of every work slot (run once, when the grid is set
+// up): the source lookups that the fill would otherwise repeat every time,
+// each a chain of dependent loads through the level tables.  Slots that are
+// not same-level copies (prolongations, and lookups that fail) are also
+// listed in rest (count: *nRest; order free), for k_amr_prolong.
+__global__ void __launch_bounds__(kThreads) k_amr_plan(AmrLaunch A,
+                                                        const int2* __restrict__ chunks,
+                                                        int2* __restrict__ plan,
+                                                        const double** __restrict__ src,
+                                                        int* __restrict__ rest,
+                                                        unsigned* __restrict__ nRest) {
+  AmrGuardCell gc;
+  int2 e = make_int2(kPlanNone, 0);
+  if (amr_guard_cell(A, chunks, blockIdx.x, threadIdx.x, gc)) {
+    const int lev = A.blocks[gc.b].level;
     int sb;
     long sc;
-    if (amr_find(A, lev, g, sb, sc)) {
-      const AmrBlock& S = A.blocks[sb];
-      const DevParams& PS = A.cls[S.cls];
-      const int v0 = A.onlyVar >= 0 ? A.onlyVar : 0, v1 = A.onlyVar >= 0 ? A.onlyVar + 1 : P.nunk;
-      const gdouble* Us = as_global(S.U);
-      for (int v = v0; v < v1; ++v) Ud[(long)v * P.ncell + cd] = Us[(long)v * PS.ncell + sc];
-      return;
+    if (amr_find(A, lev, gc.g, sb, sc)) {
+      e = make_int2(sb, (int)sc);
+    } else {
+      e = make_int2(kPlanBad, 0);
+      const int c3[3] = {gc.g[0] / 2, gc.g[1] / 2, gc.g[2] / 2};
+      if (lev > 0 && amr_find(A, lev - 1, c3, sb, sc)) e = make_int2(kPlanCoarse - sb, (int)sc);
     }
-    // prolongation from level lev - 1 (the coarse cell and its neighbours)
-    if (lev == 0) {
-      report_error(A.status, 3, b, i, j, k, -1);
-      return;
+  }
+  const int slot = (int)blockIdx.x * kThreads + (int)threadIdx.x;
+  plan[slot] = e;
+  // a copy's source (variable-0 slot of the source cell), else null
+  src[slot] = e.x >= 0 ? A.blocks[e.x].U + e.y : nullptr;
+  const bool listed = e.x <= kPlanBad;
+  const unsigned long long mask = __ballot(listed);
+  if (mask) {
+    const int lane = (int)(threadIdx.x & 63), leader = __builtin_ctzll(mask);
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(nRest, (unsigned)__popcll(mask));
+    base = __shfl(base, leader);
+    if (listed) rest[base + (unsigned)__popcll(mask & ((1ULL << lane) - 1ULL))] = slot;
+  }
+}
+
+// Guard cells of every block, in two kernels over one plan (k_amr_plan):
+// k_amr_fill copies the same-level guard cells (from src, the plan's source
+// addresses), k_amr_prolong fills the
+// others by conservative linear prolongation from the next coarser level
+// followed by the EOS (MODE_DENS_EI).  The copy kernel's workgroup w takes
+// kThreads guard cells of one block, chunks[w] = (block, first guard cell
+// in its class's list), so the block and its parameters are uniform over
+// the workgroup and every lane has a guard cell (the lists hold guard cells
+// only, in box order).  Apart, the copy kernel keeps few registers.
+__global__ void __launch_bounds__(kThreads) k_amr_fill(AmrLaunch A,
+                                                        const int2* __restrict__ chunks,
+                                                        const double* const* __restrict__ src) {
+  const double* const sp = src[(long)blockIdx.x * kThreads + threadIdx.x];
+  if (!sp) return;  // idle lane, or a k_amr_prolong slot
+  const int2 ch = chunks[blockIdx.x];
+  const AmrBlock& B = A.blocks[ch.x];
+  const DevParams& P = A.cls[B.cls];
+  const long cd = A.guard[A.guardOff[B.cls] + ch.y + (int)threadIdx.x];
+  gdouble* Ud = as_global(B.U);
+  // blocks of one level have one block shape, hence one box size
+  const gdouble* Us = as_global(sp);
+  const int v0 = A.onlyVar >= 0 ? A.onlyVar : 0, v1 = A.onlyVar >= 0 ? A.onlyVar + 1 : P.nunk;
+  // eight variables' loads in flight at a time
+  for (int v = v0; v < v1; v += 8) {
+    double t[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (v + q < v1) t[q] = Us[(long)(v + q) * P.ncell];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (v + q < v1) Ud[(long)(v + q) * P.ncell + cd] = t[q];
+  }
+}
+
+// Prolongation in two passes.  Every fine guard cell under one coarse cell
+// takes the same coarse value q0 and limited slopes s_a * alpha of each
+// variable (the fine cells differ only in their offsets +-1/4), so
+// k_amr_coarse computes them once per distinct coarse cell (a record,
+// listed when the grid is set up) and k_amr_prolong combines them per fine
+// cell: q = q0 + sum_a (+-0.25) (s_a alpha), in the order a one-pass
+// interpolation would take.  Record r of n holds, per variable v,
+// cv[(v * 4 + e) * n + r]: e = 0 the conserved value q0, 1..3 s_a * alpha;
+// recOk[r] = 0 when the coarse cell's neighbourhood is incomplete.
+struct AmrCoarseRec {
+  int block;  // coarse block
+  int cell;   // the coarse cell's index in its box
+};
+
+__global__ void __launch_bounds__(kThreads) k_amr_coarse(AmrLaunch A,
+                                                          const AmrCoarseRec* __restrict__ recs,
+                                                          int nRec, double* __restrict__ cv,
+                                                          int* __restrict__ recOk) {
+  auto one = [&](int rr) {
+    const AmrCoarseRec rc = recs[rr];
+    const AmrBlock& Cb = A.blocks[rc.block];
+    const DevParams& PC = A.cls[Cb.cls];
+    const int lev = Cb.level;
+    const AmrLevelDev& Lc = A.levels[lev];
+    const int ndim = PC.ndim;
+    // the coarse cell's global position at its level
+    const int loc[3] = {rc.cell % PC.nx, (rc.cell / PC.nx) % PC.ny, rc.cell / (PC.nx * PC.ny)};
+    int gc[3] = {0, 0, 0};
+    bool inner = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (a >= ndim) break;
+      const int lc = loc[a] - PC.lo[a];
+      gc[a] = Cb.pos[a] * Lc.nb[a] + lc;
+      if (lc < 1 || lc > Lc.nb[a] - 2) inner = false;
     }
-    int gc[3] = {g[0] / 2, g[1] / 2, g[2] / 2};
-    int off[3] = {g[0] - 2 * gc[0], g[1] - 2 * gc[1], g[2] - 2 * gc[2]};
-    int cb;
-    long cc;
-    if (!amr_find(A, lev - 1, gc, cb, cc)) {  // the levels are not properly nested
-      report_error(A.status, 3, b, i, j, k, -1);
-      return;
-    }
-    const AmrBlock& Cb = A.blocks[cb];
-    const DevParams& PC = A.cls[Cb.cls];  // every block of level lev - 1 has this class
-    // the coarse cell's 3^ndim neighbourhood (centre included), as addresses
-    // of each cell's variable-0 slot; hood[(dz+1)*9 + (dy+1)*3 + (dx+1)]
+    // the 3^ndim neighbourhood (centre included), as addresses of each
+    // cell's variable-0 slot; hood[(dz+1)*9 + (dy+1)*3 + (dx+1)].  A cell
+    // one or more cells inside its block's interior on every axis has the
+    // whole neighbourhood in that block: no level lookups.
     const gdouble* hood[27];
     bool ok = true;
+    const gdouble* const cU = as_global(Cb.U) + rc.cell;
 #pragma unroll
     for (int o = 0; o < 27; ++o) {
       const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
       hood[o] = nullptr;
-      if ((P.ndim < 2 && dy != 0) || (P.ndim < 3 && dz != 0)) continue;
+      if ((ndim < 2 && dy != 0) || (ndim < 3 && dz != 0)) continue;
+      if (inner || o == 13) {
+        hood[o] = cU + ((long)dz * PC.ny + dy) * PC.nx + dx;
+        continue;
+      }
       int gn[3] = {gc[0] + dx, gc[1] + dy, gc[2] + dz};
-      amr_domain(A, lev - 1, P.ndim, gn);
+      amr_domain(A, lev, ndim, gn);
       int hb;
       long hc;
-      if (amr_find(A, lev - 1, gn, hb, hc)) hood[o] = as_global(A.blocks[hb].U) + hc;
+      if (amr_find(A, lev, gn, hb, hc)) hood[o] = as_global(A.blocks[hb].U) + hc;
       else ok = false;
     }
-    if (!ok) {
-      report_error(A.status, 3, b, i, j, k, -1);
-      return;
-    }
+    recOk[rr] = ok ? 1 : 0;
+    if (!ok) return;
     const long dOff = PC.voff[FX_DENS];
     // conserved-form value of variable v at neighbourhood cell o
     auto qv = [&](int o, int v) {
       const double x = hood[o][(long)v * PC.ncell];
       return (A.perMass >> v) & 1ULL ? hood[o][dOff] * x : x;
     };
-    // the interpolated conserved value of variable v at this fine cell
-    auto interp = [&](int v) {
+    const int densSlot = (int)(dOff / PC.ncell);
+    for (int v = 0; v < PC.nunk; ++v) {
+      if (A.onlyVar >= 0 && v != A.onlyVar && v != densSlot) continue;
       const double q0 = qv(13, v);
-      double s[3] = {0.0, 0.0, 0.0};
-      s[0] = amr_mcslope(q0, qv(12, v), qv(14, v));
-      if (P.ndim > 1) s[1] = amr_mcslope(q0, qv(10, v), qv(16, v));
-      if (P.ndim > 2) s[2] = amr_mcslope(q0, qv(4, v), qv(22, v));
+      double sl[3] = {0.0, 0.0, 0.0};
+      sl[0] = amr_mcslope(q0, qv(12, v), qv(14, v));
+      if (ndim > 1) sl[1] = amr_mcslope(q0, qv(10, v), qv(16, v));
+      if (ndim > 2) sl[2] = amr_mcslope(q0, qv(4, v), qv(22, v));
       double alpha = 1.0;
-      if (s[0] != 0.0 || s[1] != 0.0 || s[2] != 0.0) {
+      if (sl[0] != 0.0 || sl[1] != 0.0 || sl[2] != 0.0) {
         double dumax = 0.0;
-        for (int a = 0; a < P.ndim; ++a) dumax = dumax + fabs(s[a]) * 0.25;
+        for (int a = 0; a < ndim; ++a) dumax = dumax + fabs(sl[a]) * 0.25;
         double umin = q0, umax = q0;
 #pragma unroll
         for (int o = 0; o < 27; ++o) {
@@ -203,15 +300,53 @@ __global__ void __launch_bounds__(kThreads) k_amr_fill(AmrLaunch A,
         if (dumax * alpha > (umax - q0)) alpha = (umax - q0) / dumax;
         if (dumax * alpha > (q0 - umin)) alpha = (q0 - umin) / dumax;
       }
-      double q = q0;
-      for (int a = 0; a < P.ndim; ++a) q = q + (off[a] ? 0.25 : -0.25) * (s[a] * alpha);
-      return q;
+      cv[((long)v * 4 + 0) * nRec + rr] = q0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) cv[((long)v * 4 + 1 + a) * nRec + rr] = sl[a] * alpha;
+    }
+  };
+  for (int rr = blockIdx.x * blockDim.x + threadIdx.x; rr < nRec; rr += gridDim.x * blockDim.x)
+    one(rr);
+}
+
+// The listed slots of the plan (rest[0 .. n-1]): the fine guard cell from
+// its coarse record recOf[q] (k_amr_coarse), then the EOS; or the error of
+// a guard cell that no level holds (recOf[q] < 0, or an incomplete record).
+__global__ void __launch_bounds__(kThreads) k_amr_prolong(AmrLaunch A,
+                                                           const int2* __restrict__ chunks,
+                                                           const int* __restrict__ rest,
+                                                           const int* __restrict__ recOf,
+                                                           int nRest, const double* __restrict__ cv,
+                                                           const int* __restrict__ recOk,
+                                                           int nRec) {
+  auto one = [&](int q) {
+    const int slot = rest[q];
+    const int w = slot / kThreads, t = slot - w * kThreads;
+    AmrGuardCell gcl;
+    amr_guard_cell(A, chunks, w, t, gcl);
+    const AmrBlock& B = A.blocks[gcl.b];
+    const DevParams& P = A.cls[B.cls];
+    const int rr = recOf[q];
+    if (rr < 0 || !recOk[rr]) {
+      report_error(A.status, 3, gcl.b, gcl.loc[0], gcl.loc[1], gcl.loc[2], -1);
+      return;
+    }
+    gdouble* Ud = as_global(B.U);
+    const long cd = (long)gcl.r;
+    double h[3] = {0.0, 0.0, 0.0};  // the fine cell's offsets in its coarse cell
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (a < P.ndim) h[a] = (gcl.g[a] & 1) ? 0.25 : -0.25;
+    auto interp = [&](int v) {
+      double qq = cv[((long)v * 4) * nRec + rr];
+      for (int a = 0; a < P.ndim; ++a) qq = qq + h[a] * cv[((long)v * 4 + 1 + a) * nRec + rr];
+      return qq;
     };
     const double dens = interp((int)(P.voff[FX_DENS] / P.ncell));
     const int v0 = A.onlyVar >= 0 ? A.onlyVar : 0, v1 = A.onlyVar >= 0 ? A.onlyVar + 1 : P.nunk;
     for (int v = v0; v < v1; ++v) {
-      const double q = interp(v);
-      Ud[(long)v * P.ncell + cd] = (A.perMass >> v) & 1ULL ? q / dens : q;
+      const double qq = interp(v);
+      Ud[(long)v * P.ncell + cd] = (A.perMass >> v) & 1ULL ? qq / dens : qq;
     }
     if (A.onlyVar >= 0) return;
     const EosOut eo = eos_dens_ei(P, dens, Ud[P.voff[FX_VELX] + cd], Ud[P.voff[FX_VELY] + cd],
@@ -223,7 +358,9 @@ __global__ void __launch_bounds__(kThreads) k_amr_fill(AmrLaunch A,
     Ud[P.voff[FX_EINT] + cd] = eo.eint;
     Ud[P.voff[FX_GAMC] + cd] = P.gamma;
     Ud[P.voff[FX_GAME] + cd] = eo.game;
-  }
+  };
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nRest; q += gridDim.x * blockDim.x)
+    one(q);
 }
 
 // Restriction of one level onto its parent level: every interior cell of

@@ -840,6 +840,16 @@ struct Lib {
     int* dwBox = nullptr;                     // find_tile_w tables of the two prefixes
     int* dwCell = nullptr;
     int2* dchunks = nullptr;                  // guard-fill work: (block, first guard cell)
+    int2* dplan = nullptr;                    // guard-fill source of every work slot (k_amr_plan)
+    const double** dsrc = nullptr;            // the copy slots' source addresses
+    int* drest = nullptr;                     // plan slots that are not same-level copies
+    unsigned* dnRest = nullptr;               // their count (on the device)
+    unsigned nRest = 0;
+    AmrCoarseRec* drecs = nullptr;            // distinct coarse cells under prolonged guard cells
+    int nRec = 0;
+    int* drecOf = nullptr;                    // per rest slot: its record, or -1
+    double* dcv = nullptr;                    // per record and variable: q0, s_a * alpha
+    int* drecOk = nullptr;
     int* dguard = nullptr;                    // guard-cell box indices, class by class
     int* dguardOff = nullptr;                 // per class: first entry in dguard (+ total)
     long nChunks = 0;
@@ -1799,8 +1809,10 @@ int tune_flux_stages(int nTiles, double dt, int reps, fx_tune_result* out, int c
 // ---------------------------------------------------------------------
 void amr_free() {
   Lib::Amr& A = g.amr;
-  void* bufs[] = {A.dblocks, A.dlevels, A.dboxStart, A.dcellStart, A.dwBox, A.dwCell, A.flux,
-                  A.pavg, A.dchunks, A.dguard, A.dguardOff};
+  void* bufs[] = {A.dblocks, A.dlevels, A.dboxStart, A.dcellStart, A.dwBox,  A.dwCell,
+                  A.flux,    A.pavg,    A.dchunks,   A.dplan,      (void*)A.dsrc, A.drest,
+                  A.dnRest,  A.drecs,   A.drecOf,    A.dcv,        A.drecOk, A.dguard,
+                  A.dguardOff};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (int* q : A.dindex) if (q) (void)hipFree(q);
@@ -1858,7 +1870,14 @@ void amr_launch_fill(int onlyVar = -1) {
   AmrLaunch L = amr_launch();
   L.onlyVar = onlyVar;
   hipLaunchKernelGGL(k_amr_fill, dim3((unsigned)g.amr.nChunks), dim3(kThreads), 0, g.stream, L,
-                     g.amr.dchunks);
+                     g.amr.dchunks, g.amr.dsrc);
+  const Lib::Amr& A = g.amr;
+  if (A.nRec)
+    hipLaunchKernelGGL(k_amr_coarse, dim3(grid_for(A.nRec)), dim3(kThreads), 0, g.stream, L,
+                       A.drecs, A.nRec, A.dcv, A.drecOk);
+  if (A.nRest)
+    hipLaunchKernelGGL(k_amr_prolong, dim3(grid_for(A.nRest)), dim3(kThreads), 0, g.stream, L,
+                       A.dchunks, A.drest, A.drecOf, (int)A.nRest, A.dcv, A.drecOk, A.nRec);
 }
 
 // fine-to-coarse flux correction, finest level pair first
@@ -2672,6 +2691,61 @@ int fx_amr_setup(const fx_tile* tiles, const fx_amr_block* blocks, int nBlocks, 
     if (!rc && !faces.empty())
       if (hipMalloc((void**)&A.dreg[L], sizeof(double) * FX_NFLUX * faces.size()) != hipSuccess)
         rc = 4;
+  }
+  // the guard fill's sources, fixed by the layout: looked up once here
+  if (!rc && A.nChunks > 0) {
+    const size_t slots = (size_t)A.nChunks * kThreads;
+    if (hipMalloc((void**)&A.dplan, sizeof(int2) * slots) != hipSuccess ||
+        hipMalloc((void**)&A.dsrc, sizeof(double*) * slots) != hipSuccess ||
+        hipMalloc((void**)&A.drest, sizeof(int) * slots) != hipSuccess ||
+        hipMalloc((void**)&A.dnRest, sizeof(unsigned)) != hipSuccess) {
+      rc = 4;
+    } else {
+      hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, g.stream, A.dnRest);
+      hipLaunchKernelGGL(k_amr_plan, dim3((unsigned)A.nChunks), dim3(kThreads), 0, g.stream,
+                         amr_launch(), A.dchunks, A.dplan, A.dsrc, A.drest, A.dnRest);
+      if (hipMemcpyAsync(&A.nRest, A.dnRest, sizeof(unsigned), hipMemcpyDeviceToHost,
+                         g.stream) != hipSuccess ||
+          hipStreamSynchronize(g.stream) != hipSuccess)
+        rc = 4;
+    }
+    // the distinct coarse cells under the prolonged guard cells, and each
+    // prolonged slot's record
+    if (!rc && A.nRest) {
+      std::vector<int> rest(A.nRest);
+      std::vector<int2> plan(slots);
+      if (hipMemcpy(rest.data(), A.drest, sizeof(int) * A.nRest, hipMemcpyDeviceToHost) !=
+              hipSuccess ||
+          hipMemcpy(plan.data(), A.dplan, sizeof(int2) * slots, hipMemcpyDeviceToHost) !=
+              hipSuccess) {
+        rc = 4;
+      } else {
+        std::vector<std::pair<long long, int>> keys;
+        keys.reserve(A.nRest);
+        for (unsigned q = 0; q < A.nRest; ++q) {
+          const int2 pe = plan[rest[q]];
+          if (pe.x <= kPlanCoarse)
+            keys.emplace_back(((long long)(kPlanCoarse - pe.x) << 32) | (unsigned)pe.y, (int)q);
+        }
+        std::sort(keys.begin(), keys.end());
+        std::vector<int> recOf(A.nRest, -1);
+        std::vector<AmrCoarseRec> recs;
+        for (size_t e = 0; e < keys.size(); ++e) {
+          if (e == 0 || keys[e].first != keys[e - 1].first)
+            recs.push_back(AmrCoarseRec{(int)(keys[e].first >> 32),
+                                        (int)(keys[e].first & 0xffffffffLL)});
+          recOf[keys[e].second] = (int)recs.size() - 1;
+        }
+        A.nRec = (int)recs.size();
+        rc = upload_vec(&A.drecs, recs, st);
+        if (!rc) rc = upload_vec(&A.drecOf, recOf, st);
+        if (!rc && A.nRec &&
+            (hipMalloc((void**)&A.dcv, sizeof(double) * 4 * (size_t)g.prm.nunk * A.nRec) !=
+                 hipSuccess ||
+             hipMalloc((void**)&A.drecOk, sizeof(int) * (size_t)A.nRec) != hipSuccess))
+          rc = 4;
+      }
+    }
   }
   if (rc) {
     amr_free();
