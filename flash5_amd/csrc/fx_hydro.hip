@@ -814,6 +814,10 @@ struct Lib {
   size_t dwCellCap = 0;
   int* dwFace = nullptr;
   size_t dwFaceCap = 0;
+  std::vector<int2> hgateChunks;       // multi-class: (tile, first face) per kThreads faces
+  int2* dgateChunks = nullptr;
+  size_t gateChunksCap = 0;
+  int nGateChunks = 0;
   MarchItem* ditems = nullptr;         // column tiles of a multi-class march
   size_t ditemsCap = 0;
   int nItems = 0;
@@ -1232,6 +1236,17 @@ int upload_tiles(const fx_tile* tiles, int nTiles, bool needFlux, fx_status* st,
                            hipMemcpyHostToDevice, g.stream),
             st);
     g.nItems = 0;  // march items are built for the tile shape of the call
+    // the learned seed's gate pass: each tile's faces in chunks of kThreads
+    std::vector<int2>& ch = g.hgateChunks;
+    ch.clear();
+    for (int t = 0; t < nTiles; ++t)
+      for (long f = 0; f < g.hfaceStart[t + 1] - g.hfaceStart[t]; f += kThreads)
+        ch.push_back(make_int2(t, (int)f));
+    g.nGateChunks = (int)ch.size();
+    if (grow(&g.dgateChunks, &g.gateChunksCap, ch.size(), st)) return 4;
+    HIPCALL(hipMemcpyAsync(g.dgateChunks, ch.data(), sizeof(int2) * ch.size(),
+                           hipMemcpyHostToDevice, g.stream),
+            st);
   }
   return 0;
 }
@@ -1318,13 +1333,13 @@ void launch_face_solve(const DevParams& P, const FaceLaunch& F, long faces, bool
     float* y = withUpdate ? nullptr : g.seedY;
     hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, g.stream, g.strongCount);
     if (multi)
-      hipLaunchKernelGGL((k_face_gate<true, R>), gf, dim3(kThreads), 0, g.stream, P, F,
-                         make_seed_gate(g.seedGate), y, g.strongList, g.strongCount,
-                         (unsigned)faces);
+      hipLaunchKernelGGL((k_face_gate<true, R>), dim3((unsigned)g.nGateChunks), dim3(kThreads), 0,
+                         g.stream, P, F, make_seed_gate(g.seedGate), y, g.strongList,
+                         g.strongCount, (unsigned)faces, g.dgateChunks);
     else
       hipLaunchKernelGGL((k_face_gate<false, R>), gf, dim3(kThreads), 0, g.stream, P, F,
                          make_seed_gate(g.seedGate), y, g.strongList, g.strongCount,
-                         (unsigned)faces);
+                         (unsigned)faces, nullptr);
   }
   if (multi && P.learned)
     hipLaunchKernelGGL((k_face_seed<true, R>), wave_grid(faces), dim3(kLearnThreads), 0, g.stream,
@@ -1982,7 +1997,7 @@ int fx_hydro_init(const fx_params* p) {
 }
 
 void fx_hydro_finalize(void) {
-  void* bufs[] = {g.strongList, g.strongCount, g.dwCell, g.dwFace, g.states, g.flux, g.pavg, g.dtiles, g.dinplace, g.status, g.partial,
+  void* bufs[] = {g.strongList, g.strongCount, g.dwCell, g.dwFace, g.dgateChunks, g.states, g.flux, g.pavg, g.dtiles, g.dinplace, g.status, g.partial,
                   g.dregTiles, g.dfaces, g.mlpPacked, g.seedPacked, g.seedY, g.fstates,
                   g.dnbr, g.dtBuf, g.dcls, g.dtcls, g.dcellStart, g.dfaceStart,
                   g.ditems, g.dregCls, g.seedFlux, g.seedBits};

@@ -560,37 +560,25 @@ __global__ void __launch_bounds__(kThreads) k_face_gate(DevParams P0, FaceLaunch
                                                         float* __restrict__ y,
                                                         unsigned* __restrict__ list,
                                                         unsigned* __restrict__ count,
-                                                        unsigned cap) {
-  const long total = face_total<MULTI>(P0, F);
+                                                        unsigned cap,
+                                                        const int2* __restrict__ chunks) {
   const int lane = (int)(threadIdx.x & 63);
-  // the loop bound is the same for every lane of a workgroup, so whole
-  // wavefronts reach the ballot together
-  for (long g0 = (long)blockIdx.x * blockDim.x; g0 < total; g0 += (long)gridDim.x * blockDim.x) {
-    const long gid = g0 + threadIdx.x;
-    bool strong = false;
-    if (gid < total) {
-      int t;
-      long rt;
-      if constexpr (!MULTI) {  // face numbers fit 32 bits (the strong list holds them)
-        const unsigned pt = (unsigned)(P0.nface[0] + P0.nface[1] + P0.nface[2]);
-        const unsigned ut = (unsigned)gid / pt;
-        t = (int)ut;
-        rt = (long)((unsigned)gid - ut * pt);
-      }
-      const DevParams& P = MULTI ? face_tile<MULTI>(P0, F, gid, t, rt) : P0;
-      int r = (int)rt, d = 0;
-      while (d < P.ndim - 1 && r >= (int)P.nface[d]) {
-        r -= (int)P.nface[d];
-        ++d;
-      }
-      const S* b = face_states_at<S, MULTI>(P, F, t, d);
-      const long nf = P.nface[d];
-      const double rl = b[r], ul = b[(1 + d) * nf + r], pl = b[4 * nf + r];
-      const double rr = b[6 * nf + r], ur = b[(7 + d) * nf + r], pr = b[10 * nf + r];
-      strong = rl > 0. && rr > 0. && pl > 0. && pr > 0. &&
-               seed_gate_pass(G, P.gamma, rl, ul, pl, rr, ur, pr);
-      if (y) y[gid] = kNoSeed;
+  // one face: its gate test, and its seed slot reset
+  auto gate_one = [&](const DevParams& P, int t, long gid, int r) {
+    int d = 0;
+    while (d < P.ndim - 1 && r >= (int)P.nface[d]) {
+      r -= (int)P.nface[d];
+      ++d;
     }
+    const S* b = face_states_at<S, MULTI>(P, F, t, d);
+    const long nf = P.nface[d];
+    const double rl = b[r], ul = b[(1 + d) * nf + r], pl = b[4 * nf + r];
+    const double rr = b[6 * nf + r], ur = b[(7 + d) * nf + r], pr = b[10 * nf + r];
+    if (y) y[gid] = kNoSeed;
+    return rl > 0. && rr > 0. && pl > 0. && pr > 0. &&
+           seed_gate_pass(G, P.gamma, rl, ul, pl, rr, ur, pr);
+  };
+  auto append = [&](bool strong, long gid) {
     const unsigned long long mask = __ballot(strong);
     if (mask) {
       const int leader = __builtin_ctzll(mask);
@@ -599,6 +587,34 @@ __global__ void __launch_bounds__(kThreads) k_face_gate(DevParams P0, FaceLaunch
       base = __shfl(base, leader);
       const unsigned slot = base + (unsigned)__popcll(mask & ((1ULL << lane) - 1ULL));
       if (strong && slot < cap) list[slot] = (unsigned)gid;
+    }
+  };
+  if constexpr (MULTI) {
+    // workgroup w takes kThreads faces of one tile, chunks[w] = (tile, first
+    // face of the tile): the tile's class is uniform, no per-face search
+    const int2 ch = chunks[blockIdx.x];
+    const int t = ch.x;
+    const DevParams& P = F.cls[F.tcls[t]];
+    const long perTile = P.nface[0] + P.nface[1] + P.nface[2];
+    const int rt = ch.y + (int)threadIdx.x;
+    const long gid = F.faceStart[t] + rt;
+    const bool strong = rt < perTile && gate_one(P, t, gid, rt);
+    append(strong, gid);
+  } else {
+    // face numbers fit 32 bits (the strong list holds them); the loop bound
+    // is the same for every lane of a workgroup, so whole wavefronts reach
+    // the ballot together
+    const unsigned pt = (unsigned)(P0.nface[0] + P0.nface[1] + P0.nface[2]);
+    const long total = (long)pt * F.nTiles;
+    for (long g0 = (long)blockIdx.x * blockDim.x; g0 < total;
+         g0 += (long)gridDim.x * blockDim.x) {
+      const long gid = g0 + threadIdx.x;
+      bool strong = false;
+      if (gid < total) {
+        const unsigned ut = (unsigned)gid / pt;
+        strong = gate_one(P0, (int)ut, gid, (int)((unsigned)gid - ut * pt));
+      }
+      append(strong, gid);
     }
   }
 }
