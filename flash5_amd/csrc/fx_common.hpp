@@ -237,12 +237,9 @@ __device__ __forceinline__ void upd_dir(const DevParams& P, double dt, double de
   for (int q = 0; q < 5; ++q) U0[q] = U0[q] - dt / del * (FR[q] - FL[q]);
 }
 
-// x and y terms from the cell's primitive state and its four side faces
-__device__ __forceinline__ UpdPart upd_start(const DevParams& P, double dt, double dx, double dy,
-                                             double dens0, double vx, double vy, double vz,
-                                             double ener, double eint, double pres,
-                                             const double FLx[8], const double FRx[8],
-                                             const double FLy[8], const double FRy[8]) {
+// the conserved state before any flux term, from the cell's primitives
+__device__ __forceinline__ UpdPart upd_init(const DevParams& P, double dens0, double vx, double vy,
+                                            double vz, double ener, double eint, double pres) {
   UpdPart u;
   u.U0[0] = dens0;
   u.U0[1] = vx * dens0;
@@ -252,6 +249,17 @@ __device__ __forceinline__ UpdPart upd_start(const DevParams& P, double dt, doub
   u.IntEner = 0.;
   if (P.aux) u.IntEner = dens0 * eint;
   u.pres = pres;
+  u.S1 = u.S2 = 0.;
+  return u;
+}
+
+// x and y terms from the cell's primitive state and its four side faces
+__device__ __forceinline__ UpdPart upd_start(const DevParams& P, double dt, double dx, double dy,
+                                             double dens0, double vx, double vy, double vz,
+                                             double ener, double eint, double pres,
+                                             const double FLx[8], const double FRx[8],
+                                             const double FLy[8], const double FRy[8]) {
+  UpdPart u = upd_init(P, dens0, vx, vy, vz, ener, eint, pres);
   upd_dir(P, dt, dx, pres, FLx, FRx, u.U0, u.IntEner, u.S1);
   upd_dir(P, dt, dy, pres, FLy, FRy, u.U0, u.IntEner, u.S2);
   return u;

@@ -12,7 +12,10 @@
 //   k_update   one thread per interior cell: conservative update, aux
 //              internal-energy update, energy fix, EOS
 //              (hy_unsplitUpdate.F90:440-925, hy_energyFix.F90:188-251,
-//              Eos_wrapped MODE_DENS_EI)
+//              Eos_wrapped MODE_DENS_EI); 3-D tiles of one class take
+//              k_update_cols, one thread per column, instead
+// The column march (fx_march.hpp) replaces k_states + k_fluxes where the
+// block shape allows it (kernel path 0).
 // State cells of direction d are the cells whose d-coordinate lies in
 // [lo-1, hi+1] and whose other coordinates are interior: exactly the cells
 // the reference stores face states for (hy_getRiemannState.F90:553-591).
@@ -495,6 +498,64 @@ __global__ void __launch_bounds__(kThreads) k_update(DevParams P, Launch L,
        gid += (long)gridDim.x * blockDim.x) {
     const int t = (int)(gid / perTile);
     update_cell<FROM_STATES, WRITE_PAVG_ONLY>(P, L, t, gid - (long)t * perTile, inPlace);
+  }
+}
+
+// The update of 3-D tiles by columns: one thread per (tile, i, j) column
+// marches k upward and keeps the z face between two cells in registers, so
+// each z-face flux is read once (per cell, k_update reads both of its z
+// faces): 1.79 against 1.84 ms on 4096 x 16^3.  upd_start /
+// upd_finish take each quantity's x, y and z terms in update_cell's order,
+// so the bits are update_cell's.
+__global__ void __launch_bounds__(kThreads)
+k_update_cols(DevParams P, Launch L, const unsigned char* __restrict__ inPlace) {
+  const int nx = P.nxb, ny = P.nyb;
+  const long ncol = (long)nx * ny;
+  const long total = ncol * L.nTiles;
+  const double dt = step_dt(L.dt, L.dtp);
+  for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+       gid += (long)gridDim.x * blockDim.x) {
+    const int t = (int)(gid / ncol);
+    if (inPlace && !inPlace[t]) continue;
+    const int col = (int)(gid - (long)t * ncol);
+    const int i = col % nx + P.lo[0], j = col / nx + P.lo[1];
+    const fx_tile& T = L.tiles[t];
+    gdouble* const U = as_global(T.U);
+    const gdouble* const Fx = as_global(T.flux[0]);
+    const gdouble* const Fy = as_global(T.flux[1]);
+    const gdouble* const Fz = as_global(T.flux[2]);
+    const long nfx = P.nface[0], nfy = P.nface[1], nfz = P.nface[2];
+    const long zStep = (long)P.fext[2][0] * P.fext[2][1];
+    const double dx = T.del[0], dy = T.del[1], dz = T.del[2];
+    long hz = face_index(P, 2, i, j, P.lo[2]);
+    double FLz[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) FLz[q] = Fz[q * nfz + hz];
+    for (int k = P.lo[2]; k <= P.hi[2]; ++k) {
+      const long c = cidx(P, i, j, k);
+      const long f0 = face_index(P, 0, i, j, k), g0 = face_index(P, 1, i, j, k);
+      const long r = ((long)(k - P.lo[2]) * ny + (j - P.lo[1])) * nx + (i - P.lo[0]);
+      double FLx[8], FRx[8], FLy[8], FRy[8], FRz[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        FLx[q] = Fx[q * nfx + f0];
+        FRx[q] = Fx[q * nfx + f0 + 1];
+        FLy[q] = Fy[q * nfy + g0];
+        FRy[q] = Fy[q * nfy + g0 + P.fext[1][0]];
+        FRz[q] = Fz[q * nfz + hz + zStep];
+      }
+      const UpdPart part = upd_start(P, dt, dx, dy, U[P.voff[FX_DENS] + c], U[P.voff[FX_VELX] + c],
+                                     U[P.voff[FX_VELY] + c], U[P.voff[FX_VELZ] + c],
+                                     U[P.voff[FX_ENER] + c], U[P.voff[FX_EINT] + c],
+                                     P.aux ? as_global(T.pavg)[r] : 0., FLx, FRx, FLy, FRy);
+      double out[10];
+      upd_finish(P, dt, dz, part, FLz, FRz, P.aux ? 0. : U[P.voff[FX_EINT] + c], out);
+#pragma unroll
+      for (int v = 0; v < 10; ++v) U[P.voff[upd_var(v)] + c] = out[v];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) FLz[q] = FRz[q];
+      hz += zStep;
+    }
   }
 }
 
@@ -1419,6 +1480,10 @@ void launch_update(int nTiles, double dt, const unsigned char* dIn) {
     const ClassMap C{g.dcls, g.dtcls, g.dcellStart, g.dwCell};
     hipLaunchKernelGGL(k_update_multi, dim3(grid_for(g.hcellStart[nTiles])), dim3(kThreads), 0,
                        g.stream, L, C, dIn);
+  } else if (g.cur.ndim == 3) {
+    hipLaunchKernelGGL(k_update_cols,
+                       dim3(grid_for((long)g.cur.nxb * g.cur.nyb * nTiles)), dim3(kThreads), 0,
+                       g.stream, g.cur, L, dIn);
   } else {
     hipLaunchKernelGGL((k_update<false, false>), dim3(grid_for(g.cur.icells * nTiles)),
                        dim3(kThreads), 0, g.stream, g.cur, L, dIn);
@@ -2094,8 +2159,12 @@ int fx_hydro_advance_batch_noalloc(double* U, long nBlocks, long blockStride,
   if (P.riemann == FX_HLLC) launch_march<FX_HLLC>(P, M, TX, grid);
   else launch_march<FX_HYBR>(P, M, TX, grid);
   Launch L{tiles, (int)nBlocks, nullptr, status, dt, nullptr};
-  hipLaunchKernelGGL((k_update<false, false>), dim3(grid_for(P.icells * nBlocks)), dim3(kThreads),
-                     0, g.stream, P, L, nullptr);
+  if (P.ndim == 3)
+    hipLaunchKernelGGL(k_update_cols, dim3(grid_for((long)P.nxb * P.nyb * nBlocks)),
+                       dim3(kThreads), 0, g.stream, P, L, nullptr);
+  else
+    hipLaunchKernelGGL((k_update<false, false>), dim3(grid_for(P.icells * nBlocks)),
+                       dim3(kThreads), 0, g.stream, P, L, nullptr);
   HIPCALL(hipGetLastError(), st);
   return finish_from(status, st);
 }
