@@ -7,9 +7,10 @@
 // Per z-plane k the workgroup computes, each value once (the work of
 // hy_getRiemannState.F90:356-1327 regrouped by quantity instead of by cell):
 //   stage A  transverse fluxes sigma_x, sigma_y of plane k+1 into 3-plane LDS
-//            rings (they feed the 3-D CTU terms of planes k and k+2),
-//            sigma_z of plane k, limited characteristic slopes along x and y
-//            of plane k, and along z (with the z flattening tilde) of plane
+//            rings (they feed the 3-D CTU terms of planes k and k+2) and,
+//            from the same stencils, the limited characteristic slopes
+//            along x and y of plane k+1 into 2-plane rings; sigma_z of
+//            plane k; slopes along z (with the z flattening tilde) of plane
 //            k+1 into 3-plane rings
 //   stage B1 x-, y- and z-face states of plane k: PPM with characteristic
 //            tracing, 2-D CTU transverse corrections, positivity fallback
@@ -40,6 +41,10 @@ template <int TX, int TY> struct TileIdx {
   static __device__ __forceinline__ int ly(int a, int b) { return (b + 2) * TX + a; }
   static constexpr int NRX = TX * (TY + 2), NRY = (TX + 2) * TY, NC = (TX + 2) * (TY + 2);
   static constexpr int NLX = (TX + 4) * TY, NLY = TX * (TY + 4);
+  // stage-A items along x: the cells of R_x and of the x slopes, rows
+  // b = -1, [0,TY), TY; along y: the cells of R_y and of the y slopes,
+  // rows b = -2, -1, [0,TY), TY, TY+1
+  static constexpr int NUX = 2 * TX + (TX + 4) * TY, NUY = 4 * TX + (TX + 2) * TY;
 };
 
 // LDS of one column tile (march_tile), as offsets into one array of R
@@ -49,9 +54,10 @@ template <int NDIM, int TX, int TY> struct MarchLds {
   static constexpr int SX = 0;                                // sigma_x ring
   static constexpr int SY = SX + NR * 6 * I::NRX;             // sigma_y ring
   static constexpr int SZ = SY + NR * 6 * I::NRY;             // sigma_z of plane k
-  static constexpr int LX = SZ + (Z ? 6 * I::NC : 1);         // slopes along x, plane k
-  static constexpr int LY = LX + 6 * I::NLX;                  // slopes along y, plane k
-  static constexpr int LZ = LY + 6 * I::NLY;                  // slopes along z, ring
+  static constexpr int NRL = NDIM == 3 ? 2 : 1;               // slope ring depth
+  static constexpr int LX = SZ + (Z ? 6 * I::NC : 1);         // slopes along x, ring
+  static constexpr int LY = LX + NRL * 6 * I::NLX;            // slopes along y, ring
+  static constexpr int LZ = LY + NRL * 6 * I::NLY;            // slopes along z, ring
   static constexpr int FZ = LZ + (Z ? 3 * 6 * NCOL : 1);      // flattening tilde along z, ring
   static constexpr int WX = FZ + (Z ? 3 * NCOL : 1);          // x-face states (Wp, Wn)
   static constexpr int WY = WX + 12 * I::NRY;                 // y-face states
@@ -381,6 +387,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
   bool bad = false;  // eigen failure seen by this thread
 
   auto ring = [](int k) { return NR == 3 ? (k % 3) : 0; };
+  auto ringL = [](int k) { return M::NRL == 2 ? (k & 1) : 0; };
 
   // stage-A items: sigma_x (w < NRX) and sigma_y (w < NRY) of plane kk
   auto sig_x_item = [&](int w, int kk) {
@@ -394,6 +401,79 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
     const int a = w % (TX + 2) - 1, b = w / (TX + 2);
     if (!tv_sigma<1>(P, tv, cell_b(P, ox + a, oy + b, kk), syb, sg)) { bad = true; return; }
     lds_put6(SY + ring(kk) * 6 * I::NRY, I::NRY, w, sg);
+  };
+  // stage-A items along x (w < NUX) and y (w < NUY) at plane p: one cell's
+  // three-cell stencil and eigen system serve its sigma (sig: p is a plane
+  // the sigma ring takes, and the cell lies in R_x / R_y) and its limited
+  // slope (sl: p is interior, and the cell lies in the slope rows)
+  auto x_item = [&](int w, int p, bool sig, bool sl) {
+    int a, b;
+    if (w < TX) {
+      a = w;
+      b = -1;
+    } else if (w < TX + (TX + 4) * TY) {
+      const int u = w - TX;
+      a = u % (TX + 4) - 2;
+      b = u / (TX + 4);
+    } else {
+      a = w - TX - (TX + 4) * TY;
+      b = TY;
+    }
+    const bool doS = sig && a >= 0 && a < TX;
+    const bool doL = sl && b >= 0 && b < TY;
+    if (!doS && !doL) return;
+    const unsigned cb = cell_b(P, ox + a, oy + b, p);
+    R Vm[8], Vc[8], Vp[8];
+    tv_prim(tv, cb - sxb, Vm);
+    tv_prim(tv, cb, Vc);
+    tv_prim(tv, cb + sxb, Vp);
+    EigT<R> e;
+    if (!eig_make<0>(Vc, e)) { bad = true; return; }
+    if (doS) {
+      R sg[8];
+      trans_sigma<0>(P, Vm, Vc, Vp, e, sg);
+      lds_put6(SX + ring(p) * 6 * I::NRX, I::NRX, I::rx(a, b), sg);
+    }
+    if (doL) {
+      R db[8];
+      tvd_slope<0>(P, Vm, Vc, Vp, e, db);
+      lds_put6(LX + ringL(p) * 6 * I::NLX, I::NLX, I::lx(a, b), db);
+    }
+  };
+  auto y_item = [&](int w, int p, bool sig, bool sl) {
+    int a, b;
+    if (w < 2 * TX) {
+      a = w % TX;
+      b = w / TX - 2;
+    } else if (w < 2 * TX + (TX + 2) * TY) {
+      const int u = w - 2 * TX;
+      a = u % (TX + 2) - 1;
+      b = u / (TX + 2);
+    } else {
+      const int u = w - 2 * TX - (TX + 2) * TY;
+      a = u % TX;
+      b = TY + u / TX;
+    }
+    const bool doS = sig && b >= 0 && b < TY;
+    const bool doL = sl && a >= 0 && a < TX;
+    if (!doS && !doL) return;
+    const unsigned cb = cell_b(P, ox + a, oy + b, p);
+    R Vm[8], Vc[8], Vp[8];
+    tv_prim(tv, cb - syb, Vm);
+    tv_prim(tv, cb, Vc);
+    tv_prim(tv, cb + syb, Vp);
+    EigT<R> e;
+    if (!eig_make<1>(Vc, e)) { bad = true; return; }
+    if (doS) {
+      R sg[8];
+      trans_sigma<1>(P, Vm, Vc, Vp, e, sg);
+      lds_put6(SY + ring(p) * 6 * I::NRY, I::NRY, I::ry(a, b), sg);
+    }
+    if (doL) {
+      R db[8];
+      tvd_slope<1>(P, Vm, Vc, Vp, e, db);
+      lds_put6(LY + ringL(p) * 6 * I::NLY, I::NLY, I::ly(a, b), db);
+    }
   };
   // stage-A item: z-slope and z flattening tilde of column w at plane m
   auto zslope_item = [&](int w, int m) {
@@ -425,72 +505,48 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
   for (int k = kBeg; k <= kEnd; ++k) {
     const bool interior = !Z || (k >= klo && k <= khi);
     // ------------------------------ stage A ------------------------------
+    // Plane k+1's sigma_x/y (the ring's next plane) and x/y slopes (the
+    // slope ring's next plane), plane k's sigma_z, plane k+1's z slopes.
+    // (2-D: plane 0's sigmas and slopes.)
     {
-      const int nSig = (!Z || k + 1 <= kEnd) ? I::NRX + I::NRY : 0;
+      const int pA = Z ? k + 1 : 0;
+      const bool wSig = !Z || k + 1 <= kEnd;
+      const bool wL = !Z || (k + 1 >= klo && k + 1 <= khi);
+      const bool uXY = wSig || wL;
       const int nSz = (Z && interior) ? I::NC : 0;
-      const int nL = interior ? I::NLX + I::NLY : 0;
       const int nZs = Z ? NCOL : 0;
-      const int nA = nSig + nSz + nL + nZs;
-      constexpr int PX = wave_up(I::NRX), PY = wave_up(I::NRY), PZ = wave_up(I::NC);
-      constexpr int PLX = wave_up(I::NLX), PLY = wave_up(I::NLY);
-      // Each stage's work item is a lambda whose early exits are returns,
-      // not loop continues: the compiler then allocates registers per item
-      // type.  With loop-level continues the FP64 16x8 march spilled 272 B
-      // per lane at 256 VGPRs; as lambdas it needs 251 and spills nothing
-      // (FP64 march 11.7 -> 9.3 ms on 4096 x 16^3, FP32 10.4 -> 7.7 ms).
-      auto itemA = [&](int w) {
-        int r = w;
-        if (r < nSig) { if (r < I::NRX) sig_x_item(r, Z ? k + 1 : 0); else sig_y_item(r - I::NRX, Z ? k + 1 : 0); return; }
-        r -= nSig;
-        if (r < nSz) {
+      constexpr int PUX = wave_up(I::NUX), PUY = wave_up(I::NUY), PZ = wave_up(I::NC);
+      // Segments in wave-aligned order (x items, y items, sigma_z, z
+      // slopes): one item kind per wavefront.  Each stage's work item is a
+      // lambda whose early exits are returns, not loop continues: the
+      // compiler then allocates registers per item kind.  With loop-level
+      // continues the FP64 16x8 march spilled 272 B per lane at 256 VGPRs;
+      // as lambdas it needs 251 and spills nothing (FP64 march 11.7 -> 9.3
+      // ms on 4096 x 16^3, FP32 10.4 -> 7.7 ms).  One x item serves a
+      // cell's sigma_x and x slope (one stencil, one eigen system), which
+      // keeps the slopes of two planes in LDS.
+      const int s1 = uXY ? PUX + PUY : 0, s2 = s1 + (nSz ? PZ : 0), sEnd = s2 + nZs;
+      auto itemA = [&](int v) {
+        if (v < s1) {
+          if (v < PUX) {
+            if (v < I::NUX) x_item(v, pA, wSig, wL);
+          } else if (v - PUX < I::NUY) {
+            y_item(v - PUX, pA, wSig, wL);
+          }
+          return;
+        }
+        if (v < s2) {
+          const int r = v - s1;
+          if (r >= I::NC) return;
           R v8[8];
           const int a = r % (TX + 2) - 1, b = r / (TX + 2) - 1;
           if (!tv_sigma<2>(P, tv, cell_b(P, ox + a, oy + b, k), szb, v8)) { bad = true; return; }
           lds_put6(SZ, I::NC, r, v8);
           return;
         }
-        r -= nSz;
-        if (r < nL) {
-          R v8[8];
-          if (r < I::NLX) {
-            const int a = r % (TX + 4) - 2, b = r / (TX + 4);
-            if (!tv_slope<0>(P, tv, cell_b(P, ox + a, oy + b, k), sxb, v8)) { bad = true; return; }
-            lds_put6(LX, I::NLX, r, v8);
-          } else {
-            r -= I::NLX;
-            const int a = r % TX, b = r / TX - 2;
-            if (!tv_slope<1>(P, tv, cell_b(P, ox + a, oy + b, k), syb, v8)) { bad = true; return; }
-            lds_put6(LY, I::NLY, r, v8);
-          }
-          return;
-        }
-        r -= nL;
-        zslope_item(r, k + 1);
+        zslope_item(v - s2, k + 1);
       };
-      // wave-aligned segments (sigma_x, sigma_y, sigma_z, x slopes, y
-      // slopes, z slopes): one item type per wavefront, three passes of NT
-      // in place of two passes that mixed types in five of the waves
-      {
-        const int s1 = nSig ? PX + PY : 0, s2 = s1 + (nSz ? PZ : 0), s3 = s2 + (nL ? PLX + PLY : 0);
-        const int sEnd = s3 + nZs;
-        for (int v = tid; v < sEnd; v += NT) {
-          int w;
-          if (v < s1) {
-            if (v < PX) { if (v >= I::NRX) continue; w = v; }
-            else { if (v - PX >= I::NRY) continue; w = I::NRX + (v - PX); }
-          } else if (v < s2) {
-            if (v - s1 >= I::NC) continue;
-            w = nSig + (v - s1);
-          } else if (v < s3) {
-            const int u = v - s2;
-            if (u < PLX) { if (u >= I::NLX) continue; w = nSig + nSz + u; }
-            else { if (u - PLX >= I::NLY) continue; w = nSig + nSz + I::NLX + (u - PLX); }
-          } else {
-            w = nSig + nSz + nL + (v - s3);
-          }
-          itemA(w);
-        }
-      }
+      for (int v = tid; v < sEnd; v += NT) itemA(v);
     }
     __syncthreads();
 
@@ -512,9 +568,10 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           tv_prim(tv, cb, Vc);
           tv_prim(tv, cb + sxb, Vp);
           const R flat = P.flat ? tv_flat_coeff<0, R>(tv, cb, sxb) : R(0.);
-          lds_get6(LX, I::NLX, I::lx(a - 1, b), dbN);
-          lds_get6(LX, I::NLX, I::lx(a, b), db0);
-          lds_get6(LX, I::NLX, I::lx(a + 1, b), dbP);
+          const R* lxk = LX + ringL(k) * 6 * I::NLX;
+          lds_get6(lxk, I::NLX, I::lx(a - 1, b), dbN);
+          lds_get6(lxk, I::NLX, I::lx(a, b), db0);
+          lds_get6(lxk, I::NLX, I::lx(a + 1, b), dbP);
           if (!eig_make<0>(Vc, e0)) { bad = true; return; }
           const R dmm = P.steep ? R(ld_at(tv.v[P_DENS], cb - 2 * sxb)) : R(0.);
           const R dpp = P.steep ? R(ld_at(tv.v[P_DENS], cb + 2 * sxb)) : R(0.);
@@ -538,9 +595,10 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           tv_prim(tv, cb, Vc);
           tv_prim(tv, cb + syb, Vp);
           const R flat = P.flat ? tv_flat_coeff<1, R>(tv, cb, syb) : R(0.);
-          lds_get6(LY, I::NLY, I::ly(a, b - 1), dbN);
-          lds_get6(LY, I::NLY, I::ly(a, b), db0);
-          lds_get6(LY, I::NLY, I::ly(a, b + 1), dbP);
+          const R* lyk = LY + ringL(k) * 6 * I::NLY;
+          lds_get6(lyk, I::NLY, I::ly(a, b - 1), dbN);
+          lds_get6(lyk, I::NLY, I::ly(a, b), db0);
+          lds_get6(lyk, I::NLY, I::ly(a, b + 1), dbP);
           if (!eig_make<1>(Vc, e0)) { bad = true; return; }
           const R dmm = P.steep ? R(ld_at(tv.v[P_DENS], cb - 2 * syb)) : R(0.);
           const R dpp = P.steep ? R(ld_at(tv.v[P_DENS], cb + 2 * syb)) : R(0.);
