@@ -75,9 +75,22 @@ __device__ __forceinline__ void report_error(DevStatus* s, int code, int tile, i
   }
 }
 
+// Adds a thread's Newton counters to the status.  A whole wavefront sums
+// its lanes first and adds once per counter: one atomic per lane on one
+// address serialises (a face stage with a thread per face spent a second
+// on them).
 __device__ __forceinline__ void flush_exact(DevStatus* s, const ExactStats& es) {
-  if (es.iters) atomicAdd(&s->newtonIters, es.iters);
-  if (es.fallbacks) atomicAdd(&s->newtonFallbacks, es.fallbacks);
+  unsigned long long it = es.iters, fb = es.fallbacks;
+  if (__activemask() == ~0ULL) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      it += __shfl_xor(it, o);
+      fb += __shfl_xor(fb, o);
+    }
+    if ((threadIdx.x & 63) != 0) return;
+  }
+  if (it) atomicAdd(&s->newtonIters, it);
+  if (fb) atomicAdd(&s->newtonFallbacks, fb);
 }
 
 __device__ __forceinline__ long cidx(const DevParams& P, int i, int j, int k) {
