@@ -412,20 +412,21 @@ __global__ void __launch_bounds__(kThreads) k_amr_restrict(AmrLaunch A) {
   }
 }
 
-// per-workgroup max of (|v_d| + c) / del_d over the leaf cells of every
-// level: interior cells that no block of the next finer level covers
-__global__ void __launch_bounds__(kThreads) k_amr_dt(AmrLaunch A, double* __restrict__ partial) {
-  __shared__ double red[kThreads];
-  const long total = A.start[A.nBlocks];
-  double m = 0.;
-  for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
-       gid += (long)gridDim.x * blockDim.x) {
-    const int b = find_tile_w(A.start, A.wtab, A.nBlocks, gid);
-    const int r = (int)(gid - A.start[b]);
-    const AmrBlock& B = A.blocks[b];
-    const DevParams& P = A.cls[B.cls];
-    const AmrLevelDev& L = A.levels[B.level];
-    const int nxi = P.nxb, nyi = P.ndim > 1 ? P.nyb : 1;
+// the CFL dt of each block (one workgroup per block, as dt_pick_block
+// describes) over its leaf cells: interior cells that no block of the next
+// finer level covers.  A block whose cells are all covered gets 1e300.
+// The CFL is cflDev[0] when cflDev is set, else cfl.
+__global__ void __launch_bounds__(kThreads) k_amr_dt(AmrLaunch A, double cfl, const double* cflDev,
+                                                     double* __restrict__ dtBlock) {
+  const double c = cflDev ? cflDev[0] : cfl;
+  const int b = blockIdx.x;
+  const AmrBlock& B = A.blocks[b];
+  const DevParams& P = A.cls[B.cls];
+  const AmrLevelDev& L = A.levels[B.level];
+  const int nxi = P.nxb, nyi = P.ndim > 1 ? P.nyb : 1, n = (int)P.icells;
+  const gdouble* U = as_global(B.U);
+  DtPick best = dt_pick_none();
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
     const int ii = r % nxi, jj = (r / nxi) % nyi, kk = r / (nxi * nyi);
     if (B.level + 1 < A.nLevels) {
       // Driver_computeDt.F90:219 takes leaf blocks only: skip a cell that
@@ -436,19 +437,13 @@ __global__ void __launch_bounds__(kThreads) k_amr_dt(AmrLaunch A, double* __rest
       const int fz = P.ndim > 2 ? 2 * (B.pos[2] * P.nzb + kk) / F.nb[2] : 0;
       if (F.index[((long)fz * F.ext[1] + fy) * F.ext[0] + fx] >= 0) continue;
     }
-    const long c = cidx(P, ii + P.lo[0], jj + P.lo[1], kk + P.lo[2]);
-    const gdouble* U = as_global(B.U);
-    const double cs = sqrt(U[P.voff[FX_GAMC] + c] * U[P.voff[FX_PRES] + c] / U[P.voff[FX_DENS] + c]);
-    double dtl = (fabs(U[P.voff[FX_VELX] + c] - 0.) + cs) * (1. / L.del[0]);
-    if (P.ndim > 1) dtl = fmax(dtl, (fabs(U[P.voff[FX_VELY] + c] - 0.) + cs) * (1. / L.del[1]));
-    if (P.ndim > 2) dtl = fmax(dtl, (fabs(U[P.voff[FX_VELZ] + c] - 0.) + cs) * (1. / L.del[2]));
-    m = fmax(m, dtl);
+    const long cc = cidx(P, ii + P.lo[0], jj + P.lo[1], kk + P.lo[2]);
+    const double cs = sqrt(U[P.voff[FX_GAMC] + cc] * U[P.voff[FX_PRES] + cc] / U[P.voff[FX_DENS] + cc]);
+    double dtl = (fabs(U[P.voff[FX_VELX] + cc] - 0.) + cs) * (1. / L.del[0]);
+    if (P.ndim > 1) dtl = fmax(dtl, (fabs(U[P.voff[FX_VELY] + cc] - 0.) + cs) * (1. / L.del[1]));
+    if (P.ndim > 2) dtl = fmax(dtl, (fabs(U[P.voff[FX_VELZ] + cc] - 0.) + cs) * (1. / L.del[2]));
+    dt_pick_cell(best, dtl, c, r);
   }
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int s = kThreads / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+  const double dt = dt_pick_block(best, c);
+  if (threadIdx.x == 0) dtBlock[b] = dt;
 }

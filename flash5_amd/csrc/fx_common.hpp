@@ -320,4 +320,47 @@ __device__ __forceinline__ int upd_var(int v) {
   return slots[v];
 }
 
+// ---- the CFL dt of one block (Hydro_computeDt.F90:186-283) ----
+// The reference's cell loop keeps a cell's (|v_d| + c) / del_d when its
+// product with the CFL exceeds the kept one's product, so a block's value
+// is that of the first cell in loop order (i fastest) whose product rounds
+// to the largest product, and the block's dt is the CFL over that value.
+// Of two cells whose products round alike the later one is never taken,
+// even when its own value is larger; an exact maximum would differ from
+// the reference in the last place of dt there.
+struct DtPick {
+  double prod, dtl;
+  int idx;  // interior cell index in loop order; INT_MAX: none taken
+};
+
+__device__ __forceinline__ DtPick dt_pick_none() { return DtPick{0., 0., 0x7fffffff}; }
+
+// one lane's cells come in increasing order: the strict test keeps the first
+__device__ __forceinline__ void dt_pick_cell(DtPick& a, double dtl, double cfl, int idx) {
+  const double prod = dtl * cfl;
+  if (prod > a.prod) a = DtPick{prod, dtl, idx};
+}
+
+// the workgroup's pick (kThreads lanes): the largest product, then the
+// lowest cell index; lane 0 returns the block's dt (1e300 when no cell had
+// a positive value, as the reference's huge(1.0))
+__device__ __forceinline__ double dt_pick_block(DtPick a, double cfl) {
+  __shared__ double sp[kThreads], sd[kThreads];
+  __shared__ int si[kThreads];
+  const int t = threadIdx.x;
+  sp[t] = a.prod;
+  sd[t] = a.dtl;
+  si[t] = a.idx;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (t < s && (sp[t + s] > sp[t] || (sp[t + s] == sp[t] && si[t + s] < si[t]))) {
+      sp[t] = sp[t + s];
+      sd[t] = sd[t + s];
+      si[t] = si[t + s];
+    }
+    __syncthreads();
+  }
+  return sp[0] > 0. ? cfl / sd[0] : 1.e300;
+}
+
 }  // namespace fx

@@ -570,69 +570,70 @@ __global__ void __launch_bounds__(kThreads) k_update_multi(Launch L, ClassMap M,
   }
 }
 
-// per-workgroup max of (|v_d| + c)/del_d over interior cells
+// the CFL dt of each block (one workgroup per block), as dt_pick_block
+// describes; the CFL is cflDev[0] when cflDev is set (a lowered CFL kept on
+// the device), else cfl
 __global__ void __launch_bounds__(kThreads) k_dt(DevParams P, const double* __restrict__ Ubase,
-                                                 long nBlocks, long stride, double idx, double idy,
-                                                 double idz, double* __restrict__ partial) {
-  __shared__ double red[kThreads];
-  const long perTile = P.icells;
-  const long total = perTile * nBlocks;
-  const int nxi = P.nxb, nyi = P.ndim > 1 ? P.nyb : 1;
-  double m = 0.;
-  for (long gid = (long)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
-       gid += (long)gridDim.x * blockDim.x) {
-    const long b = gid / perTile;
-    const long r = gid - b * perTile;
-    const int i = (int)(r % nxi) + P.lo[0];
-    const int j = (int)((r / nxi) % nyi) + P.lo[1];
-    const int k = (int)(r / ((long)nxi * nyi)) + P.lo[2];
-    const double* U = Ubase + b * stride;
-    const long c = cidx(P, i, j, k);
-    const double snd2 = U[P.voff[FX_GAMC] + c] * U[P.voff[FX_PRES] + c] / U[P.voff[FX_DENS] + c];
+                                                 long stride, double idx, double idy, double idz,
+                                                 double cfl, const double* cflDev,
+                                                 double* __restrict__ dtBlock) {
+  const double c = cflDev ? cflDev[0] : cfl;
+  const double* U = Ubase + (long)blockIdx.x * stride;
+  const int nxi = P.nxb, nyi = P.ndim > 1 ? P.nyb : 1, n = (int)P.icells;
+  DtPick best = dt_pick_none();
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    const int i = r % nxi + P.lo[0];
+    const int j = (r / nxi) % nyi + P.lo[1];
+    const int k = r / (nxi * nyi) + P.lo[2];
+    const long cc = cidx(P, i, j, k);
+    const double snd2 = U[P.voff[FX_GAMC] + cc] * U[P.voff[FX_PRES] + cc] / U[P.voff[FX_DENS] + cc];
     const double cs = sqrt(snd2);
-    double dtl = (fabs(U[P.voff[FX_VELX] + c] - 0.) + cs) * idx;
-    if (P.ndim > 1) dtl = fmax(dtl, (fabs(U[P.voff[FX_VELY] + c] - 0.) + cs) * idy);
-    if (P.ndim > 2) dtl = fmax(dtl, (fabs(U[P.voff[FX_VELZ] + c] - 0.) + cs) * idz);
-    m = fmax(m, dtl);
+    double dtl = (fabs(U[P.voff[FX_VELX] + cc] - 0.) + cs) * idx;
+    if (P.ndim > 1) dtl = fmax(dtl, (fabs(U[P.voff[FX_VELY] + cc] - 0.) + cs) * idy);
+    if (P.ndim > 2) dtl = fmax(dtl, (fabs(U[P.voff[FX_VELZ] + cc] - 0.) + cs) * idz);
+    dt_pick_cell(best, dtl, c, r);
   }
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int s = kThreads / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+  const double dt = dt_pick_block(best, c);
+  if (threadIdx.x == 0) dtBlock[blockIdx.x] = dt;
 }
 
-// dt = cfl / max of the partial maxima, kept on the device; t += dt.  With
-// cflDev set, the CFL is cflDev[0] (shockLowerCFL's CFL of this step).
-__global__ void __launch_bounds__(kThreads) k_dt_final(const double* __restrict__ partial, int n,
-                                                       double cfl, const double* cflDev,
+// dt = the minimum of the blocks' dts (Hydro_computeDt.F90:284-288), kept on
+// the device; t += dt
+__global__ void __launch_bounds__(kThreads) k_dt_final(const double* __restrict__ dtBlock, int n,
                                                        double* dtDev, double* tDev) {
   __shared__ double red[kThreads];
-  double m = 0.;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, partial[i]);
+  double m = 1.e300;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmin(m, dtBlock[i]);
   red[threadIdx.x] = m;
   __syncthreads();
   for (int s = kThreads / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    const double c = cflDev ? cflDev[0] : cfl;
-    const double dt = red[0] != 0. ? c / red[0] : 1.e300;
-    dtDev[0] = dt;
-    tDev[0] = tDev[0] + dt;
+    dtDev[0] = red[0];
+    tDev[0] = tDev[0] + red[0];
   }
 }
 
-// shockLowerCFL (Hydro.F90:104-108 and 128-141, hy_shockDetectBlk.F90:
-// 228-240): each step resets the CFL to cfl before the detector, a flagged
-// cell lowers it to min(cfl, cflMax), and the recovery before the fluxes
-// raises it by 1.2 up to cfl, so the next dt's CFL is min(cfl, 1.2 cflMax)
-// after a step with a flagged cell and cfl otherwise.
-__global__ void k_shock_cfl(const DevStatus* s, double cfl, double cap, double* cflNext) {
-  cflNext[0] = s->shockTiles ? fmin(cfl, cap) : cfl;
+// hy_cfl from one graph-captured step to the next, launched at the end of
+// each step.  cfl[0] holds the CFL of the next dt, cfl[1] the run's fallback
+// count at the end of the previous step.  A step resets the CFL to orig
+// before the detector when the detector runs (Hydro.F90:104-108), a flagged
+// cell lowers it to shockCap under shockLowerCFL (hy_shockDetectBlk.F90:
+// 228-240), the recovery before the fluxes raises it by 1.2 up to orig
+// (Hydro.F90:128-141), and under fallbackLowerCFL a state that fell back to
+// first order lowers it to fallbackCap (hy_getRiemannState.F90:934-941,
+// 1303-1319).  A cap of 0 is off.
+__global__ void k_step_cfl(const DevStatus* s, double orig, int detect, double shockCap,
+                           double fallbackCap, double* cfl) {
+  double c = detect ? orig : cfl[0];
+  if (shockCap > 0. && s->shockTiles) c = fmin(c, shockCap);
+  c = 1.2 * c < orig ? 1.2 * c : orig;
+  const double fb = (double)s->fallbacks;
+  if (fallbackCap > 0. && fb != cfl[1]) c = fmin(c, fallbackCap);
+  cfl[1] = fb;
+  cfl[0] = c;
 }
 
 __global__ void k_zero_shock_tiles(DevStatus* s) { s->shockTiles = 0; }
@@ -838,7 +839,7 @@ struct Lib {
   unsigned* seedBits = nullptr;        // one bit per face: flux in seedFlux (kept all zero between steps)
   size_t seedBitsCap = 0;
   const double* dtDev = nullptr;       // set while capturing steps whose dt lives on the device
-  double* dtBuf = nullptr;             // [dt, t, next dt's CFL] of graph-captured runs
+  double* dtBuf = nullptr;             // [dt, t, next dt's CFL, fallbacks so far] of graph runs
   double runCfl = 0.;                  // the CFL argument of the graph-captured run
   hipStream_t capStream = nullptr;     // library stream for graph capture (default priority)
   std::vector<int> hnbr;               // neighbour table last uploaded (host copy)
@@ -901,9 +902,7 @@ struct Lib {
     AmrLevelDev* dlevels = nullptr;
     std::vector<int*> dindex;                 // per level
     long* dboxStart = nullptr;                // guard-inclusive cells before each block
-    long* dcellStart = nullptr;               // interior cells before each block
-    int* dwBox = nullptr;                     // find_tile_w tables of the two prefixes
-    int* dwCell = nullptr;
+    int* dwBox = nullptr;                     // find_tile_w table of the prefix
     int2* dchunks = nullptr;                  // guard-fill work: (block, first guard cell)
     int2* dplan = nullptr;                    // guard-fill source of every work slot (k_amr_plan)
     const double** dsrc = nullptr;            // the copy slots' source addresses
@@ -924,7 +923,7 @@ struct Lib {
     std::vector<fx_fluxreg_face*> dfaces;     // per level pair (coarse level L, L+1)
     std::vector<int> nfaces;
     std::vector<double*> dreg;
-    long boxCells = 0, cells = 0;
+    long boxCells = 0;
     double* flux = nullptr;                   // the grid's own flux and pavg arrays
     double* pavg = nullptr;
     unsigned long gen = 0;                    // upload generation of the grid's tile list
@@ -1080,6 +1079,7 @@ int validate(const fx_params* p) {
     return 3;
   if (p->precision != FX_PREC_FP64 && p->precision != FX_PREC_MIXED) return 3;
   if (p->learnedSeed && p->riemann != FX_EXACT) return 3;
+  if (p->fallbackLowerCFL && !(p->cflFallbackFactor > 0.0)) return 3;
   if (shock_on(*p)) {  // SHOK_VAR: its own slot of U (hy_shockDetectBlk.F90:119-122)
     if (p->shokVar < 0 || p->shokVar >= p->nunk) return 3;
     for (int v = 0; v < FX_NVAR_USED; ++v)
@@ -1092,6 +1092,18 @@ int validate(const fx_params* p) {
 
 // hy_shockDetectBlk.F90:228-235: the CFL cap of a step with a flagged cell
 double shock_cfl_max(int ndim) { return ndim == 1 ? 0.60 : (ndim == 2 ? 0.45 : 0.25); }
+
+// hy_fallbackLowerCFL: the CFL cap of a step in which a state fell back to
+// first order, hy_cflFallbackFactor / NDIM (hy_dataReconstOneStep.F90:
+// 1125-1129, hy_getRiemannState.F90:1473); 0 when off.  In 1-D the
+// reference never copies the lowered cell CFL into hy_cfl (the assignment
+// at hy_getRiemannState.F90:934-941 sits under NDIM > 1).
+double fallback_cfl_cap(const fx_params& p) {
+  return p.fallbackLowerCFL && p.ndim > 1 ? p.cflFallbackFactor / double(p.ndim) : 0.0;
+}
+
+// a CFL lowered by the step's flags or fallbacks
+bool cfl_varies(const fx_params& p) { return p.shockLowerCFL || fallback_cfl_cap(p) > 0.0; }
 
 // Synchronise and copy the device status record ds into st (nothing in
 // async mode, where the call returns before its kernels end).
@@ -1112,6 +1124,7 @@ int finish_from(const DevStatus* ds, fx_status* st) {
     st->newton_fallbacks = (long long)h.newtonFallbacks;
     // hy_shockDetectBlk.F90:211-227: the CFL cap of a step with a flagged cell
     if (g.prm.shockLowerCFL && h.shockTiles) st->min_cell_cfl = shock_cfl_max(g.prm.ndim);
+    if (h.fallbacks) st->fallback_cfl = fallback_cfl_cap(g.prm);
   }
   return h.code;
 }
@@ -1158,19 +1171,20 @@ NoAllocLayout noalloc_layout(const DevParams& P, long nBlocks) {
   return L;
 }
 
-// Inside a graph-captured run under shockLowerCFL: the device slot holding
-// the next dt's CFL, or null (dt takes the CFL argument).
-const double* shock_cfl_dev() {
-  return g.prm.shockLowerCFL && g.dtDev ? g.dtDev + 2 : nullptr;
-}
+// Inside a graph-captured run under shockLowerCFL or fallbackLowerCFL: the
+// device slot holding the next dt's CFL, or null (dt takes the CFL argument).
+const double* step_cfl_dev() { return cfl_varies(g.prm) && g.dtDev ? g.dtDev + 2 : nullptr; }
 
-// Inside a graph-captured run under shockLowerCFL: the next dt's CFL from
-// this step's flags (k_shock_cfl).  Eager callers get the cap in
-// fx_status.min_cell_cfl instead.
-void launch_shock_cfl() {
-  if (!g.prm.shockLowerCFL || !g.dtDev) return;   // g.dtDev == g.dtBuf while capturing
-  hipLaunchKernelGGL(k_shock_cfl, dim3(1), dim3(1), 0, g.stream, g.status, g.runCfl,
-                     1.2 * shock_cfl_max(g.prm.ndim), g.dtBuf + 2);
+// Inside a graph-captured run under shockLowerCFL or fallbackLowerCFL, at
+// the end of a step: the next dt's CFL from this step's flags and fallbacks
+// (k_step_cfl).  Eager callers get the caps in fx_status.min_cell_cfl and
+// fallback_cfl instead.
+void launch_step_cfl() {
+  if (!cfl_varies(g.prm) || !g.dtDev) return;   // g.dtDev == g.dtBuf while capturing
+  hipLaunchKernelGGL(k_step_cfl, dim3(1), dim3(1), 0, g.stream, g.status, g.runCfl,
+                     shock_on(g.prm) ? 1 : 0,
+                     g.prm.shockLowerCFL ? shock_cfl_max(g.prm.ndim) : 0.0,
+                     fallback_cfl_cap(g.prm), g.dtBuf + 2);
 }
 
 // k_shock_detect over the uploaded tiles, one workgroup each, with the LDS
@@ -1889,7 +1903,7 @@ int tune_flux_stages(int nTiles, double dt, int reps, fx_tune_result* out, int c
 // ---------------------------------------------------------------------
 void amr_free() {
   Lib::Amr& A = g.amr;
-  void* bufs[] = {A.dblocks, A.dlevels, A.dboxStart, A.dcellStart, A.dwBox,  A.dwCell,
+  void* bufs[] = {A.dblocks, A.dlevels, A.dboxStart, A.dwBox,
                   A.flux,    A.pavg,    A.dchunks,   A.dplan,      (void*)A.dsrc, A.drest,
                   A.dnRest,  A.drecs,   A.drecOf,    A.dcv,        A.drecOk, A.dguard,
                   A.dguardOff};
@@ -1996,10 +2010,10 @@ void amr_launch_restrict() {
   }
 }
 
-// dt = cfl / max((|v| + c) / del) over the leaf cells of every level, into
-// g.dtBuf[0]; t += dt
+// dt over the leaf cells of every level (the minimum of the leaf blocks'
+// dts, as k_amr_dt picks them), into g.dtBuf[0]; t += dt
 int amr_launch_dt(double cfl, fx_status* st) {
-  const int grid = grid_for(g.amr.cells);
+  const int grid = g.amr.nBlocks;
   if (g.partialCap < grid) {
     if (g.partial) HIPCALL(hipFree(g.partial), st);
     g.partial = nullptr;
@@ -2007,11 +2021,10 @@ int amr_launch_dt(double cfl, fx_status* st) {
     g.partialCap = grid;
   }
   AmrLaunch L = amr_launch();
-  L.start = g.amr.dcellStart;
-  L.wtab = g.amr.dwCell;
-  hipLaunchKernelGGL(k_amr_dt, dim3(grid), dim3(kThreads), 0, g.stream, L, g.partial);
-  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kThreads), 0, g.stream, g.partial, grid, cfl,
-                     shock_cfl_dev(), g.dtBuf, g.dtBuf + 1);
+  hipLaunchKernelGGL(k_amr_dt, dim3(grid), dim3(kThreads), 0, g.stream, L, cfl, step_cfl_dev(),
+                     g.partial);
+  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kThreads), 0, g.stream, g.partial, grid, g.dtBuf,
+                     g.dtBuf + 1);
   return 0;
 }
 
@@ -2025,13 +2038,13 @@ int amr_launch_step(double dt) {
     if (g.capturing)
       hipLaunchKernelGGL(k_zero_shock_tiles, dim3(1), dim3(1), 0, g.stream, g.status);
     launch_shock_detect(g.amr.nBlocks);
-    launch_shock_cfl();
     amr_launch_fill(g.prm.shokVar);
   }
   if (launch_flux_stages(g.amr.nBlocks, dt, g.amr.TX, g.amr.TY)) return 4;
   amr_launch_register();
   launch_update(g.amr.nBlocks, dt, nullptr);
   amr_launch_restrict();
+  launch_step_cfl();
   return 0;
 }
 
@@ -2204,23 +2217,25 @@ int fx_hydro_shock_detect(const fx_tile* tiles, int nTiles, fx_status* st) {
 int fx_hydro_compute_dt(const double* U, long nBlocks, long blockStride, const double del[3],
                         double cfl, double* dtOut) {
   if (!g.init || !U || nBlocks <= 0 || !dtOut) return 3;
+  if (nBlocks > 0x7fffffffL) return 3;
   const DevParams& P = g.dp;
-  const int grid = grid_for(P.icells * nBlocks);
+  const int grid = (int)nBlocks;  // one workgroup per block
   if (g.partialCap < grid) {
     if (g.partial) (void)hipFree(g.partial);
+    g.partial = nullptr;
     if (hipMalloc((void**)&g.partial, sizeof(double) * grid) != hipSuccess) return 4;
     g.partialCap = grid;
   }
-  hipLaunchKernelGGL(k_dt, dim3(grid), dim3(kThreads), 0, g.stream, P, U, nBlocks, blockStride,
-                     1. / del[0], 1. / del[1], 1. / del[2], g.partial);
+  hipLaunchKernelGGL(k_dt, dim3(grid), dim3(kThreads), 0, g.stream, P, U, blockStride,
+                     1. / del[0], 1. / del[1], 1. / del[2], cfl, nullptr, g.partial);
   std::vector<double> h((size_t)grid);
   if (hipMemcpyAsync(h.data(), g.partial, sizeof(double) * grid, hipMemcpyDeviceToHost, g.stream) !=
       hipSuccess)
     return 4;
   if (hipStreamSynchronize(g.stream) != hipSuccess) return 4;
-  double m = 0.;
-  for (double v : h) m = v > m ? v : m;
-  *dtOut = m != 0. ? cfl / m : 1.e300;
+  double m = 1.e300;  // Hydro_computeDt.F90:284-288: the smallest block dt
+  for (double v : h) m = v < m ? v : m;
+  *dtOut = m;
   return 0;
 }
 
@@ -2336,23 +2351,23 @@ int fx_grid_run_graph(double* U, long nBlocks, long blockStride, const int* nbr,
   if (!rc) rc = ensure_step_scratch((int)nBlocks, st);
   if (rc) return rc;
   const DevParams& P = g.dp;
-  const int dtGrid = grid_for(P.icells * nBlocks);
+  const int dtGrid = (int)nBlocks;  // one workgroup per block
   if (g.partialCap < dtGrid) {
     if (g.partial) HIPCALL(hipFree(g.partial), st);
     g.partial = nullptr;
     HIPCALL(hipMalloc((void**)&g.partial, sizeof(double) * dtGrid), st);
     g.partialCap = dtGrid;
   }
-  if (!g.dtBuf) HIPCALL(hipMalloc((void**)&g.dtBuf, 3 * sizeof(double)), st);
+  if (!g.dtBuf) HIPCALL(hipMalloc((void**)&g.dtBuf, 4 * sizeof(double)), st);
   if (!g.capStream) HIPCALL(hipStreamCreateWithFlags(&g.capStream, hipStreamNonBlocking), st);
   const hipStream_t saved = g.stream;
   const int savedAsync = g.async, savedTiming = g.timing;
   // the clock starts at zero; dt and t stay on the device for the whole run
   if (saved) HIPCALL(hipStreamSynchronize(saved), st);
   g.stream = g.capStream;
-  // t = 0, and a graph run starts at the original CFL
-  const double init3[3] = {0.0, 0.0, cfl};
-  HIPCALL(hipMemcpy(g.dtBuf, init3, sizeof init3, hipMemcpyHostToDevice), st);
+  // t = 0, and a graph run starts at the original CFL with no fallbacks
+  const double init4[4] = {0.0, 0.0, cfl, 0.0};
+  HIPCALL(hipMemcpy(g.dtBuf, init4, sizeof init4, hipMemcpyHostToDevice), st);
   g.runCfl = cfl;
   g.async = 1;
   g.timing = 0;
@@ -2364,19 +2379,19 @@ int fx_grid_run_graph(double* U, long nBlocks, long blockStride, const int* nbr,
   if (hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) err = 4;
   for (int s = 0; !err && s < stepsPerGraph; ++s) {
     // dt from the interior cells (the guard fill does not change them), with
-    // the CFL the previous step's flags left
-    hipLaunchKernelGGL(k_dt, dim3(dtGrid), dim3(kThreads), 0, g.stream, P, U, nBlocks, blockStride,
-                       1. / del[0], 1. / del[1], 1. / del[2], g.partial);
-    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kThreads), 0, g.stream, g.partial, dtGrid, cfl,
-                       shock_cfl_dev(), g.dtBuf, g.dtBuf + 1);
+    // the CFL the previous step's flags and fallbacks left
+    hipLaunchKernelGGL(k_dt, dim3(dtGrid), dim3(kThreads), 0, g.stream, P, U, blockStride,
+                       1. / del[0], 1. / del[1], 1. / del[2], cfl, step_cfl_dev(), g.partial);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kThreads), 0, g.stream, g.partial, dtGrid,
+                       g.dtBuf, g.dtBuf + 1);
     launch_fill(U, nBlocks, blockStride);
     if (P.shokOff >= 0) {  // Hydro.F90:94-123: flag, then fill SHOK_VAR's guard cells
       hipLaunchKernelGGL(k_zero_shock_tiles, dim3(1), dim3(1), 0, g.stream, g.status);
       launch_shock_detect((int)nBlocks);
-      launch_shock_cfl();
       launch_fill(U, nBlocks, blockStride, g.prm.shokVar);
     }
     if (run_compute(nullptr, (int)nBlocks, 0.0, nullptr, nullptr, true)) err = 4;
+    launch_step_cfl();
   }
   if (hipStreamEndCapture(g.stream, &graph) != hipSuccess) err = 4;
   g.capturing = false;
@@ -2633,13 +2648,11 @@ int fx_amr_setup(const fx_tile* tiles, const fx_amr_block* blocks, int nBlocks, 
   }
   if (g.multi && build_march_items(nBlocks, A.TX, st)) return 4;
   if (ensure_step_scratch(nBlocks, st)) return 4;
-  // prefixes: guard-inclusive box cells and interior cells per block
+  // prefix of guard-inclusive box cells per block
   std::vector<long> box(nBlocks + 1, 0);
   for (int b = 0; b < nBlocks; ++b) box[b + 1] = box[b] + g.hcls[A.hblocks[b].cls].ncell;
   A.boxCells = box[nBlocks];
-  A.cells = g.hcellStart[nBlocks];
   int rc = upload_vec(&A.dboxStart, box, st);
-  if (!rc) rc = upload_vec(&A.dcellStart, g.hcellStart, st);
   if (!rc) rc = upload_vec(&A.dwBox, wave_table(box), st);
   {  // guard-fill work: each class's guard cells in box order, and chunks of
      // kThreads guard cells of one block
@@ -2665,7 +2678,6 @@ int fx_amr_setup(const fx_tile* tiles, const fx_amr_block* blocks, int nBlocks, 
     if (!rc) rc = upload_vec(&A.dguard, glist, st);
     if (!rc) rc = upload_vec(&A.dguardOff, goff, st);
   }
-  if (!rc) rc = upload_vec(&A.dwCell, wave_table(g.hcellStart), st);
   for (int L = 0; !rc && L < nLevels; ++L) {
     int* d = nullptr;
     rc = upload_vec(&d, index[L], st);
@@ -2836,7 +2848,7 @@ int fx_amr_setup(const fx_tile* tiles, const fx_amr_block* blocks, int nBlocks, 
     if (st) st->code = rc;
     return rc;
   }
-  if (!g.dtBuf) HIPCALL(hipMalloc((void**)&g.dtBuf, 3 * sizeof(double)), st);
+  if (!g.dtBuf) HIPCALL(hipMalloc((void**)&g.dtBuf, 4 * sizeof(double)), st);
   A.ready = true;
   return 0;
 }
@@ -2934,8 +2946,8 @@ int fx_amr_run_graph(double cfl, int stepsPerGraph, int nReplays, double* tOut, 
     return 3;
   }
   if (amr_ensure_uploaded(st)) return 4;
-  {  // partial maxima of the dt reduction, sized before capture
-    const int grid = grid_for(g.amr.cells);
+  {  // the blocks' dts of the dt reduction, sized before capture (amr_launch_dt)
+    const int grid = g.amr.nBlocks;
     if (g.partialCap < grid) {
       if (g.partial) HIPCALL(hipFree(g.partial), st);
       g.partial = nullptr;
@@ -2949,9 +2961,9 @@ int fx_amr_run_graph(double cfl, int stepsPerGraph, int nReplays, double* tOut, 
   if (saved) HIPCALL(hipStreamSynchronize(saved), st);
   HIPCALL(hipDeviceSynchronize(), st);
   g.stream = g.capStream;
-  // t = 0, and a graph run starts at the original CFL
-  const double init3[3] = {0.0, 0.0, cfl};
-  HIPCALL(hipMemcpy(g.dtBuf, init3, sizeof init3, hipMemcpyHostToDevice), st);
+  // t = 0, and a graph run starts at the original CFL with no fallbacks
+  const double init4[4] = {0.0, 0.0, cfl, 0.0};
+  HIPCALL(hipMemcpy(g.dtBuf, init4, sizeof init4, hipMemcpyHostToDevice), st);
   g.runCfl = cfl;
   g.async = 1;
   g.timing = 0;

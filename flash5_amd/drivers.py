@@ -142,6 +142,28 @@ def shock_cfl(cfl: float, cap: float) -> float:
     return min(cfl, 1.2 * cap) if cap > 0 else cfl
 
 
+def step_cfl(cfl: float, last: float, detect: bool, shock_cap: float,
+             fallback_cap: float) -> float:
+    """hy_cfl at the end of a step, which the next dt takes.  cfl is the
+    original CFL, last the value the previous step left (cfl before the
+    first step), shock_cap and fallback_cap the step's fx_status
+    min_cell_cfl and fallback_cfl (0: none).  The reference resets hy_cfl to
+    the original before the detector when detection is on
+    (Hydro.F90:104-108), lowers it to the shock cap when a cell is flagged
+    (hy_shockDetectBlk.F90:228-240), raises it by 1.2 up to the original
+    before the fluxes (Hydro.F90:128-141) and, under hy_fallbackLowerCFL,
+    lowers it to the fallback cap when a state fell back to first order
+    (hy_getRiemannState.F90:934-941, 1303-1319).  Without fallbacks and with
+    detection on this is shock_cfl(cfl, shock_cap)."""
+    c = cfl if detect else last
+    if shock_cap > 0:
+        c = min(c, shock_cap)
+    c = 1.2 * c if 1.2 * c < cfl else cfl
+    if fallback_cap > 0:
+        c = min(c, fallback_cap)
+    return c
+
+
 class UniformGrid:
     """A single-level grid of equal blocks advanced by the hot path."""
 
@@ -159,7 +181,7 @@ class UniformGrid:
         self.nsteps = 0
         self.detect = bool(params.shockDetect or params.shockLowerCFL or
                            params.RiemannSolver.lower() == "hybrid")
-        self._cfl_next = cfl          # shockLowerCFL: the CFL the last step's flags left
+        self._cfl_next = cfl          # the CFL the last step's flags and fallbacks left
 
     def fill_guards(self) -> None:
         import ctypes as C
@@ -174,16 +196,18 @@ class UniformGrid:
     def step(self, dt_max: float = math.inf) -> float:
         """Guard fill, CFL dt (capped by dt_max), unsplit update in place.
         With shock detection on: fill, flag, fill again (SHOK_VAR's guard
-        cells too), as Hydro.F90:94-123 orders them.  Under shockLowerCFL the
-        dt takes the CFL that the previous step's flags left (shock_cfl)."""
+        cells too), as Hydro.F90:94-123 orders them.  Under shockLowerCFL or
+        hy_fallbackLowerCFL the dt takes the CFL that the previous step's
+        flags and fallbacks left (step_cfl)."""
         cfl = self._cfl_next
         self.fill_guards()
+        shock_cap = 0.0
         if self.detect:
-            st = self.hydro.shock_detect_batch(self.U, self.deltas)
+            shock_cap = self.hydro.shock_detect_batch(self.U, self.deltas).min_cell_cfl
             self.fill_guards()
-            self._cfl_next = shock_cfl(self.cfl, st.min_cell_cfl)
         dt = min(self.hydro.compute_dt(self.U, self.deltas, cfl), dt_max)
-        self.hydro.advance_batch(self.U, dt, self.deltas)
+        st = self.hydro.advance_batch(self.U, dt, self.deltas)
+        self._cfl_next = step_cfl(self.cfl, cfl, self.detect, shock_cap, st.fallback_cfl)
         self.t += dt
         self.nsteps += 1
         return dt
@@ -396,18 +420,30 @@ class AmrRun:
         self.hydro.amr_setup(tiles, [L + 1 for L, _ in grid.blocks], [p for _, p in grid.blocks],
                              grid.ext, grid.periodic)
         self.t = 0.0
+        self.detect = bool(params.shockDetect or params.shockLowerCFL or
+                           params.RiemannSolver.lower() == "hybrid")
+        self.cfl = None               # the original CFL, from the first dt() call
+        self._cfl_next = None         # the CFL the last step's flags and fallbacks left
         self._flag_cap = 0.0          # shockLowerCFL: the cap the last step's flags set
 
     def step(self, dt: float) -> FxStatus:
         st = self.hydro.amr_step(dt)
         self.t += dt
         self._flag_cap = st.min_cell_cfl
+        if self.cfl is not None:
+            last = self._cfl_next if self._cfl_next is not None else self.cfl
+            self._cfl_next = step_cfl(self.cfl, last, self.detect, st.min_cell_cfl,
+                                      st.fallback_cfl)
         return st
 
     def dt(self, cfl: float = 0.8) -> float:
-        """CFL dt over every level; under shockLowerCFL with the CFL that the
-        last step's flags left (shock_cfl)."""
-        return self.hydro.amr_compute_dt(shock_cfl(cfl, self._flag_cap))
+        """CFL dt over every level; under shockLowerCFL or hy_fallbackLowerCFL
+        with the CFL that the last step's flags and fallbacks left
+        (step_cfl).  cfl is the original CFL; a run keeps the first one."""
+        if self.cfl is None:
+            self.cfl = cfl
+        c = self._cfl_next if self._cfl_next is not None else self.cfl
+        return self.hydro.amr_compute_dt(c)
 
     def run_graph(self, cfl: float, steps_per_graph: int, replays: int) -> float:
         self.t += self.hydro.amr_run_graph(cfl, steps_per_graph, replays)

@@ -51,6 +51,7 @@ class FxParams(C.Structure):
         ("exactMaxIter", C.c_int), ("exactTol", C.c_double),
         ("precision", C.c_int), ("learnedSeed", C.c_int),
         ("shockDetect", C.c_int), ("shockLowerCFL", C.c_int), ("shokVar", C.c_int),
+        ("fallbackLowerCFL", C.c_int), ("cflFallbackFactor", C.c_double),
     ]
 
 
@@ -75,7 +76,8 @@ class FxStatus(C.Structure):
     _fields_ = [("code", C.c_int), ("first_tile", C.c_int), ("first_i", C.c_int),
                 ("first_j", C.c_int), ("first_k", C.c_int), ("first_dir", C.c_int),
                 ("fallbacks", C.c_longlong), ("newton_iters", C.c_longlong),
-                ("newton_fallbacks", C.c_longlong), ("min_cell_cfl", C.c_double)]
+                ("newton_fallbacks", C.c_longlong), ("min_cell_cfl", C.c_double),
+                ("fallback_cfl", C.c_double)]
 
 
 class FxFluxRegFace(C.Structure):
@@ -210,6 +212,8 @@ class HydroParams:
     shockDetect: bool = False
     shockLowerCFL: bool = False
     shokVar: int = -1            # slot of SHOK_VAR in U's var axis (-1: none)
+    hy_fallbackLowerCFL: bool = False
+    hy_cflFallbackFactor: float = 0.9
 
     def to_c(self) -> FxParams:
         p = FxParams()
@@ -236,6 +240,8 @@ class HydroParams:
         p.precision, p.learnedSeed = self.precision, int(self.learnedSeed)
         p.shockDetect, p.shockLowerCFL = int(self.shockDetect), int(self.shockLowerCFL)
         p.shokVar = self.shokVar
+        p.fallbackLowerCFL = int(self.hy_fallbackLowerCFL)
+        p.cflFallbackFactor = self.hy_cflFallbackFactor
         return p
 
     def block_shape(self):

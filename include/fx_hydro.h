@@ -92,6 +92,10 @@ typedef struct {
   int shokVar;              /* 0-based slot of SHOK_VAR in U's var axis (the
                                unsplit Config's "shok"); read only while
                                shock detection is on */
+  /* CFL after the positivity fallback (unsplit/Config:137-138) */
+  int fallbackLowerCFL;     /* hy_fallbackLowerCFL */
+  double cflFallbackFactor; /* hy_cflFallbackFactor (0.9 in the Config);
+                               must be > 0 while fallbackLowerCFL is on */
 } fx_params;
 
 /* One tile as the reference's tile iterator yields it with tiling off
@@ -130,6 +134,15 @@ typedef struct {
                                shockLowerCFL is on and a cell was flagged
                                (the reference sets hy_cfl = min(hy_cfl, cap),
                                hy_shockDetectBlk.F90:211-227); else 0 */
+  double fallback_cfl;      /* calls that compute fluxes, NDIM 2 and 3, with
+                               fallbackLowerCFL on: cflFallbackFactor / NDIM
+                               when a state fell back to first order
+                               (fallbacks > 0; graph-captured runs: in any
+                               step), which the reference applies after the
+                               fluxes as hy_cfl = min(hy_cfl, that)
+                               (hy_dataReconstOneStep.F90:1125-1129,
+                               hy_getRiemannState.F90:934-941, 1303-1319,
+                               1473); else 0 */
 } fx_status;
 
 /* Hydro_init: set knobs, allocate nothing (scratch grows on first use). */
@@ -284,8 +297,11 @@ int fx_grid_fill_guards(double* U, long nBlocks, long blockStride, const int* nb
  * elapsed simulation time (the clock starts at 0).  With shock detection on,
  * each step fills, flags and fills SHOK_VAR's guard cells (Hydro.F90:94-123);
  * under shockLowerCFL a step after one with a flagged cell takes CFL
- * min(cfl, 1.2 * cap) (Hydro.F90:104-141), carried on the device across
- * steps and replays, starting at cfl.  Not with learnedSeed. */
+ * min(cfl, 1.2 * cap) (Hydro.F90:104-141).  Under fallbackLowerCFL a step
+ * with a fallback lowers the CFL to at most cflFallbackFactor / NDIM, and
+ * each later step raises it by 1.2 up to cfl (Hydro.F90:128-141; reset to
+ * cfl before detection when the detector runs).  The CFL is carried on the
+ * device across steps and replays, starting at cfl.  Not with learnedSeed. */
 int fx_grid_run_graph(double* U, long nBlocks, long blockStride, const int* nbr,
                       const double del[3], double cfl, int stepsPerGraph, int nReplays,
                       double* tOut, fx_status* st);
@@ -387,16 +403,18 @@ int fx_amr_restrict(fx_status* st);
 /* dt = cfl / max over all levels' leaf cells (interior cells no finer block
  * covers, as Driver_computeDt.F90:219 iterates LEAF blocks) of
  * (|v_d| + c) / del_d (Hydro_computeDt.F90:186-270 without the reductions
- * over ranks).  Under shockLowerCFL the caller passes the lowered CFL: the
- * last step's fx_status.min_cell_cfl gives the cap. */
+ * over ranks).  Under shockLowerCFL or fallbackLowerCFL the caller passes
+ * the lowered CFL: the last step's fx_status.min_cell_cfl and fallback_cfl
+ * give the caps. */
 int fx_amr_compute_dt(double cfl, double* dtOut);
 /* One step of every level: guard fill, fluxes of all blocks (one persistent
  * launch per stage), flux correction at fine-coarse faces, update, then
  * restriction. */
 int fx_amr_step(double dt, fx_status* st);
 /* nReplays of a HIP graph of stepsPerGraph steps (dt on the device each
- * step, as fx_amr_compute_dt; under shockLowerCFL the lowered CFL is carried
- * on the device, as in fx_grid_run_graph); *tOut = simulated time. */
+ * step, as fx_amr_compute_dt; under shockLowerCFL or fallbackLowerCFL the
+ * lowered CFL is carried on the device, as in fx_grid_run_graph); *tOut =
+ * simulated time. */
 int fx_amr_run_graph(double cfl, int stepsPerGraph, int nReplays, double* tOut, fx_status* st);
 /* Library version string. */
 const char* fx_hydro_version(void);

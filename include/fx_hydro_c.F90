@@ -26,6 +26,8 @@ module fx_hydro_c
      integer(c_int) :: precision, learnedSeed
      integer(c_int) :: shockDetect, shockLowerCFL
      integer(c_int) :: shokVar          ! SHOK_VAR - 1 (the unsplit Config's "shok")
+     integer(c_int) :: fallbackLowerCFL ! hy_fallbackLowerCFL
+     real(c_double) :: cflFallbackFactor ! hy_cflFallbackFactor
   end type
 
   type, bind(C) :: fx_tile
@@ -43,7 +45,7 @@ module fx_hydro_c
   type, bind(C) :: fx_status
      integer(c_int)       :: code, first_tile, first_i, first_j, first_k, first_dir
      integer(c_long_long) :: fallbacks, newton_iters, newton_fallbacks
-     real(c_double)       :: min_cell_cfl
+     real(c_double)       :: min_cell_cfl, fallback_cfl
   end type
 
   type, bind(C) :: fx_fluxreg_face

@@ -45,9 +45,10 @@ def _c_layout():
 int main(void){
   printf("fx_params %zu\nfx_tile %zu\nfx_status %zu\n", sizeof(fx_params), sizeof(fx_tile), sizeof(fx_status));
   O(fx_params,gamma); O(fx_params,limiter); O(fx_params,fP); O(fx_params,exactTol);
-  O(fx_params,precision); O(fx_params,learnedSeed);
+  O(fx_params,precision); O(fx_params,learnedSeed); O(fx_params,shokVar);
+  O(fx_params,fallbackLowerCFL); O(fx_params,cflFallbackFactor);
   O(fx_tile,pavg); O(fx_tile,del); O(fx_tile,level);
-  O(fx_status,fallbacks); O(fx_status,min_cell_cfl);
+  O(fx_status,fallbacks); O(fx_status,min_cell_cfl); O(fx_status,fallback_cfl);
   return 0;}
 """
     with tempfile.TemporaryDirectory() as d:
@@ -64,13 +65,15 @@ def test_ctypes_structs_match_c_layout():
     assert C.sizeof(hydro.FxParams) == lay["fx_params"]
     assert C.sizeof(hydro.FxTile) == lay["fx_tile"]
     assert C.sizeof(hydro.FxStatus) == lay["fx_status"]
-    for f in ("gamma", "limiter", "fP", "exactTol", "precision", "learnedSeed"):
+    for f in ("gamma", "limiter", "fP", "exactTol", "precision", "learnedSeed", "shokVar",
+              "fallbackLowerCFL", "cflFallbackFactor"):
         assert getattr(hydro.FxParams, f).offset == lay[f"fx_params.{f}"], f
     assert hydro.FxTile.pavg.offset == lay["fx_tile.pavg"]
     assert hydro.FxTile.del_.offset == lay["fx_tile.del"]
     assert hydro.FxTile.level.offset == lay["fx_tile.level"]
     assert hydro.FxStatus.fallbacks.offset == lay["fx_status.fallbacks"]
     assert hydro.FxStatus.min_cell_cfl.offset == lay["fx_status.min_cell_cfl"]
+    assert hydro.FxStatus.fallback_cfl.offset == lay["fx_status.fallback_cfl"]
 
 
 def test_init_rejects_bad_parameters():
@@ -79,6 +82,8 @@ def test_init_rejects_bad_parameters():
     assert L.fx_hydro_init(C.byref(p)) != 0          # NGUARD must be 4
     p = hydro.HydroParams(ndim=3, gamma=1.0).to_c()
     assert L.fx_hydro_init(C.byref(p)) != 0          # gamma > 1
+    p = hydro.HydroParams(ndim=3, hy_fallbackLowerCFL=True, hy_cflFallbackFactor=0.0).to_c()
+    assert L.fx_hydro_init(C.byref(p)) != 0          # a positive fallback factor
 
 
 def test_hydro_refuses_without_gpu():

@@ -24,11 +24,12 @@ STRUCTS = {
                   "use3dFullCTU", "useAuxEint", "riemann", "fP", "smlrho", "smallp", "smallE",
                   "eintSwitch", "tiny", "gasConstant", "singleSpeciesA", "exactMaxIter",
                   "exactTol", "precision", "learnedSeed", "shockDetect", "shockLowerCFL",
-                  "shokVar"],
+                  "shokVar", "fallbackLowerCFL", "cflFallbackFactor"],
     "fx_tile": ["U", "flux", "pavg", "del", "level", "block_class"],
     "fx_amr_block": ["level", "pos"],
     "fx_status": ["code", "first_tile", "first_i", "first_j", "first_k", "first_dir",
-                  "fallbacks", "newton_iters", "newton_fallbacks", "min_cell_cfl"],
+                  "fallbacks", "newton_iters", "newton_fallbacks", "min_cell_cfl",
+                  "fallback_cfl"],
     "fx_fluxreg_face": ["coarse_tile", "fine_tile", "dir", "ci", "cj", "ck", "fi", "fj", "fk"],
 }
 
@@ -75,7 +76,7 @@ def _fortran_program():
         # argument checks that return before any GPU call
         "  p = fx_params(3, 16, 16, 16, 4, 10, [(i, i = 0, 9)], 5d0/3d0, FX_MC, 1d0, 1, 1, 0, 1,"
         " 1, FX_HLLC, 0d0, 1d-10, 1d-10, 1d-10, 1d-4, 1d-10, 8.3144598d7, 1d0, 40, 1d-12,"
-        " FX_PREC_FP64, 0, 0, 0, -1)",
+        " FX_PREC_FP64, 0, 0, 0, -1, 0, 0.9d0)",
         "  p%ng = 3",
         '  write(*, "(A,1X,I0)") "init_ng3", fx_hydro_init(p)',
         "  p%ng = 4",
@@ -84,6 +85,10 @@ def _fortran_program():
         "  p%gamma = 5d0/3d0",
         "  p%precision = 1",
         '  write(*, "(A,1X,I0)") "init_prec1", fx_hydro_init(p)',
+        "  p%precision = FX_PREC_FP64",
+        "  p%fallbackLowerCFL = 1",
+        "  p%cflFallbackFactor = 0d0",
+        '  write(*, "(A,1X,I0)") "init_fbfactor0", fx_hydro_init(p)',
         "  call c_f_pointer(fx_hydro_version(), ver, [64])",
         "  i = 1",
         "  do while (ver(i) /= c_null_char .and. i < 64)",
@@ -118,4 +123,5 @@ def test_fortran_module_matches_the_c_abi():
     for key, off in c.items():
         assert f[key] == off, f"{key}: Fortran {f[key]} vs C {off}"
     assert f["init_ng3"] == "3" and f["init_gamma1"] == "3" and f["init_prec1"] == "3"
+    assert f["init_fbfactor0"] == "3"
     assert "gfx950" in f["version"]

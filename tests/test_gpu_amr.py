@@ -193,28 +193,21 @@ def test_amr_graph_steps_match_eager_steps():
 
 
 def _leaf_dt(grid, U, cfl):
-    """cfl / max over leaf cells (cells no finer block covers) of
-    (|v_a| + c) / del_a, in the device's operation order."""
+    """The smallest oracle block dt (Hydro_computeDt.F90:186-288) over the
+    leaf blocks: blocks whose cells no finer block covers (a refined block
+    is covered whole)."""
     idx = OA._index(grid.as_dict())
-    m = 0.0
+    dt = 1e300
     for b, (L, pos) in enumerate(grid.blocks):
         nb = grid.nb[L]
-        sl = tuple(slice(4, 4 + nb[a]) if a < grid.ndim else slice(0, 1) for a in (2, 1, 0))
-        u = U[b]
-        cs = np.sqrt(u[(O.GAMC,) + sl] * u[(O.PRES,) + sl] / u[(O.DENS,) + sl])
-        d = grid.dels(L)
-        s = (np.abs(u[(O.VELX,) + sl] - 0.0) + cs) * (1.0 / d[0])
-        for a, v in ((1, O.VELY), (2, O.VELZ))[:grid.ndim - 1]:
-            s = np.fmax(s, (np.abs(u[(v,) + sl] - 0.0) + cs) * (1.0 / d[a]))
         if L + 1 < len(grid.nb):
-            k, j, i = np.indices(s.shape)
-            g = np.array([pos[0] * nb[0] + i, pos[1] * nb[1] + j,
-                          pos[2] * nb[2] + k if grid.ndim > 2 else 0 * k]).reshape(3, -1)
             nbf = grid.nb[L + 1]
-            bp = [2 * g[a] // nbf[a] for a in range(3)]
-            s = np.where((idx[L + 1][bp[2], bp[1], bp[0]] >= 0).reshape(s.shape), 0.0, s)
-        m = max(m, s.max())
-    return cfl / m
+            f = [2 * pos[a] * nb[a] // nbf[a] if a < grid.ndim else 0 for a in range(3)]
+            if idx[L + 1][f[2], f[1], f[0]] >= 0:
+                continue
+        p = OA.level_params(grid.as_dict(), L, GAMMA)
+        dt = min(dt, O.block_dt(p, U[b], grid.dels(L), cfl))
+    return dt
 
 
 def test_amr_dt_takes_leaf_cells_only():

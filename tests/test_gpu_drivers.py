@@ -107,3 +107,24 @@ def test_graph_captured_steps_match_eager_steps(kw):
                                       eager.U.cpu().numpy()[:, :, :, 4:12, 4:12],
                                       err_msg=f"replay round {rep}")
         del junk
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_compute_dt_keeps_the_first_cell_of_a_rounding_tie(ndim):
+    """Blocks with two cells whose (|v| + c) / dx times the CFL round alike:
+    the device takes the earlier cell of each block, as the reference's loop
+    does (Hydro_computeDt.F90:264-272), and the smallest block dt, bit for
+    bit with the oracle; alone and in a batch."""
+    from flash5_amd.hydro import Hydro
+    from test_oracle_block import tie_block
+    p = O.params(ndim, 8, 1.4)
+    U1, va, vb = tie_block(p, 0.8, first_smaller=True)
+    U2, _, _ = tie_block(p, 0.8, first_smaller=False)
+    dels = (1.0 / 8,) * 3
+    hy = Hydro(HydroParams(ndim=ndim, nxb=8, nyb=8, nzb=8, gamma=1.4))
+    want = [O.block_dt(p, U, dels, 0.8) for U in (U1, U2)]
+    assert want[0] == 0.8 / va and want[0] != 0.8 / vb
+    for U, w in zip((U1, U2), want):
+        assert hy.compute_dt(torch.from_numpy(U[None].copy()).cuda(), dels, 0.8) == w
+    both = torch.from_numpy(np.stack([U1, U2])).cuda()
+    assert hy.compute_dt(both, dels, 0.8) == min(want)

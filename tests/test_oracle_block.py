@@ -431,3 +431,54 @@ def test_block_dt_is_set_by_the_fastest_signal(ndim):
     speeds = [(abs(vel[d][hot]) + c_hot) / dels[d] for d in range(ndim)]
     want = 0.8 / max(speeds)
     assert O.block_dt(p, U, dels, 0.8) == pytest.approx(want, rel=1e-15)
+
+
+def _cell_dtl(p, U, k, j, i, dels):
+    """(|v_d| + c) / del_d of one cell in the reference's operation order
+    (Hydro_computeDt.F90:231-264), numpy float64."""
+    c = np.sqrt(U[O.GAMC, k, j, i] * U[O.PRES, k, j, i] / U[O.DENS, k, j, i])
+    v = (abs(U[O.VELX, k, j, i] - 0.0) + c) * (1.0 / dels[0])
+    for d in range(1, p.ndim):
+        v = max(v, (abs(U[O.VELX + d, k, j, i] - 0.0) + c) * (1.0 / dels[d]))
+    return v
+
+
+def tie_block(p, cfl, first_smaller=True):
+    """A block at rest (density 1, pressure 1) but for two interior cells
+    moving along x whose (|v| + c) / dx differ while their products with
+    the CFL round to the same double.  The earlier cell in loop order
+    (i fastest) holds the smaller value when first_smaller.  Returns U and
+    the two values (earlier cell's first)."""
+    shape = O.block_shape(p)[1:]
+    vel = [np.zeros(shape) for _ in range(3)]
+    g = p.ng
+    a = (g if p.ndim > 2 else 0, g + 1 if p.ndim > 1 else 0, g + 2)
+    b = (g + 1 if p.ndim > 2 else 0, g + 2 if p.ndim > 1 else 0, g + 5)
+    dels = (1.0 / p.nxb,) * 3
+    # values near 44: times 0.8 they stay in [32, 64), whose spacing the
+    # product's 0.8-times-smaller steps cannot all resolve
+    for step in range(400):
+        u_lo = 4.3 + step * 2.0 ** -50
+        u_hi = u_lo + 2.0 ** -50
+        vel[0][a], vel[0][b] = (u_lo, u_hi) if first_smaller else (u_hi, u_lo)
+        U = make_block(p, np.ones(shape), vel, np.ones(shape))
+        va, vb = _cell_dtl(p, U, *a, dels), _cell_dtl(p, U, *b, dels)
+        if va != vb and va * cfl == vb * cfl and cfl / va != cfl / vb:
+            return U, va, vb
+    raise AssertionError("no rounding tie found")
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+@pytest.mark.parametrize("first_smaller", [True, False])
+def test_block_dt_keeps_the_first_cell_of_a_rounding_tie(ndim, first_smaller):
+    """Hydro_computeDt.F90:264-272 keeps a cell only when its value times
+    the CFL exceeds the kept value times the CFL.  Of two cells whose
+    products round alike the earlier one stays, so dt = cfl / (the earlier
+    cell's value), which differs in the last place from cfl / max."""
+    p = O.params(ndim, 8, G)
+    cfl = 0.8
+    U, va, vb = tie_block(p, cfl, first_smaller)
+    assert (va < vb) == first_smaller
+    dt = O.block_dt(p, U, (1.0 / 8,) * 3, cfl)
+    assert dt == cfl / va
+    assert dt != cfl / max(va, vb) or not first_smaller

@@ -1,4 +1,4 @@
-"""The CFL rule of shockLowerCFL (CPU).
+"""The CFL rules of shockLowerCFL and hy_fallbackLowerCFL (CPU).
 
 flash5_amd.drivers.shock_cfl gives the next dt's CFL in one expression; the
 reference reaches it in three steps per Hydro call: reset to the original
@@ -6,6 +6,9 @@ before the detector (Hydro.F90:104-108), lower to the cap when a cell is
 flagged (hy_shockDetectBlk.F90:228-240), then recover by a factor 1.2 up to
 the original (Hydro.F90:128-141).  The three steps are run here as written,
 over originals below, at and above the caps, and must agree bit for bit.
+flash5_amd.drivers.step_cfl carries hy_cfl from step to step with the
+fallback cap as well; it is checked against the reference's per-cell
+minimum over step sequences.
 """
 import itertools
 
@@ -37,3 +40,69 @@ def test_known_values():
     assert shock_cfl(0.8, 0.45) == 1.2 * 0.45          # 2-D
     assert shock_cfl(0.8, 0.0) == 0.8                  # nothing flagged
     assert shock_cfl(0.28, 0.25) == 0.28               # 1.2 * 0.25 is above the original
+
+
+def _reference_hydro_step(original, hy_cfl, detect, flagged, cflmax, cells_fell_back, ndim,
+                          factor, fallback_lower):
+    """hy_cfl through one Hydro call, as the reference orders it: the reset
+    before the detector and its cap (Hydro.F90:104-108,
+    hy_shockDetectBlk.F90:228-240), the recovery (Hydro.F90:128-141), then
+    the cell loop of hy_getRiemannState: each cell's cellCfl starts at
+    hy_cfl and a cell whose state fell back takes min(cellCfl, factor/NDIM)
+    (hy_dataReconstOneStep.F90:1125-1129, :1473), minCfl is their minimum
+    (:829) and is copied into hy_cfl for NDIM > 1 (:934-941) and, in 3-D,
+    after the corner loop (:1303-1319)."""
+    if detect:
+        hy_cfl = original
+        if flagged:
+            hy_cfl = min(hy_cfl, cflmax)
+    if 1.2 * hy_cfl < original:
+        hy_cfl = 1.2 * hy_cfl
+    else:
+        hy_cfl = original
+    min_cfl = hy_cfl
+    for fell_back in cells_fell_back:
+        cell_cfl = hy_cfl
+        if fell_back:
+            cell_cfl = min(cell_cfl, factor / float(ndim))
+        min_cfl = min(min_cfl, cell_cfl)
+    if ndim > 1 and fallback_lower:
+        hy_cfl = min_cfl
+    if ndim == 3 and fallback_lower:
+        hy_cfl = min(hy_cfl, min_cfl)
+    return hy_cfl
+
+
+def test_step_cfl_follows_the_reference_over_step_sequences():
+    """step_cfl, fed the caps the library reports (min_cell_cfl under
+    shockLowerCFL, fallback_cfl = factor / NDIM under hy_fallbackLowerCFL
+    in 2-D and 3-D), against the reference's order of operations over
+    sequences of steps with and without flags and fallbacks."""
+    from flash5_amd.drivers import step_cfl
+    pattern = [(False, True), (False, False), (True, False), (False, False), (True, True),
+               (False, False), (False, False), (False, True), (False, False), (False, False),
+               (False, False), (False, False)]
+    for ndim, original, factor, detect, shock_lower, fallback_lower in itertools.product(
+            (1, 2, 3), (0.3, 0.5, 0.8, 1.0), (0.9, 0.6), (False, True), (False, True),
+            (False, True)):
+        if shock_lower and not detect:
+            continue                        # shockLowerCFL turns detection on
+        cflmax = {1: 0.60, 2: 0.45, 3: 0.25}[ndim]
+        ref = ours = original
+        for flagged, fell_back in pattern:
+            flagged = flagged and detect
+            ref = _reference_hydro_step(original, ref, detect, flagged and shock_lower, cflmax,
+                                        [False, fell_back, False], ndim, factor, fallback_lower)
+            shock_cap = cflmax if (shock_lower and flagged) else 0.0
+            fb_cap = factor / ndim if (fallback_lower and fell_back and ndim > 1) else 0.0
+            ours = step_cfl(original, ours, detect, shock_cap, fb_cap)
+            assert ours == ref, (ndim, original, factor, detect, shock_lower, fallback_lower)
+
+
+def test_step_cfl_recovers_by_1_2_after_a_fallback():
+    from flash5_amd.drivers import step_cfl
+    c, seq = 0.8, []
+    for fb in (0.45, 0.0, 0.0, 0.0, 0.0):
+        c = step_cfl(0.8, c, False, 0.0, fb)
+        seq.append(c)
+    assert seq == [0.45, 1.2 * 0.45, 1.2 * (1.2 * 0.45), 1.2 * (1.2 * (1.2 * 0.45)), 0.8]
