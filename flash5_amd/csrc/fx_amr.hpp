@@ -198,14 +198,14 @@ __global__ void __launch_bounds__(kThreads) k_amr_fill(AmrLaunch A,
   // blocks of one level have one block shape, hence one box size
   const gdouble* Us = as_global(sp);
   const int v0 = A.onlyVar >= 0 ? A.onlyVar : 0, v1 = A.onlyVar >= 0 ? A.onlyVar + 1 : P.nunk;
-  // eight variables' loads in flight at a time
-  for (int v = v0; v < v1; v += 8) {
-    double t[8];
+  // up to sixteen variables' loads in flight at a time
+  for (int v = v0; v < v1; v += 16) {
+    double t[16];
 #pragma unroll
-    for (int q = 0; q < 8; ++q)
+    for (int q = 0; q < 16; ++q)
       if (v + q < v1) t[q] = Us[(long)(v + q) * P.ncell];
 #pragma unroll
-    for (int q = 0; q < 8; ++q)
+    for (int q = 0; q < 16; ++q)
       if (v + q < v1) Ud[(long)(v + q) * P.ncell + cd] = t[q];
   }
 }
