@@ -63,8 +63,8 @@ template <int NDIM, int TX, int TY> struct MarchLds {
   static constexpr int WY = WX + 12 * I::NRY;                 // y-face states
   static constexpr int WZ = WY + 12 * I::NRX;                 // z-face states of plane k
   static constexpr int WZP = WZ + (Z ? 12 * NCOL : 1);        // z-face Wp of plane k-1
-  static constexpr int CST = WZP + (Z ? 6 * NCOL : 1);        // the step's ratios (FP32)
-  static constexpr int total = CST + 9;
+  static constexpr int CST = WZP + (Z ? 6 * NCOL : 1);        // the step's ratios, widths (FP32)
+  static constexpr int total = CST + 12;
 };
 
 // components stored for sigma and slopes: dens, vx, vy, vz, pres, eint
@@ -386,21 +386,22 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
   const int klo = P.lo[2], khi = P.hi[2];
   DevStatus* st = L.status;
   bool bad = false;  // eigen failure seen by this thread
-  // The step's ratios of dt and the cell widths, uniform over the tile.  The
-  // FP64 march keeps them in registers (the compiler hoists them); the FP32
-  // march reads them from LDS, written once per column tile by one thread
-  // (stage B1 first reads them after stage A's barrier, and the previous
-  // column tile ended on a barrier): 9 fewer live registers, mixed-mode
-  // march 6.29 -> 6.17 ms (HLLC) and 7.85 -> 7.10 ms (exact-solver face
-  // states) on 4096 x 16^3 at 4 waves per SIMD, bit for bit the same.
-  const R rat[9] = {R(dt / dx), R(dt / dy), R(dt / dz),
-                    R(0.5 * dt / dx), R(0.5 * dt / dy), R(0.5 * dt / dz),
-                    R(dt * dt / (6. * dy * dz)), R(dt * dt / (6. * dz * dx)),
-                    R(dt * dt / (6. * dx * dy))};
+  // The step's ratios of dt and the cell widths, and the widths, uniform
+  // over the tile.  The FP64 march keeps them in registers (the compiler
+  // hoists them); the FP32 march reads them from LDS, written once per
+  // column tile by one thread (stage B1 first reads them after stage A's
+  // barrier, and the previous column tile ended on a barrier): 12 fewer live
+  // registers, mixed-mode march 6.29 -> 6.08 ms (HLLC) and 7.85 -> 7.08 ms
+  // (exact-solver face states) on 4096 x 16^3 at 4 waves per SIMD, bit for
+  // bit the same.
+  const R rat[12] = {R(dt / dx), R(dt / dy), R(dt / dz),
+                     R(0.5 * dt / dx), R(0.5 * dt / dy), R(0.5 * dt / dz),
+                     R(dt * dt / (6. * dy * dz)), R(dt * dt / (6. * dz * dx)),
+                     R(dt * dt / (6. * dx * dy)), R(dx), R(dy), R(dz)};
   R* const CST = smem + M::CST;
   if constexpr (sizeof(R) == 4) {
     if (tid == 0)
-      for (int i = 0; i < 9; ++i) CST[i] = rat[i];
+      for (int i = 0; i < 12; ++i) CST[i] = rat[i];
   }
   auto ratio = [&](int i) -> R {
     if constexpr (sizeof(R) == 4) return CST[i];
@@ -596,7 +597,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           if (!eig_make<0>(Vc, e0)) { bad = true; return; }
           const R dmm = P.steep ? R(ld_at(tv.v[P_DENS], cb - 2 * sxb)) : R(0.);
           const R dpp = P.steep ? R(ld_at(tv.v[P_DENS], cb + 2 * sxb)) : R(0.);
-          ppm_trace<0>(P, ratio(0), R(dx), Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
+          ppm_trace<0>(P, ratio(0), ratio(9), Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
           R sg[8];
           lds_get6(syk, I::NRY, I::ry(a, b), sg);
           sub_sigma(Wp, Wn, sg, ratio(4));
@@ -623,7 +624,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           if (!eig_make<1>(Vc, e0)) { bad = true; return; }
           const R dmm = P.steep ? R(ld_at(tv.v[P_DENS], cb - 2 * syb)) : R(0.);
           const R dpp = P.steep ? R(ld_at(tv.v[P_DENS], cb + 2 * syb)) : R(0.);
-          ppm_trace<1>(P, ratio(1), R(dy), Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
+          ppm_trace<1>(P, ratio(1), ratio(10), Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
           R sg[8];
           lds_get6(sxk, I::NRX, I::rx(a, b), sg);
           sub_sigma(Wp, Wn, sg, ratio(3));
@@ -653,7 +654,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           if (!eig_make<2>(Vc, e0)) { bad = true; return; }
           const R dmm = P.steep ? R(ld_at(tv.v[P_DENS], cb - 2 * szb)) : R(0.);
           const R dpp = P.steep ? R(ld_at(tv.v[P_DENS], cb + 2 * szb)) : R(0.);
-          ppm_trace<2>(P, ratio(2), R(dz), Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
+          ppm_trace<2>(P, ratio(2), ratio(11), Vm, Vc, Vp, dmm, dpp, dbN, db0, dbP, flat, e0, Wp, Wn);
           R sg[8];
           lds_get6(sxk, I::NRX, I::rx(a, b), sg);
           sub_sigma(Wp, Wn, sg, ratio(3));
