@@ -280,6 +280,20 @@ __device__ __forceinline__ double tv_ld(const TileView& tv, unsigned ob, unsigne
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(tv.rs, (int)cb, (int)ob, 0));
 }
 
+// A stage's item number.  In the FP32 march it passes through an empty asm
+// statement, which the compiler cannot see through: the index arithmetic of
+// each stage's items (item kind, column, cell offsets) is then redone per
+// plane instead of being hoisted out of the plane loop and kept live.  That
+// took the FP32 16x8 march from 160 registers (128 with 21 spilled at 4 waves
+// per SIMD) to 104, and 4096 x 16^3 from 5.95 to 4.81 ms (HLLC) and 6.82 to
+// 5.97 ms (exact-solver face states).  The FP64 march, which LDS holds at 2
+// waves per SIMD whatever its registers, keeps the hoisted form (1 % faster).
+template <typename R>
+__device__ __forceinline__ int item_index(int w) {
+  if constexpr (sizeof(R) == 4) asm volatile("" : "+v"(w));
+  return w;
+}
+
 // byte offset of cell (i, j, k) in its tile
 __device__ __forceinline__ unsigned cell_b(const DevParams& P, int i, int j, int k) {
   return (unsigned)((k * P.ny + j) * P.nx + i) * 8u;
@@ -587,7 +601,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
         }
         zslope_item(v - s2, k + 1);
       };
-      for (int v = tid; v < sEnd; v += NT) itemA(v);
+      for (int v = tid; v < sEnd; v += NT) itemA(item_index<R>(v));
     }
     __syncthreads();
 
@@ -683,7 +697,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           put_state(WZ, NCOL, w2, Wp, Wn);
         }
       };
-      for (int w = tid; w < B3; w += NT) itemB1(w);
+      for (int w = tid; w < B3; w += NT) itemB1(item_index<R>(w));
     }
     __syncthreads();
 
@@ -759,7 +773,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
         if (nonpositive(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st, own);
         put_state(base, n, idx, Wp, Wn);
       };
-      for (int w = tid; w < C3; w += NT) itemB2(w);
+      for (int w = tid; w < C3; w += NT) itemB2(item_index<R>(w));
       __syncthreads();
     }
 
@@ -846,7 +860,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           }
         }
       };
-      for (int w = tid; w < D3; w += NT) itemC(w);
+      for (int w = tid; w < D3; w += NT) itemC(item_index<R>(w));
     }
     __syncthreads();
   }
