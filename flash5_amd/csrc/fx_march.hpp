@@ -280,17 +280,19 @@ __device__ __forceinline__ double tv_ld(const TileView& tv, unsigned ob, unsigne
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(tv.rs, (int)cb, (int)ob, 0));
 }
 
-// A stage's item number.  In the FP32 march it passes through an empty asm
+// A stage's item number.  With OPAQUE it passes through an empty asm
 // statement, which the compiler cannot see through: the index arithmetic of
 // each stage's items (item kind, column, cell offsets) is then redone per
 // plane instead of being hoisted out of the plane loop and kept live.  That
 // took the FP32 16x8 march from 160 registers (128 with 21 spilled at 4 waves
 // per SIMD) to 104, and 4096 x 16^3 from 5.95 to 4.81 ms (HLLC) and 6.82 to
-// 5.97 ms (exact-solver face states).  The FP64 march, which LDS holds at 2
-// waves per SIMD whatever its registers, keeps the hoisted form (1 % faster).
-template <typename R>
+// 5.97 ms (exact-solver face states); the FP64 multi-class kernels from 256
+// registers with 51 spilled to 195.  The single-class FP64 march, which LDS
+// holds at 2 waves per SIMD whatever its registers, keeps the hoisted form
+// (1 % faster).
+template <bool OPAQUE>
 __device__ __forceinline__ int item_index(int w) {
-  if constexpr (sizeof(R) == 4) asm volatile("" : "+v"(w));
+  if constexpr (OPAQUE) asm volatile("" : "+v"(w));
   return w;
 }
 
@@ -384,7 +386,7 @@ __device__ __forceinline__ void write_flux_b(const DevParams& P, gdouble* Fo, un
 // compile-time divisors: x-face states over a in [-1, TX], x faces over
 // a in [0, TX] (y likewise); the high row (a = TX) is skipped unless the
 // tile sits at the block's high side, where it owns that row.
-template <int NDIM, int TX, int TY, int NT, int RS, typename R>
+template <int NDIM, int TX, int TY, int NT, int RS, typename R, bool OPAQUE = (sizeof(R) == 4)>
 __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch& L, int t, int ox,
                                            int oy, R* fsb, ExactStats& es,
                                            R* smem) {
@@ -601,7 +603,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
         }
         zslope_item(v - s2, k + 1);
       };
-      for (int v = tid; v < sEnd; v += NT) itemA(item_index<R>(v));
+      for (int v = tid; v < sEnd; v += NT) itemA(item_index<OPAQUE>(v));
     }
     __syncthreads();
 
@@ -697,7 +699,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           put_state(WZ, NCOL, w2, Wp, Wn);
         }
       };
-      for (int w = tid; w < B3; w += NT) itemB1(item_index<R>(w));
+      for (int w = tid; w < B3; w += NT) itemB1(item_index<OPAQUE>(w));
     }
     __syncthreads();
 
@@ -773,7 +775,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
         if (nonpositive(Wp, Wn)) fallback_first_order(Vc, Wp, Wn, st, own);
         put_state(base, n, idx, Wp, Wn);
       };
-      for (int w = tid; w < C3; w += NT) itemB2(item_index<R>(w));
+      for (int w = tid; w < C3; w += NT) itemB2(item_index<OPAQUE>(w));
       __syncthreads();
     }
 
@@ -860,7 +862,7 @@ __device__ __forceinline__ void march_tile(const DevParams& P, const MarchLaunch
           }
         }
       };
-      for (int w = tid; w < D3; w += NT) itemC(item_index<R>(w));
+      for (int w = tid; w < D3; w += NT) itemC(item_index<OPAQUE>(w));
     }
     __syncthreads();
   }
@@ -907,7 +909,7 @@ k_march_multi(const DevParams* __restrict__ cls,
     const DevParams& P = cls[w.cls];
     const long faceBase = is_states(RS) ? faceStart[w.t] : 0;
     R* fsb = is_states(RS) ? static_cast<R*>(L.fstates) + kFaceComps * faceBase : nullptr;
-    march_tile<NDIM, TX, TY, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, es, smem);
+    march_tile<NDIM, TX, TY, NT, RS, R, true>(P, L, w.t, w.ox, w.oy, fsb, es, smem);
   }
   flush_exact(L.status, es);
 }
@@ -927,8 +929,8 @@ k_march_mixw(const DevParams* __restrict__ cls, MarchLaunch L,
     const DevParams& P = cls[w.cls];
     const long faceBase = is_states(RS) ? faceStart[w.t] : 0;
     R* fsb = is_states(RS) ? static_cast<R*>(L.fstates) + kFaceComps * faceBase : nullptr;
-    if (w.tx == 16) march_tile<NDIM, 16, 8, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, es, smem);
-    else march_tile<NDIM, 8, 8, NT, RS, R>(P, L, w.t, w.ox, w.oy, fsb, es, smem);
+    if (w.tx == 16) march_tile<NDIM, 16, 8, NT, RS, R, true>(P, L, w.t, w.ox, w.oy, fsb, es, smem);
+    else march_tile<NDIM, 8, 8, NT, RS, R, true>(P, L, w.t, w.ox, w.oy, fsb, es, smem);
   }
   flush_exact(L.status, es);
 }
