@@ -23,6 +23,12 @@ void launch_march_r(const DevParams& P, const MarchLaunch& M, int TX, dim3 grid,
       return;
     }
   }
+  if constexpr (sizeof(R) == 8) {  // FP64 3-D 16x8: 768 threads, 3 waves per SIMD
+    if (P.ndim == 3 && TX == 16) {
+      hipLaunchKernelGGL((k_march<3, 16, 8, 768, RS, R, 3>), grid, dim3(768), 0, H.stream, P, M);
+      return;
+    }
+  }
   if (P.ndim == 3 && TX == 16) hipLaunchKernelGGL((k_march<3, 16, 8, 512, RS, R>), grid, dim3(512), 0, H.stream, P, M);
   else if (P.ndim == 3) hipLaunchKernelGGL((k_march<3, 8, 8, 256, RS, R>), grid, dim3(256), 0, H.stream, P, M);
   else if (TX == 16) hipLaunchKernelGGL((k_march<2, 16, 8, 256, RS, R>), grid, dim3(256), 0, H.stream, P, M);
@@ -84,6 +90,12 @@ void launch_march_multi_r(int ndim, int TX, const MarchLaunch& M, const MarchHos
     if (ndim == 3 && H.tuneOcc == 4) {
       if (TX == 16) launch_march_multi_k<3, 16, 8, 512, RS, R, 4>(M, H);
       else launch_march_multi_k<3, 8, 8, 256, RS, R, 4>(M, H);
+      return;
+    }
+  }
+  if constexpr (sizeof(R) == 8) {  // FP64 3-D 16x8: 768 threads, as k_march
+    if (ndim == 3 && TX == 16) {
+      launch_march_multi_k<3, 16, 8, 768, RS, R, 3>(M, H);
       return;
     }
   }
